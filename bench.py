@@ -1,0 +1,215 @@
+"""Headline benchmark: Mpts/s per RK4 step of the 256^3 3-D scalar wave.
+
+Workload (one GPU): u_t + a.grad u = 0, a = (1,1,1)/sqrt(3), on [0,1]^3 with
+256^3 points, an embedded Dirichlet sphere (centre 0.5, radius 0.3), Dirichlet
+inflow faces (xmin, ymin, zmin) and floating outflow faces, 4th-order cut-cell
+operators (scheme order 1 "E4", alpha = {0.1, 0.7}), FP64, gaussian pulse
+(amplitude 1, width 0.05, centre 0.25), dt = 0.5 h.  Inputs are resident in
+HBM when the timed region starts.
+
+Forms measured:
+  * fused   -- RHS + stage update + boundary update in one kernel per RK4
+               stage over three registers (the `value`);
+  * plain   -- the reference's schedule, one kernel per reference operation
+               (zero fills, block + full-length CSR per direction, dot, slot
+               ops, boundary update, u0 <- u1 copy).
+
+Prints ONE JSON line.  Also times the CPU restatement (oracle) on a bounded
+sample on the host cores (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_SIDE = 256
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per point of one fused RK4 step over three FP64 registers:
+# stage 1 R u0 W k W u1 (24), stages 2-3 R us R u0 R k W k W us' (40 each),
+# stage 4 R us R u0 R k W u0 (32)
+FUSED_BYTES_PER_PT_STEP = 136.0
+# the reference's schedule (init 32 + assign 72 + 4 x RHS 152 + accumulate 96
+# + final 24) plus the driver's u0 <- u1 copy (16)
+PLAIN_BYTES_PER_PT_STEP = 848.0
+
+
+def workload_cfg(n=N_SIDE):
+    a = 1.0 / 3.0 ** 0.5
+    import shoccs_amd as S
+
+    return dict(kind=S.SYS_WAVE_LINEAR, n=(n, n, n), lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                spheres=[[0.5, 0.5, 0.5, 0.3]], shape_bc=[S.BC_DIRICHLET],
+                grid_bc=[S.BC_DIRICHLET, S.BC_FLOATING] * 3, scheme=S.E4_1, alpha=(0.1, 0.7),
+                params=[a, a, a, 0.25, 0.25, 0.25, 0.05, 1.0])
+
+
+def read_traffic():
+    """HBM bytes per launch of the fused stage kernel from the committed
+    rocprofv3 --pmc summary, when present."""
+    path = os.path.join(ROOT, "profiles", "rk4_stage_wave256_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch"):
+            try:
+                return float(line.split("=")[1])
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(steps: int):
+    """Oracle (reference-structured FP64 schedule, OpenMP) on the host cores:
+    `steps` RK4 steps of the same 256^3 workload after one warm-up step."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+
+    import oracle_ffi as O
+
+    threads = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        threads = min(threads, int(env))
+    O.lib().orc_set_threads(threads)
+    a = 1.0 / 3.0 ** 0.5
+    st = O.stencil(O.E4_1, (0.1, 0.7))
+    osys = O.OracleSystem(O.SYS_WAVE_LINEAR, (N_SIDE,) * 3, (0, 0, 0), (1, 1, 1),
+                          [[0.5, 0.5, 0.5, 0.3]], [O.DIRICHLET], [O.DIRICHLET, O.FLOATING] * 3, st,
+                          [a, a, a, 0.25, 0.25, 0.25, 0.05, 1.0])
+    u0 = osys.field()
+    osys.initialize(u0, 0.0)
+    osys.update_boundary(u0, 0.0)
+    out, rk, srhs = osys.field(), osys.field(), osys.field()
+    dt = osys.dt(0.5, 1.0)
+    t = 0.0
+
+    def step():
+        nonlocal t
+        osys.rk4_step(u0, out, rk, srhs, t, dt)
+        u0.D[:] = out.D
+        for d in range(3):
+            u0.R[d][:] = out.R[d]
+        t += dt
+
+    step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    sec = (time.perf_counter() - t0) / steps
+    N = float(N_SIDE) ** 3
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    del np
+    return {"value": N / sec / 1e6, "unit": "Mpts/s per RK4 step", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} RK4 steps of the 256^3 workload after 1 warm-up step "
+                      f"({sec:.2f} s/step, reference-structured FP64 schedule, OpenMP, {cpu_model})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--n", type=int, default=N_SIDE)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); every box here has one "
+              "GPU and the multi-GPU slab decomposition is design-only.", file=sys.stderr)
+        sys.exit(2)
+
+    import shoccs_amd as S
+
+    ctx = S.Context(0)
+    cfg = workload_cfg(args.n)
+    prob = S.Problem(ctx, cfg["kind"], cfg["n"], cfg["lo"], cfg["hi"], spheres=cfg["spheres"],
+                     shape_bc=cfg["shape_bc"], grid_bc=cfg["grid_bc"], scheme=cfg["scheme"],
+                     alpha=cfg["alpha"], params=cfg["params"])
+    N = prob.N
+    dt = prob.dt(0.5, 1.0)
+    u0 = prob.field()
+    prob.initialize(u0, 0.0)
+    prob.update_boundary(u0, 0.0)
+
+    # ---- fused form: warm-up, then exactly K timed steps
+    prob.rk4_fused(u0, 0.0, dt, args.warmup)
+    t_start = args.warmup * dt
+    ctx.sync()
+    w0 = time.perf_counter()
+    ms_step = prob.time_fused(u0, t_start, dt, args.steps, 1)
+    ctx.sync()
+    wall_ms_step = (time.perf_counter() - w0) * 1e3 / args.steps
+    t_end = t_start + args.steps * dt
+    stats = prob.stats(u0, t_end)
+
+    # ---- dominant kernel: the fused stage kernel, event-timed per launch
+    ms_kernel = prob.time_fused_stage_kernel(u0, t_end, dt, 5)
+    achieved = FUSED_BYTES_PER_PT_STEP / 4.0 * N / (ms_kernel * 1e-3) / 1e9
+    traffic = read_traffic()
+
+    # ---- plain (reference-structured) FP64 form
+    p_out, p_rk, p_srhs = prob.field(), prob.field(), prob.field()
+    prob.time_unfused(u0, p_out, p_rk, p_srhs, 0.0, dt, 1)
+    ms_plain = prob.time_unfused(u0, p_out, p_rk, p_srhs, 0.0, dt, args.plain_steps)
+
+    mpts = N / (ms_step * 1e-3) / 1e6
+    result = {
+        "metric": "Mpts/s per RK4 step at 256^3 (plain FP64 vs batched FP32), % HBM roofline",
+        "value": mpts,
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3-D scalar wave u_t + a.grad u = 0, {args.n}^3, Dirichlet sphere r=0.3, "
+                        "E4 cut-cell (alpha 0.1, 0.7), gaussian pulse, dt = 0.5h, RK4",
+            "form": "fused FP64: one kernel per RK4 stage over three registers",
+            "parallelism": "1 GPU",
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic},
+        "fused_fp64": {"mpts": mpts, "ms_per_step": ms_step, "wall_ms_per_step": wall_ms_step,
+                       "stage_kernel_ms": ms_kernel,
+                       "algorithmic_bytes_per_pt_step": FUSED_BYTES_PER_PT_STEP},
+        "plain_fp64": {"mpts": N / (ms_plain * 1e-3) / 1e6, "ms_per_step": ms_plain,
+                       "algorithmic_bytes_per_pt_step": PLAIN_BYTES_PER_PT_STEP,
+                       "hbm_frac": PLAIN_BYTES_PER_PT_STEP * N / (ms_plain * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "fused_over_plain": ms_plain / ms_step,
+        "linf_error_vs_exact": stats[0],
+        "final_time": t_end,
+    }
+    for f in (p_out, p_rk, p_srhs, u0):
+        f.free()
+    prob.close()
+    ctx.close()
+    if not args.no_cpu_baseline and args.cpu_steps > 0 and args.n == N_SIDE:
+        result["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

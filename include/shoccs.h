@@ -1,0 +1,83 @@
+/*
+ * shoccs.h -- host-side C entry points of the SHOCCS MI355X library: build a
+ * problem (mesh with embedded spheres, boundary conditions, stencil, system)
+ * with the C++20 host library, inspect its assembled structures, and move
+ * fields between host and device.  The compute calls themselves are the
+ * shx_* functions of shoccs_hip.h, applied to the handles returned here.
+ *
+ * A problem replaces what the reference builds from a Lua `simulation` table
+ * in system::from_lua (src/systems/system.cpp:103-132): mesh::from_lua
+ * (src/mesh/mesh.cpp:209-220), bcs::from_lua (src/operators/boundaries.cpp:41-90),
+ * stencil::from_lua (src/stencils/stencil.cpp:8-106) and the system ctor
+ * (src/systems/heat.cpp:22-51, src/systems/scalar_wave.cpp:44-102).
+ */
+#ifndef SHOCCS_H
+#define SHOCCS_H
+
+#include <stdint.h>
+
+#include "shoccs_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* stencil schemes: scheme{order=2,type="E2"|"E4"}, scheme{order=1,type="E4"|"E4u"} */
+#define SHOCCS_E2_2 0
+#define SHOCCS_E4_2 1
+#define SHOCCS_E4_1 2
+#define SHOCCS_E4U_1 3
+
+typedef struct shoccs_problem shoccs_problem;
+
+typedef struct shoccs_problem_desc {
+    int32_t kind;          /* SHX_SYS_WAVE_RADIAL, SHX_SYS_WAVE_LINEAR, SHX_SYS_HEAT */
+    int32_t n[3];          /* index_extents (1 for unused axes) */
+    double min[3], max[3]; /* domain_bounds */
+    int32_t nshapes;
+    const double* spheres; /* nshapes x {cx, cy, cz, r} */
+    const int32_t* shape_bc; /* per shape boundary_condition */
+    int32_t grid_bc[6];    /* xmin xmax ymin ymax zmin zmax */
+    int32_t scheme;
+    double alpha[2];
+    /* system parameters:
+     *  WAVE_RADIAL: cx cy cz radius [max_error]
+     *  WAVE_LINEAR: vx vy vz x0 y0 z0 width amplitude [max_error]
+     *  HEAT:        diffusivity mms_kind [n_gauss, n_gauss x {c[3], var[3], amp, freq}] */
+    const double* params;
+    int32_t nparams;
+} shoccs_problem_desc;
+
+int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_problem** out);
+int shoccs_problem_destroy(shoccs_problem* p);
+shx_system* shoccs_problem_system(shoccs_problem* p);
+shx_derivative* shoccs_problem_derivative(shoccs_problem* p, int dir);
+/* which = 0: mesh points N; 1, 2, 3: |Rx|, |Ry|, |Rz|; 4: fluid points */
+int64_t shoccs_problem_size(const shoccs_problem* p, int which);
+/* system::timestep_size (heat.cpp:356-361, scalar_wave.cpp:357-362) */
+double shoccs_problem_dt(const shoccs_problem* p, double h_cfl, double p_cfl);
+
+/* assembled structures, for bit-exact comparison with the reference's */
+int shoccs_mesh_r(const shoccs_problem* p, int dir, double* psi, double* pos, int32_t* solid,
+                  int32_t* ray_outside, int32_t* shape_id);
+int64_t shoccs_mesh_nlines(const shoccs_problem* p, int dir);
+/* per line: start[dir], end[dir], start R index or -1, end R index or -1,
+ * slow, fast, stride */
+int shoccs_mesh_lines(const shoccs_problem* p, int dir, int64_t* out);
+int shoccs_mesh_fluid(const shoccs_problem* p, int64_t* out);
+int64_t shoccs_deriv_nblocks(const shoccs_problem* p, int dir);
+int shoccs_deriv_blocks(const shoccs_problem* p, int dir, shx_line_block* out);
+int64_t shoccs_deriv_npool(const shoccs_problem* p, int dir);
+int shoccs_deriv_pool(const shoccs_problem* p, int dir, double* out);
+int shoccs_deriv_interior(const shoccs_problem* p, int dir, double* out9);
+
+/* device fields of the problem's shape: D then Rx, Ry, Rz */
+int shoccs_field_alloc(shoccs_problem* p, shx_field* f);
+int shoccs_field_free(shoccs_problem* p, shx_field* f);
+int shoccs_field_upload(shoccs_problem* p, shx_field* f, const double* flat);
+int shoccs_field_download(shoccs_problem* p, const shx_field* f, double* flat);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

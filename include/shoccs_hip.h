@@ -1,0 +1,226 @@
+/*
+ * shoccs_hip.h -- the device boundary of the SHOCCS discrete-operator path on
+ * one MI355X (gfx950).
+ *
+ * Plain C ABI: device pointers, host pointers and sizes only.  Every function
+ * returns 0 on success and a negative code on error; shx_last_error() returns
+ * the message of the last failure on a context.  No exception crosses this
+ * boundary.  All work of one context runs on that context's single HIP stream;
+ * calls on one context must be serialized by the caller.  Entry points that
+ * replace a synchronous reference call are synchronous unless their name ends
+ * in _async.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference tree).  The reference is host-only C++ (Kokkos host
+ * backend); it has no FFI of its own, so this header is the binding point a
+ * maintainer would target from the reference's C++ classes.
+ */
+#ifndef SHOCCS_HIP_H
+#define SHOCCS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHX_OK 0
+#define SHX_EINVAL (-1)
+#define SHX_EHIP (-2)
+#define SHX_EUNSUPPORTED (-3)
+#define SHX_ENOMEM (-4)
+
+/* boundary condition types, src/operators/boundaries.hpp:13-20 */
+#define SHX_BC_DIRICHLET 0
+#define SHX_BC_FLOATING 1
+#define SHX_BC_NEUMANN 2
+
+typedef struct shx_ctx shx_ctx;
+typedef struct shx_derivative shx_derivative;
+typedef struct shx_csr shx_csr;
+typedef struct shx_system shx_system;
+
+/* A scalar field: the mesh buffer D (nx*ny*nz, z fastest) and the three
+ * cut-point buffers Rx, Ry, Rz.  Replaces ccs::scalar_span / scalar_view
+ * (src/fields/scalar.hpp:13-67).  All four are device pointers. */
+typedef struct shx_field {
+    double* D;
+    double* R[3];
+} shx_field;
+
+/* One line segment of a derivative operator: a [dense | circulant | dense]
+ * inner block placed along direction `dir` on the line identified by `pair`
+ * (pair = slow * n_fast + fast, slow/fast as src/indexing.hpp:21-37).
+ * Positions are coordinates along `dir`.  Replaces the per-line
+ * inner_block_meta of src/matrices/inner_block_meta.hpp:9-23 with positions
+ * instead of flat offsets, plus the B-matrix entries of the line's body
+ * closures (src/operators/derivative.cpp:362-365,407-412) folded in. */
+typedef struct shx_line_block {
+    int32_t pair;  /* line identifier */
+    int32_t lo;    /* position of the first row */
+    int32_t nl;    /* left closure rows */
+    int32_t ni;    /* circulant (interior) rows; negative when closures overlap */
+    int32_t nr;    /* right closure rows */
+    int32_t lcols; /* left closure columns */
+    int32_t lcol0; /* position of the left closure's first column */
+    int32_t lcoef; /* offset of the left closure (row-major) in the pool */
+    int32_t rcols;
+    int32_t rcol0;
+    int32_t rcoef;
+    int32_t lobj;  /* R(dir) index of the body at the left end, or -1 */
+    int32_t lbw;   /* pool offset of the nl B weights of the left body */
+    int32_t robj;  /* R(dir) index of the body at the right end, or -1 */
+    int32_t rbw;   /* pool offset of the nr B weights of the right body */
+    int32_t flags; /* bit0 ldd, bit1 rdd (src/matrices/common.hpp:24-25) */
+} shx_line_block;
+
+/* Everything the device needs for one direction's derivative. */
+typedef struct shx_derivative_desc {
+    int32_t dir;
+    int32_t n[3];            /* mesh extents */
+    int64_t nR[3];           /* |R(x)|, |R(y)|, |R(z)| */
+    int32_t width;           /* interior stencil width 2p+1 (<= 9) */
+    double interior[9];      /* interior stencil coefficients */
+    const shx_line_block* blocks; /* sorted by (pair, lo) */
+    int64_t nblocks;
+    const double* pool;      /* closure coefficients and B weights */
+    int64_t npool;
+    /* Neumann rows, N matrix (derivative.cpp:375-379,422-426): row, col, w */
+    const int64_t* neu_row;
+    const int64_t* neu_col;
+    const double* neu_w;
+    int64_t nneu;
+} shx_derivative_desc;
+
+/* ---------------------------------------------------------------- context */
+int shx_init(int device, shx_ctx** ctx);
+int shx_finalize(shx_ctx* ctx);
+const char* shx_last_error(const shx_ctx* ctx);
+int shx_sync(shx_ctx* ctx);
+int shx_alloc(shx_ctx* ctx, size_t bytes, void** dptr);
+int shx_free(shx_ctx* ctx, void* dptr);
+int shx_h2d(shx_ctx* ctx, void* dst, const void* src, size_t bytes);
+int shx_d2h(shx_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* dst[i] = value for i < n (a kernel, safe inside stream capture) */
+int shx_fill(shx_ctx* ctx, double* dst, double value, int64_t n);
+
+/* -------------------------------------------------------------- operators */
+/* ccs::derivative::derivative (src/operators/derivative.cpp:452-473): upload
+ * one direction's assembled operator. */
+int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc,
+                          shx_derivative** out);
+int shx_derivative_destroy(shx_derivative* d);
+
+/* ccs::derivative::operator()(u, du, op) and (u, nu, du, op)
+ * (src/operators/derivative.cpp:502-519): du.D op= O u.D, du.D += B u.R[dir]
+ * (+= N nu.D when nu is given).  accumulate = 0 is eq_t, 1 is plus_eq_t.
+ * R-space rows of du are untouched (bodies are Dirichlet).  Synchronous. */
+int shx_derivative_apply(shx_derivative* d, const shx_field* u, const shx_field* nu,
+                         shx_field* du, int accumulate);
+
+/* ccs::matrix::block::operator()(x, b, op) (src/matrices/block.hpp:185-194):
+ * the O part alone, every line of one direction in one launch. */
+int shx_block_apply(shx_derivative* d, const double* x, double* b, int accumulate);
+
+/* ccs::matrix::csr (src/matrices/csr.hpp:16-92, csr.cpp:10-51) in the
+ * reference's own full-length layout (one row pointer per row); apply is
+ * always b[row] += sum w*x[col]. */
+int shx_csr_create(shx_ctx* ctx, int64_t rows, const int64_t* rowptr, const int64_t* col,
+                   const double* w, shx_csr** out);
+int shx_csr_destroy(shx_csr* A);
+int shx_csr_apply(shx_csr* A, const double* x, double* b);
+
+/* ccs::gradient::operator() (src/operators/gradient.cpp:34-46): zero the
+ * three outputs, then dx, dy, dz with eq_t on the directions with extent > 1.
+ * d[i] may be NULL for a direction of extent 1. */
+int shx_gradient_apply(shx_derivative* const d[3], const shx_field* u, shx_field* const du[3]);
+
+/* ccs::laplacian::operator() (src/operators/laplacian.cpp:37-58): zero du,
+ * then dx, dy, dz with plus_eq_t (and N nu when nu is given). */
+int shx_laplacian_apply(shx_derivative* const d[3], const shx_field* u, const shx_field* nu,
+                        shx_field* du);
+
+/* ---------------------------------------------------------------- systems */
+#define SHX_SYS_WAVE_RADIAL 0 /* src/systems/scalar_wave.cpp:28-40 */
+#define SHX_SYS_WAVE_LINEAR 1 /* constant velocity, translated gaussian */
+#define SHX_SYS_HEAT 2        /* src/systems/heat.cpp */
+#define SHX_MMS_GAUSS 0       /* src/mms/gauss3d.cpp */
+#define SHX_MMS_SINPROD 1     /* sin(2 pi x) sin(2 pi y) [sin(2 pi z)] exp(-t) */
+
+typedef struct shx_system_desc {
+    int32_t kind;
+    int32_t n[3];
+    const double* x; /* host coordinates, n[0], n[1], n[2] values */
+    const double* y;
+    const double* z;
+    double h[3];
+    int32_t grid_bc[6];   /* xmin xmax ymin ymax zmin zmax */
+    int64_t nR[3];
+    const double* r_pos[3];     /* host, 3 * nR[d] cut-point positions */
+    const int32_t* r_dirichlet[3]; /* host, nR[d] flags: body is Dirichlet */
+    const int64_t* fluid;       /* host fluid gather (mesh::fluid_desc) */
+    int64_t nfluid;
+    shx_derivative* d[3];       /* NULL for a direction of extent 1 */
+    /* parameters */
+    double center[3], radius;   /* radial wave */
+    double vel[3], x0[3], width, amp; /* linear wave */
+    double diffusivity;         /* heat */
+    int32_t mms_kind, mms_dims, mms_n;
+    const double* mms;          /* mms_n x {center[3], variance[3], amplitude, frequency} */
+} shx_system_desc;
+
+int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** out);
+int shx_system_destroy(shx_system* s);
+
+/* system::rhs / submit_rhs_graph (src/systems/system.cpp:34-69), fused:
+ * scalar wave (scalar_wave.cpp:195-239): rhs = gG . grad u
+ * heat (heat.cpp:119-162): rhs = k lap u + (S_t - k lap S) on the fluid, 0 on
+ * Dirichlet faces and Dirichlet cut points.  One kernel for D. */
+int shx_system_rhs(shx_system* s, const shx_field* u, shx_field* rhs, double t);
+
+/* The same RHS in the reference's unfused schedule: zero fills, one block
+ * kernel and one full-length CSR kernel per direction, then the dot /
+ * scale / source / fill kernels.  Used as the plain FP64 GPU form. */
+int shx_system_rhs_unfused(shx_system* s, const shx_field* u, shx_field* rhs, double t);
+
+/* system::update_boundary (heat.cpp:296-354, scalar_wave.cpp:329-355):
+ * Dirichlet grid faces of D and Dirichlet cut points take the exact solution. */
+int shx_system_update_boundary(shx_system* s, shx_field* u, double t);
+/* initialize_scalar_field (src/systems/detail/scalar_system_utils.hpp:186-219) */
+int shx_system_initialize(shx_system* s, shx_field* u, double t);
+/* eval_at_locations of the exact solution */
+int shx_system_exact(shx_system* s, shx_field* out, double t);
+/* compute_scalar_stats (scalar_system_utils.hpp:77-181): 11 doubles */
+int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[11]);
+
+/* integrators::rk4::operator() (src/temporal/rk4.cpp:16-47) with the
+ * reference's slot schedule (zero rk/srhs, copy, 4 x {assign_lc,
+ * update_boundary, rhs, accumulate}, final assign_lc + update_boundary), one
+ * kernel per reference operation: the plain FP64 GPU form. */
+int shx_rk4_step(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk,
+                 shx_field* srhs, double t, double dt);
+
+/* The same step with the RHS, the stage update and the boundary update fused
+ * into one kernel per stage over three registers (u0, stage, accumulator):
+ * u0 is advanced in place by nsteps steps of size dt from time t.  Scratch
+ * registers are owned by the system.  Asynchronous. */
+int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt, int nsteps);
+
+/* ------------------------------------------------------------- measurement */
+/* Run `nsteps` fused steps `reps` times and report the mean device time per
+ * step in milliseconds, measured with HIP events on the context's stream. */
+int shx_time_fused_steps(shx_system* s, shx_field* u0, double t, double dt, int nsteps,
+                         int reps, double* ms_per_step);
+int shx_time_unfused_steps(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk,
+                           shx_field* srhs, double t, double dt, int nsteps,
+                           double* ms_per_step);
+/* Mean device time of one fused stage kernel over `nsteps` fused steps, each
+ * launch bracketed by a HIP event pair on the context's stream. */
+int shx_time_fused_stage_kernel(shx_system* s, shx_field* u0, double t, double dt, int nsteps,
+                                double* ms_per_kernel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,200 @@
+/*
+ * oracle.h -- CPU restatement of the SHOCCS discrete-operator path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against.  It may be linked or loaded only by tests/, by
+ * __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.  Nothing in
+ * the product (shoccs_amd/) may include, link or call it.
+ *
+ * It restates, in plain C and in the reference's own evaluation order, the
+ * functions listed below (file:line cites are relative to the reference tree):
+ *
+ *   geometry   src/mesh/cartesian.cpp:13-44, src/fields/lazy_views.hpp:563-569,
+ *              src/mesh/sphere.cpp:15-45, src/mesh/object_geometry.cpp:17-116,
+ *              src/mesh/mesh.cpp:29-111,154-168
+ *   stencils   src/stencils/E4_1.cpp:66-342, E4u_1.cpp:49-126,
+ *              E4_2.cpp:33-42,185-296, E2_2.cpp:95-175
+ *   assembly   src/operators/derivative.cpp:259-449 (domain_discretization),
+ *              src/matrices/csr.cpp:10-35 (builder), block.hpp:30-94 (meta)
+ *   apply      src/matrices/block.hpp:117-177 (matvec_functor),
+ *              src/matrices/csr.cpp:37-51, src/operators/derivative.cpp:475-519,
+ *              src/operators/gradient.cpp:34-46, laplacian.cpp:37-58
+ *   systems    src/systems/scalar_wave.cpp:28-362, heat.cpp:111-361,
+ *              src/systems/detail/scalar_system_utils.hpp:16-219
+ *   temporal   src/temporal/rk4.cpp:11-47, slot_ops.hpp:11-59
+ *
+ * Arithmetic is compiled with -ffp-contract=off so that every sum and product
+ * rounds exactly as the reference's x86-64 build does.
+ *
+ * Not restated (documented as gaps): cut_discretization for non-Dirichlet
+ * bodies (src/operators/derivative.cpp:159-257); such configurations are
+ * rejected with an error code.
+ */
+#ifndef SHOCCS_ORACLE_H
+#define SHOCCS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* boundary condition types, src/operators/boundaries.hpp:13-20 */
+enum { ORC_DIRICHLET = 0, ORC_FLOATING = 1, ORC_NEUMANN = 2 };
+
+/* stencil families, src/stencils/stencil.cpp:8-106 */
+enum { ORC_E2_2 = 0, ORC_E4_2 = 1, ORC_E4_1 = 2, ORC_E4U_1 = 3 };
+
+typedef struct { int p, r, t, x; } orc_info;
+typedef struct { int kind; double alpha[2]; } orc_stencil;
+
+/* src/mesh/mesh_types.hpp:30-38 */
+typedef struct {
+    double psi;
+    double pos[3];
+    double normal[3];
+    int ray_outside;
+    int solid[3];
+    int shape_id;
+} orc_rinfo;
+
+/* src/mesh/mesh_types.hpp:8-24 */
+typedef struct {
+    int mesh[3];
+    int has_obj;
+    long obj_coord;
+    long obj_id;
+    double psi;
+} orc_bound;
+
+typedef struct {
+    long stride;
+    orc_bound start, end;
+} orc_line;
+
+typedef struct {
+    int n[3];
+    double min[3], max[3], h[3];
+    int dims;
+    double* xyz[3];
+    int nshapes;
+    double* spheres; /* nshapes x {cx, cy, cz, r} */
+    orc_rinfo* R[3];
+    long nR[3];
+    orc_line* lines[3];
+    long nlines[3];
+    long* fluid;
+    long nfluid;
+} orc_mesh;
+
+/* one inner_block in block-kernel (matvec_functor) terms, block.hpp:30-94 */
+typedef struct {
+    long row_offset, col_offset, stride;
+    long left_rows, left_cols;
+    double* left;
+    long interior_rows;
+    long right_rows, right_cols, right_col_offset;
+    double* right;
+    int flags; /* 1 = ldd, 2 = rdd (src/matrices/common.hpp:24-25) */
+} orc_iblock;
+
+typedef struct {
+    long rows;
+    long nnz;
+    long* u; /* row pointer, rows + 1 */
+    long* v; /* column */
+    double* w;
+} orc_csr;
+
+typedef struct {
+    int dir;
+    long N;
+    int w;
+    double interior[9];
+    orc_iblock* blocks;
+    long nblocks;
+    orc_csr B, Nm;
+} orc_deriv;
+
+/* a scalar field: D plus the three cut-point buffers, src/fields/scalar.hpp */
+typedef struct {
+    double* D;
+    double* R[3];
+} orc_field;
+
+/* systems */
+enum { ORC_SYS_WAVE_RADIAL = 0, ORC_SYS_WAVE_LINEAR = 1, ORC_SYS_HEAT = 2 };
+/* manufactured solutions for heat */
+enum { ORC_MMS_GAUSS = 0, ORC_MMS_SINPROD = 1 };
+
+typedef struct {
+    int kind;
+    orc_mesh* m;
+    int grid_bc[6];
+    int* obj_bc;
+    orc_deriv* d[3];
+    /* radial scalar wave (scalar_wave.cpp:28-40) */
+    double center[3], radius;
+    /* linear advection extension: constant velocity, gaussian pulse */
+    double vel[3], x0[3], width, amp;
+    double max_error;
+    /* heat */
+    double diffusivity;
+    int mms_kind;
+    int mms_dims;
+    int mms_n;
+    double* mms; /* per gaussian: center[3], variance[3], amplitude, frequency */
+    /* work buffers */
+    orc_field gG[3];
+    orc_field du[3];
+    orc_field src;
+    orc_field neu;
+} orc_system;
+
+/* ------------------------------------------------------------------ API */
+void orc_set_threads(int n);
+
+orc_mesh* orc_mesh_create(const int n[3], const double min[3], const double max[3],
+                          int nshapes, const double* spheres);
+void orc_mesh_destroy(orc_mesh*);
+long orc_mesh_size(const orc_mesh*);
+int orc_dirichlet_line(const orc_mesh* m, const int start[3], int dir, const int bc[6]);
+
+orc_info orc_query(const orc_stencil* st, int bc);
+orc_info orc_query_max(const orc_stencil* st);
+void orc_interior(const orc_stencil* st, double h, double* c);
+void orc_nbs(const orc_stencil* st, double h, int bc, double psi, int right, double* c,
+             double* extra);
+
+/* returns NULL on unsupported configuration */
+orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
+                            const int grid_bc[6], const int* obj_bc);
+void orc_deriv_destroy(orc_deriv*);
+void orc_block_apply(const orc_deriv* d, const double* x, double* b, int plus_eq);
+void orc_csr_apply(const orc_csr* A, const double* x, double* b);
+void orc_deriv_apply(const orc_deriv* d, const orc_field* u, const orc_field* nu,
+                     orc_field* du, int plus_eq);
+
+orc_system* orc_system_create(int kind, const int n[3], const double min[3],
+                              const double max[3], int nshapes, const double* spheres,
+                              const int* obj_bc, const int grid_bc[6],
+                              const orc_stencil* st, const double* params, int nparams);
+void orc_system_destroy(orc_system*);
+orc_mesh* orc_system_mesh(orc_system*);
+orc_deriv* orc_system_deriv(orc_system*, int dir);
+void orc_system_rhs(orc_system*, const orc_field* u, orc_field* rhs, double t);
+void orc_system_update_boundary(orc_system*, orc_field* u, double t);
+void orc_system_initialize(orc_system*, orc_field* u, double t);
+void orc_system_exact(orc_system*, orc_field* out, double t);
+void orc_system_stats(orc_system*, const orc_field* u, double t, double stats[11]);
+double orc_system_dt(orc_system*, double h_cfl, double p_cfl);
+void orc_rk4_step(orc_system*, orc_field* u0, orc_field* out, orc_field* rk,
+                  orc_field* srhs, double t, double dt);
+
+void orc_gradient_apply(const orc_system* s, const orc_field* u, orc_field* dux,
+                        orc_field* duy, orc_field* duz);
+void orc_laplacian_apply(const orc_system* s, const orc_field* u, const orc_field* nu,
+                         orc_field* du);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,299 @@
+"""shoccs_amd -- the SHOCCS discrete-operator path on one MI355X.
+
+Host-side mirror of the reference's operator interface over the C ABI of
+libshoccs.so:
+
+* ``Problem``   -- mesh + boundary conditions + stencil + system, as built by
+  ``system::from_lua`` (src/systems/system.cpp:103-132);
+* ``Field``     -- a device scalar {D, Rx, Ry, Rz} (src/fields/scalar.hpp);
+* ``Problem.derivative(dir)(u, du, op)`` -- ``derivative::operator()``
+  (src/operators/derivative.cpp:502-519);
+* ``Problem.gradient(u)(dux, duy, duz)`` / ``Problem.laplacian(u)(du)``
+  (src/operators/gradient.cpp:34-46, laplacian.cpp:37-58);
+* ``Problem.rhs``, ``update_boundary``, ``initialize``, ``stats``
+  (src/systems/system.hpp:22-65);
+* ``Problem.rk4`` -- ``integrators::rk4::operator()`` (src/temporal/rk4.cpp:16-47)
+  and ``Problem.rk4_fused`` -- the same step fused into one kernel per stage.
+
+No compute call has a CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, E2_2, E4_1, E4_2, E4U_1, MMS_GAUSS,
+                   MMS_SINPROD, SYS_HEAT, SYS_WAVE_LINEAR, SYS_WAVE_RADIAL, ProblemDesc,
+                   ShoccsError, ShxField, ShxLineBlock, check, lib)
+
+__all__ = [
+    "Context", "Problem", "Field", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
+    "E2_2", "E4_2", "E4_1", "E4U_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
+    "MMS_GAUSS", "MMS_SINPROD", "eq", "plus_eq",
+]
+
+eq = 0
+plus_eq = 1
+
+
+class Context:
+    """One device and one HIP stream (shx_init)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        p = C.c_void_p()
+        rc = L.shx_init(device, C.byref(p))
+        if rc != 0:
+            raise ShoccsError(f"shx_init({device}) failed with code {rc}: no usable GPU")
+        self.p = p
+
+    def sync(self):
+        check(lib().shx_sync(self.p), self.p, "shx_sync")
+
+    def close(self):
+        if getattr(self, "p", None):
+            lib().shx_finalize(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Field:
+    """A device scalar field of a problem's shape."""
+
+    def __init__(self, problem: "Problem"):
+        self.problem = problem
+        self.c = ShxField()
+        check(lib().shoccs_field_alloc(problem.p, C.byref(self.c)), problem.ctx.p, "field alloc")
+
+    def ref(self):
+        return C.byref(self.c)
+
+    def upload(self, flat: np.ndarray):
+        a = np.ascontiguousarray(flat, dtype=np.float64)
+        assert a.size == self.problem.flat_size
+        check(lib().shoccs_field_upload(self.problem.p, C.byref(self.c),
+                                        a.ctypes.data_as(C.POINTER(C.c_double))),
+              self.problem.ctx.p, "field upload")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.problem.flat_size)
+        check(lib().shoccs_field_download(self.problem.p, C.byref(self.c),
+                                          out.ctypes.data_as(C.POINTER(C.c_double))),
+              self.problem.ctx.p, "field download")
+        return out
+
+    def free(self):
+        if getattr(self, "c", None) is not None and self.problem.p:
+            lib().shoccs_field_free(self.problem.p, C.byref(self.c))
+            self.c = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Problem:
+    """Mesh, boundary conditions, stencil and system on one device.
+
+    ``ctx=None`` builds the host structures only (no device, no compute).
+    """
+
+    def __init__(self, ctx: Context | None, kind: int, n, lo, hi, spheres=(), shape_bc=(),
+                 grid_bc=(BC_FLOATING,) * 6, scheme: int = E4_1, alpha=(0.0, 0.0), params=()):
+        L = lib()
+        self.ctx = ctx
+        d = ProblemDesc()
+        d.kind = kind
+        n = list(n) + [1] * (3 - len(n))
+        for a in range(3):
+            d.n[a] = int(n[a])
+            d.min[a] = float(lo[a])
+            d.max[a] = float(hi[a])
+        sp = np.ascontiguousarray(np.array(spheres, dtype=np.float64).reshape(-1, 4))
+        bc = np.ascontiguousarray(np.array(list(shape_bc) + [0], dtype=np.int32))
+        pr = np.ascontiguousarray(np.array(list(params) + [0.0], dtype=np.float64))
+        self._keep = (sp, bc, pr)
+        d.nshapes = len(sp)
+        d.spheres = sp.ctypes.data_as(C.POINTER(C.c_double))
+        d.shape_bc = bc.ctypes.data_as(C.POINTER(C.c_int32))
+        for f in range(6):
+            d.grid_bc[f] = int(grid_bc[f])
+        d.scheme = scheme
+        d.alpha[0], d.alpha[1] = alpha
+        d.params = pr.ctypes.data_as(C.POINTER(C.c_double))
+        d.nparams = len(pr) - 1
+        p = C.c_void_p()
+        rc = L.shoccs_problem_create(ctx.p if ctx else None, C.byref(d), C.byref(p))
+        check(rc, ctx.p if ctx else None, "shoccs_problem_create")
+        self.p = p
+        self.n = tuple(n)
+        self.kind = kind
+        self.N = L.shoccs_problem_size(p, 0)
+        self.nR = tuple(L.shoccs_problem_size(p, a + 1) for a in range(3))
+        self.flat_size = self.N + sum(self.nR)
+        self.sys = L.shoccs_problem_system(p)
+
+    def close(self):
+        if getattr(self, "p", None):
+            lib().shoccs_problem_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -------------------------------------------------------- structures
+    def r_arrays(self, d: int) -> dict:
+        n = self.nR[d]
+        psi = np.zeros(max(n, 1))
+        pos = np.zeros((max(n, 1), 3))
+        solid = np.zeros((max(n, 1), 3), np.int32)
+        ro = np.zeros(max(n, 1), np.int32)
+        sid = np.zeros(max(n, 1), np.int32)
+        check(lib().shoccs_mesh_r(self.p, d, psi.ctypes.data_as(C.POINTER(C.c_double)),
+                                  pos.ctypes.data_as(C.POINTER(C.c_double)),
+                                  solid.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  ro.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  sid.ctypes.data_as(C.POINTER(C.c_int32))), None, "mesh_r")
+        return dict(psi=psi[:n], pos=pos[:n], solid=solid[:n], ray_outside=ro[:n], shape_id=sid[:n])
+
+    def line_array(self, d: int) -> np.ndarray:
+        n = lib().shoccs_mesh_nlines(self.p, d)
+        a = np.zeros((max(n, 1), 7), np.int64)
+        check(lib().shoccs_mesh_lines(self.p, d, a.ctypes.data_as(C.POINTER(C.c_int64))), None, "lines")
+        return a[:n]
+
+    def fluid(self) -> np.ndarray:
+        n = lib().shoccs_problem_size(self.p, 4)
+        a = np.zeros(max(n, 1), np.int64)
+        check(lib().shoccs_mesh_fluid(self.p, a.ctypes.data_as(C.POINTER(C.c_int64))), None, "fluid")
+        return a[:n]
+
+    def blocks(self, d: int) -> np.ndarray:
+        n = lib().shoccs_deriv_nblocks(self.p, d)
+        arr = (ShxLineBlock * max(n, 1))()
+        check(lib().shoccs_deriv_blocks(self.p, d, arr), None, "blocks")
+        names = [f[0] for f in ShxLineBlock._fields_]
+        raw = np.frombuffer(arr, dtype=np.int32).reshape(-1, len(names))[:n].copy()
+        return np.rec.fromarrays(raw.T, names=names)
+
+    def pool(self, d: int) -> np.ndarray:
+        n = lib().shoccs_deriv_npool(self.p, d)
+        a = np.zeros(max(n, 1))
+        check(lib().shoccs_deriv_pool(self.p, d, a.ctypes.data_as(C.POINTER(C.c_double))), None, "pool")
+        return a[:n]
+
+    def interior(self, d: int) -> np.ndarray:
+        a = np.zeros(9)
+        w = lib().shoccs_deriv_interior(self.p, d, a.ctypes.data_as(C.POINTER(C.c_double)))
+        return a[:w]
+
+    def dt(self, h_cfl=1.0, p_cfl=1.0) -> float:
+        return lib().shoccs_problem_dt(self.p, h_cfl, p_cfl)
+
+    # ---------------------------------------------------------- compute
+    def _need_device(self):
+        if self.ctx is None or not self.sys:
+            raise ShoccsError("problem was built without a device context")
+
+    def field(self) -> Field:
+        self._need_device()
+        return Field(self)
+
+    def derivative(self, d: int):
+        self._need_device()
+        dev = lib().shoccs_problem_derivative(self.p, d)
+
+        def apply(u: Field, du: Field, op: int = eq, nu: Field | None = None):
+            check(lib().shx_derivative_apply(dev, u.ref(), nu.ref() if nu else None, du.ref(), op),
+                  self.ctx.p, "shx_derivative_apply")
+
+        return apply
+
+    def _dev_ops(self):
+        arr = (C.c_void_p * 3)()
+        for a in range(3):
+            arr[a] = lib().shoccs_problem_derivative(self.p, a) if self.n[a] > 1 else None
+        return arr
+
+    def gradient(self, u: Field):
+        self._need_device()
+
+        def apply(dux: Field, duy: Field, duz: Field):
+            outs = (C.POINTER(ShxField) * 3)(C.pointer(dux.c), C.pointer(duy.c), C.pointer(duz.c))
+            check(lib().shx_gradient_apply(self._dev_ops(), u.ref(), outs), self.ctx.p, "gradient")
+
+        return apply
+
+    def laplacian(self, u: Field):
+        self._need_device()
+
+        def apply(du: Field):
+            check(lib().shx_laplacian_apply(self._dev_ops(), u.ref(), None, du.ref()), self.ctx.p,
+                  "laplacian")
+
+        return apply
+
+    def rhs(self, u: Field, out: Field, t: float, fused: bool = True):
+        self._need_device()
+        f = lib().shx_system_rhs if fused else lib().shx_system_rhs_unfused
+        check(f(self.sys, u.ref(), out.ref(), t), self.ctx.p, "rhs")
+
+    def update_boundary(self, u: Field, t: float):
+        self._need_device()
+        check(lib().shx_system_update_boundary(self.sys, u.ref(), t), self.ctx.p, "update_boundary")
+
+    def initialize(self, u: Field, t: float = 0.0):
+        self._need_device()
+        check(lib().shx_system_initialize(self.sys, u.ref(), t), self.ctx.p, "initialize")
+
+    def exact(self, out: Field, t: float):
+        self._need_device()
+        check(lib().shx_system_exact(self.sys, out.ref(), t), self.ctx.p, "exact")
+
+    def stats(self, u: Field, t: float) -> np.ndarray:
+        self._need_device()
+        s = np.zeros(11)
+        check(lib().shx_system_stats(self.sys, u.ref(), t, s.ctypes.data_as(C.POINTER(C.c_double))),
+              self.ctx.p, "stats")
+        return s
+
+    def rk4(self, u0: Field, out: Field, rk: Field, srhs: Field, t: float, dt: float):
+        self._need_device()
+        check(lib().shx_rk4_step(self.sys, u0.ref(), out.ref(), rk.ref(), srhs.ref(), t, dt),
+              self.ctx.p, "rk4")
+
+    def rk4_fused(self, u0: Field, t: float, dt: float, nsteps: int = 1):
+        self._need_device()
+        check(lib().shx_rk4_fused_steps_async(self.sys, u0.ref(), t, dt, nsteps), self.ctx.p,
+              "rk4_fused")
+        self.ctx.sync()
+
+    def time_fused(self, u0: Field, t: float, dt: float, nsteps: int, reps: int = 1) -> float:
+        ms = C.c_double()
+        check(lib().shx_time_fused_steps(self.sys, u0.ref(), t, dt, nsteps, reps, C.byref(ms)),
+              self.ctx.p, "time_fused")
+        return ms.value
+
+    def time_fused_stage_kernel(self, u0: Field, t: float, dt: float, nsteps: int) -> float:
+        ms = C.c_double()
+        check(lib().shx_time_fused_stage_kernel(self.sys, u0.ref(), t, dt, nsteps, C.byref(ms)),
+              self.ctx.p, "time_fused_stage_kernel")
+        return ms.value
+
+    def time_unfused(self, u0: Field, out: Field, rk: Field, srhs: Field, t: float, dt: float,
+                     nsteps: int) -> float:
+        ms = C.c_double()
+        check(lib().shx_time_unfused_steps(self.sys, u0.ref(), out.ref(), rk.ref(), srhs.ref(), t, dt,
+                                           nsteps, C.byref(ms)), self.ctx.p, "time_unfused")
+        return ms.value

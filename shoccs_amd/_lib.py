@@ -1,0 +1,144 @@
+"""ctypes binding of libshoccs.so (include/shoccs_hip.h and include/shoccs.h).
+
+The library is the product: HIP kernels for gfx950 behind a C ABI plus the
+C++20 host library that assembles the operators.  There is no CPU fallback:
+when the library is missing or no GPU is present the calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libshoccs.so")
+
+SHX_OK = 0
+BC_DIRICHLET, BC_FLOATING, BC_NEUMANN = 0, 1, 2
+SYS_WAVE_RADIAL, SYS_WAVE_LINEAR, SYS_HEAT = 0, 1, 2
+MMS_GAUSS, MMS_SINPROD = 0, 1
+E2_2, E4_2, E4_1, E4U_1 = 0, 1, 2, 3
+
+dp = C.POINTER(C.c_double)
+i64p = C.POINTER(C.c_int64)
+i32p = C.POINTER(C.c_int32)
+
+
+class ShxField(C.Structure):
+    _fields_ = [("D", C.c_void_p), ("R", C.c_void_p * 3)]
+
+
+class ShxLineBlock(C.Structure):
+    _fields_ = [(name, C.c_int32) for name in (
+        "pair", "lo", "nl", "ni", "nr", "lcols", "lcol0", "lcoef", "rcols", "rcol0", "rcoef",
+        "lobj", "lbw", "robj", "rbw", "flags")]
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("n", C.c_int32 * 3),
+        ("min", C.c_double * 3),
+        ("max", C.c_double * 3),
+        ("nshapes", C.c_int32),
+        ("spheres", dp),
+        ("shape_bc", i32p),
+        ("grid_bc", C.c_int32 * 6),
+        ("scheme", C.c_int32),
+        ("alpha", C.c_double * 2),
+        ("params", dp),
+        ("nparams", C.c_int32),
+    ]
+
+
+class ShoccsError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# (name, restype, argtypes) of every exported entry point
+_SIGS = [
+    ("shx_init", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    ("shx_finalize", C.c_int, [C.c_void_p]),
+    ("shx_last_error", C.c_char_p, [C.c_void_p]),
+    ("shx_sync", C.c_int, [C.c_void_p]),
+    ("shx_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("shx_free", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("shx_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("shx_d2h", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("shx_fill", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int64]),
+    ("shx_derivative_create", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("shx_derivative_destroy", C.c_int, [C.c_void_p]),
+    ("shx_derivative_apply", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
+                                       C.POINTER(ShxField), C.c_int]),
+    ("shx_block_apply", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    ("shx_csr_create", C.c_int, [C.c_void_p, C.c_int64, i64p, i64p, dp, C.POINTER(C.c_void_p)]),
+    ("shx_csr_destroy", C.c_int, [C.c_void_p]),
+    ("shx_csr_apply", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("shx_gradient_apply", C.c_int, [C.c_void_p * 3, C.POINTER(ShxField), C.POINTER(ShxField) * 3]),
+    ("shx_laplacian_apply", C.c_int, [C.c_void_p * 3, C.POINTER(ShxField), C.POINTER(ShxField),
+                                      C.POINTER(ShxField)]),
+    ("shx_system_create", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("shx_system_destroy", C.c_int, [C.c_void_p]),
+    ("shx_system_rhs", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField), C.c_double]),
+    ("shx_system_rhs_unfused", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
+                                         C.c_double]),
+    ("shx_system_update_boundary", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double]),
+    ("shx_system_initialize", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double]),
+    ("shx_system_exact", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double]),
+    ("shx_system_stats", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, dp]),
+    ("shx_rk4_step", C.c_int, [C.c_void_p] + [C.POINTER(ShxField)] * 4 + [C.c_double, C.c_double]),
+    ("shx_rk4_fused_steps_async", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double,
+                                            C.c_int]),
+    ("shx_time_fused_steps", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double,
+                                       C.c_int, C.c_int, dp]),
+    ("shx_time_unfused_steps", C.c_int, [C.c_void_p] + [C.POINTER(ShxField)] * 4 +
+     [C.c_double, C.c_double, C.c_int, dp]),
+    ("shx_time_fused_stage_kernel", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double,
+                                              C.c_double, C.c_int, dp]),
+    ("shoccs_problem_create", C.c_int, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
+    ("shoccs_problem_destroy", C.c_int, [C.c_void_p]),
+    ("shoccs_problem_system", C.c_void_p, [C.c_void_p]),
+    ("shoccs_problem_derivative", C.c_void_p, [C.c_void_p, C.c_int]),
+    ("shoccs_problem_size", C.c_int64, [C.c_void_p, C.c_int]),
+    ("shoccs_problem_dt", C.c_double, [C.c_void_p, C.c_double, C.c_double]),
+    ("shoccs_mesh_r", C.c_int, [C.c_void_p, C.c_int, dp, dp, i32p, i32p, i32p]),
+    ("shoccs_mesh_nlines", C.c_int64, [C.c_void_p, C.c_int]),
+    ("shoccs_mesh_lines", C.c_int, [C.c_void_p, C.c_int, i64p]),
+    ("shoccs_mesh_fluid", C.c_int, [C.c_void_p, i64p]),
+    ("shoccs_deriv_nblocks", C.c_int64, [C.c_void_p, C.c_int]),
+    ("shoccs_deriv_blocks", C.c_int, [C.c_void_p, C.c_int, C.POINTER(ShxLineBlock)]),
+    ("shoccs_deriv_npool", C.c_int64, [C.c_void_p, C.c_int]),
+    ("shoccs_deriv_pool", C.c_int, [C.c_void_p, C.c_int, dp]),
+    ("shoccs_deriv_interior", C.c_int, [C.c_void_p, C.c_int, dp]),
+    ("shoccs_field_alloc", C.c_int, [C.c_void_p, C.POINTER(ShxField)]),
+    ("shoccs_field_free", C.c_int, [C.c_void_p, C.POINTER(ShxField)]),
+    ("shoccs_field_upload", C.c_int, [C.c_void_p, C.POINTER(ShxField), dp]),
+    ("shoccs_field_download", C.c_int, [C.c_void_p, C.POINTER(ShxField), dp]),
+]
+
+EXPORTED = [s[0] for s in _SIGS]
+
+
+def lib():
+    """Load libshoccs.so (building it first only if it is absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        from . import build as _build
+
+        _build.build()
+    L = C.CDLL(LIB_PATH)
+    for name, res, args in _SIGS:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None, what: str = ""):
+    if rc != SHX_OK:
+        msg = lib().shx_last_error(ctx).decode() if ctx else ""
+        raise ShoccsError(f"{what} failed with code {rc}: {msg}")

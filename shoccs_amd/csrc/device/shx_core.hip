@@ -1,0 +1,486 @@
+// shx_core.hip -- context, memory, derivative / block / CSR operators,
+// gradient and laplacian of the SHOCCS device library (gfx950).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "shx_internal.hpp"
+#include "shx_state.hpp"
+
+using namespace shx;
+
+// ------------------------------------------------------------------ kernels
+namespace {
+
+__global__ void k_fill(double* __restrict__ dst, double v, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+template <bool PLUS, bool WITH_B>
+__global__ __launch_bounds__(kBlock) void k_derivative(Tiling t, DevDeriv d,
+                                                       const double* __restrict__ u,
+                                                       const double* __restrict__ uR,
+                                                       double* __restrict__ du)
+{
+    int i, j, k;
+    long idx;
+    if (!tile_point(t, i, j, k, idx)) return;
+    const RowVal r = row_at<WITH_B>(d, u, uR, idx, i, j, k, t.n1, t.n2);
+    if (!r.row) return;
+    du[idx] = PLUS ? row_plus(du[idx], r) : row_eq(r);
+}
+
+// N matrix rows, compressed to the rows that hold entries.
+__global__ void k_rows_csr(long nrows, const long* __restrict__ rows, const int* __restrict__ ptr,
+                           const long* __restrict__ col, const double* __restrict__ w,
+                           const double* __restrict__ x, double* __restrict__ b)
+{
+    const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const long row = rows[r];
+    double acc = b[row];
+    for (int e = ptr[r]; e < ptr[r + 1]; ++e) acc += w[e] * x[col[e]];
+    b[row] = acc;
+}
+
+// The reference's csr apply over every row pointer (src/matrices/csr.cpp:37-51).
+__global__ void k_csr_full(long nrows, const long* __restrict__ u, const long* __restrict__ v,
+                           const double* __restrict__ w, const double* __restrict__ x,
+                           double* __restrict__ b)
+{
+    const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    const long e0 = u[row], e1 = u[row + 1];
+    if (e0 == e1) return;
+    double acc = b[row];
+    for (long e = e0; e < e1; ++e) acc += w[e] * x[v[e]];
+    b[row] = acc;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ helpers
+namespace shx {
+
+int fail(shx_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_fail(shx_ctx* ctx, hipError_t e, const char* what)
+{
+    return fail(ctx, SHX_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int launch_fill(shx_ctx* ctx, double* dst, double v, long n)
+{
+    if (n <= 0) return SHX_OK;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    k_fill<<<grid, 256, 0, ctx->stream>>>(dst, v, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "fill");
+}
+
+int launch_derivative(shx_derivative* d, const double* u, const double* uR, double* du, bool plus,
+                      bool with_b)
+{
+    shx_ctx* ctx = d->ctx;
+    if (!d->dev.present) return SHX_OK;
+    const Tiling t = make_tiling(d->n);
+    if (plus && with_b)
+        k_derivative<true, true><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->dev, u, uR, du);
+    else if (plus)
+        k_derivative<true, false><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->dev, u, uR, du);
+    else if (with_b)
+        k_derivative<false, true><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->dev, u, uR, du);
+    else
+        k_derivative<false, false><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->dev, u, uR, du);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "derivative kernel");
+}
+
+int launch_neumann(shx_derivative* d, const double* nu, double* du)
+{
+    if (d->nneu_rows == 0) return SHX_OK;
+    shx_ctx* ctx = d->ctx;
+    const unsigned grid = (unsigned)((d->nneu_rows + 255) / 256);
+    k_rows_csr<<<grid, 256, 0, ctx->stream>>>(d->nneu_rows, d->neu_rows, d->neu_ptr, d->neu_col,
+                                             d->neu_w, nu, du);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "neumann kernel");
+}
+
+int launch_csr_full(shx_csr* A, const double* x, double* b)
+{
+    if (A->rows == 0) return SHX_OK;
+    const unsigned grid = (unsigned)((A->rows + 255) / 256);
+    k_csr_full<<<grid, 256, 0, A->ctx->stream>>>(A->rows, A->u, A->v, A->w, x, b);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SHX_OK : hip_fail(A->ctx, e, "csr kernel");
+}
+
+int zero_field(shx_ctx* ctx, const shx_field* f, const int n[3], const long nR[3])
+{
+    int rc = launch_fill(ctx, f->D, 0.0, (long)n[0] * n[1] * n[2]);
+    for (int r = 0; r < 3 && rc == SHX_OK; ++r)
+        if (nR[r] > 0) rc = launch_fill(ctx, f->R[r], 0.0, nR[r]);
+    return rc;
+}
+
+int sync(shx_ctx* ctx)
+{
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "stream synchronize");
+}
+
+template <typename T>
+int upload(shx_ctx* ctx, T** dst, const T* src, size_t n)
+{
+    *dst = nullptr;
+    if (n == 0) return SHX_OK;
+    hipError_t e = hipMalloc((void**)dst, n * sizeof(T));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc");
+    e = hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpy");
+    return SHX_OK;
+}
+
+template int upload<double>(shx_ctx*, double**, const double*, size_t);
+template int upload<int>(shx_ctx*, int**, const int*, size_t);
+template int upload<long>(shx_ctx*, long**, const long*, size_t);
+template int upload<unsigned>(shx_ctx*, unsigned**, const unsigned*, size_t);
+template int upload<int8_t>(shx_ctx*, int8_t**, const int8_t*, size_t);
+
+}  // namespace shx
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int shx_init(int device, shx_ctx** out)
+{
+    if (!out) return SHX_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return SHX_EHIP;
+    shx_ctx* ctx = new (std::nothrow) shx_ctx();
+    if (!ctx) return SHX_ENOMEM;
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return SHX_EHIP;
+    }
+    *out = ctx;
+    return SHX_OK;
+}
+
+int shx_finalize(shx_ctx* ctx)
+{
+    if (!ctx) return SHX_EINVAL;
+    hipStreamSynchronize(ctx->stream);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SHX_OK;
+}
+
+const char* shx_last_error(const shx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int shx_sync(shx_ctx* ctx) { return ctx ? sync(ctx) : SHX_EINVAL; }
+
+int shx_alloc(shx_ctx* ctx, size_t bytes, void** dptr)
+{
+    if (!ctx || !dptr) return SHX_EINVAL;
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 8);
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "hipMalloc");
+}
+
+int shx_free(shx_ctx* ctx, void* dptr)
+{
+    if (!ctx) return SHX_EINVAL;
+    hipError_t e = hipFree(dptr);
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "hipFree");
+}
+
+int shx_h2d(shx_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx) return SHX_EINVAL;
+    if (!bytes) return SHX_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "h2d");
+    return sync(ctx);
+}
+
+int shx_d2h(shx_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx) return SHX_EINVAL;
+    if (!bytes) return SHX_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "d2h");
+    return sync(ctx);
+}
+
+int shx_fill(shx_ctx* ctx, double* dst, double value, int64_t n)
+{
+    if (!ctx) return SHX_EINVAL;
+    return launch_fill(ctx, dst, value, n);
+}
+
+int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_derivative** out)
+{
+    if (!ctx || !desc || !out) return SHX_EINVAL;
+    *out = nullptr;
+    const int dir = desc->dir;
+    if (dir < 0 || dir > 2) return fail(ctx, SHX_EINVAL, "derivative: dir out of range");
+    for (int a = 0; a < 3; ++a)
+        if (desc->n[a] < 1 || desc->n[a] > 65535)
+            return fail(ctx, SHX_EINVAL, "derivative: extents must be in [1, 65535]");
+    if (desc->width < 1 || desc->width > 9 || (desc->width % 2) == 0)
+        return fail(ctx, SHX_EINVAL, "derivative: interior width must be odd and <= 9");
+    const long N = (long)desc->n[0] * desc->n[1] * desc->n[2];
+    if (N > 0x7fffffffL) return fail(ctx, SHX_EINVAL, "derivative: mesh too large");
+
+    auto* d = new (std::nothrow) shx_derivative();
+    if (!d) return SHX_ENOMEM;
+    d->ctx = ctx;
+    d->dir = dir;
+    for (int a = 0; a < 3; ++a) {
+        d->n[a] = desc->n[a];
+        d->nR[a] = desc->nR[a];
+    }
+    std::memset(&d->dev, 0, sizeof d->dev);
+    d->dev.dir = dir;
+    d->dev.nq = desc->n[dir];
+    d->dev.width = desc->width;
+    for (int jj = 0; jj < desc->width; ++jj) d->dev.c[jj] = desc->interior[jj];
+    d->dev.stride = dir == 0 ? (long)desc->n[1] * desc->n[2] : dir == 1 ? desc->n[2] : 1;
+    const int nslow = dir == 0 ? desc->n[1] : desc->n[0];
+    const int nfast = dir == 2 ? desc->n[1] : desc->n[2];
+    d->dev.nfast = nfast;
+    d->dev.present = desc->n[dir] > 1;
+    const long npairs = (long)nslow * nfast;
+
+    // validate the segment table and build pair_ptr and the fast descriptors
+    std::vector<int> pair_ptr(npairs + 1, 0);
+    std::vector<unsigned long long> fast(npairs, 0ull);
+    std::vector<int> seen(npairs, 0);
+    long prev_pair = -1;
+    int prev_hi = -1;
+    for (long b = 0; b < desc->nblocks; ++b) {
+        const shx_line_block& blk = desc->blocks[b];
+        const int total = blk.nl + blk.ni + blk.nr;
+        if (blk.pair < 0 || blk.pair >= npairs || blk.lo < 0 || total < 0 ||
+            blk.lo + total > desc->n[dir] || blk.nl < 0 || blk.nr < 0) {
+            delete d;
+            return fail(ctx, SHX_EINVAL, "derivative: segment out of range");
+        }
+        const long nRd = desc->nR[dir];
+        const bool bad_ref =
+            blk.lobj < -1 || blk.lobj >= nRd || blk.robj < -1 || blk.robj >= nRd ||
+            blk.lcols < 0 || blk.rcols < 0 || blk.lcoef < 0 || blk.rcoef < 0 ||
+            (long)blk.lcoef + (long)blk.nl * blk.lcols > desc->npool ||
+            (long)blk.rcoef + (long)blk.nr * blk.rcols > desc->npool ||
+            (blk.lobj >= 0 && (blk.lbw < 0 || (long)blk.lbw + blk.nl > desc->npool)) ||
+            (blk.robj >= 0 && (blk.rbw < 0 || (long)blk.rbw + blk.nr > desc->npool));
+        if (bad_ref) {
+            delete d;
+            return fail(ctx, SHX_EINVAL, "derivative: segment references out of range");
+        }
+        if (total < blk.nl || total < blk.nr) {
+            delete d;
+            return fail(ctx, SHX_EUNSUPPORTED,
+                        "derivative: a line segment is shorter than its closures");
+        }
+        // Overlapping closures of short segments may read past the line ends
+        // (into the neighbouring line, as the reference's flat offsets do) but
+        // never outside the mesh buffer.
+        {
+            const int s = blk.pair / nfast, f = blk.pair % nfast;
+            long base;
+            if (dir == 0) base = (long)s * desc->n[2] + f;
+            else if (dir == 1) base = (long)s * desc->n[1] * desc->n[2] + f;
+            else base = ((long)s * desc->n[1] + f) * desc->n[2];
+            const long st = d->dev.stride;
+            const long l0 = base + (long)blk.lcol0 * st, l1 = base + (long)(blk.lcol0 + blk.lcols - 1) * st;
+            const long r0 = base + (long)blk.rcol0 * st, r1 = base + (long)(blk.rcol0 + blk.rcols - 1) * st;
+            if ((blk.nl > 0 && (l0 < 0 || l1 >= N)) || (blk.nr > 0 && (r0 < 0 || r1 >= N))) {
+                delete d;
+                return fail(ctx, SHX_EUNSUPPORTED,
+                            "derivative: closure columns of a short segment fall outside the mesh");
+            }
+        }
+        if (blk.pair < prev_pair || (blk.pair == prev_pair && blk.lo < prev_hi)) {
+            delete d;
+            return fail(ctx, SHX_EINVAL, "derivative: segments not sorted by (pair, lo)");
+        }
+        if (blk.pair != prev_pair) prev_hi = -1;
+        prev_pair = blk.pair;
+        prev_hi = blk.lo + total;
+        pair_ptr[blk.pair + 1]++;
+        const int s = seen[blk.pair]++;
+        if (s < 2 && blk.ni > 0) {
+            const unsigned long long lo = (unsigned long long)(blk.lo + blk.nl);
+            const unsigned long long hi = (unsigned long long)(blk.lo + blk.nl + blk.ni);
+            fast[blk.pair] |= (lo | (hi << 16)) << (32 * s);
+        }
+    }
+    for (long p = 0; p < npairs; ++p) pair_ptr[p + 1] += pair_ptr[p];
+
+    int rc = upload(ctx, &d->pair_ptr, pair_ptr.data(), pair_ptr.size());
+    if (rc == SHX_OK) rc = upload(ctx, &d->fast, fast.data(), fast.size());
+    if (rc == SHX_OK && desc->nblocks)
+        rc = upload(ctx, &d->blocks, desc->blocks, (size_t)desc->nblocks);
+    if (rc == SHX_OK) rc = upload(ctx, &d->pool, desc->pool, (size_t)desc->npool);
+
+    // Neumann entries -> row-compressed CSR sorted by (row, col, w)
+    if (rc == SHX_OK && desc->nneu > 0) {
+        std::vector<long> order(desc->nneu);
+        for (long e = 0; e < desc->nneu; ++e) order[e] = e;
+        std::sort(order.begin(), order.end(), [&](long a, long b) {
+            if (desc->neu_row[a] != desc->neu_row[b]) return desc->neu_row[a] < desc->neu_row[b];
+            if (desc->neu_col[a] != desc->neu_col[b]) return desc->neu_col[a] < desc->neu_col[b];
+            return desc->neu_w[a] < desc->neu_w[b];
+        });
+        std::vector<long> rows, col;
+        std::vector<int> ptr{0};
+        std::vector<double> w;
+        for (long e : order) {
+            if (rows.empty() || rows.back() != desc->neu_row[e]) {
+                if (!rows.empty()) ptr.push_back((int)w.size());
+                rows.push_back(desc->neu_row[e]);
+            }
+            col.push_back(desc->neu_col[e]);
+            w.push_back(desc->neu_w[e]);
+        }
+        ptr.push_back((int)w.size());
+        d->nneu_rows = (long)rows.size();
+        rc = upload(ctx, &d->neu_rows, rows.data(), rows.size());
+        if (rc == SHX_OK) rc = upload(ctx, &d->neu_ptr, ptr.data(), ptr.size());
+        if (rc == SHX_OK) rc = upload(ctx, &d->neu_col, col.data(), col.size());
+        if (rc == SHX_OK) rc = upload(ctx, &d->neu_w, w.data(), w.size());
+    }
+    if (rc != SHX_OK) {
+        shx_derivative_destroy(d);
+        return rc;
+    }
+    d->h_blocks.assign(desc->blocks, desc->blocks + desc->nblocks);
+    d->h_pool.assign(desc->pool, desc->pool + desc->npool);
+    d->dev.pair_ptr = d->pair_ptr;
+    d->dev.fast = d->fast;
+    d->dev.blocks = d->blocks;
+    d->dev.pool = d->pool;
+    *out = d;
+    return SHX_OK;
+}
+
+int shx_derivative_destroy(shx_derivative* d)
+{
+    if (!d) return SHX_EINVAL;
+    hipFree(d->pair_ptr);
+    hipFree(d->fast);
+    hipFree(d->blocks);
+    hipFree(d->pool);
+    hipFree(d->neu_rows);
+    hipFree(d->neu_ptr);
+    hipFree(d->neu_col);
+    hipFree(d->neu_w);
+    delete d;
+    return SHX_OK;
+}
+
+int shx_derivative_apply(shx_derivative* d, const shx_field* u, const shx_field* nu, shx_field* du,
+                         int accumulate)
+{
+    if (!d || !u || !du) return SHX_EINVAL;
+    int rc = launch_derivative(d, u->D, u->R[d->dir], du->D, accumulate != 0, true);
+    if (rc == SHX_OK && nu) rc = launch_neumann(d, nu->D, du->D);
+    if (rc == SHX_OK) rc = sync(d->ctx);
+    return rc;
+}
+
+int shx_block_apply(shx_derivative* d, const double* x, double* b, int accumulate)
+{
+    if (!d || !x || !b) return SHX_EINVAL;
+    int rc = launch_derivative(d, x, nullptr, b, accumulate != 0, false);
+    if (rc == SHX_OK) rc = sync(d->ctx);
+    return rc;
+}
+
+int shx_csr_create(shx_ctx* ctx, int64_t rows, const int64_t* rowptr, const int64_t* col,
+                   const double* w, shx_csr** out)
+{
+    if (!ctx || !out || rows < 0 || (rows > 0 && !rowptr)) return SHX_EINVAL;
+    auto* A = new (std::nothrow) shx_csr();
+    if (!A) return SHX_ENOMEM;
+    A->ctx = ctx;
+    A->rows = rows;
+    const long nnz = rows > 0 ? (long)rowptr[rows] : 0;
+    A->nnz = nnz;
+    int rc = upload(ctx, &A->u, (const long*)rowptr, rows > 0 ? (size_t)rows + 1 : 0);
+    if (rc == SHX_OK) rc = upload(ctx, &A->v, (const long*)col, (size_t)nnz);
+    if (rc == SHX_OK) rc = upload(ctx, &A->w, w, (size_t)nnz);
+    if (rc != SHX_OK) {
+        shx_csr_destroy(A);
+        return rc;
+    }
+    *out = A;
+    return SHX_OK;
+}
+
+int shx_csr_destroy(shx_csr* A)
+{
+    if (!A) return SHX_EINVAL;
+    hipFree(A->u);
+    hipFree(A->v);
+    hipFree(A->w);
+    delete A;
+    return SHX_OK;
+}
+
+int shx_csr_apply(shx_csr* A, const double* x, double* b)
+{
+    if (!A) return SHX_EINVAL;
+    int rc = launch_csr_full(A, x, b);
+    if (rc == SHX_OK) rc = sync(A->ctx);
+    return rc;
+}
+
+int shx_gradient_apply(shx_derivative* const d[3], const shx_field* u, shx_field* const du[3])
+{
+    shx_derivative* any = d[0] ? d[0] : d[1] ? d[1] : d[2];
+    if (!any || !u || !du[0] || !du[1] || !du[2]) return SHX_EINVAL;
+    shx_ctx* ctx = any->ctx;
+    int rc = SHX_OK;
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = zero_field(ctx, du[c], any->n, any->nR);
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c)
+        if (d[c] && d[c]->n[c] > 1) rc = launch_derivative(d[c], u->D, u->R[c], du[c]->D, false, true);
+    if (rc == SHX_OK) rc = sync(ctx);
+    return rc;
+}
+
+int shx_laplacian_apply(shx_derivative* const d[3], const shx_field* u, const shx_field* nu,
+                        shx_field* du)
+{
+    shx_derivative* any = d[0] ? d[0] : d[1] ? d[1] : d[2];
+    if (!any || !u || !du) return SHX_EINVAL;
+    shx_ctx* ctx = any->ctx;
+    int rc = zero_field(ctx, du, any->n, any->nR);
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+        if (!d[c] || d[c]->n[c] < 2) continue;
+        rc = launch_derivative(d[c], u->D, u->R[c], du->D, true, true);
+        if (rc == SHX_OK && nu) rc = launch_neumann(d[c], nu->D, du->D);
+    }
+    if (rc == SHX_OK) rc = sync(ctx);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// shx_state.hpp -- host-side state behind the opaque handles of shoccs_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "shoccs_hip.h"
+#include "shx_internal.hpp"
+
+struct shx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+struct shx_derivative {
+    shx_ctx* ctx = nullptr;
+    int dir = 0;
+    int n[3] = {1, 1, 1};
+    long nR[3] = {0, 0, 0};
+    shx::DevDeriv dev;
+    int* pair_ptr = nullptr;
+    unsigned long long* fast = nullptr;
+    shx_line_block* blocks = nullptr;
+    double* pool = nullptr;
+    long nneu_rows = 0;
+    long* neu_rows = nullptr;
+    int* neu_ptr = nullptr;
+    long* neu_col = nullptr;
+    double* neu_w = nullptr;
+    // host copies kept to build the reference's full-length B matrix
+    std::vector<shx_line_block> h_blocks;
+    std::vector<double> h_pool;
+};
+
+struct shx_csr {
+    shx_ctx* ctx = nullptr;
+    long rows = 0;
+    long nnz = 0;
+    long* u = nullptr;
+    long* v = nullptr;
+    double* w = nullptr;
+};
+
+namespace shx {
+
+int fail(shx_ctx* ctx, int code, const std::string& msg);
+int hip_fail(shx_ctx* ctx, hipError_t e, const char* what);
+int launch_fill(shx_ctx* ctx, double* dst, double v, long n);
+int launch_derivative(shx_derivative* d, const double* u, const double* uR, double* du, bool plus,
+                      bool with_b);
+int launch_neumann(shx_derivative* d, const double* nu, double* du);
+int launch_csr_full(shx_csr* A, const double* x, double* b);
+int zero_field(shx_ctx* ctx, const shx_field* f, const int n[3], const long nR[3]);
+int sync(shx_ctx* ctx);
+template <typename T>
+int upload(shx_ctx* ctx, T** dst, const T* src, size_t n);
+
+}  // namespace shx

@@ -1,0 +1,1071 @@
+// shx_systems.hip -- scalar-wave and heat right-hand sides, boundary updates,
+// statistics and the RK4 step (reference slot schedule and fused form) of the
+// SHOCCS device library (gfx950).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "shx_internal.hpp"
+#include "shx_state.hpp"
+
+using namespace shx;
+
+namespace shx {
+
+// Device view of a system (kernel argument).
+struct DevSys {
+    int kind;
+    int n0, n1, n2;
+    const double* x;
+    const double* y;
+    const double* z;
+    int bc[6];
+    double center[3], radius;
+    double vel[3], x0[3], width, amp;
+    double k;
+    int mms_kind, mms_dims, mms_n;
+    const double* mms;
+    const unsigned* fluid_mask;
+};
+
+constexpr double kTwoPi = 6.283185307179586;  // 2 * pi, as 2 * std::numbers::pi
+
+__device__ __forceinline__ double dot3(const double* a, const double* b)
+{
+    return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]);
+}
+
+// scalar_wave.cpp:35-40 (radial) and the translated gaussian of the
+// constant-velocity wave.
+__device__ __forceinline__ double wave_exact(const DevSys& s, const double* loc, double t)
+{
+    if (s.kind == SHX_SYS_WAVE_RADIAL) {
+        const double r[3] = {loc[0] - s.center[0], loc[1] - s.center[1], loc[2] - s.center[2]};
+        return sin(kTwoPi * (sqrt(dot3(r, r)) - s.radius - t));
+    }
+    double q[3];
+    for (int c = 0; c < 3; ++c) q[c] = loc[c] - s.x0[c] - s.vel[c] * t;
+    return s.amp * exp(-dot3(q, q) / (2 * s.width * s.width));
+}
+
+// gG component (scalar_wave.cpp:28-33): -(x - c)/|x - c| or -a.
+__device__ __forceinline__ double wave_neg_g(const DevSys& s, int comp, const double* loc)
+{
+    if (s.kind == SHX_SYS_WAVE_RADIAL) {
+        const double r[3] = {loc[0] - s.center[0], loc[1] - s.center[1], loc[2] - s.center[2]};
+        return -(loc[comp] - s.center[comp]) / sqrt(dot3(r, r));
+    }
+    return -s.vel[comp];
+}
+
+__device__ __forceinline__ double g3_expo(const double* g, const double* loc)
+{
+    const double *c = g, *v = g + 3;
+    return (-0.5 * pow((loc[0] - c[0]), 2) * pow(v[0], -2) +
+            (-0.5 * pow((loc[1] - c[1]), 2) * pow(v[1], -2) +
+             -0.5 * pow((loc[2] - c[2]), 2) * pow(v[2], -2)));
+}
+
+__device__ double mms_call(const DevSys& s, double t, const double* loc)
+{
+    if (s.mms_kind == SHX_MMS_SINPROD) {
+        double v = sin(kTwoPi * loc[0]) * sin(kTwoPi * loc[1]);
+        if (s.mms_dims == 3) v = v * sin(kTwoPi * loc[2]);
+        return v * exp(-t);
+    }
+    double sol = 0;
+    for (int i = 0; i < s.mms_n; ++i) {
+        const double* g = s.mms + 8 * i;
+        sol += exp(g3_expo(g, loc)) * g[6] * cos(t * g[7]);
+    }
+    return sol;
+}
+
+__device__ double mms_ddt(const DevSys& s, double t, const double* loc)
+{
+    if (s.mms_kind == SHX_MMS_SINPROD) return -mms_call(s, t, loc);
+    double sol = 0;
+    for (int i = 0; i < s.mms_n; ++i) {
+        const double* g = s.mms + 8 * i;
+        sol += -exp(g3_expo(g, loc)) * g[6] * g[7] * sin(t * g[7]);
+    }
+    return sol;
+}
+
+__device__ double mms_lap(const DevSys& s, double t, const double* loc)
+{
+    if (s.mms_kind == SHX_MMS_SINPROD)
+        return -(s.mms_dims * kTwoPi * kTwoPi) * mms_call(s, t, loc);
+    double sol = 0;
+    const double x = loc[0], y = loc[1], z = loc[2];
+    for (int i = 0; i < s.mms_n; ++i) {
+        const double* c = s.mms + 8 * i;
+        const double* v = c + 3;
+        sol += exp(0.5 * ((-1 * pow((z - c[2]), 2) * pow(v[2], -2) -
+                           pow((y - c[1]), 2) * pow(v[1], -2)) -
+                          pow((x - c[0]), 2) * pow(v[0], -2))) *
+               c[6] * cos(t * c[7]) * pow(v[0], -4) * pow(v[1], -4) * pow(v[2], -4) *
+               ((((pow(z, 2) * pow(v[0], 4) * pow(v[1], 4) +
+                   (-2 * z * c[2] * pow(v[0], 4) * pow(v[1], 4) +
+                    (pow(c[2], 2) * pow(v[0], 4) * pow(v[1], 4) +
+                     (pow(y, 2) * pow(v[0], 4) * pow(v[2], 4) +
+                      (-2 * y * c[1] * pow(v[0], 4) * pow(v[2], 4) +
+                       (pow(c[1], 2) * pow(v[0], 4) * pow(v[2], 4) +
+                        (pow(x, 2) * pow(v[1], 4) * pow(v[2], 4) +
+                         (-2 * x * c[0] * pow(v[1], 4) * pow(v[2], 4) +
+                          pow(c[0], 2) * pow(v[1], 4) * pow(v[2], 4))))))))) -
+                  pow(v[0], 2) * pow(v[1], 4) * pow(v[2], 4)) -
+                 pow(v[0], 4) * pow(v[1], 2) * pow(v[2], 4)) -
+                pow(v[0], 4) * pow(v[1], 4) * pow(v[2], 2));
+    }
+    return sol;
+}
+
+__device__ __forceinline__ double exact_at(const DevSys& s, const double* loc, double t)
+{
+    return s.kind == SHX_SYS_HEAT ? mms_call(s, t, loc) : wave_exact(s, loc, t);
+}
+
+// point on a Dirichlet grid face (for_each_grid_bc_desc, selection_desc.hpp:211-222)
+__device__ __forceinline__ bool on_dirichlet_face(const DevSys& s, int i, int j, int k)
+{
+    return (s.bc[0] == SHX_BC_DIRICHLET && i == 0) || (s.bc[1] == SHX_BC_DIRICHLET && i == s.n0 - 1) ||
+           (s.bc[2] == SHX_BC_DIRICHLET && j == 0) || (s.bc[3] == SHX_BC_DIRICHLET && j == s.n1 - 1) ||
+           (s.bc[4] == SHX_BC_DIRICHLET && k == 0) || (s.bc[5] == SHX_BC_DIRICHLET && k == s.n2 - 1);
+}
+
+__device__ __forceinline__ bool is_fluid(const DevSys& s, long idx)
+{
+    return (s.fluid_mask[idx >> 5] >> (idx & 31)) & 1u;
+}
+
+struct DevOps {
+    DevDeriv d[3];
+};
+
+struct RPtrs {
+    const double* r[3];
+};
+
+// Right-hand side at one point of D.
+template <int KIND>
+__device__ __forceinline__ double rhs_point(const DevSys& s, const DevOps& ops,
+                                            const double* __restrict__ u, const RPtrs& uR,
+                                            long idx, int i, int j, int k, double t)
+{
+    if (KIND == SHX_SYS_HEAT) {
+        // laplacian: du = 0; du += O_x u; du += B_x u_Rx; then y, z (laplacian.cpp:37-46)
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (!ops.d[c].present) continue;
+            const RowVal r = row_at<true>(ops.d[c], u, uR.r[c], idx, i, j, k, s.n1, s.n2);
+            if (r.row) acc = row_plus(acc, r);
+        }
+        acc = acc * s.k;
+        if (on_dirichlet_face(s, i, j, k)) return 0.0;
+        if (is_fluid(s, idx)) {
+            const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+            acc += mms_ddt(s, t, loc) - s.k * mms_lap(s, t, loc);
+        }
+        return acc;
+    } else {
+        double du[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (!ops.d[c].present) continue;
+            const RowVal r = row_at<true>(ops.d[c], u, uR.r[c], idx, i, j, k, s.n1, s.n2);
+            if (r.row) du[c] = row_eq(r);
+        }
+        double g[3] = {0.0, 0.0, 0.0};
+        if (!on_dirichlet_face(s, i, j, k)) {
+            const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g[c] = wave_neg_g(s, c, loc);
+        }
+        return g[0] * du[0] + g[1] * du[1] + g[2] * du[2];
+    }
+}
+
+}  // namespace shx
+
+// --------------------------------------------------------------------- kernels
+namespace {
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_rhs(Tiling t, DevSys s, DevOps ops,
+                                                const double* __restrict__ u, RPtrs uR,
+                                                double* __restrict__ rhs, double time)
+{
+    int i, j, k;
+    long idx;
+    if (!tile_point(t, i, j, k, idx)) return;
+    rhs[idx] = rhs_point<KIND>(s, ops, u, uR, idx, i, j, k, time);
+}
+
+// One fused RK4 stage over three registers (rk4.cpp:27-46 for one point):
+//   rhs = RHS(us) at t_s;  acc' = acc + c_acc * rhs  (acc = 0 on stage 0)
+//   stages 0-2: out = u0 + c_next * rhs, Dirichlet faces take exact(t_next)
+//   stage 3:    out = u0 + 1.0 * acc'  (out may alias u0), faces exact(t_next)
+template <int KIND, int STAGE>
+__global__ __launch_bounds__(kBlock) void k_rk4_stage(Tiling t, DevSys s, DevOps ops,
+                                                      const double* __restrict__ us, RPtrs usR,
+                                                      const double* u0, double* __restrict__ acc,
+                                                      double* out, double t_s, double c_acc,
+                                                      double c_next, double t_next)
+{
+    int i, j, k;
+    long idx;
+    if (!tile_point(t, i, j, k, idx)) return;
+    const double rhs = rhs_point<KIND>(s, ops, us, usR, idx, i, j, k, t_s);
+    const double a = (STAGE == 0 ? 0.0 : acc[idx]) + c_acc * rhs;
+    const double base = u0[idx];
+    double v;
+    if (STAGE < 3) {
+        acc[idx] = a;
+        v = base + c_next * rhs;
+    } else {
+        v = base + 1.0 * a;
+    }
+    if (on_dirichlet_face(s, i, j, k)) {
+        const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+        v = exact_at(s, loc, t_next);
+    }
+    out[idx] = v;
+}
+
+// Dirichlet cut points take the exact solution (update_boundary on R).
+__global__ void k_r_exact(DevSys s, long n, const double* __restrict__ pos,
+                          const signed char* __restrict__ dirich, double* __restrict__ uR,
+                          double t)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !dirich[i]) return;
+    uR[i] = exact_at(s, pos + 3 * i, t);
+}
+
+// Dirichlet grid faces of D take the exact solution.
+__global__ void k_face_exact(DevSys s, long n, const int* __restrict__ faces,
+                             double* __restrict__ u, double t)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long idx = faces[q];
+    int i, j, k;
+    point_coords(idx, s.n1, s.n2, i, j, k);
+    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+    u[idx] = exact_at(s, loc, t);
+}
+
+__global__ void k_exact_D(DevSys s, long N, double* __restrict__ out, double t)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    int i, j, k;
+    point_coords(idx, s.n1, s.n2, i, j, k);
+    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+    out[idx] = exact_at(s, loc, t);
+}
+
+__global__ void k_exact_R(DevSys s, long n, const double* __restrict__ pos, double* __restrict__ out,
+                          double t)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = exact_at(s, pos + 3 * i, t);
+}
+
+// initialize_scalar_field on the fluid gather
+__global__ void k_init_fluid(DevSys s, long n, const int* __restrict__ fluid, double* __restrict__ u,
+                             double t)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long idx = fluid[q];
+    int i, j, k;
+    point_coords(idx, s.n1, s.n2, i, j, k);
+    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+    u[idx] = exact_at(s, loc, t);
+}
+
+// gG fields of the reference (scalar_wave.cpp:70-95), stored for the unfused form
+__global__ void k_gG_D(DevSys s, long N, double* __restrict__ gx, double* __restrict__ gy,
+                       double* __restrict__ gz)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    int i, j, k;
+    point_coords(idx, s.n1, s.n2, i, j, k);
+    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+    const bool f = on_dirichlet_face(s, i, j, k);
+    gx[idx] = f ? 0.0 : wave_neg_g(s, 0, loc);
+    gy[idx] = f ? 0.0 : wave_neg_g(s, 1, loc);
+    gz[idx] = f ? 0.0 : wave_neg_g(s, 2, loc);
+}
+
+// scalar_wave.cpp:294-317, one buffer
+__global__ void k_dot3(long n, const double* __restrict__ gx, const double* __restrict__ dx,
+                       const double* __restrict__ gy, const double* __restrict__ dy,
+                       const double* __restrict__ gz, const double* __restrict__ dz,
+                       double* __restrict__ o)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    o[i] = gx[i] * dx[i] + gy[i] * dy[i] + gz[i] * dz[i];
+}
+
+__global__ void k_scale(long n, double* __restrict__ a, double c)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] *= c;
+}
+
+// heat fill_source into a stored buffer (heat.cpp:111-117)
+__global__ void k_src_D(DevSys s, long N, double* __restrict__ src, double t)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    int i, j, k;
+    point_coords(idx, s.n1, s.n2, i, j, k);
+    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+    src[idx] = mms_ddt(s, t, loc) - s.k * mms_lap(s, t, loc);
+}
+
+__global__ void k_gather_add(long n, const int* __restrict__ sel, const double* __restrict__ src,
+                             double* __restrict__ dst)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long idx = sel[q];
+    dst[idx] += src[idx];
+}
+
+__global__ void k_gather_fill(long n, const int* __restrict__ sel, double* __restrict__ dst, double v)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[sel[q]] = v;
+}
+
+// slot_ops.hpp:23-59
+__global__ void k_assign_lc(long n, double* __restrict__ d, const double* __restrict__ s0, double c,
+                            const double* __restrict__ r)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s0[i] + c * r[i];
+}
+
+__global__ void k_accumulate(long n, double* __restrict__ d, double c, const double* __restrict__ r)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] += c * r[i];
+}
+
+__global__ void k_copy(long n, double* __restrict__ d, const double* __restrict__ s)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+// compute_scalar_stats partials over the fluid gather: per workgroup
+// (max |u - exact|, its index, min u, max u)
+__global__ __launch_bounds__(256) void k_stats(DevSys s, long n, const int* __restrict__ fluid,
+                                               const double* __restrict__ u, double t,
+                                               double* __restrict__ part_err,
+                                               long* __restrict__ part_idx,
+                                               double* __restrict__ part_min,
+                                               double* __restrict__ part_max)
+{
+    __shared__ double se[256], smin[256], smax[256];
+    __shared__ long si[256];
+    double e = -DBL_MAX, mn = DBL_MAX, mx = -DBL_MAX;
+    long ei = 0x7fffffffffffffffL;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const long idx = fluid[q];
+        int i, j, k;
+        point_coords(idx, s.n1, s.n2, i, j, k);
+        const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+        const double v = u[idx];
+        const double err = fabs(v - exact_at(s, loc, t));
+        if (err > e || (err == e && idx < ei)) {
+            e = err;
+            ei = idx;
+        }
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    se[threadIdx.x] = e;
+    si[threadIdx.x] = ei;
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            const int o = threadIdx.x + w;
+            if (se[o] > se[threadIdx.x] || (se[o] == se[threadIdx.x] && si[o] < si[threadIdx.x])) {
+                se[threadIdx.x] = se[o];
+                si[threadIdx.x] = si[o];
+            }
+            smin[threadIdx.x] = smin[o] < smin[threadIdx.x] ? smin[o] : smin[threadIdx.x];
+            smax[threadIdx.x] = smax[o] > smax[threadIdx.x] ? smax[o] : smax[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_err[blockIdx.x] = se[0];
+        part_idx[blockIdx.x] = si[0];
+        part_min[blockIdx.x] = smin[0];
+        part_max[blockIdx.x] = smax[0];
+    }
+}
+
+inline unsigned grid_for(long n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- system state
+struct shx_system {
+    shx_ctx* ctx = nullptr;
+    int kind = 0;
+    int n[3] = {1, 1, 1};
+    long N = 0;
+    long nR[3] = {0, 0, 0};
+    shx_derivative* d[3] = {nullptr, nullptr, nullptr};
+    DevSys dev;
+    DevOps ops;
+    Tiling tiling;
+    // device buffers
+    double* xyz[3] = {nullptr, nullptr, nullptr};
+    double* mms = nullptr;
+    unsigned* fluid_mask = nullptr;
+    int* fluid = nullptr;
+    long nfluid = 0;
+    int* faces = nullptr;
+    long nfaces = 0;
+    double* rpos[3] = {nullptr, nullptr, nullptr};
+    signed char* rdir[3] = {nullptr, nullptr, nullptr};
+    // fused RK4 registers (stage ping-pong and accumulator), allocated lazily
+    double* stageD[2] = {nullptr, nullptr};
+    double* stageR[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    double* accD = nullptr;
+    // reference-schedule buffers, allocated lazily
+    bool unfused_ready = false;
+    double* gG[3][4] = {};
+    double* du[3][4] = {};
+    double* srcD = nullptr;
+    shx_csr* Bfull[3] = {nullptr, nullptr, nullptr};
+    // statistics partials
+    double* part = nullptr;
+    long* part_idx = nullptr;
+};
+
+namespace {
+
+int check(shx_ctx* ctx, const char* what)
+{
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, what);
+}
+
+int alloc_d(shx_ctx* ctx, double** p, long n)
+{
+    hipError_t e = hipMalloc((void**)p, (size_t)(n > 0 ? n : 1) * sizeof(double));
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "hipMalloc");
+}
+
+RPtrs rptrs(const shx_field* f)
+{
+    RPtrs r;
+    for (int c = 0; c < 3; ++c) r.r[c] = f->R[c];
+    return r;
+}
+
+int r_exact_all(shx_system* s, double* const R[3], double t)
+{
+    for (int c = 0; c < 3; ++c) {
+        if (s->nR[c] == 0) continue;
+        k_r_exact<<<grid_for(s->nR[c]), 256, 0, s->ctx->stream>>>(s->dev, s->nR[c], s->rpos[c],
+                                                                  s->rdir[c], R[c], t);
+    }
+    return check(s->ctx, "cut-point boundary kernel");
+}
+
+int face_exact(shx_system* s, double* D, double t)
+{
+    if (s->nfaces == 0) return SHX_OK;
+    k_face_exact<<<grid_for(s->nfaces), 256, 0, s->ctx->stream>>>(s->dev, s->nfaces, s->faces, D, t);
+    return check(s->ctx, "face boundary kernel");
+}
+
+int ensure_fused(shx_system* s)
+{
+    if (s->accD) return SHX_OK;
+    int rc = alloc_d(s->ctx, &s->accD, s->N);
+    for (int b = 0; b < 2 && rc == SHX_OK; ++b) {
+        rc = alloc_d(s->ctx, &s->stageD[b], s->N);
+        for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = alloc_d(s->ctx, &s->stageR[b][c], s->nR[c]);
+    }
+    return rc;
+}
+
+// Full-length B (rows = N) rebuilt from the segment table in the builder's
+// (row, col, value) order, src/matrices/csr.cpp:10-35.
+int build_bfull(shx_system* s, int c)
+{
+    shx_derivative* d = s->d[c];
+    struct E {
+        long row, col;
+        double w;
+    };
+    std::vector<E> e;
+    const long stride = d->dev.stride;
+    const int n1 = s->n[1], n2 = s->n[2];
+    const int nfast = c == 2 ? n1 : n2;
+    for (const shx_line_block& b : d->h_blocks) {
+        const int slow = b.pair / nfast, fast = b.pair % nfast;
+        long base;
+        if (c == 0) base = (long)slow * n2 + fast;                     // (j, k)
+        else if (c == 1) base = (long)slow * n1 * n2 + fast;          // (i, k)
+        else base = ((long)slow * n1 + fast) * n2;                    // (i, j)
+        const int total = b.nl + b.ni + b.nr;
+        if (b.lobj >= 0)
+            for (int r = 0; r < b.nl; ++r)
+                e.push_back({base + (long)(b.lo + r) * stride, b.lobj, d->h_pool[b.lbw + r]});
+        if (b.robj >= 0)
+            for (int r = 0; r < b.nr; ++r)
+                e.push_back({base + (long)(b.lo + total - b.nr + r) * stride, b.robj,
+                             d->h_pool[b.rbw + r]});
+    }
+    std::sort(e.begin(), e.end(), [](const E& a, const E& b) {
+        if (a.row != b.row) return a.row < b.row;
+        if (a.col != b.col) return a.col < b.col;
+        return a.w < b.w;
+    });
+    std::vector<int64_t> u(s->N + 1, 0), v(e.size());
+    std::vector<double> w(e.size());
+    for (size_t q = 0; q < e.size(); ++q) {
+        u[e[q].row + 1]++;
+        v[q] = e[q].col;
+        w[q] = e[q].w;
+    }
+    for (long r = 0; r < s->N; ++r) u[r + 1] += u[r];
+    return shx_csr_create(s->ctx, s->N, u.data(), v.data(), w.data(), &s->Bfull[c]);
+}
+
+int ensure_unfused(shx_system* s)
+{
+    if (s->unfused_ready) return SHX_OK;
+    int rc = SHX_OK;
+    if (s->kind != SHX_SYS_HEAT) {
+        for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+            rc = alloc_d(s->ctx, &s->gG[c][0], s->N);
+            for (int r = 0; r < 3 && rc == SHX_OK; ++r) rc = alloc_d(s->ctx, &s->gG[c][r + 1], s->nR[r]);
+            rc = rc == SHX_OK ? alloc_d(s->ctx, &s->du[c][0], s->N) : rc;
+            for (int r = 0; r < 3 && rc == SHX_OK; ++r) rc = alloc_d(s->ctx, &s->du[c][r + 1], s->nR[r]);
+        }
+        if (rc != SHX_OK) return rc;
+        k_gG_D<<<grid_for(s->N), 256, 0, s->ctx->stream>>>(s->dev, s->N, s->gG[0][0], s->gG[1][0],
+                                                          s->gG[2][0]);
+        // gG on cut points: every body is Dirichlet, so the reference zeroes them
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) rc = rc == SHX_OK ? launch_fill(s->ctx, s->gG[c][r + 1], 0.0, s->nR[r]) : rc;
+        rc = rc == SHX_OK ? check(s->ctx, "gG kernel") : rc;
+    } else {
+        rc = alloc_d(s->ctx, &s->srcD, s->N);
+    }
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c)
+        if (s->d[c] && s->n[c] > 1) rc = build_bfull(s, c);
+    if (rc == SHX_OK) rc = sync(s->ctx);
+    s->unfused_ready = rc == SHX_OK;
+    return rc;
+}
+
+int rhs_fused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double t)
+{
+    const Tiling& tl = s->tiling;
+    if (s->kind == SHX_SYS_HEAT)
+        k_rhs<SHX_SYS_HEAT><<<tl.nblocks, kBlock, 0, s->ctx->stream>>>(tl, s->dev, s->ops, u->D,
+                                                                        rptrs(u), rhs->D, t);
+    else
+        k_rhs<SHX_SYS_WAVE_LINEAR><<<tl.nblocks, kBlock, 0, s->ctx->stream>>>(
+            tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
+    int rc = check(s->ctx, "rhs kernel");
+    // every body is Dirichlet: the cut-point rows of the rhs are zero
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = launch_fill(s->ctx, rhs->R[c], 0.0, s->nR[c]);
+    return rc;
+}
+
+int rhs_unfused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double t)
+{
+    int rc = ensure_unfused(s);
+    if (rc != SHX_OK) return rc;
+    shx_ctx* ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    if (s->kind != SHX_SYS_HEAT) {
+        // gradient: zero 12 buffers, then O and B per direction (gradient.hpp:35-122)
+        for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+            rc = launch_fill(ctx, s->du[c][0], 0.0, s->N);
+            for (int r = 0; r < 3 && rc == SHX_OK; ++r) rc = launch_fill(ctx, s->du[c][r + 1], 0.0, s->nR[r]);
+        }
+        for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+            if (!s->d[c] || s->n[c] < 2) continue;
+            rc = launch_derivative(s->d[c], u->D, nullptr, s->du[c][0], false, false);
+            if (rc == SHX_OK) rc = launch_csr_full(s->Bfull[c], u->R[c], s->du[c][0]);
+        }
+        if (rc != SHX_OK) return rc;
+        k_dot3<<<grid_for(s->N), 256, 0, st>>>(s->N, s->gG[0][0], s->du[0][0], s->gG[1][0], s->du[1][0],
+                                               s->gG[2][0], s->du[2][0], rhs->D);
+        for (int r = 0; r < 3; ++r)
+            if (s->nR[r] > 0)
+                k_dot3<<<grid_for(s->nR[r]), 256, 0, st>>>(s->nR[r], s->gG[0][r + 1], s->du[0][r + 1],
+                                                           s->gG[1][r + 1], s->du[1][r + 1],
+                                                           s->gG[2][r + 1], s->du[2][r + 1], rhs->R[r]);
+        return check(ctx, "dot kernel");
+    }
+    // heat: laplacian (zero, O+B per direction, plus_eq), scale, source, fills
+    rc = launch_fill(ctx, rhs->D, 0.0, s->N);
+    for (int r = 0; r < 3 && rc == SHX_OK; ++r) rc = launch_fill(ctx, rhs->R[r], 0.0, s->nR[r]);
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+        if (!s->d[c] || s->n[c] < 2) continue;
+        rc = launch_derivative(s->d[c], u->D, nullptr, rhs->D, true, false);
+        if (rc == SHX_OK) rc = launch_csr_full(s->Bfull[c], u->R[c], rhs->D);
+    }
+    if (rc != SHX_OK) return rc;
+    k_scale<<<grid_for(s->N), 256, 0, st>>>(s->N, rhs->D, s->dev.k);
+    for (int r = 0; r < 3; ++r)
+        if (s->nR[r] > 0) k_scale<<<grid_for(s->nR[r]), 256, 0, st>>>(s->nR[r], rhs->R[r], s->dev.k);
+    k_src_D<<<grid_for(s->N), 256, 0, st>>>(s->dev, s->N, s->srcD, t);
+    if (s->nfluid) k_gather_add<<<grid_for(s->nfluid), 256, 0, st>>>(s->nfluid, s->fluid, s->srcD, rhs->D);
+    if (s->nfaces) k_gather_fill<<<grid_for(s->nfaces), 256, 0, st>>>(s->nfaces, s->faces, rhs->D, 0.0);
+    rc = check(ctx, "heat source kernels");
+    // every body is Dirichlet: the cut-point rows are filled with zero
+    for (int r = 0; r < 3 && rc == SHX_OK; ++r) rc = launch_fill(ctx, rhs->R[r], 0.0, s->nR[r]);
+    return rc;
+}
+
+int update_boundary_launch(shx_system* s, shx_field* u, double t)
+{
+    int rc = face_exact(s, u->D, t);
+    if (rc == SHX_OK) rc = r_exact_all(s, u->R, t);
+    return rc;
+}
+
+int slot_assign_lc(shx_system* s, shx_field* d, const shx_field* s0, double c, const shx_field* r)
+{
+    hipStream_t st = s->ctx->stream;
+    k_assign_lc<<<grid_for(s->N), 256, 0, st>>>(s->N, d->D, s0->D, c, r->D);
+    for (int q = 0; q < 3; ++q)
+        if (s->nR[q] > 0) k_assign_lc<<<grid_for(s->nR[q]), 256, 0, st>>>(s->nR[q], d->R[q], s0->R[q], c, r->R[q]);
+    return check(s->ctx, "assign_lc kernel");
+}
+
+int slot_accumulate(shx_system* s, shx_field* d, double c, const shx_field* r)
+{
+    hipStream_t st = s->ctx->stream;
+    k_accumulate<<<grid_for(s->N), 256, 0, st>>>(s->N, d->D, c, r->D);
+    for (int q = 0; q < 3; ++q)
+        if (s->nR[q] > 0) k_accumulate<<<grid_for(s->nR[q]), 256, 0, st>>>(s->nR[q], d->R[q], c, r->R[q]);
+    return check(s->ctx, "accumulate kernel");
+}
+
+int slot_copy(shx_system* s, shx_field* d, const shx_field* src)
+{
+    hipStream_t st = s->ctx->stream;
+    k_copy<<<grid_for(s->N), 256, 0, st>>>(s->N, d->D, src->D);
+    for (int q = 0; q < 3; ++q)
+        if (s->nR[q] > 0) k_copy<<<grid_for(s->nR[q]), 256, 0, st>>>(s->nR[q], d->R[q], src->R[q]);
+    return check(s->ctx, "copy kernel");
+}
+
+int zero_slot(shx_system* s, shx_field* f) { return zero_field(s->ctx, f, s->n, s->nR); }
+
+constexpr double kRki[4] = {0.0, 0.5, 0.5, 1.0};
+constexpr double kRkf[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+// rk4.cpp:16-47, one kernel per reference operation
+int rk4_unfused(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk, shx_field* srhs,
+                double t, double dt)
+{
+    int rc = zero_slot(s, rk);
+    if (rc == SHX_OK) rc = zero_slot(s, srhs);
+    if (rc == SHX_OK) rc = slot_copy(s, out, u0);
+    for (int i = 0; i < 4 && rc == SHX_OK; ++i) {
+        if (i > 0) {
+            rc = slot_assign_lc(s, out, u0, dt * kRki[i], srhs);
+            if (rc == SHX_OK) rc = update_boundary_launch(s, out, t + dt * kRki[i]);
+        }
+        if (rc == SHX_OK) rc = rhs_unfused_launch(s, out, srhs, t + dt * kRki[i]);
+        if (rc == SHX_OK) rc = slot_accumulate(s, rk, dt * kRkf[i], srhs);
+    }
+    if (rc == SHX_OK) rc = slot_assign_lc(s, out, u0, 1.0, rk);
+    if (rc == SHX_OK) rc = update_boundary_launch(s, out, t + dt);
+    return rc;
+}
+
+template <int KIND>
+int launch_stage(shx_system* s, int stage, const double* usD, RPtrs usR, const double* u0,
+                 double* acc, double* out, double ts, double c_acc, double c_next, double tn)
+{
+    const Tiling& tl = s->tiling;
+    hipStream_t st = s->ctx->stream;
+    switch (stage) {
+    case 0:
+        k_rk4_stage<KIND, 0><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, usD, usR, u0, acc, out, ts, c_acc, c_next, tn);
+        break;
+    case 1:
+        k_rk4_stage<KIND, 1><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, usD, usR, u0, acc, out, ts, c_acc, c_next, tn);
+        break;
+    case 2:
+        k_rk4_stage<KIND, 2><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, usD, usR, u0, acc, out, ts, c_acc, c_next, tn);
+        break;
+    default:
+        k_rk4_stage<KIND, 3><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, usD, usR, u0, acc, out, ts, c_acc, c_next, tn);
+    }
+    return check(s->ctx, "rk4 stage kernel");
+}
+
+// One fused step; when ev is given, event pairs bracket each stage kernel.
+int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t* ev)
+{
+    int rc = ensure_fused(s);
+    for (int stage = 0; stage < 4 && rc == SHX_OK; ++stage) {
+        const double* usD = stage == 0 ? u0->D : s->stageD[(stage - 1) & 1];
+        RPtrs usR;
+        for (int c = 0; c < 3; ++c) usR.r[c] = stage == 0 ? u0->R[c] : s->stageR[(stage - 1) & 1][c];
+        double* outD = stage == 3 ? u0->D : s->stageD[stage & 1];
+        double* outR[3];
+        for (int c = 0; c < 3; ++c) outR[c] = stage == 3 ? u0->R[c] : s->stageR[stage & 1][c];
+        const double ts = t + dt * kRki[stage];
+        const double c_acc = dt * kRkf[stage];
+        const double c_next = stage < 3 ? dt * kRki[stage + 1] : 1.0;
+        const double tn = stage < 3 ? t + dt * kRki[stage + 1] : t + dt;
+        if (ev) hipEventRecord(ev[2 * stage], s->ctx->stream);
+        if (s->kind == SHX_SYS_HEAT)
+            rc = launch_stage<SHX_SYS_HEAT>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
+        else
+            rc = launch_stage<SHX_SYS_WAVE_LINEAR>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
+        if (ev) hipEventRecord(ev[2 * stage + 1], s->ctx->stream);
+        if (rc == SHX_OK) rc = r_exact_all(s, outR, tn);
+    }
+    return rc;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------ C ABI
+extern "C" {
+
+int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** out)
+{
+    if (!ctx || !desc || !out) return SHX_EINVAL;
+    *out = nullptr;
+    if (desc->kind < SHX_SYS_WAVE_RADIAL || desc->kind > SHX_SYS_HEAT)
+        return fail(ctx, SHX_EINVAL, "system: unknown kind");
+    for (int c = 0; c < 3; ++c)
+        for (long i = 0; i < desc->nR[c]; ++i)
+            if (!desc->r_dirichlet[c][i])
+                return fail(ctx, SHX_EUNSUPPORTED, "system: only Dirichlet bodies are supported");
+    for (int f = 0; f < 6; ++f)
+        if (desc->grid_bc[f] == SHX_BC_NEUMANN)
+            return fail(ctx, SHX_EUNSUPPORTED, "system: Neumann grid faces are not supported");
+
+    auto* s = new (std::nothrow) shx_system();
+    if (!s) return SHX_ENOMEM;
+    s->ctx = ctx;
+    s->kind = desc->kind;
+    for (int c = 0; c < 3; ++c) {
+        s->n[c] = desc->n[c];
+        s->nR[c] = desc->nR[c];
+        s->d[c] = desc->d[c];
+    }
+    s->N = (long)s->n[0] * s->n[1] * s->n[2];
+    s->tiling = make_tiling(s->n);
+
+    std::memset(&s->dev, 0, sizeof s->dev);
+    DevSys& dv = s->dev;
+    dv.kind = desc->kind;
+    dv.n0 = s->n[0];
+    dv.n1 = s->n[1];
+    dv.n2 = s->n[2];
+    for (int f = 0; f < 6; ++f) dv.bc[f] = desc->grid_bc[f];
+    for (int c = 0; c < 3; ++c) {
+        dv.center[c] = desc->center[c];
+        dv.vel[c] = desc->vel[c];
+        dv.x0[c] = desc->x0[c];
+    }
+    dv.radius = desc->radius;
+    dv.width = desc->width;
+    dv.amp = desc->amp;
+    dv.k = desc->diffusivity;
+    dv.mms_kind = desc->mms_kind;
+    dv.mms_dims = desc->mms_dims;
+    dv.mms_n = desc->mms_n;
+
+    int rc = upload(ctx, &s->xyz[0], desc->x, (size_t)s->n[0]);
+    if (rc == SHX_OK) rc = upload(ctx, &s->xyz[1], desc->y, (size_t)s->n[1]);
+    if (rc == SHX_OK) rc = upload(ctx, &s->xyz[2], desc->z, (size_t)s->n[2]);
+    if (rc == SHX_OK && desc->mms_n > 0) rc = upload(ctx, &s->mms, desc->mms, (size_t)8 * desc->mms_n);
+    // fluid gather and bit mask
+    std::vector<unsigned> mask((s->N + 31) / 32, 0u);
+    std::vector<int> fl(desc->nfluid);
+    for (long q = 0; q < desc->nfluid; ++q) {
+        const long idx = desc->fluid[q];
+        mask[idx >> 5] |= 1u << (idx & 31);
+        fl[q] = (int)idx;
+    }
+    s->nfluid = desc->nfluid;
+    if (rc == SHX_OK) rc = upload(ctx, &s->fluid_mask, mask.data(), mask.size());
+    if (rc == SHX_OK) rc = upload(ctx, &s->fluid, fl.data(), fl.size());
+    // Dirichlet face list in for_each_grid_bc_desc order (duplicates kept)
+    std::vector<int> faces;
+    const int nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    for (int dir = 0; dir < 3; ++dir)
+        for (int side = 0; side < 2; ++side) {
+            if (desc->grid_bc[2 * dir + side] != SHX_BC_DIRICHLET) continue;
+            const int p = side ? s->n[dir] - 1 : 0;
+            if (dir == 0)
+                for (long q = 0; q < (long)ny * nz; ++q) faces.push_back((int)((long)p * ny * nz + q));
+            else if (dir == 1)
+                for (int i = 0; i < nx; ++i)
+                    for (int k = 0; k < nz; ++k) faces.push_back((int)((long)i * ny * nz + (long)p * nz + k));
+            else
+                for (long q = 0; q < (long)nx * ny; ++q) faces.push_back((int)(p + q * nz));
+        }
+    s->nfaces = (long)faces.size();
+    if (rc == SHX_OK) rc = upload(ctx, &s->faces, faces.data(), faces.size());
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+        rc = upload(ctx, &s->rpos[c], desc->r_pos[c], (size_t)(3 * s->nR[c]));
+        std::vector<signed char> fl8(s->nR[c]);
+        for (long i = 0; i < s->nR[c]; ++i) fl8[i] = desc->r_dirichlet[c][i] ? 1 : 0;
+        if (rc == SHX_OK)
+            rc = upload(ctx, reinterpret_cast<int8_t**>(&s->rdir[c]),
+                        reinterpret_cast<const int8_t*>(fl8.data()), fl8.size());
+    }
+    if (rc == SHX_OK) {
+        const long nparts = 1024;
+        hipError_t e = hipMalloc((void**)&s->part, (size_t)nparts * 3 * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc((void**)&s->part_idx, (size_t)nparts * sizeof(long));
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "hipMalloc");
+    }
+    if (rc != SHX_OK) {
+        shx_system_destroy(s);
+        return rc;
+    }
+    dv.x = s->xyz[0];
+    dv.y = s->xyz[1];
+    dv.z = s->xyz[2];
+    dv.mms = s->mms;
+    dv.fluid_mask = s->fluid_mask;
+    for (int c = 0; c < 3; ++c) {
+        if (s->d[c]) s->ops.d[c] = s->d[c]->dev;
+        else std::memset(&s->ops.d[c], 0, sizeof(DevDeriv));
+    }
+    *out = s;
+    return SHX_OK;
+}
+
+int shx_system_destroy(shx_system* s)
+{
+    if (!s) return SHX_EINVAL;
+    hipStreamSynchronize(s->ctx->stream);
+    for (int c = 0; c < 3; ++c) {
+        hipFree(s->xyz[c]);
+        hipFree(s->rpos[c]);
+        hipFree(s->rdir[c]);
+        for (int b = 0; b < 2; ++b) hipFree(s->stageR[b][c]);
+        for (int r = 0; r < 4; ++r) {
+            hipFree(s->gG[c][r]);
+            hipFree(s->du[c][r]);
+        }
+        if (s->Bfull[c]) shx_csr_destroy(s->Bfull[c]);
+    }
+    hipFree(s->stageD[0]);
+    hipFree(s->stageD[1]);
+    hipFree(s->accD);
+    hipFree(s->srcD);
+    hipFree(s->mms);
+    hipFree(s->fluid_mask);
+    hipFree(s->fluid);
+    hipFree(s->faces);
+    hipFree(s->part);
+    hipFree(s->part_idx);
+    delete s;
+    return SHX_OK;
+}
+
+int shx_system_rhs(shx_system* s, const shx_field* u, shx_field* rhs, double t)
+{
+    if (!s || !u || !rhs) return SHX_EINVAL;
+    int rc = rhs_fused_launch(s, u, rhs, t);
+    return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
+int shx_system_rhs_unfused(shx_system* s, const shx_field* u, shx_field* rhs, double t)
+{
+    if (!s || !u || !rhs) return SHX_EINVAL;
+    int rc = rhs_unfused_launch(s, u, rhs, t);
+    return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
+int shx_system_update_boundary(shx_system* s, shx_field* u, double t)
+{
+    if (!s || !u) return SHX_EINVAL;
+    int rc = update_boundary_launch(s, u, t);
+    return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
+int shx_system_exact(shx_system* s, shx_field* out, double t)
+{
+    if (!s || !out) return SHX_EINVAL;
+    hipStream_t st = s->ctx->stream;
+    k_exact_D<<<grid_for(s->N), 256, 0, st>>>(s->dev, s->N, out->D, t);
+    for (int c = 0; c < 3; ++c)
+        if (s->nR[c] > 0) k_exact_R<<<grid_for(s->nR[c]), 256, 0, st>>>(s->dev, s->nR[c], s->rpos[c], out->R[c], t);
+    int rc = check(s->ctx, "exact kernel");
+    return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
+int shx_system_initialize(shx_system* s, shx_field* u, double t)
+{
+    if (!s || !u) return SHX_EINVAL;
+    hipStream_t st = s->ctx->stream;
+    int rc = launch_fill(s->ctx, u->D, 0.0, s->N);
+    if (rc == SHX_OK && s->nfluid)
+        k_init_fluid<<<grid_for(s->nfluid), 256, 0, st>>>(s->dev, s->nfluid, s->fluid, u->D, t);
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c)
+        if (s->nR[c] > 0) k_exact_R<<<grid_for(s->nR[c]), 256, 0, st>>>(s->dev, s->nR[c], s->rpos[c], u->R[c], t);
+    if (rc == SHX_OK) rc = check(s->ctx, "initialize kernels");
+    return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
+int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[11])
+{
+    if (!s || !u || !stats) return SHX_EINVAL;
+    const long nparts = 1024;
+    double* pe = s->part;
+    double* pmin = s->part + nparts;
+    double* pmax = s->part + 2 * nparts;
+    for (int q = 0; q < 11; ++q) stats[q] = 0.0;
+    if (s->nfluid > 0) {
+        const unsigned grid = (unsigned)std::min<long>(nparts, (s->nfluid + 255) / 256);
+        k_stats<<<grid, 256, 0, s->ctx->stream>>>(s->dev, s->nfluid, s->fluid, u->D, t, pe, s->part_idx,
+                                                   pmin, pmax);
+        int rc = check(s->ctx, "stats kernel");
+        if (rc != SHX_OK) return rc;
+        std::vector<double> h(3 * nparts);
+        std::vector<long> hi(nparts);
+        hipError_t e = hipMemcpyAsync(h.data(), s->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hi.data(), s->part_idx, hi.size() * sizeof(long), hipMemcpyDeviceToHost, s->ctx->stream);
+        if (e != hipSuccess) return hip_fail(s->ctx, e, "stats copy");
+        rc = sync(s->ctx);
+        if (rc != SHX_OK) return rc;
+        double err = -DBL_MAX, mn = DBL_MAX, mx = -DBL_MAX;
+        long ei = 0;
+        for (unsigned b = 0; b < grid; ++b) {
+            if (h[b] > err || (h[b] == err && hi[b] < ei)) {
+                err = h[b];
+                ei = hi[b];
+            }
+            mn = std::min(mn, h[nparts + b]);
+            mx = std::max(mx, h[2 * nparts + b]);
+        }
+        stats[0] = err;
+        stats[1] = mn;
+        stats[2] = mx;
+        stats[3] = err;
+        stats[4] = (double)ei;
+    }
+    return SHX_OK;
+}
+
+int shx_rk4_step(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk, shx_field* srhs,
+                 double t, double dt)
+{
+    if (!s || !u0 || !out || !rk || !srhs) return SHX_EINVAL;
+    int rc = rk4_unfused(s, u0, out, rk, srhs, t, dt);
+    return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
+int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt, int nsteps)
+{
+    if (!s || !u0 || nsteps < 0) return SHX_EINVAL;
+    int rc = SHX_OK;
+    for (int n = 0; n < nsteps && rc == SHX_OK; ++n) rc = rk4_fused_step(s, u0, t + n * dt, dt, nullptr);
+    return rc;
+}
+
+int shx_time_fused_steps(shx_system* s, shx_field* u0, double t, double dt, int nsteps, int reps,
+                         double* ms_per_step)
+{
+    if (!s || !u0 || nsteps < 1 || reps < 1 || !ms_per_step) return SHX_EINVAL;
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    int rc = ensure_fused(s);
+    if (rc == SHX_OK) rc = sync(s->ctx);
+    hipEventRecord(a, s->ctx->stream);
+    for (int r = 0; r < reps && rc == SHX_OK; ++r)
+        for (int n = 0; n < nsteps && rc == SHX_OK; ++n) rc = rk4_fused_step(s, u0, t + n * dt, dt, nullptr);
+    hipEventRecord(b, s->ctx->stream);
+    if (rc == SHX_OK) rc = sync(s->ctx);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    *ms_per_step = (double)ms / ((double)reps * nsteps);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return rc;
+}
+
+int shx_time_unfused_steps(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk,
+                           shx_field* srhs, double t, double dt, int nsteps, double* ms_per_step)
+{
+    if (!s || !u0 || !out || !rk || !srhs || nsteps < 1 || !ms_per_step) return SHX_EINVAL;
+    int rc = ensure_unfused(s);
+    if (rc != SHX_OK) return rc;
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, s->ctx->stream);
+    for (int n = 0; n < nsteps && rc == SHX_OK; ++n) {
+        rc = rk4_unfused(s, u0, out, rk, srhs, t + n * dt, dt);
+        // simulation_cycle.cpp:122-125: u0 <- u1 by copy
+        if (rc == SHX_OK) rc = slot_copy(s, u0, out);
+    }
+    hipEventRecord(b, s->ctx->stream);
+    if (rc == SHX_OK) rc = sync(s->ctx);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    *ms_per_step = (double)ms / nsteps;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    return rc;
+}
+
+// Mean device time of one fused stage kernel, bracketed by HIP events on the
+// stream it is launched on (the cut-point kernels are outside the brackets).
+int shx_time_fused_stage_kernel(shx_system* s, shx_field* u0, double t, double dt, int nsteps,
+                                double* ms_per_kernel)
+{
+    if (!s || !u0 || nsteps < 1 || !ms_per_kernel) return SHX_EINVAL;
+    int rc = ensure_fused(s);
+    if (rc == SHX_OK) rc = sync(s->ctx);
+    std::vector<hipEvent_t> ev(8 * (size_t)nsteps);
+    for (auto& e : ev) hipEventCreate(&e);
+    for (int n = 0; n < nsteps && rc == SHX_OK; ++n) rc = rk4_fused_step(s, u0, t + n * dt, dt, &ev[8 * (size_t)n]);
+    if (rc == SHX_OK) rc = sync(s->ctx);
+    double total = 0.0;
+    for (int q = 0; q < 4 * nsteps; ++q) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]);
+        total += ms;
+    }
+    *ms_per_kernel = total / (4.0 * nsteps);
+    for (auto& e : ev) hipEventDestroy(e);
+    return rc;
+}
+
+}  // extern "C"

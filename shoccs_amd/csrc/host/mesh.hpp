@@ -1,0 +1,108 @@
+// mesh.hpp -- Cartesian mesh with embedded spheres, found by ray casting.
+//
+// Mirrors ccs::mesh (src/mesh/mesh.hpp:15-119) and ccs::object_geometry
+// (src/mesh/object_geometry.hpp): coordinates, cut points R(dir) ordered by
+// (slow, fast) then along dir, line segments per direction, and the fluid
+// gather.  Runs once on the host; every value that decides a cut point is
+// computed in the reference's order so that masks, line extents and psi come
+// out identical.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <optional>
+#include <vector>
+
+namespace ccs {
+
+using real = double;
+using real3 = std::array<real, 3>;
+using int3 = std::array<int, 3>;
+
+namespace bcs {
+// src/operators/boundaries.hpp:13-20
+enum class type : int { Dirichlet = 0, Floating = 1, Neumann = 2 };
+struct Line {
+    type left = type::Floating;
+    type right = type::Floating;
+};
+using Grid = std::array<Line, 3>;
+using Object = std::vector<type>;
+}  // namespace bcs
+
+// index::dir<I>::slow / fast (src/indexing.hpp:21-37)
+constexpr int slow_axis(int dir) { return dir == 0 ? 1 : 0; }
+constexpr int fast_axis(int dir) { return dir == 2 ? 1 : 2; }
+
+struct sphere_shape {
+    real3 center;
+    real radius;
+};
+
+// src/mesh/mesh_types.hpp:30-38
+struct mesh_object_info {
+    real psi;
+    real3 position;
+    real3 normal;
+    bool ray_outside;
+    int3 solid_coord;
+    int shape_id;
+};
+
+struct object_boundary {
+    int64_t object_coordinate;
+    int64_t objectID;
+    real psi;
+};
+
+struct boundary {
+    int3 mesh_coordinate;
+    std::optional<object_boundary> object;
+};
+
+struct line {
+    int64_t stride;
+    boundary start;
+    boundary end;
+};
+
+class mesh
+{
+public:
+    mesh() = default;
+    mesh(const int3& n, const real3& min, const real3& max, const std::vector<sphere_shape>& shapes);
+
+    const int3& extents() const { return n_; }
+    int64_t size() const { return (int64_t)n_[0] * n_[1] * n_[2]; }
+    int dims() const { return dims_; }
+    real h(int i) const { return h_[i]; }
+    const real3& h() const { return h_; }
+    const std::vector<real>& coords(int i) const { return xyz_[i]; }
+    const std::vector<mesh_object_info>& R(int dir) const { return R_[dir]; }
+    const std::vector<line>& lines(int dir) const { return lines_[dir]; }
+    const std::vector<int64_t>& fluid() const { return fluid_; }
+    const std::vector<sphere_shape>& shapes() const { return shapes_; }
+    int64_t stride(int dir) const;
+    int64_t ic(const int3& ijk) const
+    {
+        return (int64_t)ijk[0] * n_[1] * n_[2] + (int64_t)ijk[1] * n_[2] + ijk[2];
+    }
+    // mesh::dirichlet_line (src/mesh/mesh.cpp:154-168)
+    bool dirichlet_line(const int3& start, int dir, const bcs::Grid& grid) const;
+
+private:
+    int3 n_{1, 1, 1};
+    real3 min_{}, max_{}, h_{};
+    int dims_ = 0;
+    std::array<std::vector<real>, 3> xyz_;
+    std::vector<sphere_shape> shapes_;
+    std::array<std::vector<mesh_object_info>, 3> R_;
+    std::array<std::vector<line>, 3> lines_;
+    std::vector<int64_t> fluid_;
+
+    void cast_rays(int dir);
+    void build_lines(int dir);
+    void build_fluid();
+};
+
+}  // namespace ccs

@@ -1,0 +1,287 @@
+// problem.cpp -- C entry points of the host library (include/shoccs.h): build
+// a mesh, its boundary conditions, stencil and system, upload them through
+// shoccs_hip.h, and expose the assembled structures.
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "operators.hpp"
+#include "shoccs.h"
+
+namespace ccs {
+
+struct problem {
+    shx_ctx* ctx = nullptr;
+    int kind = 0;
+    std::unique_ptr<mesh> m;
+    bcs::Grid grid;
+    bcs::Object obj;
+    std::unique_ptr<derivative> d[3];
+    shx_system* sys = nullptr;
+    double diffusivity = 1.0;
+};
+
+}  // namespace ccs
+
+struct shoccs_problem : ccs::problem {
+};
+
+using namespace ccs;
+
+extern "C" {
+
+int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_problem** out)
+{
+    if (!desc || !out) return SHX_EINVAL;
+    *out = nullptr;
+    const double* pr = desc->params;
+    const int np = desc->nparams;
+    if (desc->kind == SHX_SYS_WAVE_RADIAL && np < 4) return SHX_EINVAL;
+    if (desc->kind == SHX_SYS_WAVE_LINEAR && np < 8) return SHX_EINVAL;
+    if (desc->kind == SHX_SYS_HEAT && np < 2) return SHX_EINVAL;
+    auto st = stencil::make(desc->scheme, desc->alpha, 2);
+    if (!st) return SHX_EINVAL;
+
+    auto p = std::unique_ptr<shoccs_problem>(new (std::nothrow) shoccs_problem());
+    if (!p) return SHX_ENOMEM;
+    p->ctx = ctx;
+    p->kind = desc->kind;
+    std::vector<sphere_shape> shapes;
+    for (int s = 0; s < desc->nshapes; ++s) {
+        const double* q = desc->spheres + 4 * s;
+        shapes.push_back(sphere_shape{{q[0], q[1], q[2]}, q[3]});
+        p->obj.push_back((bcs::type)desc->shape_bc[s]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        p->grid[a].left = (bcs::type)desc->grid_bc[2 * a];
+        p->grid[a].right = (bcs::type)desc->grid_bc[2 * a + 1];
+    }
+    p->m = std::make_unique<mesh>(int3{desc->n[0], desc->n[1], desc->n[2]},
+                                  real3{desc->min[0], desc->min[1], desc->min[2]},
+                                  real3{desc->max[0], desc->max[1], desc->max[2]}, shapes);
+    for (int a = 0; a < 3; ++a) {
+        p->d[a] = std::make_unique<derivative>(ctx, a, *p->m, *st, p->grid, p->obj);
+        if (p->d[a]->status() != SHX_OK) return p->d[a]->status();
+    }
+
+    shx_system_desc sd;
+    std::memset(&sd, 0, sizeof sd);
+    sd.kind = desc->kind;
+    const mesh& m = *p->m;
+    for (int a = 0; a < 3; ++a) {
+        sd.n[a] = m.extents()[a];
+        sd.h[a] = m.h(a);
+        sd.nR[a] = (int64_t)m.R(a).size();
+        sd.d[a] = m.extents()[a] > 1 ? p->d[a]->device() : nullptr;
+    }
+    sd.x = m.coords(0).data();
+    sd.y = m.coords(1).data();
+    sd.z = m.coords(2).data();
+    for (int f = 0; f < 6; ++f) sd.grid_bc[f] = desc->grid_bc[f];
+    std::vector<double> rpos[3];
+    std::vector<int32_t> rdir[3];
+    for (int a = 0; a < 3; ++a) {
+        for (const auto& r : m.R(a)) {
+            rpos[a].insert(rpos[a].end(), r.position.begin(), r.position.end());
+            rdir[a].push_back(p->obj[r.shape_id] == bcs::type::Dirichlet);
+        }
+        rpos[a].push_back(0.0);
+        rdir[a].push_back(0);
+        sd.r_pos[a] = rpos[a].data();
+        sd.r_dirichlet[a] = rdir[a].data();
+    }
+    sd.fluid = m.fluid().data();
+    sd.nfluid = (int64_t)m.fluid().size();
+    std::vector<double> mms;
+    if (desc->kind == SHX_SYS_WAVE_RADIAL) {
+        for (int c = 0; c < 3; ++c) sd.center[c] = pr[c];
+        sd.radius = pr[3];
+    } else if (desc->kind == SHX_SYS_WAVE_LINEAR) {
+        for (int c = 0; c < 3; ++c) {
+            sd.vel[c] = pr[c];
+            sd.x0[c] = pr[3 + c];
+        }
+        sd.width = pr[6];
+        sd.amp = pr[7];
+    } else {
+        sd.diffusivity = pr[0];
+        p->diffusivity = pr[0];
+        sd.mms_kind = (int32_t)pr[1];
+        sd.mms_dims = m.dims();
+        if (sd.mms_kind == SHX_MMS_GAUSS) {
+            if (np < 3 || np < 3 + 8 * (int)pr[2]) return SHX_EINVAL;
+            sd.mms_n = (int32_t)pr[2];
+            mms.assign(pr + 3, pr + 3 + 8 * sd.mms_n);
+            sd.mms = mms.data();
+        }
+    }
+    // a NULL context builds the host structures only (no device, no system)
+    int rc = ctx ? shx_system_create(ctx, &sd, &p->sys) : SHX_OK;
+    if (rc != SHX_OK) return rc;
+    *out = p.release();
+    return SHX_OK;
+}
+
+int shoccs_problem_destroy(shoccs_problem* p)
+{
+    if (!p) return SHX_EINVAL;
+    if (p->sys) shx_system_destroy(p->sys);
+    delete p;
+    return SHX_OK;
+}
+
+shx_system* shoccs_problem_system(shoccs_problem* p) { return p ? p->sys : nullptr; }
+
+shx_derivative* shoccs_problem_derivative(shoccs_problem* p, int dir)
+{
+    if (!p || dir < 0 || dir > 2) return nullptr;
+    return p->d[dir]->device();
+}
+
+int64_t shoccs_problem_size(const shoccs_problem* p, int which)
+{
+    if (!p) return -1;
+    if (which == 0) return p->m->size();
+    if (which >= 1 && which <= 3) return (int64_t)p->m->R(which - 1).size();
+    if (which == 4) return (int64_t)p->m->fluid().size();
+    return -1;
+}
+
+double shoccs_problem_dt(const shoccs_problem* p, double h_cfl, double p_cfl)
+{
+    const real3& h = p->m->h();
+    const double hmin = std::min(h[0], std::min(h[1], h[2]));
+    if (p->kind == SHX_SYS_HEAT) return p_cfl * hmin * hmin / (4 * p->diffusivity);
+    return h_cfl * hmin;
+}
+
+int shoccs_mesh_r(const shoccs_problem* p, int dir, double* psi, double* pos, int32_t* solid,
+                  int32_t* ray_outside, int32_t* shape_id)
+{
+    if (!p || dir < 0 || dir > 2) return SHX_EINVAL;
+    const auto& R = p->m->R(dir);
+    for (size_t i = 0; i < R.size(); ++i) {
+        if (psi) psi[i] = R[i].psi;
+        for (int c = 0; c < 3; ++c) {
+            if (pos) pos[3 * i + c] = R[i].position[c];
+            if (solid) solid[3 * i + c] = R[i].solid_coord[c];
+        }
+        if (ray_outside) ray_outside[i] = R[i].ray_outside;
+        if (shape_id) shape_id[i] = R[i].shape_id;
+    }
+    return SHX_OK;
+}
+
+int64_t shoccs_mesh_nlines(const shoccs_problem* p, int dir)
+{
+    return p && dir >= 0 && dir < 3 ? (int64_t)p->m->lines(dir).size() : -1;
+}
+
+int shoccs_mesh_lines(const shoccs_problem* p, int dir, int64_t* out)
+{
+    if (!p || dir < 0 || dir > 2 || !out) return SHX_EINVAL;
+    const int S = slow_axis(dir), F = fast_axis(dir);
+    int64_t* o = out;
+    for (const auto& l : p->m->lines(dir)) {
+        o[0] = l.start.mesh_coordinate[dir];
+        o[1] = l.end.mesh_coordinate[dir];
+        o[2] = l.start.object ? l.start.object->object_coordinate : -1;
+        o[3] = l.end.object ? l.end.object->object_coordinate : -1;
+        o[4] = l.start.mesh_coordinate[S];
+        o[5] = l.start.mesh_coordinate[F];
+        o[6] = l.stride;
+        o += 7;
+    }
+    return SHX_OK;
+}
+
+int shoccs_mesh_fluid(const shoccs_problem* p, int64_t* out)
+{
+    if (!p || !out) return SHX_EINVAL;
+    const auto& f = p->m->fluid();
+    std::memcpy(out, f.data(), f.size() * sizeof(int64_t));
+    return SHX_OK;
+}
+
+int64_t shoccs_deriv_nblocks(const shoccs_problem* p, int dir)
+{
+    return p && dir >= 0 && dir < 3 ? (int64_t)p->d[dir]->blocks().size() : -1;
+}
+
+int shoccs_deriv_blocks(const shoccs_problem* p, int dir, shx_line_block* out)
+{
+    if (!p || dir < 0 || dir > 2 || !out) return SHX_EINVAL;
+    const auto& b = p->d[dir]->blocks();
+    std::memcpy(out, b.data(), b.size() * sizeof(shx_line_block));
+    return SHX_OK;
+}
+
+int64_t shoccs_deriv_npool(const shoccs_problem* p, int dir)
+{
+    return p && dir >= 0 && dir < 3 ? (int64_t)p->d[dir]->pool().size() : -1;
+}
+
+int shoccs_deriv_pool(const shoccs_problem* p, int dir, double* out)
+{
+    if (!p || dir < 0 || dir > 2 || !out) return SHX_EINVAL;
+    const auto& v = p->d[dir]->pool();
+    std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return SHX_OK;
+}
+
+int shoccs_deriv_interior(const shoccs_problem* p, int dir, double* out9)
+{
+    if (!p || dir < 0 || dir > 2 || !out9) return SHX_EINVAL;
+    const auto& v = p->d[dir]->interior();
+    for (size_t j = 0; j < 9; ++j) out9[j] = j < v.size() ? v[j] : 0.0;
+    return (int)v.size();
+}
+
+int shoccs_field_alloc(shoccs_problem* p, shx_field* f)
+{
+    if (!p || !f) return SHX_EINVAL;
+    std::memset(f, 0, sizeof *f);
+    int rc = shx_alloc(p->ctx, sizeof(double) * (size_t)p->m->size(), (void**)&f->D);
+    for (int a = 0; a < 3 && rc == SHX_OK; ++a)
+        rc = shx_alloc(p->ctx, sizeof(double) * (p->m->R(a).size() + 1), (void**)&f->R[a]);
+    if (rc == SHX_OK) rc = shx_fill(p->ctx, f->D, 0.0, p->m->size());
+    for (int a = 0; a < 3 && rc == SHX_OK; ++a) rc = shx_fill(p->ctx, f->R[a], 0.0, (int64_t)p->m->R(a).size());
+    if (rc == SHX_OK) rc = shx_sync(p->ctx);
+    return rc;
+}
+
+int shoccs_field_free(shoccs_problem* p, shx_field* f)
+{
+    if (!p || !f) return SHX_EINVAL;
+    shx_free(p->ctx, f->D);
+    for (int a = 0; a < 3; ++a) shx_free(p->ctx, f->R[a]);
+    std::memset(f, 0, sizeof *f);
+    return SHX_OK;
+}
+
+int shoccs_field_upload(shoccs_problem* p, shx_field* f, const double* flat)
+{
+    if (!p || !f || !flat) return SHX_EINVAL;
+    int rc = shx_h2d(p->ctx, f->D, flat, sizeof(double) * (size_t)p->m->size());
+    size_t off = (size_t)p->m->size();
+    for (int a = 0; a < 3 && rc == SHX_OK; ++a) {
+        rc = shx_h2d(p->ctx, f->R[a], flat + off, sizeof(double) * p->m->R(a).size());
+        off += p->m->R(a).size();
+    }
+    return rc;
+}
+
+int shoccs_field_download(shoccs_problem* p, const shx_field* f, double* flat)
+{
+    if (!p || !f || !flat) return SHX_EINVAL;
+    int rc = shx_d2h(p->ctx, flat, f->D, sizeof(double) * (size_t)p->m->size());
+    size_t off = (size_t)p->m->size();
+    for (int a = 0; a < 3 && rc == SHX_OK; ++a) {
+        rc = shx_d2h(p->ctx, flat + off, f->R[a], sizeof(double) * p->m->R(a).size());
+        off += p->m->R(a).size();
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// stencils.hpp -- the interior stencil and numerical boundary scheme of one
+// discretisation, as the type-erased ccs::stencil of the reference
+// (src/stencils/stencil.hpp:22-300, src/stencils/stencil.cpp:8-106).
+#pragma once
+
+#include <array>
+#include <optional>
+
+#include "mesh.hpp"
+
+namespace ccs::stencils {
+
+// stencil.hpp:22-27: interior order 2p with 2p+1 points; r x t closure; extras
+struct info {
+    int p = 0;
+    int r = 0;
+    int t = 0;
+    int nextra = 0;
+};
+
+// scheme{order, type}: order 2 "E2"/"E4" and order 1 "E4"/"E4u"
+enum class scheme : int { E2_2 = 0, E4_2 = 1, E4_1 = 2, E4u_1 = 3 };
+
+class stencil
+{
+public:
+    stencil() = default;
+    stencil(scheme k, std::array<double, 2> alpha) : kind_(k), alpha_(alpha) {}
+
+    // returns nullopt for an unknown scheme or an E4_1 alpha[1] below 197/288
+    static std::optional<stencil> make(int kind, const double* alpha, int nalpha);
+
+    info query(bcs::type b) const;
+    info query_max() const;
+    void interior(double h, double* c) const;
+    // closure of r x t coefficients (row-major) for a wall at distance psi*h;
+    // Neumann closures also fill `extra`
+    void nbs(double h, bcs::type b, double psi, bool right, double* c, double* extra) const;
+    scheme kind() const { return kind_; }
+
+private:
+    scheme kind_ = scheme::E4_2;
+    std::array<double, 2> alpha_{0.0, 0.0};
+};
+
+}  // namespace ccs::stencils
+
+namespace ccs {
+using stencils::stencil;
+}

@@ -1,0 +1,168 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+FP64 operator applies are compared bit for bit: the kernels evaluate every
+reference sum in the reference's order with contraction disabled.  Results
+that involve transcendental functions (manufactured sources, exact boundary
+values) are compared with the tolerances stated in each test.
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle_ffi as O
+
+pytestmark = pytest.mark.gpu
+
+WAVE_CFGS = {
+    "wave48_E4_1": H.sphere_wave_cfg(48),
+    "wave20_E4u_overlap": H.sphere_wave_cfg(20, scheme=O.E4U_1, alpha=(0.1, 0.2)),
+    "wave30_radial_E4u": H.sphere_wave_cfg(30, linear=False, scheme=O.E4U_1),
+}
+HEAT_CFGS = {
+    "heat2d_21": H.heat2d_cfg(21),
+    "heat3d_E4_2": H.heat3d_cfg(),
+    "heat3d_E2_2": H.heat3d_cfg(scheme=O.E2_2),
+}
+ALL = {**WAVE_CFGS, **HEAT_CFGS}
+
+
+def _rand_field(osys, seed):
+    rng = np.random.default_rng(seed)
+    f = osys.field()
+    f.D[:] = rng.standard_normal(f.D.size)
+    for d in range(3):
+        f.R[d][:] = rng.standard_normal(f.R[d].size)
+    return f
+
+
+def _pair(ctx, cfg):
+    osys = H.oracle_system(cfg)
+    prob = H.product_problem(ctx, cfg)
+    return osys, prob
+
+
+@pytest.mark.parametrize("name", list(ALL))
+@pytest.mark.parametrize("plus", [False, True])
+def test_derivative_apply_bitexact(ctx, name, plus):
+    cfg = ALL[name]
+    osys, prob = _pair(ctx, cfg)
+    u = _rand_field(osys, 1)
+    for d in range(3):
+        if prob.n[d] < 2:
+            continue
+        du = _rand_field(osys, 2)  # eq must overwrite rows only; plus_eq accumulates
+        gu, gdu = prob.field(), prob.field()
+        gu.upload(u.flat())
+        gdu.upload(du.flat())
+        osys.deriv_apply(d, u, du, plus_eq=plus)
+        prob.derivative(d)(gu, gdu, 1 if plus else 0)
+        got = gdu.download()
+        want = du.flat()
+        assert np.array_equal(got, want), f"{name} dir {d}: max diff {np.max(np.abs(got - want))}"
+
+
+@pytest.mark.parametrize("name", list(WAVE_CFGS))
+def test_gradient_bitexact(ctx, name):
+    osys, prob = _pair(ctx, WAVE_CFGS[name])
+    u = _rand_field(osys, 3)
+    want = osys.gradient(u)
+    gu = prob.field()
+    gu.upload(u.flat())
+    outs = [prob.field() for _ in range(3)]
+    prob.gradient(gu)(*outs)
+    for c in range(3):
+        assert np.array_equal(outs[c].download(), want[c].flat())
+
+
+@pytest.mark.parametrize("name", list(HEAT_CFGS))
+def test_laplacian_bitexact(ctx, name):
+    osys, prob = _pair(ctx, HEAT_CFGS[name])
+    u = _rand_field(osys, 4)
+    want = osys.laplacian(u)
+    gu, gl = prob.field(), prob.field()
+    gu.upload(u.flat())
+    prob.laplacian(gu)(gl)
+    assert np.array_equal(gl.download(), want.flat())
+
+
+@pytest.mark.parametrize("name", list(WAVE_CFGS))
+@pytest.mark.parametrize("fused", [True, False])
+def test_wave_rhs_bitexact(ctx, name, fused):
+    """Scalar-wave RHS has no transcendental in the linear form; the radial
+    form computes gG with sqrt and division, both correctly rounded."""
+    osys, prob = _pair(ctx, WAVE_CFGS[name])
+    u = _rand_field(osys, 5)
+    want = osys.rhs(u, 0.3)
+    gu, gr = prob.field(), prob.field()
+    gu.upload(u.flat())
+    prob.rhs(gu, gr, 0.3, fused=fused)
+    assert np.array_equal(gr.download(), want.flat())
+
+
+@pytest.mark.parametrize("name", list(HEAT_CFGS))
+@pytest.mark.parametrize("fused", [True, False])
+def test_heat_rhs(ctx, name, fused):
+    """Heat RHS adds the manufactured source (sin, exp on the device vs glibc
+    on the host): tolerance 1e-13 relative to the max-norm of the result."""
+    osys, prob = _pair(ctx, HEAT_CFGS[name])
+    u = osys.field()
+    osys.initialize(u, 0.0)
+    want = osys.rhs(u, 0.25).flat()
+    gu, gr = prob.field(), prob.field()
+    gu.upload(u.flat())
+    prob.rhs(gu, gr, 0.25, fused=fused)
+    got = gr.download()
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(got - want)) <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("name", ["wave48_E4_1", "heat2d_21", "heat3d_E4_2"])
+@pytest.mark.parametrize("fused", [False, True])
+def test_rk4_steps(ctx, name, fused):
+    """Several RK4 steps from the exact initial condition against the
+    oracle's step; tolerance 1e-11 relative max-norm (full solves)."""
+    cfg = ALL[name]
+    osys, prob = _pair(ctx, cfg)
+    u0 = osys.field()
+    osys.initialize(u0, 0.0)
+    osys.update_boundary(u0, 0.0)
+    dt = osys.dt(0.5, 0.8)
+    assert abs(dt - prob.dt(0.5, 0.8)) == 0.0
+    nsteps = 3
+    gu = prob.field()
+    gu.upload(u0.flat())
+    out, rk, srhs = osys.field(), osys.field(), osys.field()
+    t = 0.0
+    for _ in range(nsteps):
+        osys.rk4_step(u0, out, rk, srhs, t, dt)
+        u0.D[:] = out.D
+        for d in range(3):
+            u0.R[d][:] = out.R[d]
+        t += dt
+    if fused:
+        prob.rk4_fused(gu, 0.0, dt, nsteps)
+    else:
+        g_out, g_rk, g_srhs = prob.field(), prob.field(), prob.field()
+        tt = 0.0
+        for _ in range(nsteps):
+            prob.rk4(gu, g_out, g_rk, g_srhs, tt, dt)
+            gu.upload(g_out.download())
+            tt += dt
+    got = gu.download()
+    want = u0.flat()
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    assert err <= 1e-11, f"{name}: relative max-norm error {err:.3e}"
+
+
+def test_stats_match_oracle(ctx):
+    cfg = HEAT_CFGS["heat3d_E4_2"]
+    osys, prob = _pair(ctx, cfg)
+    u = osys.field()
+    osys.initialize(u, 0.0)
+    u.D[:] += 1e-3 * np.random.default_rng(6).standard_normal(u.D.size)
+    gu = prob.field()
+    gu.upload(u.flat())
+    want = osys.stats(u, 0.1)
+    got = prob.stats(gu, 0.1)
+    assert got[4] == want[4]  # location of the max error
+    np.testing.assert_allclose(got[:4], want[:4], rtol=1e-12, atol=1e-15)
