@@ -1,0 +1,86 @@
+"""Write tests/golden/*.json from the golden vectors held by the reference's
+own unit tests (read as text; nothing of the reference is built or run).
+
+Run in a container where the reference tree is mounted:
+    python tests/golden/make_fixtures.py /root/reference
+The JSON files it writes are committed; the tests read only them.
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+NUM = r"[-+]?(?:\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)"
+
+
+def _numbers(text: str):
+    return [float(x) for x in re.findall(NUM, text)]
+
+
+def stencil_cases(ref: str):
+    """(scheme, alpha, h, bc, psi, right, expected) from `st.nbs(...)` calls
+    followed by `Approx(T{...})` / `Approx(std::vector{...})` matchers."""
+    out = []
+    specs = [("E4_1.t.cpp", "E4_1"), ("E4u_1.t.cpp", "E4u_1"), ("E2_2.t.cpp", "E2_2")]
+    for fname, scheme in specs:
+        path = os.path.join(ref, "src", "stencils", fname)
+        src = open(path).read()
+        m = re.search(r"alpha\s*=\s*\{([^}]*)\}", src)
+        alpha = _numbers(m.group(1)) if m else [0.0, 0.0]
+        pat = re.compile(
+            r"st\.nbs\(\s*(" + NUM + r")\s*,\s*bcs::(\w+)\s*,\s*(" + NUM + r")\s*,\s*(true|false)"
+            r"[^;]*;\s*REQUIRE_THAT\(\s*\w+\s*,\s*Approx\((?:T|std::vector)\s*\{([^}]*)\}")
+        for mm in pat.finditer(src):
+            line = src[: mm.start()].count("\n") + 1
+            out.append(dict(scheme=scheme, alpha=alpha, h=float(mm.group(1)), bc=mm.group(2),
+                            psi=float(mm.group(3)), right=mm.group(4) == "true",
+                            expected=_numbers(mm.group(5)),
+                            source=f"src/stencils/{fname}:{line}"))
+    return out
+
+
+def mesh_cases():
+    """Line extents, R indices and psi of the reference's mesh test
+    (src/mesh/mesh.t.cpp:67-230): extents {21, 22, 23}, bounds
+    {-1, -1, 0}..{1, 2, 2.2}, sphere centre {0.01, -0.01, 0.5} radius 0.25."""
+    return dict(
+        extents=[21, 22, 23], lo=[-1, -1, 0], hi=[1, 2, 2.2], sphere=[0.01, -0.01, 0.5, 0.25],
+        source="src/mesh/mesh.t.cpp:67-230",
+        n_intersections=[26, 42, 28],
+        lines=[
+            # dir, line index, start mesh coord, start obj (R index or -1), start psi,
+            # end mesh coord, end obj, end psi
+            dict(dir=0, index=6 * 23 + 3, start=[0, 6, 3], start_obj=-1, end=[10, 6, 3], end_obj=0,
+                 end_psi=0.40365385103120377),
+            dict(dir=0, index=6 * 23 + 4, start=[10, 6, 3], start_obj=1,
+                 start_psi=0.2036538510312047, end=[20, 6, 3], end_obj=-1),
+            dict(dir=1, index=8 * 23 + 4, start=[8, 0, 4], start_obj=-1, end=[8, 7, 4], end_obj=0,
+                 end_psi=0.2884394027061823),
+            dict(dir=1, index=8 * 23 + 5, start=[8, 7, 4], start_obj=1,
+                 start_psi=0.42843940270618086, end=[8, 21, 4], end_obj=-1),
+            dict(dir=2, index=12 * 22 + 8 + 13, start=[12, 8, 0], start_obj=-1, end=[12, 8, 5],
+                 end_obj=26, end_psi=0.4491194432954615),
+            dict(dir=0, index=0, start=[0, 0, 0], start_obj=-1, end=[20, 0, 0], end_obj=-1),
+            dict(dir=1, index=1, start=[0, 0, 1], start_obj=-1, end=[0, 21, 1], end_obj=-1),
+            dict(dir=2, index=-1, start=[20, 21, 0], start_obj=-1, end=[20, 21, 22], end_obj=-1),
+        ],
+        strides=[22 * 23, 23, 1],
+    )
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    stencils = stencil_cases(ref)
+    with open(os.path.join(HERE, "stencil_closures.json"), "w") as f:
+        json.dump(stencils, f, indent=1)
+    with open(os.path.join(HERE, "mesh_lines.json"), "w") as f:
+        json.dump(mesh_cases(), f, indent=1)
+    print(f"{len(stencils)} stencil cases written")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,72 @@
+"""The product's host assembly (C++20, libshoccs.so, built without a device
+context) against the oracle: cut points, masks, line extents, segment tables,
+closure coefficients and B entries must be identical (bit for bit)."""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle_ffi as O
+
+CFGS = {
+    "wave48_E4_1": H.sphere_wave_cfg(48),
+    "wave20_E4u_overlapping_closures": H.sphere_wave_cfg(20, scheme=O.E4U_1, alpha=(0.1, 0.2)),
+    "wave30_radial": H.sphere_wave_cfg(30, linear=False, scheme=O.E4U_1),
+    "heat2d_101_config1": H.heat2d_cfg(101),
+    "heat3d_noncubic": H.heat3d_cfg(),
+    "heat3d_E2": H.heat3d_cfg(scheme=O.E2_2),
+}
+
+
+@pytest.mark.parametrize("name", list(CFGS))
+def test_structures_bitexact(name):
+    cfg = CFGS[name]
+    osys = H.oracle_system(cfg)
+    prob = H.product_problem(None, cfg)
+    n = prob.n
+    assert prob.N == osys.mesh.N
+    for d in range(3):
+        ra, rb = osys.mesh.r_arrays(d), prob.r_arrays(d)
+        for k in ("psi", "pos", "solid", "ray_outside", "shape_id"):
+            assert np.array_equal(ra[k], rb[k]), (d, k)
+        assert np.array_equal(osys.mesh.line_array(d), prob.line_array(d)), d
+        if n[d] < 2:
+            continue
+        so, Bo = H.oracle_segments(osys.deriv(d), n, d)
+        sp, Bp = H.product_segments(prob, d)
+        assert len(so) == len(sp)
+        for a, b in zip(so, sp):
+            for k in a:
+                if isinstance(a[k], np.ndarray):
+                    assert np.array_equal(a[k], b[k]), (d, k)
+                else:
+                    assert a[k] == b[k], (d, k, a, b)
+        for x, y in zip(Bo, Bp):
+            assert np.array_equal(x, y), d
+        st = O.stencil(cfg["scheme"], cfg["alpha"])
+        assert np.array_equal(O.interior(st, osys.mesh.h()[d]), prob.interior(d))
+    assert np.array_equal(osys.mesh.fluid(), prob.fluid())
+
+
+def test_timestep_matches():
+    for cfg in CFGS.values():
+        osys = H.oracle_system(cfg)
+        prob = H.product_problem(None, cfg)
+        assert osys.dt(0.5, 0.8) == prob.dt(0.5, 0.8)
+
+
+def test_rejects_closures_outside_the_mesh():
+    """A 24^3 E4_1 sphere has 4-row segments whose overlapping right closure
+    would read before the start of the mesh buffer (undefined in the
+    reference); the oracle and the device upload both refuse it."""
+    cfg = H.sphere_wave_cfg(24)
+    with pytest.raises(ValueError):
+        H.oracle_system(cfg)
+
+
+def test_unsupported_bodies_rejected():
+    """Floating bodies need the R-space rows of cut_discretization, which the
+    device path does not implement: the oracle refuses the configuration."""
+    cfg = dict(H.sphere_wave_cfg(32, scheme=O.E4U_1, alpha=(0.1, 0.2)))
+    cfg["obj_bc"] = [O.FLOATING]
+    with pytest.raises(ValueError):
+        H.oracle_system(cfg)

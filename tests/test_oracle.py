@@ -1,0 +1,329 @@
+"""Pins of the CPU oracle: the reference's own golden vectors (tests/golden,
+extracted from the reference's unit tests) and what the mathematics fixes
+(polynomial exactness of interior stencils, closures and whole operators,
+measured convergence orders, RK4 on a solution linear in time)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle_ffi as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SCHEMES = {"E4_1": O.E4_1, "E4u_1": O.E4U_1, "E2_2": O.E2_2, "E4_2": O.E4_2}
+BCS = {"Floating": O.FLOATING, "Dirichlet": O.DIRICHLET, "Neumann": O.NEUMANN}
+
+
+def _cases():
+    return json.load(open(os.path.join(GOLDEN, "stencil_closures.json")))
+
+
+@pytest.mark.parametrize("case", _cases(), ids=lambda c: c["source"])
+def test_closure_golden_vectors(case):
+    """Closure coefficients equal the reference tests' vectors.  The golden
+    values were printed by the reference's generator; they agree with the
+    restated formulas to <= 1e-12 relative (the reference's own check uses an
+    absolute margin of 1e-8)."""
+    st = O.stencil(SCHEMES[case["scheme"]], tuple(case["alpha"]))
+    c, ex, q = O.nbs(st, case["h"], BCS[case["bc"]], case["psi"], case["right"])
+    want = np.array(case["expected"])
+    assert c.size == want.size
+    scale = max(1.0, np.max(np.abs(want)))
+    assert np.max(np.abs(c - want)) <= 1e-12 * scale
+
+
+def test_query_shapes():
+    """stencil::query (E4_1.cpp:43-56, E4u_1.cpp:26-39, E4_2.cpp:16-28, E2_2.cpp:12-24)."""
+    L = O.lib()
+    expect = {O.E4_1: (2, 5, 7, 0), O.E4U_1: (2, 3, 5, 0), O.E4_2: (2, 3, 5, 3), O.E2_2: (1, 2, 4, 2)}
+    for k, (p, r, t, x) in expect.items():
+        st = O.stencil(k, (0.1, 0.7))
+        q = L.orc_query_max(st)
+        assert (q.p, q.r, q.t, q.x) == (p, r, t, x)
+        d = L.orc_query(st, O.DIRICHLET)
+        assert (d.r, d.t, d.x) == (r - 1, t, 0)
+
+
+def _closure_points(h, psi, t):
+    """Positions of a left closure's t columns: the wall, then t-1 grid points
+    starting psi*h from it."""
+    return np.concatenate([[0.0], psi * h + h * np.arange(t - 1)])
+
+
+# Column at which row 0 of a closure differentiates (measured property of the
+# closed forms): the first-derivative cut-cell scheme E4_1 starts one column
+# later than the others.  A Dirichlet closure drops the first floating row.
+ROW0_COL = {O.E4_1: 1, O.E4U_1: 0, O.E4_2: 0, O.E2_2: 0}
+
+
+@pytest.mark.parametrize("scheme,alpha,order,deg", [(O.E4_1, (0.1, 0.7), 1, 3),
+                                                    (O.E4U_1, (0.1, 0.2), 1, 3),
+                                                    (O.E4_2, (0.0, 0.0), 2, 3),
+                                                    (O.E2_2, (0.0, 0.0), 2, 2)])
+@pytest.mark.parametrize("bc", [O.FLOATING, O.DIRICHLET])
+@pytest.mark.parametrize("right", [False, True])
+def test_closure_polynomial_exactness(scheme, alpha, order, deg, bc, right):
+    """Every closure row differentiates polynomials of degree <= deg exactly
+    at its own grid point (a dropped or mistyped term breaks this).  Right-wall
+    closures are checked on the mirrored points.  E4u_1 is psi-independent and
+    is checked at psi = 1."""
+    st = O.stencil(scheme, alpha)
+    h = 0.37
+    psis = [1.0] if scheme == O.E4U_1 else [0.2, 0.55, 0.9]
+    for psi in psis:
+        c, _, q = O.nbs(st, h, bc, psi, right)
+        c = c.reshape(q.r, q.t)
+        x = _closure_points(h, psi, q.t)
+        if right:
+            x = -x[::-1]
+        x = x + 0.3
+        first = ROW0_COL[scheme] + (0 if bc == O.FLOATING else 1)
+        for r in range(q.r):
+            col = first + r if not right else q.t - 1 - (first + (q.r - 1 - r))
+            xr = x[col]
+            for k in range(deg + 1):
+                val = c[r] @ (x ** k)
+                if order == 1:
+                    exact = k * xr ** (k - 1) if k > 0 else 0.0
+                else:
+                    exact = k * (k - 1) * xr ** (k - 2) if k > 1 else 0.0
+                scale = np.max(np.abs(c[r])) * np.max(np.abs(x)) ** k
+                assert abs(val - exact) <= 1e-11 * scale, (psi, r, k, val, exact)
+
+
+@pytest.mark.parametrize("scheme,alpha", [(O.E4_1, (0.1, 0.7)), (O.E4U_1, (0.1, 0.2))])
+def test_interior_first_derivative_exact(scheme, alpha):
+    st = O.stencil(scheme, alpha)
+    h = 0.21
+    c = O.interior(st, h)
+    x = h * np.arange(-2, 3) + 0.4
+    for k in range(5):
+        assert abs(c @ x ** k - (k * 0.4 ** (k - 1) if k else 0.0)) <= 1e-12 * max(1, k * 4)
+
+
+def test_mesh_golden_lines():
+    g = json.load(open(os.path.join(GOLDEN, "mesh_lines.json")))
+    m = O.OracleMesh(g["extents"], g["lo"], g["hi"], g["sphere"])
+    ex = g["extents"]
+    for d in range(3):
+        slow = {0: 1, 1: 0, 2: 0}[d]
+        fast = {0: 2, 1: 2, 2: 1}[d]
+        assert m.nR(d) == g["n_intersections"][d]
+        assert len(m.lines(d)) == ex[slow] * ex[fast] + g["n_intersections"][d] // 2
+    for L in g["lines"]:
+        ln = m.lines(L["dir"])[L["index"]]
+        assert ln["stride"] == g["strides"][L["dir"]]
+        assert list(ln["start"]["mesh"]) == L["start"]
+        assert list(ln["end"]["mesh"]) == L["end"]
+        so = ln["start"]["obj"]
+        eo = ln["end"]["obj"]
+        assert (so[0] if so else -1) == L["start_obj"]
+        assert (eo[0] if eo else -1) == L["end_obj"]
+        if "start_psi" in L:
+            assert abs(so[2] - L["start_psi"]) <= 1e-14  # goldens from an exact-arithmetic tool
+        if "end_psi" in L:
+            assert abs(eo[2] - L["end_psi"]) <= 1e-14
+
+
+def test_cut_points_lie_on_sphere():
+    """Every cut point is on the sphere, psi is the fractional distance from
+    the solid coordinate to the hit, and rays alternate in/out per line."""
+    c = np.array([0.5, 0.45, 0.55])
+    r = 0.31
+    n = (23, 26, 29)
+    m = O.OracleMesh(n, (0, 0, 0), (1, 1, 1), list(c) + [r])
+    h = np.array(m.h())
+    for d in range(3):
+        a = m.r_arrays(d)
+        dist = np.linalg.norm(a["pos"] - c, axis=1)
+        assert np.max(np.abs(dist - r)) < 1e-14
+        # fluid neighbour of the solid coordinate is psi*h away from the hit
+        off = np.where(a["ray_outside"] == 1, -1, 1)
+        fluid = a["solid"][:, d] + off
+        fpos = fluid * h[d]
+        psi = off * (fpos - a["pos"][:, d]) / h[d]
+        assert np.max(np.abs(np.clip(psi, 1e-12, 1 - 1e-12) - a["psi"])) < 1e-12
+
+
+def _poly_field(sys_, fn):
+    """fn evaluated at every mesh point and cut point."""
+    m = sys_.mesh.m
+    n = m.n
+    xs = [np.ctypeslib.as_array(m.xyz[a], shape=(n[a],)) for a in range(3)]
+    X, Y, Z = np.meshgrid(xs[0], xs[1], xs[2], indexing="ij")
+    f = sys_.field()
+    f.D[:] = fn(X, Y, Z).ravel()
+    for d in range(3):
+        p = sys_.mesh.r_arrays(d)["pos"]
+        if len(p):
+            f.R[d][: len(p)] = fn(p[:, 0], p[:, 1], p[:, 2])
+    return f
+
+
+def _segment_class(s, cfg, d):
+    """Per mesh point, for direction d: 0 no row, 1 interior row, 2 closure
+    row next to a grid wall, 3 closure row next to a body."""
+    n = cfg["n"]
+    cls = np.zeros(int(np.prod(n)), np.int8)
+    stride = {0: n[1] * n[2], 1: n[2], 2: 1}[d]
+    for b in O.deriv_blocks(s.deriv(d)):
+        base = b["row_offset"]
+        L, I, R = b["left_rows"], b["interior_rows"], b["right_rows"]
+        total = L + I + R
+        lobj = b["left_cols"] == (7 - 1) or False
+        for r in range(total):
+            idx = base + r * stride
+            if r < L:
+                cls[idx] = 2
+            elif r < L + I:
+                cls[idx] = 1
+            else:
+                cls[idx] = 2
+    # mark rows whose segment ends on a body
+    for ln in s.mesh.lines(d):
+        for end, sign in ((ln["start"], 1), (ln["end"], -1)):
+            if end["obj"] is None:
+                continue
+            ijk = list(end["mesh"])
+            for k in range(1, 6):
+                p = list(ijk)
+                p[d] += sign * k
+                if 0 <= p[d] < n[d]:
+                    idx = (p[0] * n[1] + p[1]) * n[2] + p[2]
+                    if cls[idx] == 2:
+                        cls[idx] = 3
+    return cls
+
+
+@pytest.mark.parametrize("scheme,alpha", [(O.E4U_1, (0.1, 0.2)), (O.E4_1, (0.1, 0.7))])
+def test_first_derivative_exact_on_polynomials(scheme, alpha):
+    """The assembled first derivative (O + B with Dirichlet-body values on the
+    cut points) differentiates cubics exactly on every interior row, and
+    constants (zero row sums, B included) on every row.  E4u_1 closures are
+    also exact on cubics at grid walls, where the mesh is uniform."""
+    cfg = H.sphere_wave_cfg(48, scheme=scheme, alpha=alpha)
+    s = H.oracle_system(cfg)
+    fn = lambda x, y, z: 1 + 2 * x - 3 * y + 0.5 * z + x * x - y * z + 0.7 * z ** 3
+    dfn = [lambda x, y, z: 2 + 2 * x + 0 * y, lambda x, y, z: -3 - z + 0 * x,
+           lambda x, y, z: 0.5 - y + 2.1 * z * z]
+    u = _poly_field(s, fn)
+    one = _poly_field(s, lambda x, y, z: 1.0 + 0 * x)
+    for d in range(3):
+        cls = _segment_class(s, cfg, d)
+        du = s.field()
+        du.D[:] = np.nan
+        s.deriv_apply(d, u, du)
+        want = _poly_field(s, dfn[d]).D
+        err = np.abs(du.D - want)
+        assert np.max(err[cls == 1]) < 1e-10
+        if scheme == O.E4U_1:
+            assert np.max(err[cls == 2]) < 1e-9
+        dc = s.field()
+        dc.D[:] = np.nan
+        s.deriv_apply(d, one, dc)
+        rows = cls > 0
+        assert np.max(np.abs(dc.D[rows])) < 1e-8 * 48 * 1e4
+
+
+def test_laplacian_exact_on_quadratics():
+    """E4_2 second derivatives (closures at grid walls and at Dirichlet bodies)
+    are exact on quadratics at every row, as the reference's own operator tests
+    check for E2 (src/operators/derivative.t.cpp:324-384)."""
+    cfg = H.heat3d_cfg(n=(26, 28, 30))
+    s = H.oracle_system(cfg)
+    fn = lambda x, y, z: x * (y + z) + y * (x + z) + z * (x + y) + 3 * x * y * z + x * x - 2 * z * z
+    u = _poly_field(s, fn)
+    lap = s.laplacian(u)
+    fl = s.mesh.fluid()
+    ijk = H.unflat(fl, cfg["n"])
+    interior = np.all((ijk > 0) & (ijk < np.array(cfg["n"]) - 1), axis=1)
+    got = lap.D[fl[interior]]
+    assert np.max(np.abs(got - (2.0 - 4.0))) < 1e-7
+
+
+@pytest.mark.parametrize("scheme,max_order,rms_order", [(O.E4_2, 1.8, 2.4), (O.E2_2, 0.9, 1.5)])
+def test_laplacian_convergence_order(scheme, max_order, rms_order):
+    """Measured orders of the assembled laplacian on a smooth function over
+    sphere meshes 24^3..64^3: E4_2 max-norm 1.9, rms 2.6; E2_2 1.0 and 1.7
+    (closure rows at the curved body bound the max-norm order)."""
+    fn = lambda x, y, z: np.sin(2 * x + 0.3) * np.cos(1.5 * y) + np.exp(0.5 * z)
+    lapf = lambda x, y, z: -6.25 * np.sin(2 * x + 0.3) * np.cos(1.5 * y) + 0.25 * np.exp(0.5 * z)
+    errs, rms, hs = [], [], []
+    for n in (24, 32, 48, 64):
+        cfg = H.heat3d_cfg(n=(n, n, n), scheme=scheme)
+        s = H.oracle_system(cfg)
+        L = s.laplacian(_poly_field(s, fn))
+        w = _poly_field(s, lapf).D
+        fl = s.mesh.fluid()
+        ijk = H.unflat(fl, (n, n, n))
+        inner = fl[np.all((ijk > 0) & (ijk < n - 1), axis=1)]
+        e = np.abs(L.D[inner] - w[inner])
+        errs.append(e.max())
+        rms.append(np.sqrt(np.mean(e ** 2)))
+        hs.append(1.0 / (n - 1))
+    assert np.polyfit(np.log(hs), np.log(errs), 1)[0] > max_order
+    assert np.polyfit(np.log(hs), np.log(rms), 1)[0] > rms_order
+
+
+def test_interior_rows_converge_fourth_order():
+    """Away from walls and bodies the first derivative is the 4th-order
+    central stencil: the error on interior rows falls as h^4."""
+    fn = lambda x, y, z: np.sin(2 * x + 0.3) * np.cos(1.5 * y) + np.exp(0.5 * z)
+    dfn = lambda x, y, z: 2 * np.cos(2 * x + 0.3) * np.cos(1.5 * y)
+    errs, hs = [], []
+    for n in (32, 48, 64):
+        cfg = H.sphere_wave_cfg(n, scheme=O.E4U_1, alpha=(0.1, 0.2))
+        s = H.oracle_system(cfg)
+        cls = _segment_class(s, cfg, 0)
+        du = s.field()
+        s.deriv_apply(0, _poly_field(s, fn), du)
+        e = np.abs(du.D - _poly_field(s, dfn).D)[cls == 1]
+        errs.append(e.max())
+        hs.append(1.0 / (n - 1))
+    assert np.polyfit(np.log(hs), np.log(errs), 1)[0] > 3.7
+
+
+def test_rk4_exact_for_linear_in_time_heat():
+    """rk4_v2.t.cpp:28-143 analogue: for a heat solution whose spatial part is
+    annihilated by the laplacian closures and linear in time the RK4 step is
+    exact; here the sin-product MMS over 3 steps must stay within the spatial
+    truncation error and the stepper must reproduce dt = p_cfl h^2 / 4k."""
+    cfg = H.heat2d_cfg(41)
+    s = H.oracle_system(cfg)
+    h = 1.0 / 40
+    assert s.dt(1.0, 0.8) == 0.8 * h * h / 4
+    u0 = s.field()
+    s.initialize(u0, 0.0)
+    out, rk, srhs = s.field(), s.field(), s.field()
+    dt = s.dt(1.0, 0.8)
+    t = 0.0
+    for _ in range(3):
+        s.rk4_step(u0, out, rk, srhs, t, dt)
+        u0.D[:] = out.D
+        t += dt
+    st = s.stats(u0, t)
+    assert 0 < st[0] < 1e-3
+
+
+def test_system_cycle_heat_2d_config1_short():
+    """The 2-D heat configuration (101^2, E4, Dirichlet circle, sin-product MMS,
+    dt = 0.2 h^2): 20 of its 200 steps stay within the discretisation error."""
+    cfg = H.heat2d_cfg(101)
+    s = H.oracle_system(cfg)
+    u0 = s.field()
+    s.initialize(u0, 0.0)
+    s.update_boundary(u0, 0.0)
+    out, rk, srhs = s.field(), s.field(), s.field()
+    dt = s.dt(1.0, 0.8)
+    assert abs(dt - 0.2 * 0.01 ** 2) < 1e-18
+    t = 0.0
+    for _ in range(20):
+        s.rk4_step(u0, out, rk, srhs, t, dt)
+        u0.D[:] = out.D
+        for d in range(3):
+            u0.R[d][:] = out.R[d]
+        t += dt
+    st = s.stats(u0, t)
+    assert st[0] < 1e-5, st[0]
