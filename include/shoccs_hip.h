@@ -207,6 +207,17 @@ int shx_rk4_step(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk,
  * registers are owned by the system.  Asynchronous. */
 int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt, int nsteps);
 
+/* Fused-stage kernel choice (no reference counterpart; the reference has one
+ * Kokkos TeamPolicy shape, src/matrices/block.hpp:185-194): variant 0 is the
+ * point-wise kernel on 64 x 4 tiles, variant 1 marches tiles of march_ty x 64
+ * points along x over chunks of march_xchunk planes (scalar wave with 5-point
+ * interior stencils; other systems use variant 0). */
+int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xchunk);
+int shx_system_get_tuning(const shx_system* s, int out[3]);
+/* Autotuner: times every candidate on a copy of u0 (u0 is not modified),
+ * keeps the fastest and returns its ms per step. */
+int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt, double* best_ms);
+
 /* ------------------------------------------------------------- measurement */
 /* Run `nsteps` fused steps `reps` times and report the mean device time per
  * step in milliseconds, measured with HIP events on the context's stream. */

@@ -279,6 +279,25 @@ class Problem:
               "rk4_fused")
         self.ctx.sync()
 
+    def set_tuning(self, variant: int, march_ty: int = 4, march_xchunk: int = 32):
+        """Fused-stage kernel: 0 point-wise tiles, 1 x-marching tiles."""
+        self._need_device()
+        check(lib().shx_system_set_tuning(self.sys, variant, march_ty, march_xchunk), self.ctx.p,
+              "set_tuning")
+
+    def tuning(self) -> tuple:
+        out = (C.c_int * 3)()
+        check(lib().shx_system_get_tuning(self.sys, out), self.ctx.p, "get_tuning")
+        return tuple(out)
+
+    def autotune(self, u0: Field, t: float, dt: float) -> tuple:
+        """Time every fused-stage candidate on a copy of u0 and keep the
+        fastest; returns (variant, march_ty, march_xchunk, ms_per_step)."""
+        self._need_device()
+        ms = C.c_double()
+        check(lib().shx_system_autotune(self.sys, u0.ref(), t, dt, C.byref(ms)), self.ctx.p, "autotune")
+        return self.tuning() + (ms.value,)
+
     def time_fused(self, u0: Field, t: float, dt: float, nsteps: int, reps: int = 1) -> float:
         ms = C.c_double()
         check(lib().shx_time_fused_steps(self.sys, u0.ref(), t, dt, nsteps, reps, C.byref(ms)),

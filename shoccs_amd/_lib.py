@@ -96,6 +96,9 @@ _SIGS = [
      [C.c_double, C.c_double, C.c_int, dp]),
     ("shx_time_fused_stage_kernel", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double,
                                               C.c_double, C.c_int, dp]),
+    ("shx_system_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    ("shx_system_get_tuning", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("shx_system_autotune", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double, dp]),
     ("shoccs_problem_create", C.c_int, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
     ("shoccs_problem_destroy", C.c_int, [C.c_void_p]),
     ("shoccs_problem_system", C.c_void_p, [C.c_void_p]),

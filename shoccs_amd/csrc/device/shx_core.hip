@@ -23,7 +23,7 @@ __global__ void k_fill(double* __restrict__ dst, double v, long n)
     if (i < n) dst[i] = v;
 }
 
-template <bool PLUS, bool WITH_B>
+template <int DIR, int W, bool PLUS, bool WITH_B>
 __global__ __launch_bounds__(kBlock) void k_derivative(Tiling t, DevDeriv d,
                                                        const double* __restrict__ u,
                                                        const double* __restrict__ uR,
@@ -32,7 +32,7 @@ __global__ __launch_bounds__(kBlock) void k_derivative(Tiling t, DevDeriv d,
     int i, j, k;
     long idx;
     if (!tile_point(t, i, j, k, idx)) return;
-    const RowVal r = row_at<WITH_B>(d, u, uR, idx, i, j, k, t.n1, t.n2);
+    const RowVal r = row_at<DIR, W, WITH_B>(d, u, uR, idx, i, j, k, t.n1, t.n2);
     if (!r.row) return;
     du[idx] = PLUS ? row_plus(du[idx], r) : row_eq(r);
 }
@@ -95,14 +95,22 @@ int launch_derivative(shx_derivative* d, const double* u, const double* uR, doub
     shx_ctx* ctx = d->ctx;
     if (!d->dev.present) return SHX_OK;
     const Tiling t = make_tiling(d->n);
-    if (plus && with_b)
-        k_derivative<true, true><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->dev, u, uR, du);
-    else if (plus)
-        k_derivative<true, false><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->dev, u, uR, du);
-    else if (with_b)
-        k_derivative<false, true><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->dev, u, uR, du);
-    else
-        k_derivative<false, false><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->dev, u, uR, du);
+    const int key = d->dir * 100 + d->dev.width * 10 + (plus ? 2 : 0) + (with_b ? 1 : 0);
+    switch (key) {
+#define SHX_DERIV_CASE(D, W, P, B) \
+    case D * 100 + W * 10 + (P ? 2 : 0) + (B ? 1 : 0): \
+        k_derivative<D, W, P, B><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->dev, u, uR, du); \
+        break;
+#define SHX_DERIV_DIR(D, W) \
+    SHX_DERIV_CASE(D, W, false, false) SHX_DERIV_CASE(D, W, false, true) \
+    SHX_DERIV_CASE(D, W, true, false) SHX_DERIV_CASE(D, W, true, true)
+        SHX_DERIV_DIR(0, 3) SHX_DERIV_DIR(1, 3) SHX_DERIV_DIR(2, 3)
+        SHX_DERIV_DIR(0, 5) SHX_DERIV_DIR(1, 5) SHX_DERIV_DIR(2, 5)
+#undef SHX_DERIV_DIR
+#undef SHX_DERIV_CASE
+    default:
+        return fail(ctx, SHX_EUNSUPPORTED, "derivative: interior width must be 3 or 5");
+    }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "derivative kernel");
 }
@@ -244,8 +252,8 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     for (int a = 0; a < 3; ++a)
         if (desc->n[a] < 1 || desc->n[a] > 65535)
             return fail(ctx, SHX_EINVAL, "derivative: extents must be in [1, 65535]");
-    if (desc->width < 1 || desc->width > 9 || (desc->width % 2) == 0)
-        return fail(ctx, SHX_EINVAL, "derivative: interior width must be odd and <= 9");
+    if (desc->width != 3 && desc->width != 5)
+        return fail(ctx, SHX_EUNSUPPORTED, "derivative: interior width must be 3 or 5");
     const long N = (long)desc->n[0] * desc->n[1] * desc->n[2];
     if (N > 0x7fffffffL) return fail(ctx, SHX_EINVAL, "derivative: mesh too large");
 
@@ -261,7 +269,6 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     d->dev.dir = dir;
     d->dev.nq = desc->n[dir];
     d->dev.width = desc->width;
-    for (int jj = 0; jj < desc->width; ++jj) d->dev.c[jj] = desc->interior[jj];
     d->dev.stride = dir == 0 ? (long)desc->n[1] * desc->n[2] : dir == 1 ? desc->n[2] : 1;
     const int nslow = dir == 0 ? desc->n[1] : desc->n[0];
     const int nfast = dir == 2 ? desc->n[1] : desc->n[2];
@@ -340,6 +347,7 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     if (rc == SHX_OK && desc->nblocks)
         rc = upload(ctx, &d->blocks, desc->blocks, (size_t)desc->nblocks);
     if (rc == SHX_OK) rc = upload(ctx, &d->pool, desc->pool, (size_t)desc->npool);
+    if (rc == SHX_OK) rc = upload(ctx, &d->coef, desc->interior, 9);
 
     // Neumann entries -> row-compressed CSR sorted by (row, col, w)
     if (rc == SHX_OK && desc->nneu > 0) {
@@ -378,6 +386,7 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     d->dev.fast = d->fast;
     d->dev.blocks = d->blocks;
     d->dev.pool = d->pool;
+    d->dev.coef = d->coef;
     *out = d;
     return SHX_OK;
 }
@@ -389,6 +398,7 @@ int shx_derivative_destroy(shx_derivative* d)
     hipFree(d->fast);
     hipFree(d->blocks);
     hipFree(d->pool);
+    hipFree(d->coef);
     hipFree(d->neu_rows);
     hipFree(d->neu_ptr);
     hipFree(d->neu_col);

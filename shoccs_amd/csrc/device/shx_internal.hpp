@@ -32,14 +32,16 @@ constexpr int kTileZ = 64;  // lanes along the contiguous axis (one wavefront)
 constexpr int kTileY = 4;   // wavefronts per workgroup, stacked along y
 constexpr int kBlock = kTileZ * kTileY;
 
-// Device view of one direction's derivative operator (kernel argument).
+// Device view of one direction's derivative operator (kernel argument).  Only
+// scalars and pointers: a kernel argument that is indexed dynamically or whose
+// address escapes into a call is copied to scratch memory for every thread.
 struct DevDeriv {
     int dir;
     int nq;          // extent along dir
     int nfast;       // extent of the fast axis of the pair
     int width;       // interior stencil width
     long stride;     // D stride along dir
-    double c[9];     // interior stencil
+    const double* coef;  // interior stencil (device, `width` values)
     const int* pair_ptr;
     const unsigned long long* fast;
     const shx_line_block* blocks;
@@ -82,9 +84,14 @@ __device__ __forceinline__ void pair_of(int dir, int i, int j, int k, int n1, in
     }
 }
 
-// Slow path: scan the segments of the line for the one holding row q.
+// Slow path: scan the segments of the line for the one holding row q.  All
+// arguments are scalars or pointers so that nothing is spilled to scratch.
 template <bool WITH_B>
-__device__ __noinline__ RowVal row_slow(const DevDeriv& d, const double* __restrict__ u,
+__device__ __noinline__ RowVal row_slow(const int* __restrict__ pair_ptr,
+                                        const shx_line_block* __restrict__ blocks,
+                                        const double* __restrict__ pool,
+                                        const double* __restrict__ coef, int width, long stride,
+                                        const double* __restrict__ u,
                                         const double* __restrict__ uR, long idx, int pair, int q)
 {
     RowVal r;
@@ -92,38 +99,38 @@ __device__ __noinline__ RowVal row_slow(const DevDeriv& d, const double* __restr
     r.b0 = r.b1 = 0.0;
     r.nb = 0;
     r.row = false;
-    const int b_begin = d.pair_ptr[pair];
-    const int b_end = d.pair_ptr[pair + 1];
-    const long base = idx - (long)q * d.stride;
+    const int b_begin = pair_ptr[pair];
+    const int b_end = pair_ptr[pair + 1];
+    const long base = idx - (long)q * stride;
     for (int b = b_begin; b < b_end; ++b) {
-        const shx_line_block blk = d.blocks[b];
+        const shx_line_block blk = blocks[b];
         const int lr = q - blk.lo;
         if (lr < 0) break;
         const int total = blk.nl + blk.ni + blk.nr;
         if (lr >= total) continue;
         double dot = 0.0;
         if (lr < blk.nl) {
-            const double* a = d.pool + blk.lcoef + (long)lr * blk.lcols;
+            const double* a = pool + blk.lcoef + (long)lr * blk.lcols;
             for (int jj = 0; jj < blk.lcols; ++jj)
-                dot += a[jj] * u[base + (long)(blk.lcol0 + jj) * d.stride];
+                dot += a[jj] * u[base + (long)(blk.lcol0 + jj) * stride];
         } else if (lr < blk.nl + blk.ni) {
-            const int hw = d.width / 2;
-            for (int jj = 0; jj < d.width; ++jj) dot += d.c[jj] * u[idx + (long)(jj - hw) * d.stride];
+            const int hw = width / 2;
+            for (int jj = 0; jj < width; ++jj) dot += coef[jj] * u[idx + (long)(jj - hw) * stride];
         } else {
             const int rr = lr - blk.nl - blk.ni;
-            const double* a = d.pool + blk.rcoef + (long)rr * blk.rcols;
+            const double* a = pool + blk.rcoef + (long)rr * blk.rcols;
             for (int jj = 0; jj < blk.rcols; ++jj)
-                dot += a[jj] * u[base + (long)(blk.rcol0 + jj) * d.stride];
+                dot += a[jj] * u[base + (long)(blk.rcol0 + jj) * stride];
         }
         r.dot = dot;
         r.row = true;
         if (WITH_B) {
             if (blk.lobj >= 0 && lr < blk.nl) {
-                r.b0 = d.pool[blk.lbw + lr] * uR[blk.lobj];
+                r.b0 = pool[blk.lbw + lr] * uR[blk.lobj];
                 r.nb = 1;
             }
             if (blk.robj >= 0 && lr >= total - blk.nr) {
-                const double v = d.pool[blk.rbw + (lr - (total - blk.nr))] * uR[blk.robj];
+                const double v = pool[blk.rbw + (lr - (total - blk.nr))] * uR[blk.robj];
                 if (r.nb == 0) r.b0 = v;
                 else r.b1 = v;
                 r.nb++;
@@ -134,31 +141,40 @@ __device__ __noinline__ RowVal row_slow(const DevDeriv& d, const double* __restr
     return r;
 }
 
-// Row of direction d at point idx (coordinates i, j, k).
-template <bool WITH_B>
+// Row of direction DIR (stencil width W) at point idx = (i, j, k).  The W taps
+// are loaded before the interval test, at positions clamped to the line, so
+// that their loads are issued together; they are used only when q lies in one
+// of the line's interior intervals (where no clamping occurs).
+template <int DIR, int W, bool WITH_B>
 __device__ __forceinline__ RowVal row_at(const DevDeriv& d, const double* __restrict__ u,
                                          const double* __restrict__ uR, long idx, int i, int j,
                                          int k, int n1, int n2)
 {
-    int pair, q;
-    pair_of(d.dir, i, j, k, n1, n2, pair, q);
+    constexpr int hw = W / 2;
+    const int pair = DIR == 0 ? j * n2 + k : DIR == 1 ? i * n2 + k : i * n1 + j;
+    const int q = DIR == 0 ? i : DIR == 1 ? j : k;
     const unsigned long long fd = d.fast[pair];
+    double v[W];
+#pragma unroll
+    for (int jj = 0; jj < W; ++jj) {
+        int qq = q + jj - hw;
+        qq = qq < 0 ? 0 : (qq > d.nq - 1 ? d.nq - 1 : qq);
+        v[jj] = u[idx + (long)(qq - q) * d.stride];
+    }
     const int lo0 = (int)(fd & 0xffffu), hi0 = (int)((fd >> 16) & 0xffffu);
     const int lo1 = (int)((fd >> 32) & 0xffffu), hi1 = (int)(fd >> 48);
     if ((q >= lo0 && q < hi0) || (q >= lo1 && q < hi1)) {
         RowVal r;
-        const int hw = d.width / 2;
         double dot = 0.0;
 #pragma unroll
-        for (int jj = 0; jj < 9; ++jj)
-            if (jj < d.width) dot += d.c[jj] * u[idx + (long)(jj - hw) * d.stride];
+        for (int jj = 0; jj < W; ++jj) dot += d.coef[jj] * v[jj];
         r.dot = dot;
         r.b0 = r.b1 = 0.0;
         r.nb = 0;
         r.row = true;
         return r;
     }
-    return row_slow<WITH_B>(d, u, uR, idx, pair, q);
+    return row_slow<WITH_B>(d.pair_ptr, d.blocks, d.pool, d.coef, W, d.stride, u, uR, idx, pair, q);
 }
 
 // eq_t result of a derivative row: dot, then the B terms.

@@ -25,6 +25,7 @@ struct shx_derivative {
     unsigned long long* fast = nullptr;
     shx_line_block* blocks = nullptr;
     double* pool = nullptr;
+    double* coef = nullptr;
     long nneu_rows = 0;
     long* neu_rows = nullptr;
     int* neu_ptr = nullptr;

@@ -12,99 +12,56 @@
 
 #include "shx_internal.hpp"
 #include "shx_state.hpp"
+#include "shx_sys.hpp"
 
 using namespace shx;
 
 namespace shx {
 
-// Device view of a system (kernel argument).
-struct DevSys {
-    int kind;
-    int n0, n1, n2;
-    const double* x;
-    const double* y;
-    const double* z;
-    int bc[6];
-    double center[3], radius;
-    double vel[3], x0[3], width, amp;
-    double k;
-    int mms_kind, mms_dims, mms_n;
-    const double* mms;
-    const unsigned* fluid_mask;
-};
-
-constexpr double kTwoPi = 6.283185307179586;  // 2 * pi, as 2 * std::numbers::pi
-
-__device__ __forceinline__ double dot3(const double* a, const double* b)
-{
-    return a[0] * b[0] + (a[1] * b[1] + a[2] * b[2]);
-}
-
-// scalar_wave.cpp:35-40 (radial) and the translated gaussian of the
-// constant-velocity wave.
-__device__ __forceinline__ double wave_exact(const DevSys& s, const double* loc, double t)
-{
-    if (s.kind == SHX_SYS_WAVE_RADIAL) {
-        const double r[3] = {loc[0] - s.center[0], loc[1] - s.center[1], loc[2] - s.center[2]};
-        return sin(kTwoPi * (sqrt(dot3(r, r)) - s.radius - t));
-    }
-    double q[3];
-    for (int c = 0; c < 3; ++c) q[c] = loc[c] - s.x0[c] - s.vel[c] * t;
-    return s.amp * exp(-dot3(q, q) / (2 * s.width * s.width));
-}
-
-// gG component (scalar_wave.cpp:28-33): -(x - c)/|x - c| or -a.
-__device__ __forceinline__ double wave_neg_g(const DevSys& s, int comp, const double* loc)
-{
-    if (s.kind == SHX_SYS_WAVE_RADIAL) {
-        const double r[3] = {loc[0] - s.center[0], loc[1] - s.center[1], loc[2] - s.center[2]};
-        return -(loc[comp] - s.center[comp]) / sqrt(dot3(r, r));
-    }
-    return -s.vel[comp];
-}
-
-__device__ __forceinline__ double g3_expo(const double* g, const double* loc)
+// gauss3d exponent (src/mms/gauss3d.cpp:14-20)
+__device__ __forceinline__ double g3_expo(const double* g, double x, double y, double z)
 {
     const double *c = g, *v = g + 3;
-    return (-0.5 * pow((loc[0] - c[0]), 2) * pow(v[0], -2) +
-            (-0.5 * pow((loc[1] - c[1]), 2) * pow(v[1], -2) +
-             -0.5 * pow((loc[2] - c[2]), 2) * pow(v[2], -2)));
+    return (-0.5 * pow((x - c[0]), 2) * pow(v[0], -2) +
+            (-0.5 * pow((y - c[1]), 2) * pow(v[1], -2) + -0.5 * pow((z - c[2]), 2) * pow(v[2], -2)));
 }
 
-__device__ double mms_call(const DevSys& s, double t, const double* loc)
+// Manufactured solutions; scalar arguments only (called out of line).
+__device__ __noinline__ double mms_call(int kind, int dims, int n, const double* __restrict__ p,
+                                        double t, double x, double y, double z)
 {
-    if (s.mms_kind == SHX_MMS_SINPROD) {
-        double v = sin(kTwoPi * loc[0]) * sin(kTwoPi * loc[1]);
-        if (s.mms_dims == 3) v = v * sin(kTwoPi * loc[2]);
+    if (kind == SHX_MMS_SINPROD) {
+        double v = sin(kTwoPi * x) * sin(kTwoPi * y);
+        if (dims == 3) v = v * sin(kTwoPi * z);
         return v * exp(-t);
     }
     double sol = 0;
-    for (int i = 0; i < s.mms_n; ++i) {
-        const double* g = s.mms + 8 * i;
-        sol += exp(g3_expo(g, loc)) * g[6] * cos(t * g[7]);
+    for (int i = 0; i < n; ++i) {
+        const double* g = p + 8 * i;
+        sol += exp(g3_expo(g, x, y, z)) * g[6] * cos(t * g[7]);
     }
     return sol;
 }
 
-__device__ double mms_ddt(const DevSys& s, double t, const double* loc)
+__device__ __noinline__ double mms_ddt(int kind, int dims, int n, const double* __restrict__ p,
+                                       double t, double x, double y, double z)
 {
-    if (s.mms_kind == SHX_MMS_SINPROD) return -mms_call(s, t, loc);
+    if (kind == SHX_MMS_SINPROD) return -mms_call(kind, dims, n, p, t, x, y, z);
     double sol = 0;
-    for (int i = 0; i < s.mms_n; ++i) {
-        const double* g = s.mms + 8 * i;
-        sol += -exp(g3_expo(g, loc)) * g[6] * g[7] * sin(t * g[7]);
+    for (int i = 0; i < n; ++i) {
+        const double* g = p + 8 * i;
+        sol += -exp(g3_expo(g, x, y, z)) * g[6] * g[7] * sin(t * g[7]);
     }
     return sol;
 }
 
-__device__ double mms_lap(const DevSys& s, double t, const double* loc)
+__device__ __noinline__ double mms_lap(int kind, int dims, int n, const double* __restrict__ p,
+                                       double t, double x, double y, double z)
 {
-    if (s.mms_kind == SHX_MMS_SINPROD)
-        return -(s.mms_dims * kTwoPi * kTwoPi) * mms_call(s, t, loc);
+    if (kind == SHX_MMS_SINPROD) return -(dims * kTwoPi * kTwoPi) * mms_call(kind, dims, n, p, t, x, y, z);
     double sol = 0;
-    const double x = loc[0], y = loc[1], z = loc[2];
-    for (int i = 0; i < s.mms_n; ++i) {
-        const double* c = s.mms + 8 * i;
+    for (int i = 0; i < n; ++i) {
+        const double* c = p + 8 * i;
         const double* v = c + 3;
         sol += exp(0.5 * ((-1 * pow((z - c[2]), 2) * pow(v[2], -2) -
                            pow((y - c[1]), 2) * pow(v[1], -2)) -
@@ -126,34 +83,30 @@ __device__ double mms_lap(const DevSys& s, double t, const double* loc)
     return sol;
 }
 
-__device__ __forceinline__ double exact_at(const DevSys& s, const double* loc, double t)
+// heat source S_t - k lap S (heat.cpp:111-117)
+__device__ __forceinline__ double heat_source(const DevSys& s, double t, double x, double y, double z)
 {
-    return s.kind == SHX_SYS_HEAT ? mms_call(s, t, loc) : wave_exact(s, loc, t);
+    return mms_ddt(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z) -
+           s.k * mms_lap(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z);
 }
 
-// point on a Dirichlet grid face (for_each_grid_bc_desc, selection_desc.hpp:211-222)
-__device__ __forceinline__ bool on_dirichlet_face(const DevSys& s, int i, int j, int k)
+__device__ __forceinline__ double exact_at(const DevSys& s, double x, double y, double z, double t)
 {
-    return (s.bc[0] == SHX_BC_DIRICHLET && i == 0) || (s.bc[1] == SHX_BC_DIRICHLET && i == s.n0 - 1) ||
-           (s.bc[2] == SHX_BC_DIRICHLET && j == 0) || (s.bc[3] == SHX_BC_DIRICHLET && j == s.n1 - 1) ||
-           (s.bc[4] == SHX_BC_DIRICHLET && k == 0) || (s.bc[5] == SHX_BC_DIRICHLET && k == s.n2 - 1);
+    return s.kind == SHX_SYS_HEAT ? mms_call(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z)
+                                  : wave_exact(s, x, y, z, t);
 }
 
-__device__ __forceinline__ bool is_fluid(const DevSys& s, long idx)
-{
-    return (s.fluid_mask[idx >> 5] >> (idx & 31)) & 1u;
-}
-
-struct DevOps {
-    DevDeriv d[3];
-};
-
-struct RPtrs {
-    const double* r[3];
-};
-
-// Right-hand side at one point of D.
+// Exact solution with the system family fixed at compile time, so that the
+// scalar-wave kernels carry no call into the manufactured solutions.
 template <int KIND>
+__device__ __forceinline__ double exact_of(const DevSys& s, double x, double y, double z, double t)
+{
+    if (KIND == SHX_SYS_HEAT) return mms_call(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z);
+    return wave_exact(s, x, y, z, t);
+}
+
+// Right-hand side at one point of D (interior stencil width W).
+template <int KIND, int W>
 __device__ __forceinline__ double rhs_point(const DevSys& s, const DevOps& ops,
                                             const double* __restrict__ u, const RPtrs& uR,
                                             long idx, int i, int j, int k, double t)
@@ -161,32 +114,43 @@ __device__ __forceinline__ double rhs_point(const DevSys& s, const DevOps& ops,
     if (KIND == SHX_SYS_HEAT) {
         // laplacian: du = 0; du += O_x u; du += B_x u_Rx; then y, z (laplacian.cpp:37-46)
         double acc = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (!ops.d[c].present) continue;
-            const RowVal r = row_at<true>(ops.d[c], u, uR.r[c], idx, i, j, k, s.n1, s.n2);
+        if (ops.d[0].present) {
+            const RowVal r = row_at<0, W, true>(ops.d[0], u, uR.r[0], idx, i, j, k, s.n1, s.n2);
+            if (r.row) acc = row_plus(acc, r);
+        }
+        if (ops.d[1].present) {
+            const RowVal r = row_at<1, W, true>(ops.d[1], u, uR.r[1], idx, i, j, k, s.n1, s.n2);
+            if (r.row) acc = row_plus(acc, r);
+        }
+        if (ops.d[2].present) {
+            const RowVal r = row_at<2, W, true>(ops.d[2], u, uR.r[2], idx, i, j, k, s.n1, s.n2);
             if (r.row) acc = row_plus(acc, r);
         }
         acc = acc * s.k;
         if (on_dirichlet_face(s, i, j, k)) return 0.0;
         if (is_fluid(s, idx)) {
-            const double loc[3] = {s.x[i], s.y[j], s.z[k]};
-            acc += mms_ddt(s, t, loc) - s.k * mms_lap(s, t, loc);
+            acc += heat_source(s, t, s.x[i], s.y[j], s.z[k]);
         }
         return acc;
     } else {
         double du[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (!ops.d[c].present) continue;
-            const RowVal r = row_at<true>(ops.d[c], u, uR.r[c], idx, i, j, k, s.n1, s.n2);
-            if (r.row) du[c] = row_eq(r);
+        if (ops.d[0].present) {
+            const RowVal r = row_at<0, W, true>(ops.d[0], u, uR.r[0], idx, i, j, k, s.n1, s.n2);
+            if (r.row) du[0] = row_eq(r);
+        }
+        if (ops.d[1].present) {
+            const RowVal r = row_at<1, W, true>(ops.d[1], u, uR.r[1], idx, i, j, k, s.n1, s.n2);
+            if (r.row) du[1] = row_eq(r);
+        }
+        if (ops.d[2].present) {
+            const RowVal r = row_at<2, W, true>(ops.d[2], u, uR.r[2], idx, i, j, k, s.n1, s.n2);
+            if (r.row) du[2] = row_eq(r);
         }
         double g[3] = {0.0, 0.0, 0.0};
         if (!on_dirichlet_face(s, i, j, k)) {
-            const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+            const double x = s.x[i], y = s.y[j], z = s.z[k];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) g[c] = wave_neg_g(s, c, loc);
+            for (int c = 0; c < 3; ++c) g[c] = wave_neg_g(s, c, x, y, z);
         }
         return g[0] * du[0] + g[1] * du[1] + g[2] * du[2];
     }
@@ -197,7 +161,7 @@ __device__ __forceinline__ double rhs_point(const DevSys& s, const DevOps& ops,
 // --------------------------------------------------------------------- kernels
 namespace {
 
-template <int KIND>
+template <int KIND, int W>
 __global__ __launch_bounds__(kBlock) void k_rhs(Tiling t, DevSys s, DevOps ops,
                                                 const double* __restrict__ u, RPtrs uR,
                                                 double* __restrict__ rhs, double time)
@@ -205,14 +169,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs(Tiling t, DevSys s, DevOps ops,
     int i, j, k;
     long idx;
     if (!tile_point(t, i, j, k, idx)) return;
-    rhs[idx] = rhs_point<KIND>(s, ops, u, uR, idx, i, j, k, time);
+    rhs[idx] = rhs_point<KIND, W>(s, ops, u, uR, idx, i, j, k, time);
 }
 
 // One fused RK4 stage over three registers (rk4.cpp:27-46 for one point):
 //   rhs = RHS(us) at t_s;  acc' = acc + c_acc * rhs  (acc = 0 on stage 0)
 //   stages 0-2: out = u0 + c_next * rhs, Dirichlet faces take exact(t_next)
 //   stage 3:    out = u0 + 1.0 * acc'  (out may alias u0), faces exact(t_next)
-template <int KIND, int STAGE>
+template <int KIND, int STAGE, int W>
 __global__ __launch_bounds__(kBlock) void k_rk4_stage(Tiling t, DevSys s, DevOps ops,
                                                       const double* __restrict__ us, RPtrs usR,
                                                       const double* u0, double* __restrict__ acc,
@@ -222,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_rk4_stage(Tiling t, DevSys s, DevOps
     int i, j, k;
     long idx;
     if (!tile_point(t, i, j, k, idx)) return;
-    const double rhs = rhs_point<KIND>(s, ops, us, usR, idx, i, j, k, t_s);
+    const double rhs = rhs_point<KIND, W>(s, ops, us, usR, idx, i, j, k, t_s);
     const double a = (STAGE == 0 ? 0.0 : acc[idx]) + c_acc * rhs;
     const double base = u0[idx];
     double v;
@@ -233,8 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_rk4_stage(Tiling t, DevSys s, DevOps
         v = base + 1.0 * a;
     }
     if (on_dirichlet_face(s, i, j, k)) {
-        const double loc[3] = {s.x[i], s.y[j], s.z[k]};
-        v = exact_at(s, loc, t_next);
+        v = exact_of<KIND>(s, s.x[i], s.y[j], s.z[k], t_next);
     }
     out[idx] = v;
 }
@@ -246,7 +209,7 @@ __global__ void k_r_exact(DevSys s, long n, const double* __restrict__ pos,
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !dirich[i]) return;
-    uR[i] = exact_at(s, pos + 3 * i, t);
+    uR[i] = exact_at(s, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], t);
 }
 
 // Dirichlet grid faces of D take the exact solution.
@@ -258,8 +221,7 @@ __global__ void k_face_exact(DevSys s, long n, const int* __restrict__ faces,
     const long idx = faces[q];
     int i, j, k;
     point_coords(idx, s.n1, s.n2, i, j, k);
-    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
-    u[idx] = exact_at(s, loc, t);
+    u[idx] = exact_at(s, s.x[i], s.y[j], s.z[k], t);
 }
 
 __global__ void k_exact_D(DevSys s, long N, double* __restrict__ out, double t)
@@ -268,8 +230,7 @@ __global__ void k_exact_D(DevSys s, long N, double* __restrict__ out, double t)
     if (idx >= N) return;
     int i, j, k;
     point_coords(idx, s.n1, s.n2, i, j, k);
-    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
-    out[idx] = exact_at(s, loc, t);
+    out[idx] = exact_at(s, s.x[i], s.y[j], s.z[k], t);
 }
 
 __global__ void k_exact_R(DevSys s, long n, const double* __restrict__ pos, double* __restrict__ out,
@@ -277,7 +238,7 @@ __global__ void k_exact_R(DevSys s, long n, const double* __restrict__ pos, doub
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    out[i] = exact_at(s, pos + 3 * i, t);
+    out[i] = exact_at(s, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], t);
 }
 
 // initialize_scalar_field on the fluid gather
@@ -289,8 +250,7 @@ __global__ void k_init_fluid(DevSys s, long n, const int* __restrict__ fluid, do
     const long idx = fluid[q];
     int i, j, k;
     point_coords(idx, s.n1, s.n2, i, j, k);
-    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
-    u[idx] = exact_at(s, loc, t);
+    u[idx] = exact_at(s, s.x[i], s.y[j], s.z[k], t);
 }
 
 // gG fields of the reference (scalar_wave.cpp:70-95), stored for the unfused form
@@ -301,11 +261,11 @@ __global__ void k_gG_D(DevSys s, long N, double* __restrict__ gx, double* __rest
     if (idx >= N) return;
     int i, j, k;
     point_coords(idx, s.n1, s.n2, i, j, k);
-    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
+    const double x = s.x[i], y = s.y[j], z = s.z[k];
     const bool f = on_dirichlet_face(s, i, j, k);
-    gx[idx] = f ? 0.0 : wave_neg_g(s, 0, loc);
-    gy[idx] = f ? 0.0 : wave_neg_g(s, 1, loc);
-    gz[idx] = f ? 0.0 : wave_neg_g(s, 2, loc);
+    gx[idx] = f ? 0.0 : wave_neg_g(s, 0, x, y, z);
+    gy[idx] = f ? 0.0 : wave_neg_g(s, 1, x, y, z);
+    gz[idx] = f ? 0.0 : wave_neg_g(s, 2, x, y, z);
 }
 
 // scalar_wave.cpp:294-317, one buffer
@@ -332,8 +292,7 @@ __global__ void k_src_D(DevSys s, long N, double* __restrict__ src, double t)
     if (idx >= N) return;
     int i, j, k;
     point_coords(idx, s.n1, s.n2, i, j, k);
-    const double loc[3] = {s.x[i], s.y[j], s.z[k]};
-    src[idx] = mms_ddt(s, t, loc) - s.k * mms_lap(s, t, loc);
+    src[idx] = heat_source(s, t, s.x[i], s.y[j], s.z[k]);
 }
 
 __global__ void k_gather_add(long n, const int* __restrict__ sel, const double* __restrict__ src,
@@ -388,10 +347,10 @@ __global__ __launch_bounds__(256) void k_stats(DevSys s, long n, const int* __re
         const long idx = fluid[q];
         int i, j, k;
         point_coords(idx, s.n1, s.n2, i, j, k);
-        const double loc[3] = {s.x[i], s.y[j], s.z[k]};
         const double v = u[idx];
-        const double err = fabs(v - exact_at(s, loc, t));
-        if (err > e || (err == e && idx < ei)) {
+        const double err = fabs(v - exact_at(s, s.x[i], s.y[j], s.z[k], t));
+        // a non-finite error wins and stays, so that system::valid() sees it
+        if ((isnan(err) && !isnan(e)) || err > e || (err == e && idx < ei)) {
             e = err;
             ei = idx;
         }
@@ -406,7 +365,8 @@ __global__ __launch_bounds__(256) void k_stats(DevSys s, long n, const int* __re
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             const int o = threadIdx.x + w;
-            if (se[o] > se[threadIdx.x] || (se[o] == se[threadIdx.x] && si[o] < si[threadIdx.x])) {
+            const double a = se[o], b = se[threadIdx.x];
+            if ((isnan(a) && !isnan(b)) || a > b || (a == b && si[o] < si[threadIdx.x])) {
                 se[threadIdx.x] = se[o];
                 si[threadIdx.x] = si[o];
             }
@@ -461,6 +421,11 @@ struct shx_system {
     // statistics partials
     double* part = nullptr;
     long* part_idx = nullptr;
+    // fused-stage kernel choice: 0 point-wise tiles, 1 x-marching tiles
+    int variant = 0;
+    int march_ty = 4;
+    int march_xchunk = 32;
+    int width = 5;  // interior stencil width shared by the three derivatives
 };
 
 namespace {
@@ -587,12 +552,16 @@ int ensure_unfused(shx_system* s)
 int rhs_fused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double t)
 {
     const Tiling& tl = s->tiling;
-    if (s->kind == SHX_SYS_HEAT)
-        k_rhs<SHX_SYS_HEAT><<<tl.nblocks, kBlock, 0, s->ctx->stream>>>(tl, s->dev, s->ops, u->D,
-                                                                        rptrs(u), rhs->D, t);
+    hipStream_t st = s->ctx->stream;
+    const bool heat = s->kind == SHX_SYS_HEAT;
+    if (heat && s->width == 3)
+        k_rhs<SHX_SYS_HEAT, 3><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
+    else if (heat)
+        k_rhs<SHX_SYS_HEAT, 5><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
+    else if (s->width == 3)
+        k_rhs<SHX_SYS_WAVE_LINEAR, 3><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
     else
-        k_rhs<SHX_SYS_WAVE_LINEAR><<<tl.nblocks, kBlock, 0, s->ctx->stream>>>(
-            tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
+        k_rhs<SHX_SYS_WAVE_LINEAR, 5><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
     int rc = check(s->ctx, "rhs kernel");
     // every body is Dirichlet: the cut-point rows of the rhs are zero
     for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = launch_fill(s->ctx, rhs->R[c], 0.0, s->nR[c]);
@@ -712,20 +681,54 @@ int launch_stage(shx_system* s, int stage, const double* usD, RPtrs usR, const d
 {
     const Tiling& tl = s->tiling;
     hipStream_t st = s->ctx->stream;
-    switch (stage) {
-    case 0:
-        k_rk4_stage<KIND, 0><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, usD, usR, u0, acc, out, ts, c_acc, c_next, tn);
+    const int key = stage * 10 + s->width;
+    switch (key) {
+#define SHX_STAGE_CASE(S, W)                                                                  \
+    case S * 10 + W:                                                                          \
+        k_rk4_stage<KIND, S, W><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, usD, usR, \
+                                                             u0, acc, out, ts, c_acc, c_next, tn); \
         break;
-    case 1:
-        k_rk4_stage<KIND, 1><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, usD, usR, u0, acc, out, ts, c_acc, c_next, tn);
-        break;
-    case 2:
-        k_rk4_stage<KIND, 2><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, usD, usR, u0, acc, out, ts, c_acc, c_next, tn);
-        break;
+        SHX_STAGE_CASE(0, 3) SHX_STAGE_CASE(1, 3) SHX_STAGE_CASE(2, 3) SHX_STAGE_CASE(3, 3)
+        SHX_STAGE_CASE(0, 5) SHX_STAGE_CASE(1, 5) SHX_STAGE_CASE(2, 5) SHX_STAGE_CASE(3, 5)
+#undef SHX_STAGE_CASE
     default:
-        k_rk4_stage<KIND, 3><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, usD, usR, u0, acc, out, ts, c_acc, c_next, tn);
+        return fail(s->ctx, SHX_EUNSUPPORTED, "rk4 stage: interior width must be 3 or 5");
     }
     return check(s->ctx, "rk4 stage kernel");
+}
+
+// The x-marching stage kernel handles the scalar wave with 5-point interior
+// stencils (the 4th-order schemes); anything else uses the point-wise tiles.
+bool march_ok(const shx_system* s)
+{
+    if (s->kind == SHX_SYS_HEAT) return false;
+    for (int c = 0; c < 3; ++c)
+        if (s->ops.d[c].present && s->ops.d[c].width != 5) return false;
+    return true;
+}
+
+int launch_march(shx_system* s, int stage, const double* usD, RPtrs usR, const double* u0,
+                 double* acc, double* out, double c_acc, double c_next, double tn)
+{
+    MarchArgs a;
+    a.s = s->dev;
+    a.ops = s->ops;
+    a.us = usD;
+    for (int c = 0; c < 3; ++c) a.usR[c] = usR.r[c];
+    a.u0 = u0;
+    a.acc = acc;
+    a.out = out;
+    a.c_acc = c_acc;
+    a.c_next = c_next;
+    a.t_next = tn;
+    a.xchunk = s->march_xchunk;
+    a.nbz = (s->n[2] + 63) / 64;
+    a.nby = (s->n[1] + s->march_ty - 1) / s->march_ty;
+    const int nbx = (s->n[0] + a.xchunk - 1) / a.xchunk;
+    a.nblocks = (unsigned)a.nbz * (unsigned)a.nby * (unsigned)nbx;
+    if (launch_wave_stage_march(s->ctx->stream, stage, s->march_ty, a) != 0)
+        return fail(s->ctx, SHX_EINVAL, "march: unsupported tile height");
+    return check(s->ctx, "rk4 march stage kernel");
 }
 
 // One fused step; when ev is given, event pairs bracket each stage kernel.
@@ -746,6 +749,8 @@ int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t
         if (ev) hipEventRecord(ev[2 * stage], s->ctx->stream);
         if (s->kind == SHX_SYS_HEAT)
             rc = launch_stage<SHX_SYS_HEAT>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
+        else if (s->variant == 1 && march_ok(s))
+            rc = launch_march(s, stage, usD, usR, u0->D, s->accD, outD, c_acc, c_next, tn);
         else
             rc = launch_stage<SHX_SYS_WAVE_LINEAR>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
         if (ev) hipEventRecord(ev[2 * stage + 1], s->ctx->stream);
@@ -860,10 +865,19 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
     dv.z = s->xyz[2];
     dv.mms = s->mms;
     dv.fluid_mask = s->fluid_mask;
+    s->width = 0;
     for (int c = 0; c < 3; ++c) {
         if (s->d[c]) s->ops.d[c] = s->d[c]->dev;
         else std::memset(&s->ops.d[c], 0, sizeof(DevDeriv));
+        if (s->ops.d[c].present) {
+            if (s->width && s->width != s->ops.d[c].width) {
+                shx_system_destroy(s);
+                return fail(ctx, SHX_EINVAL, "system: derivatives with different stencil widths");
+            }
+            s->width = s->ops.d[c].width;
+        }
     }
+    if (s->width == 0) s->width = 5;
     *out = s;
     return SHX_OK;
 }
@@ -966,7 +980,7 @@ int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[1
         double err = -DBL_MAX, mn = DBL_MAX, mx = -DBL_MAX;
         long ei = 0;
         for (unsigned b = 0; b < grid; ++b) {
-            if (h[b] > err || (h[b] == err && hi[b] < ei)) {
+            if ((std::isnan(h[b]) && !std::isnan(err)) || h[b] > err || (h[b] == err && hi[b] < ei)) {
                 err = h[b];
                 ei = hi[b];
             }
@@ -1065,6 +1079,87 @@ int shx_time_fused_stage_kernel(shx_system* s, shx_field* u0, double t, double d
     }
     *ms_per_kernel = total / (4.0 * nsteps);
     for (auto& e : ev) hipEventDestroy(e);
+    return rc;
+}
+
+int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xchunk)
+{
+    if (!s || variant < 0 || variant > 1) return SHX_EINVAL;
+    if (variant == 1 && !(march_ty == 2 || march_ty == 4 || march_ty == 8)) return SHX_EINVAL;
+    if (variant == 1 && march_xchunk < 1) return SHX_EINVAL;
+    s->variant = variant;
+    if (variant == 1) {
+        s->march_ty = march_ty;
+        s->march_xchunk = march_xchunk;
+    }
+    return SHX_OK;
+}
+
+int shx_system_get_tuning(const shx_system* s, int out[3])
+{
+    if (!s || !out) return SHX_EINVAL;
+    out[0] = s->variant;
+    out[1] = s->march_ty;
+    out[2] = s->march_xchunk;
+    return SHX_OK;
+}
+
+int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt, double* best_ms)
+{
+    if (!s || !u0) return SHX_EINVAL;
+    int rc = ensure_fused(s);
+    if (rc != SHX_OK) return rc;
+    // work on a copy so that the caller's state is untouched
+    shx_field tmp;
+    std::memset(&tmp, 0, sizeof tmp);
+    rc = alloc_d(s->ctx, &tmp.D, s->N);
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = alloc_d(s->ctx, &tmp.R[c], s->nR[c]);
+    auto reset = [&]() {
+        hipMemcpyAsync(tmp.D, u0->D, sizeof(double) * s->N, hipMemcpyDeviceToDevice, s->ctx->stream);
+        for (int c = 0; c < 3; ++c)
+            if (s->nR[c])
+                hipMemcpyAsync(tmp.R[c], u0->R[c], sizeof(double) * s->nR[c], hipMemcpyDeviceToDevice,
+                               s->ctx->stream);
+    };
+    struct cand {
+        int v, ty, xc;
+    };
+    std::vector<cand> cands{{0, 4, 32}};
+    if (march_ok(s))
+        for (int ty : {2, 4, 8})
+            for (int xc : {16, 32, 64, 128, 1 << 16}) cands.push_back({1, ty, std::min(xc, s->n[0])});
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    double best = 1e30;
+    cand pick = cands[0];
+    for (const cand& c : cands) {
+        if (rc != SHX_OK) break;
+        s->variant = c.v;
+        s->march_ty = c.ty;
+        s->march_xchunk = c.xc;
+        reset();
+        rc = rk4_fused_step(s, &tmp, t, dt, nullptr);  // warm-up
+        const int reps = 3;
+        hipEventRecord(e0, s->ctx->stream);
+        for (int r = 0; r < reps && rc == SHX_OK; ++r) rc = rk4_fused_step(s, &tmp, t + r * dt, dt, nullptr);
+        hipEventRecord(e1, s->ctx->stream);
+        if (rc == SHX_OK) rc = sync(s->ctx);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (rc == SHX_OK && ms / reps < best) {
+            best = ms / reps;
+            pick = c;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    s->variant = pick.v;
+    s->march_ty = pick.ty;
+    s->march_xchunk = pick.xc;
+    hipFree(tmp.D);
+    for (int c = 0; c < 3; ++c) hipFree(tmp.R[c]);
+    if (best_ms) *best_ms = best;
     return rc;
 }
 

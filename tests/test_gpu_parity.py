@@ -154,6 +154,52 @@ def test_rk4_steps(ctx, name, fused):
     assert err <= 1e-11, f"{name}: relative max-norm error {err:.3e}"
 
 
+@pytest.mark.parametrize("name", ["wave48_E4_1", "wave30_radial_E4u"])
+@pytest.mark.parametrize("ty,xchunk", [(2, 16), (4, 7), (8, 64), (4, 1000)])
+def test_march_kernel_bitexact(ctx, name, ty, xchunk):
+    """The x-marching fused stage kernel gives the same doubles as the
+    point-wise one (same operation order), over several RK4 steps."""
+    cfg = WAVE_CFGS[name]
+    osys, prob = _pair(ctx, cfg)
+    u0 = osys.field()
+    osys.initialize(u0, 0.0)
+    dt = osys.dt(0.5, 1.0)
+    a, b = prob.field(), prob.field()
+    a.upload(u0.flat())
+    b.upload(u0.flat())
+    prob.set_tuning(0)
+    prob.rk4_fused(a, 0.0, dt, 3)
+    prob.set_tuning(1, ty, xchunk)
+    prob.rk4_fused(b, 0.0, dt, 3)
+    assert np.array_equal(a.download(), b.download())
+
+
+def test_autotune_keeps_state(ctx):
+    cfg = WAVE_CFGS["wave48_E4_1"]
+    osys, prob = _pair(ctx, cfg)
+    u0 = osys.field()
+    osys.initialize(u0, 0.0)
+    g = prob.field()
+    g.upload(u0.flat())
+    v, ty, xc, ms = prob.autotune(g, 0.0, osys.dt(0.5, 1.0))
+    assert v in (0, 1) and ms > 0
+    assert np.array_equal(g.download(), u0.flat())
+
+
+def test_stats_flag_non_finite(ctx):
+    """A NaN injected into the solution must make the L-inf error non-finite,
+    so that system::valid() (heat.cpp:54-58, scalar_wave.cpp:108-112) stops."""
+    cfg = HEAT_CFGS["heat3d_E4_2"]
+    osys, prob = _pair(ctx, cfg)
+    u = osys.field()
+    osys.initialize(u, 0.0)
+    flat = u.flat()
+    flat[osys.mesh.fluid()[123]] = np.nan
+    gu = prob.field()
+    gu.upload(flat)
+    assert not np.isfinite(prob.stats(gu, 0.0)[0])
+
+
 def test_stats_match_oracle(ctx):
     cfg = HEAT_CFGS["heat3d_E4_2"]
     osys, prob = _pair(ctx, cfg)
