@@ -104,6 +104,15 @@ int shx_h2d(shx_ctx* ctx, void* dst, const void* src, size_t bytes);
 int shx_d2h(shx_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* dst[i] = value for i < n (a kernel, safe inside stream capture) */
 int shx_fill(shx_ctx* ctx, double* dst, double value, int64_t n);
+/* dst[i] = src[i] for i < n on the context's stream: one buffer of
+ * field_registry::deep_copy_slot (src/fields/field_registry.hpp:172-187),
+ * e.g. the u1 <- u0 copy of src/simulation/simulation_cycle.cpp:61. */
+int shx_copy(shx_ctx* ctx, double* dst, const double* src, int64_t n);
+/* Mean device time of shx_copy over `reps` launches, each bracketed by a HIP
+ * event pair on the context's stream: the achievable-copy calibration the
+ * roofline figures are read against. */
+int shx_time_copy(shx_ctx* ctx, double* dst, const double* src, int64_t n, int reps,
+                  double* ms_per_copy);
 
 /* -------------------------------------------------------------- operators */
 /* ccs::derivative::derivative (src/operators/derivative.cpp:452-473): upload
@@ -209,11 +218,18 @@ int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt,
 
 /* Fused-stage kernel choice (no reference counterpart; the reference has one
  * Kokkos TeamPolicy shape, src/matrices/block.hpp:185-194): variant 0 is the
- * point-wise kernel on 64 x 4 tiles, variant 1 marches tiles of march_ty x 64
- * points along x over chunks of march_xchunk planes (scalar wave with 5-point
- * interior stencils; other systems use variant 0). */
+ * point-wise kernel on 64 x 4 tiles; variant 1 marches tiles of march_ty x 64
+ * points (march_ty = 4, 8 or 16) along x over chunks of march_xchunk planes
+ * for interior and solid points, and evaluates the remaining points
+ * (closure rows, B terms, Dirichlet faces) from a list built once, in a
+ * second kernel that also updates the Dirichlet cut points.  Both variants
+ * give the same doubles. */
 int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xchunk);
 int shx_system_get_tuning(const shx_system* s, int out[3]);
+/* Points of D updated per stage by the main fused-stage kernel (out[0]) and
+ * by the special-point kernel (out[1], 0 for variant 0) of the current
+ * variant; builds the point classes of variant 1 if needed. */
+int shx_system_stage_points(shx_system* s, int64_t out[2]);
 /* Autotuner: times every candidate on a copy of u0 (u0 is not modified),
  * keeps the fastest and returns its ms per step. */
 int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt, double* best_ms);
@@ -226,8 +242,9 @@ int shx_time_fused_steps(shx_system* s, shx_field* u0, double t, double dt, int 
 int shx_time_unfused_steps(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk,
                            shx_field* srhs, double t, double dt, int nsteps,
                            double* ms_per_step);
-/* Mean device time of one fused stage kernel over `nsteps` fused steps, each
- * launch bracketed by a HIP event pair on the context's stream. */
+/* Mean device time of the main fused stage kernel (see
+ * shx_system_stage_points) over `nsteps` fused steps, each launch bracketed
+ * by a HIP event pair on the context's stream. */
 int shx_time_fused_stage_kernel(shx_system* s, shx_field* u0, double t, double dt, int nsteps,
                                 double* ms_per_kernel);
 

@@ -20,6 +20,7 @@ No compute call has a CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -47,12 +48,34 @@ class Context:
         if rc != 0:
             raise ShoccsError(f"shx_init({device}) failed with code {rc}: no usable GPU")
         self.p = p
+        self._problems = weakref.WeakSet()
 
     def sync(self):
         check(lib().shx_sync(self.p), self.p, "shx_sync")
 
+    def time_copy(self, n: int, reps: int = 20) -> float:
+        """Mean device milliseconds of one n-double device-to-device copy
+        (shx_time_copy) between two scratch buffers allocated here."""
+        L = lib()
+        a, b = C.c_void_p(), C.c_void_p()
+        check(L.shx_alloc(self.p, 8 * n, C.byref(a)), self.p, "alloc")
+        try:
+            check(L.shx_alloc(self.p, 8 * n, C.byref(b)), self.p, "alloc")
+            try:
+                check(L.shx_fill(self.p, a, 1.0, n), self.p, "fill")
+                ms = C.c_double()
+                check(L.shx_time_copy(self.p, b, a, n, reps, C.byref(ms)), self.p, "time_copy")
+                return ms.value
+            finally:
+                L.shx_free(self.p, b)
+        finally:
+            L.shx_free(self.p, a)
+
     def close(self):
+        """Release every problem made on this context, then the context."""
         if getattr(self, "p", None):
+            for prob in list(getattr(self, "_problems", ())):
+                prob.close()
             lib().shx_finalize(self.p)
             self.p = None
 
@@ -70,6 +93,7 @@ class Field:
         self.problem = problem
         self.c = ShxField()
         check(lib().shoccs_field_alloc(problem.p, C.byref(self.c)), problem.ctx.p, "field alloc")
+        problem._fields.add(self)
 
     def ref(self):
         return C.byref(self.c)
@@ -140,9 +164,16 @@ class Problem:
         self.nR = tuple(L.shoccs_problem_size(p, a + 1) for a in range(3))
         self.flat_size = self.N + sum(self.nR)
         self.sys = L.shoccs_problem_system(p)
+        self._fields = weakref.WeakSet()
+        if ctx is not None:
+            ctx._problems.add(self)
 
     def close(self):
+        """Free the problem's live fields, then the problem (device state
+        included); safe to call more than once."""
         if getattr(self, "p", None):
+            for f in list(getattr(self, "_fields", ())):
+                f.free()
             lib().shoccs_problem_destroy(self.p)
             self.p = None
 
@@ -279,7 +310,7 @@ class Problem:
               "rk4_fused")
         self.ctx.sync()
 
-    def set_tuning(self, variant: int, march_ty: int = 4, march_xchunk: int = 32):
+    def set_tuning(self, variant: int, march_ty: int = 8, march_xchunk: int = 64):
         """Fused-stage kernel: 0 point-wise tiles, 1 x-marching tiles."""
         self._need_device()
         check(lib().shx_system_set_tuning(self.sys, variant, march_ty, march_xchunk), self.ctx.p,
@@ -288,6 +319,14 @@ class Problem:
     def tuning(self) -> tuple:
         out = (C.c_int * 3)()
         check(lib().shx_system_get_tuning(self.sys, out), self.ctx.p, "get_tuning")
+        return tuple(out)
+
+    def stage_points(self) -> tuple:
+        """(points of D updated by the main fused-stage kernel, points updated
+        by the special-point kernel) for the current variant."""
+        self._need_device()
+        out = (C.c_int64 * 2)()
+        check(lib().shx_system_stage_points(self.sys, out), self.ctx.p, "stage_points")
         return tuple(out)
 
     def autotune(self, u0: Field, t: float, dt: float) -> tuple:

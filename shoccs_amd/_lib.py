@@ -67,6 +67,8 @@ _SIGS = [
     ("shx_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("shx_d2h", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     ("shx_fill", C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int64]),
+    ("shx_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]),
+    ("shx_time_copy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int, dp]),
     ("shx_derivative_create", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("shx_derivative_destroy", C.c_int, [C.c_void_p]),
     ("shx_derivative_apply", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
@@ -98,6 +100,7 @@ _SIGS = [
                                               C.c_double, C.c_int, dp]),
     ("shx_system_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     ("shx_system_get_tuning", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("shx_system_stage_points", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("shx_system_autotune", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double, dp]),
     ("shoccs_problem_create", C.c_int, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
     ("shoccs_problem_destroy", C.c_int, [C.c_void_p]),

@@ -23,6 +23,19 @@ __global__ void k_fill(double* __restrict__ dst, double v, long n)
     if (i < n) dst[i] = v;
 }
 
+// two doubles per lane: 16-byte loads and stores, grid-stride over the pairs
+__global__ void k_copy_vec(double2* __restrict__ dst, const double2* __restrict__ src, long npairs)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < npairs;
+         i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+__global__ void k_copy_tail(double* __restrict__ dst, const double* __restrict__ src, long n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
+}
+
 template <int DIR, int W, bool PLUS, bool WITH_B>
 __global__ __launch_bounds__(kBlock) void k_derivative(Tiling t, DevDeriv d,
                                                        const double* __restrict__ u,
@@ -87,6 +100,23 @@ int launch_fill(shx_ctx* ctx, double* dst, double v, long n)
     k_fill<<<grid, 256, 0, ctx->stream>>>(dst, v, n);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "fill");
+}
+
+static int launch_copy(shx_ctx* ctx, double* dst, const double* src, long n)
+{
+    if (n <= 0) return SHX_OK;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15) == 0;
+    if (!aligned) return fail(ctx, SHX_EINVAL, "copy: buffers must be 16-byte aligned");
+    const long pairs = n / 2;
+    if (pairs > 0) {
+        const long want = (pairs + 255) / 256;
+        const unsigned grid = (unsigned)(want < 65536 ? want : 65536);
+        k_copy_vec<<<grid, 256, 0, ctx->stream>>>(reinterpret_cast<double2*>(dst),
+                                                  reinterpret_cast<const double2*>(src), pairs);
+    }
+    if (n & 1) k_copy_tail<<<1, 64, 0, ctx->stream>>>(dst, src, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "copy");
 }
 
 int launch_derivative(shx_derivative* d, const double* u, const double* uR, double* du, bool plus,
@@ -243,6 +273,38 @@ int shx_fill(shx_ctx* ctx, double* dst, double value, int64_t n)
     return launch_fill(ctx, dst, value, n);
 }
 
+int shx_copy(shx_ctx* ctx, double* dst, const double* src, int64_t n)
+{
+    if (!ctx || n < 0 || (n > 0 && (!dst || !src))) return SHX_EINVAL;
+    return launch_copy(ctx, dst, src, n);
+}
+
+int shx_time_copy(shx_ctx* ctx, double* dst, const double* src, int64_t n, int reps,
+                  double* ms_per_copy)
+{
+    if (!ctx || !ms_per_copy || reps < 1 || n < 1 || !dst || !src) return SHX_EINVAL;
+    int rc = launch_copy(ctx, dst, src, n);  // warm-up
+    if (rc != SHX_OK) return rc;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return hip_fail(ctx, hipGetLastError(), "time_copy");
+    if (hipEventCreate(&b) != hipSuccess) {
+        hipEventDestroy(a);
+        return hip_fail(ctx, hipGetLastError(), "time_copy");
+    }
+    hipEventRecord(a, ctx->stream);
+    for (int r = 0; r < reps && rc == SHX_OK; ++r) rc = launch_copy(ctx, dst, src, n);
+    hipEventRecord(b, ctx->stream);
+    hipError_t e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc != SHX_OK) return rc;
+    if (e != hipSuccess) return hip_fail(ctx, e, "time_copy");
+    *ms_per_copy = ms / reps;
+    return SHX_OK;
+}
+
 int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_derivative** out)
 {
     if (!ctx || !desc || !out) return SHX_EINVAL;
@@ -382,6 +444,7 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     }
     d->h_blocks.assign(desc->blocks, desc->blocks + desc->nblocks);
     d->h_pool.assign(desc->pool, desc->pool + desc->npool);
+    d->h_pair_ptr = std::move(pair_ptr);
     d->dev.pair_ptr = d->pair_ptr;
     d->dev.fast = d->fast;
     d->dev.blocks = d->blocks;

@@ -1,15 +1,25 @@
-// shx_march.hip -- fused RK4 stage kernels that march along x.
+// shx_march.hip -- the x-marching form of one fused RK4 stage.
 //
-// A workgroup owns a (y, z) tile of TY x 64 points and walks a chunk of x
-// planes.  Each thread keeps the five x-neighbours of its point in registers
-// (a queue shifted by one plane per step), the current plane is staged in LDS
-// with two-point y and z halos, so every stage value is fetched from HBM about
-// once.  Rows whose position is not in one of the line's interior intervals
-// (closures, B terms, solid points) take the segment-table slow path, which
-// reads global memory.  Arithmetic is the same as the point-wise kernel of
-// shx_systems.hip, term for term, so results are bit-identical.
+// Every point of D carries a class byte, computed once per system:
+//   * interior points (all rows circulant), points with no rows, and points
+//     whose y/z closure rows (no B terms) read only values of the LDS tile
+//     are handled by k_stage_march: a workgroup owns a (y, z) tile of TY x 64
+//     points and walks a chunk of x planes.  Each thread keeps the W
+//     x-neighbours of its point in registers (a queue shifted by one plane
+//     per step); the current plane is staged in LDS, double-buffered, with
+//     W/2-point y and z halos, so each stage value is fetched about once and
+//     there is one barrier per plane.  No segment-table lookups, no
+//     divergence into slow paths;
+//   * the remaining points (closure rows, B terms, Dirichlet faces) are
+//     listed once, each with a record naming its row in every direction
+//     (interior, none, or an entry of a deduplicated row table), and
+//     evaluated by k_stage_special without segment-table scans; the same
+//     launch updates the Dirichlet cut points.
+// Both kernels use the formulas of shx_point.hpp in the same order as the
+// point-wise tiles, so results are bit-identical to them.
 #include <hip/hip_runtime.h>
 
+#include "shx_point.hpp"
 #include "shx_sys.hpp"
 
 namespace shx {
@@ -23,24 +33,80 @@ __device__ __forceinline__ bool in_fast(unsigned long long fd, int q)
     return (q >= lo0 && q < hi0) || (q >= lo1 && q < hi1);
 }
 
-// interior 5-point dot from 0.0, left to right
-__device__ __forceinline__ double dot5(const double* c, double a, double b, double m, double d,
-                                       double e)
+// Whether line `pair` of an operator has a row at position q.
+__device__ __forceinline__ bool row_exists(const DevDeriv& d, int pair, int q)
+{
+    const int b_end = d.pair_ptr[pair + 1];
+    for (int b = d.pair_ptr[pair]; b < b_end; ++b) {
+        const shx_line_block& blk = d.blocks[b];
+        const int lr = q - blk.lo;
+        if (lr < 0) return false;
+        if (lr < blk.nl + blk.ni + blk.nr) return true;
+    }
+    return false;
+}
+
+// 0 = no row, 1 = interior (circulant) row, 2 = other row
+__device__ __forceinline__ int row_kind(const DevDeriv& d, int pair, int q)
+{
+    if (in_fast(d.fast[pair], q)) return 1;
+    return row_exists(d, pair, q) ? 2 : 0;
+}
+
+__global__ void k_classify(DevSys s, DevOps ops, unsigned char* __restrict__ cls, long N)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    int i, j, k;
+    point_coords(idx, s.n1, s.n2, i, j, k);
+    int present = 0, interior = 0, none = 0;
+    if (ops.d[0].present) {
+        ++present;
+        const int r = row_kind(ops.d[0], j * s.n2 + k, i);
+        interior += r == 1;
+        none += r == 0;
+    }
+    if (ops.d[1].present) {
+        ++present;
+        const int r = row_kind(ops.d[1], i * s.n2 + k, j);
+        interior += r == 1;
+        none += r == 0;
+    }
+    if (ops.d[2].present) {
+        ++present;
+        const int r = row_kind(ops.d[2], i * s.n1 + j, k);
+        interior += r == 1;
+        none += r == 0;
+    }
+    const bool face = on_dirichlet_face(s, i, j, k);
+    const bool fluid = is_fluid(s, idx);
+    unsigned char c = kRawSpecial | (face ? kRawFace : 0) | (fluid ? kRawFluid : 0);
+    if (!face && fluid && interior == present) c = kInterior;
+    else if (!face && !fluid && none == present) c = kNoRows;
+    cls[idx] = c;
+}
+
+// One y or z row of a pattern point from the LDS tile: the closure row of n
+// columns starting at offset off (weights w, in LDS).  `c0` is the tile slot
+// of the point, `step` the slot distance of one position along the direction.
+__device__ __forceinline__ double pattern_dot(int n, int off, const double* w, const double* c0,
+                                              int step)
 {
     double s = 0.0;
-    s += c[0] * a;
-    s += c[1] * b;
-    s += c[2] * m;
-    s += c[3] * d;
-    s += c[4] * e;
+#pragma unroll 1
+    for (int m = 0; m < n; ++m) s += w[m] * c0[(off + m) * step];
     return s;
 }
 
-template <int STAGE, int TY>
-__global__ __launch_bounds__(64 * TY) void k_wave_stage_march(MarchArgs a)
+template <int KIND, int STAGE, int W, int TY>
+__global__ __launch_bounds__(64 * TY) void k_stage_march(MarchArgs a)
 {
     constexpr int TZ = 64;
-    __shared__ double tile[TY + 4][TZ + 4];
+    constexpr int hw = W / 2;
+    constexpr int LY = TY + 2 * hw, LZ = TZ + 2 * hw;
+    static_assert(TY >= 2 * hw, "y halo rows are loaded by distinct threads");
+    __shared__ double tile[2][LY][LZ];
+    __shared__ MarchPattern spat[kMaxPatterns];
 
     // workgroup -> (z tile, y tile, x chunk), contiguous per XCD, z fastest
     const unsigned L = xcd_remap(blockIdx.x, a.nblocks);
@@ -52,144 +118,338 @@ __global__ __launch_bounds__(64 * TY) void k_wave_stage_march(MarchArgs a)
     const int n0 = a.s.n0, n1 = a.s.n1, n2 = a.s.n2;
     const int k = bz * TZ + tz, j = by * TY + ty;
     const bool active = k < n2 && j < n1;
+    const int kc = min(k, n2 - 1), jc = min(j, n1 - 1);
+    const long plane = (long)n1 * n2;
+    const long col = (long)jc * n2 + kc;
+
+    // halo sources inside a plane (positions clamped to the mesh; clamped
+    // values are never used by an interior row)
+    int yslot = -1, zslot = -1;
+    long yoff = 0, zoff = 0;
+    if (ty < hw) {
+        const int jj = min(max(by * TY - hw + ty, 0), n1 - 1);
+        yslot = ty;
+        yoff = (long)jj * n2 + kc;
+    } else if (ty >= TY - hw) {
+        const int jj = min(by * TY + TY + ty - (TY - hw), n1 - 1);
+        yslot = ty + 2 * hw;
+        yoff = (long)jj * n2 + kc;
+    }
+    if (tz < hw) {
+        const int kk = min(max(bz * TZ - hw + tz, 0), n2 - 1);
+        zslot = tz;
+        zoff = (long)jc * n2 + kk;
+    } else if (tz >= TZ - hw) {
+        const int kk = min(bz * TZ + TZ + tz - (TZ - hw), n2 - 1);
+        zslot = tz + 2 * hw;
+        zoff = (long)jc * n2 + kk;
+    }
+
+    // closure patterns to LDS (read after the first barrier of the march)
+    {
+        const int nwords = a.npat * (int)(sizeof(MarchPattern) / sizeof(double));
+        const double* src = reinterpret_cast<const double*>(a.pat);
+        double* dst = reinterpret_cast<double*>(spat);
+        for (int w = threadIdx.x; w < nwords; w += 64 * TY) dst[w] = src[w];
+    }
+
+    const bool px = a.ops.d[0].present, py = a.ops.d[1].present, pz = a.ops.d[2].present;
+    double cx[W], cy[W], cz[W];
+#pragma unroll
+    for (int m = 0; m < W; ++m) {
+        cx[m] = px ? a.ops.d[0].coef[m] : 0.0;
+        cy[m] = py ? a.ops.d[1].coef[m] : 0.0;
+        cz[m] = pz ? a.ops.d[2].coef[m] : 0.0;
+    }
+
     const int x0 = bx * a.xchunk;
     const int x1 = min(x0 + a.xchunk, n0);
-    const long plane = (long)n1 * n2;
-    const long col = (long)j * n2 + k;  // offset of (j, k) inside a plane
-
-    const DevDeriv& dx = a.ops.d[0];
-    const DevDeriv& dy = a.ops.d[1];
-    const DevDeriv& dz = a.ops.d[2];
-    double cx[5], cy[5], cz[5];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        cx[q] = dx.present ? dx.coef[q] : 0.0;
-        cy[q] = dy.present ? dy.coef[q] : 0.0;
-        cz[q] = dz.present ? dz.coef[q] : 0.0;
-    }
-    const unsigned long long fx = (active && dx.present) ? dx.fast[j * n2 + k] : 0ull;
-
     const double* __restrict__ us = a.us;
-    auto ld = [&](int x) -> double { return (active && x >= 0 && x < n0) ? us[(long)x * plane + col] : 0.0; };
-    double qm2 = ld(x0 - 2), qm1 = ld(x0 - 1), q0 = ld(x0), qp1 = ld(x0 + 1), qp2;
+    // x queue: q[m] = us at plane x - hw + m; qn is the next queue element.
+    double q[W];
+#pragma unroll
+    for (int m = 0; m < W - 1; ++m) q[m] = us[(long)min(max(x0 - hw + m, 0), n0 - 1) * plane + col];
+    double qn = us[(long)min(x0 + hw, n0 - 1) * plane + col];
+    // values of the current plane; those of the next plane are loaded one
+    // iteration ahead so that their latency overlaps the current plane
+    long pcur = (long)x0 * plane;
+    double yv = yslot >= 0 ? us[pcur + yoff] : 0.0;
+    double zv = zslot >= 0 ? us[pcur + zoff] : 0.0;
+    unsigned char c = a.cls[pcur + col];
+    double u0v = a.u0[pcur + col];
+    double accv = STAGE == 0 ? 0.0 : a.acc[pcur + col];
 
     for (int x = x0; x < x1; ++x) {
-        // issue every global load of this plane before the barrier: the queue
-        // head, the LDS halos, the y/z line descriptors and u0/acc at the point
-        qp2 = ld(x + 2);
-        const long pbase = (long)x * plane;
-        double hy = 0.0, hz = 0.0;
-        int hy_row = -1, hz_col = -1;
-        if (ty < 2) {
-            const int jj = by * TY - 2 + ty;
-            hy_row = ty;
-            if (k < n2 && jj >= 0 && jj < n1) hy = us[pbase + (long)jj * n2 + k];
-        } else if (ty >= TY - 2) {
-            const int jj = by * TY + TY + (ty - (TY - 2));
-            hy_row = ty + 4;
-            if (k < n2 && jj < n1) hy = us[pbase + (long)jj * n2 + k];
-        }
-        double hy2 = 0.0;
-        int hy2_row = -1;
-        if (TY == 2) {  // two rows of threads also load the bottom halo
-            const int jj = by * TY + TY + ty;
-            hy2_row = ty + 4;
-            if (k < n2 && jj < n1) hy2 = us[pbase + (long)jj * n2 + k];
-        }
-        if (tz < 2) {
-            const int kk = bz * TZ - 2 + tz;
-            hz_col = tz;
-            if (j < n1 && kk >= 0 && kk < n2) hz = us[pbase + (long)j * n2 + kk];
-        } else if (tz >= TZ - 2) {
-            const int kk = bz * TZ + TZ + (tz - (TZ - 2));
-            hz_col = tz + 4;
-            if (j < n1 && kk < n2) hz = us[pbase + (long)j * n2 + kk];
-        }
-        const long idx = pbase + col;
-        const unsigned long long fy = (active && dy.present) ? dy.fast[x * n2 + k] : 0ull;
-        const unsigned long long fz = (active && dz.present) ? dz.fast[x * n1 + j] : 0ull;
-        const double u0v = active ? a.u0[idx] : 0.0;
-        const double accv = (active && STAGE > 0) ? a.acc[idx] : 0.0;
-
-        tile[ty + 2][tz + 2] = q0;
-        if (hy_row >= 0) tile[hy_row][tz + 2] = hy;
-        if (hy2_row >= 0) tile[hy2_row][tz + 2] = hy2;
-        if (hz_col >= 0) tile[ty + 2][hz_col] = hz;
+        const long pb = (long)x * plane;
+        q[W - 1] = qn;
+        // next plane (the last iteration reloads its own plane; unused)
+        const int xn = min(x + 1, x1 - 1);
+        const long pn = (long)xn * plane;
+        qn = us[(long)min(xn + hw, n0 - 1) * plane + col];
+        const double nyv = yslot >= 0 ? us[pn + yoff] : 0.0;
+        const double nzv = zslot >= 0 ? us[pn + zoff] : 0.0;
+        const unsigned char nc = a.cls[pn + col];
+        const double nu0 = a.u0[pn + col];
+        const double nacc = STAGE == 0 ? 0.0 : a.acc[pn + col];
+        const long idx = pb + col;
+        double(*t)[LZ] = tile[x & 1];
+        t[ty + hw][tz + hw] = q[hw];
+        if (yslot >= 0) t[yslot][tz + hw] = yv;
+        if (zslot >= 0) t[ty + hw][zslot] = zv;
         __syncthreads();
-
-        if (active) {
-            double du[3] = {0.0, 0.0, 0.0};
-            // x: queue
-            if (dx.present) {
-                if (in_fast(fx, x)) {
-                    du[0] = dot5(cx, qm2, qm1, q0, qp1, qp2);
-                } else {
-                    const RowVal r = row_slow<true>(dx.pair_ptr, dx.blocks, dx.pool, dx.coef, dx.width,
-                                                    dx.stride, us, a.usR[0], idx, j * n2 + k, x);
-                    if (r.row) du[0] = row_eq(r);
+        if (active && c != kSpecial) {
+            double rhs;
+            if (c != kNoRows) {
+                double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+                if (px) {
+                    double sx = 0.0;
+#pragma unroll
+                    for (int m = 0; m < W; ++m) sx += cx[m] * q[m];
+                    d0 = sx;
                 }
-            }
-            // y: LDS column
-            if (dy.present) {
-                if (in_fast(fy, j)) {
-                    du[1] = dot5(cy, tile[ty][tz + 2], tile[ty + 1][tz + 2], tile[ty + 2][tz + 2],
-                                 tile[ty + 3][tz + 2], tile[ty + 4][tz + 2]);
+                if (c == kInterior) {
+                    if (py) {
+                        double sy = 0.0;
+#pragma unroll
+                        for (int m = 0; m < W; ++m) sy += cy[m] * t[ty + m][tz + hw];
+                        d1 = sy;
+                    }
+                    if (pz) {
+                        double sz = 0.0;
+#pragma unroll
+                        for (int m = 0; m < W; ++m) sz += cz[m] * t[ty + hw][tz + m];
+                        d2 = sz;
+                    }
                 } else {
-                    const RowVal r = row_slow<true>(dy.pair_ptr, dy.blocks, dy.pool, dy.coef, dy.width,
-                                                    dy.stride, us, a.usR[1], idx, x * n2 + k, j);
-                    if (r.row) du[1] = row_eq(r);
+                    // y/z rows of a pattern point: interior or a closure row
+                    const MarchPattern& pt = spat[c - kPattern0];
+                    if (py) {
+                        const int yn = pt.yn;
+                        if (yn == 0) {
+                            double sy = 0.0;
+#pragma unroll
+                            for (int m = 0; m < W; ++m) sy += cy[m] * t[ty + m][tz + hw];
+                            d1 = sy;
+                        } else {
+                            d1 = pattern_dot(yn, pt.yoff, pt.wy, &t[ty + hw][tz + hw], LZ);
+                        }
+                    }
+                    if (pz) {
+                        const int zn = pt.zn;
+                        if (zn == 0) {
+                            double sz = 0.0;
+#pragma unroll
+                            for (int m = 0; m < W; ++m) sz += cz[m] * t[ty + hw][tz + m];
+                            d2 = sz;
+                        } else {
+                            d2 = pattern_dot(zn, pt.zoff, pt.wz, &t[ty + hw][tz + hw], 1);
+                        }
+                    }
                 }
-            }
-            // z: LDS row
-            if (dz.present) {
-                if (in_fast(fz, k)) {
-                    du[2] = dot5(cz, tile[ty + 2][tz], tile[ty + 2][tz + 1], tile[ty + 2][tz + 2],
-                                 tile[ty + 2][tz + 3], tile[ty + 2][tz + 4]);
+                if (KIND == SHX_SYS_HEAT) {
+                    // laplacian rows added in direction order (plus_eq)
+                    double lap = 0.0;
+                    if (px) lap += d0;
+                    if (py) lap += d1;
+                    if (pz) lap += d2;
+                    rhs = heat_finish(a.s, false, true, x, j, k, lap, a.t_s);
                 } else {
-                    const RowVal r = row_slow<true>(dz.pair_ptr, dz.blocks, dz.pool, dz.coef, dz.width,
-                                                    dz.stride, us, a.usR[2], idx, x * n1 + j, k);
-                    if (r.row) du[2] = row_eq(r);
+                    rhs = wave_combine<KIND>(a.s, false, x, j, k, d0, d1, d2);
                 }
-            }
-            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-            const bool face = on_dirichlet_face(a.s, x, j, k);
-            if (!face) {
-                const double px = a.s.x[x], py = a.s.y[j], pz = a.s.z[k];
-                g0 = wave_neg_g(a.s, 0, px, py, pz);
-                g1 = wave_neg_g(a.s, 1, px, py, pz);
-                g2 = wave_neg_g(a.s, 2, px, py, pz);
-            }
-            const double rhs = g0 * du[0] + g1 * du[1] + g2 * du[2];
-            const double acc = (STAGE == 0 ? 0.0 : accv) + a.c_acc * rhs;
-            double v;
-            if (STAGE < 3) {
-                a.acc[idx] = acc;
-                v = u0v + a.c_next * rhs;
             } else {
-                v = u0v + 1.0 * acc;
+                rhs = KIND == SHX_SYS_HEAT ? heat_finish(a.s, false, false, x, j, k, 0.0, a.t_s)
+                                           : wave_combine<KIND>(a.s, false, x, j, k, 0.0, 0.0, 0.0);
             }
-            if (face) v = wave_exact(a.s, a.s.x[x], a.s.y[j], a.s.z[k], a.t_next);
-            a.out[idx] = v;
+            a.out[idx] = stage_update<STAGE>(rhs, accv, u0v, a.acc, idx, a.c_acc, a.c_next);
         }
-        __syncthreads();
-        qm2 = qm1;
-        qm1 = q0;
-        q0 = qp1;
-        qp1 = qp2;
+#pragma unroll
+        for (int m = 0; m < W - 1; ++m) q[m] = q[m + 1];
+        yv = nyv;
+        zv = nzv;
+        c = nc;
+        u0v = nu0;
+        accv = nacc;
+    }
+}
+
+// Row of direction d at a special point from its record (the same sums as
+// row_at: interior rows tap by tap, other rows as listed, B terms after).
+template <int W>
+__device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRow* __restrict__ rows,
+                                              unsigned ref, const double* __restrict__ u,
+                                              const double* __restrict__ uR, long idx)
+{
+    RowVal r;
+    r.dot = 0.0;
+    r.b0 = r.b1 = 0.0;
+    r.nb = 0;
+    r.row = ref != kRowNone;
+    if (!r.row) return r;
+    const long st = d.stride;
+    double dot = 0.0;
+    if (ref == kRowInterior) {
+        constexpr int hw = W / 2;
+#pragma unroll
+        for (int m = 0; m < W; ++m) dot += d.coef[m] * u[idx + (long)(m - hw) * st];
+        r.dot = dot;
+        return r;
+    }
+    const SpecialRow e = rows[ref];
+    const double* __restrict__ w = d.pool + e.coef;
+    const double* __restrict__ v = u + idx + (long)e.off * st;
+    for (int c = 0; c < e.ncols; ++c) dot += w[c] * v[(long)c * st];
+    r.dot = dot;
+    r.nb = e.nb;
+    if (e.nb > 0) r.b0 = d.pool[e.bw0] * uR[e.bobj0];
+    if (e.nb > 1) r.b1 = d.pool[e.bw1] * uR[e.bobj1];
+    return r;
+}
+
+template <int KIND, int STAGE, int W>
+__global__ __launch_bounds__(256) void k_stage_special(SpecialArgs a)
+{
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < a.nspecial) {
+        const SpecialRec rec = a.recs[q];
+        const long idx = rec.idx;
+        int i, j, k;
+        point_coords(idx, a.s.n1, a.s.n2, i, j, k);
+        const DevOps& ops = a.ops;
+        double rhs;
+        if (KIND == SHX_SYS_HEAT) {
+            double lap = 0.0;
+            if (ops.d[0].present) {
+                const RowVal r = special_row<W>(ops.d[0], a.rows0, rec.ref[0], a.us, a.usR.r[0], idx);
+                if (r.row) lap = row_plus(lap, r);
+            }
+            if (ops.d[1].present) {
+                const RowVal r = special_row<W>(ops.d[1], a.rows1, rec.ref[1], a.us, a.usR.r[1], idx);
+                if (r.row) lap = row_plus(lap, r);
+            }
+            if (ops.d[2].present) {
+                const RowVal r = special_row<W>(ops.d[2], a.rows2, rec.ref[2], a.us, a.usR.r[2], idx);
+                if (r.row) lap = row_plus(lap, r);
+            }
+            rhs = heat_finish(a.s, on_dirichlet_face(a.s, i, j, k), is_fluid(a.s, idx), i, j, k, lap,
+                              a.t_s);
+        } else {
+            double du0 = 0.0, du1 = 0.0, du2 = 0.0;
+            if (ops.d[0].present) {
+                const RowVal r = special_row<W>(ops.d[0], a.rows0, rec.ref[0], a.us, a.usR.r[0], idx);
+                if (r.row) du0 = row_eq(r);
+            }
+            if (ops.d[1].present) {
+                const RowVal r = special_row<W>(ops.d[1], a.rows1, rec.ref[1], a.us, a.usR.r[1], idx);
+                if (r.row) du1 = row_eq(r);
+            }
+            if (ops.d[2].present) {
+                const RowVal r = special_row<W>(ops.d[2], a.rows2, rec.ref[2], a.us, a.usR.r[2], idx);
+                if (r.row) du2 = row_eq(r);
+            }
+            rhs = wave_combine<KIND>(a.s, on_dirichlet_face(a.s, i, j, k), i, j, k, du0, du1, du2);
+        }
+        stage_store<KIND, STAGE>(a.s, rhs, a.u0, a.acc, a.out, idx, i, j, k, a.c_acc, a.c_next,
+                                 a.t_next);
+        return;
+    }
+    q -= a.nspecial;
+    if (q < a.nR0) {
+        if (a.dir0[q])
+            a.outR0[q] = exact_of<KIND>(a.s, a.pos0[3 * q], a.pos0[3 * q + 1], a.pos0[3 * q + 2], a.t_next);
+        return;
+    }
+    q -= a.nR0;
+    if (q < a.nR1) {
+        if (a.dir1[q])
+            a.outR1[q] = exact_of<KIND>(a.s, a.pos1[3 * q], a.pos1[3 * q + 1], a.pos1[3 * q + 2], a.t_next);
+        return;
+    }
+    q -= a.nR1;
+    if (q < a.nR2) {
+        if (a.dir2[q])
+            a.outR2[q] = exact_of<KIND>(a.s, a.pos2[3 * q], a.pos2[3 * q + 1], a.pos2[3 * q + 2], a.t_next);
+    }
+}
+
+template <int KIND, int STAGE, int W>
+int march_ty(hipStream_t st, int ty, const MarchArgs& a)
+{
+    switch (ty) {
+    case 4:
+        k_stage_march<KIND, STAGE, W, 4><<<a.nblocks, 64 * 4, 0, st>>>(a);
+        return 0;
+    case 8:
+        k_stage_march<KIND, STAGE, W, 8><<<a.nblocks, 64 * 8, 0, st>>>(a);
+        return 0;
+    case 16:
+        k_stage_march<KIND, STAGE, W, 16><<<a.nblocks, 64 * 16, 0, st>>>(a);
+        return 0;
+    default:
+        return -1;
+    }
+}
+
+template <int KIND, int W>
+int march_stage(hipStream_t st, int stage, int ty, const MarchArgs& a)
+{
+    switch (stage) {
+    case 0: return march_ty<KIND, 0, W>(st, ty, a);
+    case 1: return march_ty<KIND, 1, W>(st, ty, a);
+    case 2: return march_ty<KIND, 2, W>(st, ty, a);
+    case 3: return march_ty<KIND, 3, W>(st, ty, a);
+    default: return -1;
+    }
+}
+
+template <int KIND, int W>
+int special_stage(hipStream_t st, int stage, unsigned grid, const SpecialArgs& a)
+{
+    switch (stage) {
+    case 0: k_stage_special<KIND, 0, W><<<grid, 256, 0, st>>>(a); return 0;
+    case 1: k_stage_special<KIND, 1, W><<<grid, 256, 0, st>>>(a); return 0;
+    case 2: k_stage_special<KIND, 2, W><<<grid, 256, 0, st>>>(a); return 0;
+    case 3: k_stage_special<KIND, 3, W><<<grid, 256, 0, st>>>(a); return 0;
+    default: return -1;
     }
 }
 
 }  // namespace
 
-int launch_wave_stage_march(hipStream_t st, int stage, int ty, const MarchArgs& a)
+int launch_classify(hipStream_t st, const DevSys& s, const DevOps& ops, unsigned char* cls, long N)
 {
-#define SHX_MARCH_CASE(S, T) \
-    if (stage == S && ty == T) { k_wave_stage_march<S, T><<<a.nblocks, 64 * T, 0, st>>>(a); return 0; }
-    SHX_MARCH_CASE(0, 4) SHX_MARCH_CASE(1, 4) SHX_MARCH_CASE(2, 4) SHX_MARCH_CASE(3, 4)
-    SHX_MARCH_CASE(0, 8) SHX_MARCH_CASE(1, 8) SHX_MARCH_CASE(2, 8) SHX_MARCH_CASE(3, 8)
-    SHX_MARCH_CASE(0, 2) SHX_MARCH_CASE(1, 2) SHX_MARCH_CASE(2, 2) SHX_MARCH_CASE(3, 2)
-#undef SHX_MARCH_CASE
-    return -1;
+    if (N <= 0) return 0;
+    k_classify<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(s, ops, cls, N);
+    return 0;
+}
+
+int launch_stage_march(hipStream_t st, int kind, int stage, int width, int ty, const MarchArgs& a)
+{
+    if (a.nblocks == 0) return 0;
+    switch (kind * 10 + width) {
+    case SHX_SYS_HEAT * 10 + 5: return march_stage<SHX_SYS_HEAT, 5>(st, stage, ty, a);
+    case SHX_SYS_HEAT * 10 + 3: return march_stage<SHX_SYS_HEAT, 3>(st, stage, ty, a);
+    case SHX_SYS_WAVE_LINEAR * 10 + 5: return march_stage<SHX_SYS_WAVE_LINEAR, 5>(st, stage, ty, a);
+    case SHX_SYS_WAVE_LINEAR * 10 + 3: return march_stage<SHX_SYS_WAVE_LINEAR, 3>(st, stage, ty, a);
+    case SHX_SYS_WAVE_RADIAL * 10 + 5: return march_stage<SHX_SYS_WAVE_RADIAL, 5>(st, stage, ty, a);
+    case SHX_SYS_WAVE_RADIAL * 10 + 3: return march_stage<SHX_SYS_WAVE_RADIAL, 3>(st, stage, ty, a);
+    default: return -1;
+    }
+}
+
+int launch_stage_special(hipStream_t st, int kind, int stage, int width, const SpecialArgs& a)
+{
+    const long n = a.nspecial + a.nR0 + a.nR1 + a.nR2;
+    if (n == 0) return 0;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    switch (kind * 10 + width) {
+    case SHX_SYS_HEAT * 10 + 5: return special_stage<SHX_SYS_HEAT, 5>(st, stage, grid, a);
+    case SHX_SYS_HEAT * 10 + 3: return special_stage<SHX_SYS_HEAT, 3>(st, stage, grid, a);
+    case SHX_SYS_WAVE_LINEAR * 10 + 5: return special_stage<SHX_SYS_WAVE_LINEAR, 5>(st, stage, grid, a);
+    case SHX_SYS_WAVE_LINEAR * 10 + 3: return special_stage<SHX_SYS_WAVE_LINEAR, 3>(st, stage, grid, a);
+    case SHX_SYS_WAVE_RADIAL * 10 + 5: return special_stage<SHX_SYS_WAVE_RADIAL, 5>(st, stage, grid, a);
+    case SHX_SYS_WAVE_RADIAL * 10 + 3: return special_stage<SHX_SYS_WAVE_RADIAL, 3>(st, stage, grid, a);
+    default: return -1;
+    }
 }
 
 }  // namespace shx

@@ -31,9 +31,11 @@ struct shx_derivative {
     int* neu_ptr = nullptr;
     long* neu_col = nullptr;
     double* neu_w = nullptr;
-    // host copies kept to build the reference's full-length B matrix
+    // host copies kept to build the reference's full-length B matrix and the
+    // special-point rows of the x-marching form
     std::vector<shx_line_block> h_blocks;
     std::vector<double> h_pool;
+    std::vector<int> h_pair_ptr;
 };
 
 struct shx_csr {

@@ -77,23 +77,95 @@ struct RPtrs {
 };
 
 
-// Arguments of the x-marching fused stage kernels.
+// Per-point class of the x-marching fused form (one byte per point of D):
+//   kInterior -- every present direction has an interior circulant row here,
+//                not on a Dirichlet face, fluid: register/LDS stencils only;
+//   kNoRows   -- no row in any direction, not on a face, not fluid (solid
+//                points): the right-hand side of zero derivatives;
+//   kPattern0 + p -- fluid, not on a face, interior x row, and y/z rows that
+//                are interior or closure rows without B terms whose columns
+//                lie in the workgroup's LDS tile: MarchPattern p;
+//   kSpecial  -- anything else (x closures, B terms, faces): evaluated by the
+//                special-point kernel from its record.
+// The classification kernel marks raw special points kRawSpecial | flags;
+// the host turns them into kPattern0 + p or kSpecial.
+constexpr unsigned char kInterior = 0;
+constexpr unsigned char kSpecial = 1;
+constexpr unsigned char kNoRows = 2;
+constexpr unsigned char kPattern0 = 3;
+constexpr int kMaxPatterns = 64;     // staged in LDS: 64 x 144 B
+constexpr int kMaxPatternCols = 8;
+constexpr unsigned char kRawSpecial = 0x80;
+constexpr unsigned char kRawFace = 0x01;
+constexpr unsigned char kRawFluid = 0x02;
+
+// y and z rows of a pattern point: n == 0 is the interior circulant row,
+// otherwise the dot of w[c] with the values at offsets off + c along the
+// direction, c < n (w copied from the operator's coefficient pool).
+struct MarchPattern {
+    int yoff, yn, zoff, zn;
+    double wy[kMaxPatternCols];
+    double wz[kMaxPatternCols];
+};
+
+// Arguments of the x-marching fused stage kernel.
 struct MarchArgs {
     DevSys s;
     DevOps ops;
     const double* us;
-    const double* usR[3];
     const double* u0;
     double* acc;
     double* out;
-    double c_acc, c_next, t_next;
+    const unsigned char* cls;
+    const MarchPattern* pat;
+    int npat;
+    double t_s, c_acc, c_next;
     int nbz, nby, xchunk;
     unsigned nblocks;
 };
 
-// Launch one fused stage of the constant/radial scalar wave with the
-// x-marching kernel (tile TY x 64, chunk of a.xchunk planes); returns -1 for
-// an unsupported TY.
-int launch_wave_stage_march(hipStream_t st, int stage, int ty, const MarchArgs& a);
+// A row of one direction's operator at a special point, other than an
+// interior circulant row: dot of pool[coef + c] with the D values at
+// offsets (off + c) * stride from the point, c < ncols, then nb (0-2) B
+// terms pool[bw] * R[bobj] in the order the reference adds them.
+struct SpecialRow {
+    int off, ncols, coef, nb;
+    int bobj0, bw0, bobj1, bw1;
+};
+constexpr unsigned kRowNone = 0xffffffffu;      // no row in this direction
+constexpr unsigned kRowInterior = 0xfffffffeu;  // interior circulant row
+
+// One special point: its index in D and, per direction, kRowNone,
+// kRowInterior or an index into that direction's SpecialRow table.
+struct SpecialRec {
+    unsigned idx;
+    unsigned ref[3];
+};
+
+// Arguments of the special-point kernel: the listed points of D, then the
+// cut points of the three directions (Dirichlet ones take exact(t_next)).
+struct SpecialArgs {
+    DevSys s;
+    DevOps ops;
+    const double* us;
+    RPtrs usR;
+    const double* u0;
+    double* acc;
+    double* out;
+    const SpecialRec* recs;
+    const SpecialRow *rows0, *rows1, *rows2;
+    long nspecial;
+    long nR0, nR1, nR2;
+    const double *pos0, *pos1, *pos2;
+    const signed char *dir0, *dir1, *dir2;
+    double *outR0, *outR1, *outR2;
+    double t_s, c_acc, c_next, t_next;
+};
+
+// Host launchers (shx_march.hip); each returns 0, or -1 for a template
+// combination that is not built.
+int launch_classify(hipStream_t st, const DevSys& s, const DevOps& ops, unsigned char* cls, long N);
+int launch_stage_march(hipStream_t st, int kind, int stage, int width, int ty, const MarchArgs& a);
+int launch_stage_special(hipStream_t st, int kind, int stage, int width, const SpecialArgs& a);
 
 }  // namespace shx

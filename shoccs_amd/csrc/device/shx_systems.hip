@@ -4,159 +4,21 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <new>
 #include <vector>
 
 #include "shx_internal.hpp"
 #include "shx_state.hpp"
+#include "shx_point.hpp"
 #include "shx_sys.hpp"
 
 using namespace shx;
 
-namespace shx {
-
-// gauss3d exponent (src/mms/gauss3d.cpp:14-20)
-__device__ __forceinline__ double g3_expo(const double* g, double x, double y, double z)
-{
-    const double *c = g, *v = g + 3;
-    return (-0.5 * pow((x - c[0]), 2) * pow(v[0], -2) +
-            (-0.5 * pow((y - c[1]), 2) * pow(v[1], -2) + -0.5 * pow((z - c[2]), 2) * pow(v[2], -2)));
-}
-
-// Manufactured solutions; scalar arguments only (called out of line).
-__device__ __noinline__ double mms_call(int kind, int dims, int n, const double* __restrict__ p,
-                                        double t, double x, double y, double z)
-{
-    if (kind == SHX_MMS_SINPROD) {
-        double v = sin(kTwoPi * x) * sin(kTwoPi * y);
-        if (dims == 3) v = v * sin(kTwoPi * z);
-        return v * exp(-t);
-    }
-    double sol = 0;
-    for (int i = 0; i < n; ++i) {
-        const double* g = p + 8 * i;
-        sol += exp(g3_expo(g, x, y, z)) * g[6] * cos(t * g[7]);
-    }
-    return sol;
-}
-
-__device__ __noinline__ double mms_ddt(int kind, int dims, int n, const double* __restrict__ p,
-                                       double t, double x, double y, double z)
-{
-    if (kind == SHX_MMS_SINPROD) return -mms_call(kind, dims, n, p, t, x, y, z);
-    double sol = 0;
-    for (int i = 0; i < n; ++i) {
-        const double* g = p + 8 * i;
-        sol += -exp(g3_expo(g, x, y, z)) * g[6] * g[7] * sin(t * g[7]);
-    }
-    return sol;
-}
-
-__device__ __noinline__ double mms_lap(int kind, int dims, int n, const double* __restrict__ p,
-                                       double t, double x, double y, double z)
-{
-    if (kind == SHX_MMS_SINPROD) return -(dims * kTwoPi * kTwoPi) * mms_call(kind, dims, n, p, t, x, y, z);
-    double sol = 0;
-    for (int i = 0; i < n; ++i) {
-        const double* c = p + 8 * i;
-        const double* v = c + 3;
-        sol += exp(0.5 * ((-1 * pow((z - c[2]), 2) * pow(v[2], -2) -
-                           pow((y - c[1]), 2) * pow(v[1], -2)) -
-                          pow((x - c[0]), 2) * pow(v[0], -2))) *
-               c[6] * cos(t * c[7]) * pow(v[0], -4) * pow(v[1], -4) * pow(v[2], -4) *
-               ((((pow(z, 2) * pow(v[0], 4) * pow(v[1], 4) +
-                   (-2 * z * c[2] * pow(v[0], 4) * pow(v[1], 4) +
-                    (pow(c[2], 2) * pow(v[0], 4) * pow(v[1], 4) +
-                     (pow(y, 2) * pow(v[0], 4) * pow(v[2], 4) +
-                      (-2 * y * c[1] * pow(v[0], 4) * pow(v[2], 4) +
-                       (pow(c[1], 2) * pow(v[0], 4) * pow(v[2], 4) +
-                        (pow(x, 2) * pow(v[1], 4) * pow(v[2], 4) +
-                         (-2 * x * c[0] * pow(v[1], 4) * pow(v[2], 4) +
-                          pow(c[0], 2) * pow(v[1], 4) * pow(v[2], 4))))))))) -
-                  pow(v[0], 2) * pow(v[1], 4) * pow(v[2], 4)) -
-                 pow(v[0], 4) * pow(v[1], 2) * pow(v[2], 4)) -
-                pow(v[0], 4) * pow(v[1], 4) * pow(v[2], 2));
-    }
-    return sol;
-}
-
-// heat source S_t - k lap S (heat.cpp:111-117)
-__device__ __forceinline__ double heat_source(const DevSys& s, double t, double x, double y, double z)
-{
-    return mms_ddt(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z) -
-           s.k * mms_lap(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z);
-}
-
-__device__ __forceinline__ double exact_at(const DevSys& s, double x, double y, double z, double t)
-{
-    return s.kind == SHX_SYS_HEAT ? mms_call(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z)
-                                  : wave_exact(s, x, y, z, t);
-}
-
-// Exact solution with the system family fixed at compile time, so that the
-// scalar-wave kernels carry no call into the manufactured solutions.
-template <int KIND>
-__device__ __forceinline__ double exact_of(const DevSys& s, double x, double y, double z, double t)
-{
-    if (KIND == SHX_SYS_HEAT) return mms_call(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z);
-    return wave_exact(s, x, y, z, t);
-}
-
-// Right-hand side at one point of D (interior stencil width W).
-template <int KIND, int W>
-__device__ __forceinline__ double rhs_point(const DevSys& s, const DevOps& ops,
-                                            const double* __restrict__ u, const RPtrs& uR,
-                                            long idx, int i, int j, int k, double t)
-{
-    if (KIND == SHX_SYS_HEAT) {
-        // laplacian: du = 0; du += O_x u; du += B_x u_Rx; then y, z (laplacian.cpp:37-46)
-        double acc = 0.0;
-        if (ops.d[0].present) {
-            const RowVal r = row_at<0, W, true>(ops.d[0], u, uR.r[0], idx, i, j, k, s.n1, s.n2);
-            if (r.row) acc = row_plus(acc, r);
-        }
-        if (ops.d[1].present) {
-            const RowVal r = row_at<1, W, true>(ops.d[1], u, uR.r[1], idx, i, j, k, s.n1, s.n2);
-            if (r.row) acc = row_plus(acc, r);
-        }
-        if (ops.d[2].present) {
-            const RowVal r = row_at<2, W, true>(ops.d[2], u, uR.r[2], idx, i, j, k, s.n1, s.n2);
-            if (r.row) acc = row_plus(acc, r);
-        }
-        acc = acc * s.k;
-        if (on_dirichlet_face(s, i, j, k)) return 0.0;
-        if (is_fluid(s, idx)) {
-            acc += heat_source(s, t, s.x[i], s.y[j], s.z[k]);
-        }
-        return acc;
-    } else {
-        double du[3] = {0.0, 0.0, 0.0};
-        if (ops.d[0].present) {
-            const RowVal r = row_at<0, W, true>(ops.d[0], u, uR.r[0], idx, i, j, k, s.n1, s.n2);
-            if (r.row) du[0] = row_eq(r);
-        }
-        if (ops.d[1].present) {
-            const RowVal r = row_at<1, W, true>(ops.d[1], u, uR.r[1], idx, i, j, k, s.n1, s.n2);
-            if (r.row) du[1] = row_eq(r);
-        }
-        if (ops.d[2].present) {
-            const RowVal r = row_at<2, W, true>(ops.d[2], u, uR.r[2], idx, i, j, k, s.n1, s.n2);
-            if (r.row) du[2] = row_eq(r);
-        }
-        double g[3] = {0.0, 0.0, 0.0};
-        if (!on_dirichlet_face(s, i, j, k)) {
-            const double x = s.x[i], y = s.y[j], z = s.z[k];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) g[c] = wave_neg_g(s, c, x, y, z);
-        }
-        return g[0] * du[0] + g[1] * du[1] + g[2] * du[2];
-    }
-}
-
-}  // namespace shx
 
 // --------------------------------------------------------------------- kernels
 namespace {
@@ -172,10 +34,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs(Tiling t, DevSys s, DevOps ops,
     rhs[idx] = rhs_point<KIND, W>(s, ops, u, uR, idx, i, j, k, time);
 }
 
-// One fused RK4 stage over three registers (rk4.cpp:27-46 for one point):
-//   rhs = RHS(us) at t_s;  acc' = acc + c_acc * rhs  (acc = 0 on stage 0)
-//   stages 0-2: out = u0 + c_next * rhs, Dirichlet faces take exact(t_next)
-//   stage 3:    out = u0 + 1.0 * acc'  (out may alias u0), faces exact(t_next)
+// One fused RK4 stage over three registers, point-wise tiles (stage_point).
 template <int KIND, int STAGE, int W>
 __global__ __launch_bounds__(kBlock) void k_rk4_stage(Tiling t, DevSys s, DevOps ops,
                                                       const double* __restrict__ us, RPtrs usR,
@@ -186,20 +45,8 @@ __global__ __launch_bounds__(kBlock) void k_rk4_stage(Tiling t, DevSys s, DevOps
     int i, j, k;
     long idx;
     if (!tile_point(t, i, j, k, idx)) return;
-    const double rhs = rhs_point<KIND, W>(s, ops, us, usR, idx, i, j, k, t_s);
-    const double a = (STAGE == 0 ? 0.0 : acc[idx]) + c_acc * rhs;
-    const double base = u0[idx];
-    double v;
-    if (STAGE < 3) {
-        acc[idx] = a;
-        v = base + c_next * rhs;
-    } else {
-        v = base + 1.0 * a;
-    }
-    if (on_dirichlet_face(s, i, j, k)) {
-        v = exact_of<KIND>(s, s.x[i], s.y[j], s.z[k], t_next);
-    }
-    out[idx] = v;
+    stage_point<KIND, STAGE, W>(s, ops, us, usR, u0, acc, out, idx, i, j, k, t_s, c_acc, c_next,
+                                t_next);
 }
 
 // Dirichlet cut points take the exact solution (update_boundary on R).
@@ -423,8 +270,17 @@ struct shx_system {
     long* part_idx = nullptr;
     // fused-stage kernel choice: 0 point-wise tiles, 1 x-marching tiles
     int variant = 0;
-    int march_ty = 4;
-    int march_xchunk = 32;
+    int march_ty = 8;
+    int march_xchunk = 64;
+    // x-marching form: class byte per point of D, the special-point records
+    // and the per-direction special-row tables
+    unsigned char* cls = nullptr;
+    SpecialRec* special = nullptr;
+    SpecialRow* srows[3] = {nullptr, nullptr, nullptr};
+    MarchPattern* mpat = nullptr;
+    long nspecial = 0;
+    long npatterns = 0;
+    bool classes_ready = false;
     int width = 5;  // interior stencil width shared by the three derivatives
 };
 
@@ -553,15 +409,18 @@ int rhs_fused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double t
 {
     const Tiling& tl = s->tiling;
     hipStream_t st = s->ctx->stream;
-    const bool heat = s->kind == SHX_SYS_HEAT;
-    if (heat && s->width == 3)
-        k_rhs<SHX_SYS_HEAT, 3><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
-    else if (heat)
-        k_rhs<SHX_SYS_HEAT, 5><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
-    else if (s->width == 3)
-        k_rhs<SHX_SYS_WAVE_LINEAR, 3><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
-    else
-        k_rhs<SHX_SYS_WAVE_LINEAR, 5><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t);
+    switch (s->kind * 10 + s->width) {
+#define SHX_RHS_CASE(K, W)                                                                      \
+    case K * 10 + W:                                                                            \
+        k_rhs<K, W><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t); \
+        break;
+        SHX_RHS_CASE(SHX_SYS_HEAT, 3) SHX_RHS_CASE(SHX_SYS_HEAT, 5)
+        SHX_RHS_CASE(SHX_SYS_WAVE_LINEAR, 3) SHX_RHS_CASE(SHX_SYS_WAVE_LINEAR, 5)
+        SHX_RHS_CASE(SHX_SYS_WAVE_RADIAL, 3) SHX_RHS_CASE(SHX_SYS_WAVE_RADIAL, 5)
+#undef SHX_RHS_CASE
+    default:
+        return fail(s->ctx, SHX_EUNSUPPORTED, "rhs: interior width must be 3 or 5");
+    }
     int rc = check(s->ctx, "rhs kernel");
     // every body is Dirichlet: the cut-point rows of the rhs are zero
     for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = launch_fill(s->ctx, rhs->R[c], 0.0, s->nR[c]);
@@ -697,44 +556,255 @@ int launch_stage(shx_system* s, int stage, const double* usD, RPtrs usR, const d
     return check(s->ctx, "rk4 stage kernel");
 }
 
-// The x-marching stage kernel handles the scalar wave with 5-point interior
-// stencils (the 4th-order schemes); anything else uses the point-wise tiles.
+// The x-marching form lists special points with 32-bit indices.
 bool march_ok(const shx_system* s)
 {
-    if (s->kind == SHX_SYS_HEAT) return false;
-    for (int c = 0; c < 3; ++c)
-        if (s->ops.d[c].present && s->ops.d[c].width != 5) return false;
-    return true;
+    return (s->width == 3 || s->width == 5) && s->N < (1L << 32);
 }
 
-int launch_march(shx_system* s, int stage, const double* usD, RPtrs usR, const double* u0,
-                 double* acc, double* out, double c_acc, double c_next, double tn)
+template <typename T>
+int upload_vec(shx_ctx* ctx, T** dst, const std::vector<T>& v)
+{
+    *dst = nullptr;
+    if (v.empty()) return SHX_OK;
+    hipError_t e = hipMalloc((void**)dst, v.size() * sizeof(T));
+    if (e == hipSuccess) e = hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "special-point upload");
+}
+
+// Class byte of every point and the special-point records (built once).
+int ensure_classes(shx_system* s)
+{
+    if (s->classes_ready) return SHX_OK;
+    if (!march_ok(s)) return fail(s->ctx, SHX_EUNSUPPORTED, "x-marching form: unsupported system");
+    hipError_t e = hipMalloc((void**)&s->cls, (size_t)s->N);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "classes alloc");
+    launch_classify(s->ctx->stream, s->dev, s->ops, s->cls, s->N);
+    int rc = check(s->ctx, "classify kernel");
+    std::vector<unsigned char> h((size_t)s->N);
+    if (rc == SHX_OK) {
+        e = hipMemcpyAsync(h.data(), s->cls, (size_t)s->N, hipMemcpyDeviceToHost, s->ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(s->ctx, e, "classes d2h");
+    }
+    if (rc == SHX_OK) rc = sync(s->ctx);
+    if (rc != SHX_OK) return rc;
+    // Raw special points become pattern points of the x-marching kernel when
+    // their rows allow it, or special points with a record; rows other than
+    // interior ones go to a per-direction table, deduplicated (grid-wall
+    // closures repeat from line to line).
+    std::vector<SpecialRec> recs;
+    std::vector<SpecialRow> rows[3];
+    std::map<std::array<int, 8>, unsigned> seen[3];
+    std::vector<MarchPattern> pats;
+    std::map<std::array<int, 6>, int> pat_of;
+    const std::vector<double>* ypool = s->d[1] ? &s->d[1]->h_pool : nullptr;
+    const std::vector<double>* zpool = s->d[2] ? &s->d[2]->h_pool : nullptr;
+    const int n1 = s->n[1], n2 = s->n[2];
+    const int hw = s->width / 2;
+    // 0 = no row, 1 = interior row, 2 = other row (e filled); the segment
+    // holding row q is found as the device slow path finds it
+    auto find_row = [&](int c, int pair, int q, SpecialRow& e) -> int {
+        const shx_derivative* d = s->d[c];
+        for (int b = d->h_pair_ptr[pair]; b < d->h_pair_ptr[pair + 1]; ++b) {
+            const shx_line_block& blk = d->h_blocks[b];
+            const int lr = q - blk.lo;
+            if (lr < 0) break;
+            const int total = blk.nl + blk.ni + blk.nr;
+            if (lr >= total) continue;
+            if (lr >= blk.nl && lr < blk.nl + blk.ni) return 1;
+            e = SpecialRow{};
+            if (lr < blk.nl) {
+                e.off = blk.lcol0 - q;
+                e.ncols = blk.lcols;
+                e.coef = blk.lcoef + lr * blk.lcols;
+            } else {
+                const int rr = lr - blk.nl - blk.ni;
+                e.off = blk.rcol0 - q;
+                e.ncols = blk.rcols;
+                e.coef = blk.rcoef + rr * blk.rcols;
+            }
+            e.bobj0 = e.bw0 = e.bobj1 = e.bw1 = -1;
+            if (blk.lobj >= 0 && lr < blk.nl) {
+                e.bobj0 = blk.lobj;
+                e.bw0 = blk.lbw + lr;
+                e.nb = 1;
+            }
+            if (blk.robj >= 0 && lr >= total - blk.nr) {
+                if (e.nb == 0) {
+                    e.bobj0 = blk.robj;
+                    e.bw0 = blk.rbw + (lr - (total - blk.nr));
+                } else {
+                    e.bobj1 = blk.robj;
+                    e.bw1 = blk.rbw + (lr - (total - blk.nr));
+                }
+                e.nb++;
+            }
+            return 2;
+        }
+        return 0;
+    };
+    // a closure row without B terms whose columns [q+off, q+off+n) lie in
+    // the LDS tile of every tile height (the 4-row tile is the smallest)
+    auto in_tile = [&](const SpecialRow& e, int q, int tile, int n) {
+        const int lo = std::max(0, (q / tile) * tile - hw);
+        const int hi = std::min(n, (q / tile) * tile + tile + hw);
+        return e.nb == 0 && e.ncols >= 1 && e.ncols <= kMaxPatternCols && q + e.off >= lo &&
+               q + e.off + e.ncols <= hi;
+    };
+    for (long idx = 0; idx < s->N; ++idx) {
+        const unsigned char raw = h[(size_t)idx];
+        if (!(raw & kRawSpecial)) continue;
+        const int i = (int)(idx / ((long)n1 * n2));
+        const int rem = (int)(idx - (long)i * n1 * n2);
+        const int j = rem / n2, k = rem - (rem / n2) * n2;
+        int kind[3] = {1, 1, 1};
+        SpecialRow e[3];
+        for (int c = 0; c < 3; ++c) {
+            if (!s->d[c] || !s->ops.d[c].present) continue;
+            const int pair = c == 0 ? j * n2 + k : c == 1 ? i * n2 + k : i * n1 + j;
+            const int q = c == 0 ? i : c == 1 ? j : k;
+            kind[c] = find_row(c, pair, q, e[c]);
+        }
+        const bool eligible = !(raw & kRawFace) && (raw & kRawFluid) && kind[0] == 1 &&
+                              kind[1] != 0 && kind[2] != 0 &&
+                              (kind[1] == 1 || in_tile(e[1], j, 4, n1)) &&
+                              (kind[2] == 1 || in_tile(e[2], k, 64, n2));
+        if (eligible) {
+            MarchPattern pt{};
+            std::array<int, 6> key{0, 0, 0, 0, 0, 0};
+            if (kind[1] == 2) {
+                pt.yoff = e[1].off;
+                pt.yn = e[1].ncols;
+                for (int m = 0; m < pt.yn; ++m) pt.wy[m] = (*ypool)[(size_t)e[1].coef + m];
+                key[0] = pt.yoff;
+                key[1] = pt.yn;
+                key[2] = e[1].coef;
+            }
+            if (kind[2] == 2) {
+                pt.zoff = e[2].off;
+                pt.zn = e[2].ncols;
+                for (int m = 0; m < pt.zn; ++m) pt.wz[m] = (*zpool)[(size_t)e[2].coef + m];
+                key[3] = pt.zoff;
+                key[4] = pt.zn;
+                key[5] = e[2].coef;
+            }
+            auto it = pat_of.find(key);
+            if (it == pat_of.end() && (int)pats.size() < kMaxPatterns) {
+                it = pat_of.emplace(key, (int)pats.size()).first;
+                pats.push_back(pt);
+            }
+            if (it != pat_of.end()) {
+                h[(size_t)idx] = (unsigned char)(kPattern0 + it->second);
+                continue;
+            }
+        }
+        h[(size_t)idx] = kSpecial;
+        SpecialRec r;
+        r.idx = (unsigned)idx;
+        for (int c = 0; c < 3; ++c) {
+            r.ref[c] = kind[c] == 1 ? kRowInterior : kRowNone;
+            if (kind[c] != 2) continue;
+            const SpecialRow& x = e[c];
+            const std::array<int, 8> key{x.off, x.ncols, x.coef, x.nb, x.bobj0, x.bw0, x.bobj1, x.bw1};
+            auto it = seen[c].find(key);
+            if (it == seen[c].end()) {
+                it = seen[c].emplace(key, (unsigned)rows[c].size()).first;
+                rows[c].push_back(x);
+            }
+            r.ref[c] = it->second;
+        }
+        if (!s->d[0] || !s->ops.d[0].present) r.ref[0] = kRowNone;
+        if (!s->d[1] || !s->ops.d[1].present) r.ref[1] = kRowNone;
+        if (!s->d[2] || !s->ops.d[2].present) r.ref[2] = kRowNone;
+        recs.push_back(r);
+    }
+    s->nspecial = (long)recs.size();
+    s->npatterns = (long)pats.size();
+    e = hipMemcpy(s->cls, h.data(), (size_t)s->N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "classes h2d");
+    int rc2 = upload_vec(s->ctx, &s->special, recs);
+    if (rc2 == SHX_OK) rc2 = upload_vec(s->ctx, &s->mpat, pats);
+    for (int c = 0; c < 3 && rc2 == SHX_OK; ++c) rc2 = upload_vec(s->ctx, &s->srows[c], rows[c]);
+    if (rc2 != SHX_OK) return rc2;
+    s->classes_ready = true;
+    return SHX_OK;
+}
+
+// Main kernel of the x-marching form: interior and no-row points.
+int launch_march(shx_system* s, int stage, const double* usD, const double* u0, double* acc,
+                 double* out, double ts, double c_acc, double c_next)
 {
     MarchArgs a;
     a.s = s->dev;
     a.ops = s->ops;
     a.us = usD;
-    for (int c = 0; c < 3; ++c) a.usR[c] = usR.r[c];
     a.u0 = u0;
     a.acc = acc;
     a.out = out;
+    a.cls = s->cls;
+    a.pat = s->mpat;
+    a.npat = (int)s->npatterns;
+    a.t_s = ts;
     a.c_acc = c_acc;
     a.c_next = c_next;
-    a.t_next = tn;
     a.xchunk = s->march_xchunk;
     a.nbz = (s->n[2] + 63) / 64;
     a.nby = (s->n[1] + s->march_ty - 1) / s->march_ty;
-    const int nbx = (s->n[0] + a.xchunk - 1) / a.xchunk;
-    a.nblocks = (unsigned)a.nbz * (unsigned)a.nby * (unsigned)nbx;
-    if (launch_wave_stage_march(s->ctx->stream, stage, s->march_ty, a) != 0)
-        return fail(s->ctx, SHX_EINVAL, "march: unsupported tile height");
+    const long nbx = (s->n[0] + a.xchunk - 1) / a.xchunk;
+    const long nb = (long)a.nbz * a.nby * nbx;
+    if (nb >= (1L << 31)) return fail(s->ctx, SHX_EUNSUPPORTED, "march: too many workgroups");
+    a.nblocks = (unsigned)nb;
+    if (launch_stage_march(s->ctx->stream, s->kind, stage, s->width, s->march_ty, a) != 0)
+        return fail(s->ctx, SHX_EINVAL, "march: unsupported tile height or stencil width");
     return check(s->ctx, "rk4 march stage kernel");
 }
 
-// One fused step; when ev is given, event pairs bracket each stage kernel.
+// Second kernel of the x-marching form: listed points and Dirichlet cut points.
+int launch_special(shx_system* s, int stage, const double* usD, RPtrs usR, const double* u0,
+                   double* acc, double* out, double* const outR[3], double ts, double c_acc,
+                   double c_next, double tn)
+{
+    SpecialArgs a;
+    a.s = s->dev;
+    a.ops = s->ops;
+    a.us = usD;
+    a.usR = usR;
+    a.u0 = u0;
+    a.acc = acc;
+    a.out = out;
+    a.recs = s->special;
+    a.rows0 = s->srows[0];
+    a.rows1 = s->srows[1];
+    a.rows2 = s->srows[2];
+    a.nspecial = s->nspecial;
+    a.nR0 = s->nR[0];
+    a.nR1 = s->nR[1];
+    a.nR2 = s->nR[2];
+    a.pos0 = s->rpos[0];
+    a.pos1 = s->rpos[1];
+    a.pos2 = s->rpos[2];
+    a.dir0 = s->rdir[0];
+    a.dir1 = s->rdir[1];
+    a.dir2 = s->rdir[2];
+    a.outR0 = outR[0];
+    a.outR1 = outR[1];
+    a.outR2 = outR[2];
+    a.t_s = ts;
+    a.c_acc = c_acc;
+    a.c_next = c_next;
+    a.t_next = tn;
+    if (launch_stage_special(s->ctx->stream, s->kind, stage, s->width, a) != 0)
+        return fail(s->ctx, SHX_EINVAL, "special points: unsupported stencil width");
+    return check(s->ctx, "rk4 special-point kernel");
+}
+
+// One fused step; when ev is given, event pairs bracket each stage's main
+// kernel (the point-wise tiles, or the x-marching kernel).
 int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t* ev)
 {
     int rc = ensure_fused(s);
+    const bool march = s->variant == 1;
+    if (rc == SHX_OK && march) rc = ensure_classes(s);
     for (int stage = 0; stage < 4 && rc == SHX_OK; ++stage) {
         const double* usD = stage == 0 ? u0->D : s->stageD[(stage - 1) & 1];
         RPtrs usR;
@@ -747,14 +817,20 @@ int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t
         const double c_next = stage < 3 ? dt * kRki[stage + 1] : 1.0;
         const double tn = stage < 3 ? t + dt * kRki[stage + 1] : t + dt;
         if (ev) hipEventRecord(ev[2 * stage], s->ctx->stream);
-        if (s->kind == SHX_SYS_HEAT)
+        if (march)
+            rc = launch_march(s, stage, usD, u0->D, s->accD, outD, ts, c_acc, c_next);
+        else if (s->kind == SHX_SYS_HEAT)
             rc = launch_stage<SHX_SYS_HEAT>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
-        else if (s->variant == 1 && march_ok(s))
-            rc = launch_march(s, stage, usD, usR, u0->D, s->accD, outD, c_acc, c_next, tn);
+        else if (s->kind == SHX_SYS_WAVE_RADIAL)
+            rc = launch_stage<SHX_SYS_WAVE_RADIAL>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
         else
             rc = launch_stage<SHX_SYS_WAVE_LINEAR>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
         if (ev) hipEventRecord(ev[2 * stage + 1], s->ctx->stream);
-        if (rc == SHX_OK) rc = r_exact_all(s, outR, tn);
+        if (rc != SHX_OK) break;
+        if (march)
+            rc = launch_special(s, stage, usD, usR, u0->D, s->accD, outD, outR, ts, c_acc, c_next, tn);
+        else
+            rc = r_exact_all(s, outR, tn);
     }
     return rc;
 }
@@ -900,6 +976,10 @@ int shx_system_destroy(shx_system* s)
     hipFree(s->stageD[0]);
     hipFree(s->stageD[1]);
     hipFree(s->accD);
+    hipFree(s->cls);
+    hipFree(s->special);
+    for (int c = 0; c < 3; ++c) hipFree(s->srows[c]);
+    hipFree(s->mpat);
     hipFree(s->srcD);
     hipFree(s->mms);
     hipFree(s->fluid_mask);
@@ -1085,12 +1165,27 @@ int shx_time_fused_stage_kernel(shx_system* s, shx_field* u0, double t, double d
 int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xchunk)
 {
     if (!s || variant < 0 || variant > 1) return SHX_EINVAL;
-    if (variant == 1 && !(march_ty == 2 || march_ty == 4 || march_ty == 8)) return SHX_EINVAL;
+    if (variant == 1 && !(march_ty == 4 || march_ty == 8 || march_ty == 16)) return SHX_EINVAL;
     if (variant == 1 && march_xchunk < 1) return SHX_EINVAL;
     s->variant = variant;
     if (variant == 1) {
         s->march_ty = march_ty;
         s->march_xchunk = march_xchunk;
+    }
+    return SHX_OK;
+}
+
+int shx_system_stage_points(shx_system* s, int64_t out[2])
+{
+    if (!s || !out) return SHX_EINVAL;
+    if (s->variant == 1) {
+        const int rc = ensure_classes(s);
+        if (rc != SHX_OK) return rc;
+        out[0] = s->N - s->nspecial;
+        out[1] = s->nspecial;
+    } else {
+        out[0] = s->N;
+        out[1] = 0;
     }
     return SHX_OK;
 }
@@ -1126,7 +1221,7 @@ int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt,
     };
     std::vector<cand> cands{{0, 4, 32}};
     if (march_ok(s))
-        for (int ty : {2, 4, 8})
+        for (int ty : {4, 8, 16})
             for (int xc : {16, 32, 64, 128, 1 << 16}) cands.push_back({1, ty, std::min(xc, s->n[0])});
     hipEvent_t e0, e1;
     hipEventCreate(&e0);

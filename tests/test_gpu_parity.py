@@ -154,15 +154,18 @@ def test_rk4_steps(ctx, name, fused):
     assert err <= 1e-11, f"{name}: relative max-norm error {err:.3e}"
 
 
-@pytest.mark.parametrize("name", ["wave48_E4_1", "wave30_radial_E4u"])
-@pytest.mark.parametrize("ty,xchunk", [(2, 16), (4, 7), (8, 64), (4, 1000)])
+@pytest.mark.parametrize("name", ["wave48_E4_1", "wave30_radial_E4u", "wave20_E4u_overlap",
+                                  "heat3d_E4_2", "heat3d_E2_2", "heat2d_21"])
+@pytest.mark.parametrize("ty,xchunk", [(4, 16), (8, 7), (16, 64), (8, 1000)])
 def test_march_kernel_bitexact(ctx, name, ty, xchunk):
-    """The x-marching fused stage kernel gives the same doubles as the
-    point-wise one (same operation order), over several RK4 steps."""
-    cfg = WAVE_CFGS[name]
+    """The x-marching form (interior and solid points in registers and LDS,
+    special points from a list) gives the same doubles as the point-wise
+    tiles (same operation order), over several RK4 steps, D and cut points."""
+    cfg = ALL[name]
     osys, prob = _pair(ctx, cfg)
     u0 = osys.field()
     osys.initialize(u0, 0.0)
+    osys.update_boundary(u0, 0.0)
     dt = osys.dt(0.5, 1.0)
     a, b = prob.field(), prob.field()
     a.upload(u0.flat())
@@ -171,7 +174,21 @@ def test_march_kernel_bitexact(ctx, name, ty, xchunk):
     prob.rk4_fused(a, 0.0, dt, 3)
     prob.set_tuning(1, ty, xchunk)
     prob.rk4_fused(b, 0.0, dt, 3)
-    assert np.array_equal(a.download(), b.download())
+    ga, gb = a.download(), b.download()
+    assert np.array_equal(ga, gb), f"max diff {np.max(np.abs(ga - gb))}"
+
+
+@pytest.mark.parametrize("name", ["wave48_E4_1", "heat3d_E4_2"])
+def test_march_point_classes(ctx, name):
+    """Every point of D is updated by exactly one of the two kernels, and the
+    special list holds the faces and closure rows."""
+    osys, prob = _pair(ctx, ALL[name])
+    prob.set_tuning(1)
+    main, special = prob.stage_points()
+    assert main + special == prob.N
+    assert 0 < special < prob.N
+    prob.set_tuning(0)
+    assert prob.stage_points() == (prob.N, 0)
 
 
 def test_autotune_keeps_state(ctx):
@@ -212,3 +229,38 @@ def test_stats_match_oracle(ctx):
     got = prob.stats(gu, 0.1)
     assert got[4] == want[4]  # location of the max error
     np.testing.assert_allclose(got[:4], want[:4], rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [1, 2, 1001, 1 << 16])
+def test_copy(ctx, n):
+    """shx_copy moves every double, including the odd tail."""
+    import ctypes as C
+    from shoccs_amd._lib import lib
+
+    L = lib()
+    a, b = C.c_void_p(), C.c_void_p()
+    assert L.shx_alloc(ctx.p, 8 * n, C.byref(a)) == 0
+    assert L.shx_alloc(ctx.p, 8 * n, C.byref(b)) == 0
+    src = np.random.default_rng(n).standard_normal(n)
+    assert L.shx_h2d(ctx.p, a, src.ctypes.data_as(C.c_void_p), 8 * n) == 0
+    assert L.shx_fill(ctx.p, b, -1.0, n) == 0
+    assert L.shx_copy(ctx.p, b, a, n) == 0
+    out = np.empty(n)
+    assert L.shx_d2h(ctx.p, out.ctypes.data_as(C.c_void_p), b, 8 * n) == 0
+    L.shx_free(ctx.p, a)
+    L.shx_free(ctx.p, b)
+    assert np.array_equal(out, src)
+
+
+def test_context_close_releases_problems():
+    """Closing a context first releases the problems and fields made on it,
+    so that objects outliving the context (a failing test's traceback, for
+    one) never reach a finalized context."""
+    import shoccs_amd as S
+
+    c = S.Context(0)
+    prob = H.product_problem(c, HEAT_CFGS["heat3d_E4_2"])
+    f = prob.field()
+    c.close()
+    assert prob.p is None and f.c is None
+    del f, prob
