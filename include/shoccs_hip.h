@@ -223,7 +223,8 @@ int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt,
  * for interior and solid points, and evaluates the remaining points
  * (closure rows, B terms, Dirichlet faces) from a list built once, in a
  * second kernel that also updates the Dirichlet cut points.  Both variants
- * give the same doubles. */
+ * give the same doubles.  A new system starts on variant 1 (march_ty 8,
+ * march_xchunk 16) when its stencil width is 3 or 5 and N < 2^32. */
 int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xchunk);
 int shx_system_get_tuning(const shx_system* s, int out[3]);
 /* Points of D updated per stage by the main fused-stage kernel (out[0]) and

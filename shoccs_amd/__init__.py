@@ -310,7 +310,7 @@ class Problem:
               "rk4_fused")
         self.ctx.sync()
 
-    def set_tuning(self, variant: int, march_ty: int = 8, march_xchunk: int = 64):
+    def set_tuning(self, variant: int, march_ty: int = 8, march_xchunk: int = 16):
         """Fused-stage kernel: 0 point-wise tiles, 1 x-marching tiles."""
         self._need_device()
         check(lib().shx_system_set_tuning(self.sys, variant, march_ty, march_xchunk), self.ctx.p,

@@ -196,8 +196,9 @@ __global__ __launch_bounds__(256) void k_stats(DevSys s, long n, const int* __re
         point_coords(idx, s.n1, s.n2, i, j, k);
         const double v = u[idx];
         const double err = fabs(v - exact_at(s, s.x[i], s.y[j], s.z[k], t));
-        // a non-finite error wins and stays, so that system::valid() sees it
-        if ((isnan(err) && !isnan(e)) || err > e || (err == e && idx < ei)) {
+        // strict comparisons, as the reference's MaxLoc: a NaN error never
+        // wins (scalar_system_utils.hpp:102-113); ties keep the first point
+        if (err > e || (err == e && idx < ei)) {
             e = err;
             ei = idx;
         }
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(256) void k_stats(DevSys s, long n, const int* __re
         if ((int)threadIdx.x < w) {
             const int o = threadIdx.x + w;
             const double a = se[o], b = se[threadIdx.x];
-            if ((isnan(a) && !isnan(b)) || a > b || (a == b && si[o] < si[threadIdx.x])) {
+            if (a > b || (a == b && si[o] < si[threadIdx.x])) {
                 se[threadIdx.x] = se[o];
                 si[threadIdx.x] = si[o];
             }
@@ -271,7 +272,7 @@ struct shx_system {
     // fused-stage kernel choice: 0 point-wise tiles, 1 x-marching tiles
     int variant = 0;
     int march_ty = 8;
-    int march_xchunk = 64;
+    int march_xchunk = 16;
     // x-marching form: class byte per point of D, the special-point records
     // and the per-direction special-row tables
     unsigned char* cls = nullptr;
@@ -954,6 +955,8 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
         }
     }
     if (s->width == 0) s->width = 5;
+    // the x-marching form is the default where it applies
+    s->variant = march_ok(s) ? 1 : 0;
     *out = s;
     return SHX_OK;
 }
@@ -1060,7 +1063,7 @@ int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[1
         double err = -DBL_MAX, mn = DBL_MAX, mx = -DBL_MAX;
         long ei = 0;
         for (unsigned b = 0; b < grid; ++b) {
-            if ((std::isnan(h[b]) && !std::isnan(err)) || h[b] > err || (h[b] == err && hi[b] < ei)) {
+            if (h[b] > err || (h[b] == err && hi[b] < ei)) {
                 err = h[b];
                 ei = hi[b];
             }

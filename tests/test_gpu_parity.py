@@ -203,18 +203,25 @@ def test_autotune_keeps_state(ctx):
     assert np.array_equal(g.download(), u0.flat())
 
 
-def test_stats_flag_non_finite(ctx):
-    """A NaN injected into the solution must make the L-inf error non-finite,
-    so that system::valid() (heat.cpp:54-58, scalar_wave.cpp:108-112) stops."""
+def test_stats_ignore_nan_like_the_reference(ctx):
+    """compute_scalar_stats keeps the first strict maximum (MaxLoc with `e >
+    val`, scalar_system_utils.hpp:102-113): a NaN point never wins, and an
+    all-NaN field leaves the lowest double, which system::valid() rejects."""
     cfg = HEAT_CFGS["heat3d_E4_2"]
     osys, prob = _pair(ctx, cfg)
     u = osys.field()
     osys.initialize(u, 0.0)
-    flat = u.flat()
-    flat[osys.mesh.fluid()[123]] = np.nan
+    fl = osys.mesh.fluid()
+    u.D[fl[40]] += 0.125
+    u.D[fl[123]] = np.nan
     gu = prob.field()
-    gu.upload(flat)
-    assert not np.isfinite(prob.stats(gu, 0.0)[0])
+    gu.upload(u.flat())
+    got, want = prob.stats(gu, 0.0), osys.stats(u, 0.0)
+    assert got[0] == want[0] and got[4] == want[4] == fl[40]
+    u.D[fl] = np.nan
+    gu.upload(u.flat())
+    got, want = prob.stats(gu, 0.0), osys.stats(u, 0.0)
+    assert got[0] == want[0] == -np.finfo(np.float64).max
 
 
 def test_stats_match_oracle(ctx):

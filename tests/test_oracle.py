@@ -3,6 +3,7 @@ extracted from the reference's unit tests) and what the mathematics fixes
 (polynomial exactness of interior stencils, closures and whole operators,
 measured convergence orders, RK4 on a solution linear in time)."""
 import json
+import math
 import os
 
 import numpy as np
@@ -327,3 +328,86 @@ def test_system_cycle_heat_2d_config1_short():
         t += dt
     st = s.stats(u0, t)
     assert st[0] < 1e-5, st[0]
+
+
+def _box_wave(kind, grid_bc, n=(18, 20, 22), scheme=O.E4U_1, alpha=(-0.7733323791884821, 0.1623961700641681)):
+    a = np.array([0.3, -0.5, 0.8])
+    if kind == O.SYS_WAVE_LINEAR:
+        params = [a[0], a[1], a[2], 0.4, 0.5, 0.6, 0.1, 1.0]
+    else:
+        params = [-0.25, -0.3, -0.2, 0.1]
+    cfg = dict(kind=kind, n=n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), spheres=[], obj_bc=[],
+               grid_bc=grid_bc, scheme=scheme, alpha=alpha, params=params)
+    return cfg, a
+
+
+@pytest.mark.parametrize("kind", [O.SYS_WAVE_LINEAR, O.SYS_WAVE_RADIAL])
+def test_wave_rhs_closed_form_on_linear_field(kind):
+    """scalar_wave rhs = gG . grad u with gG = -a (constant velocity) or
+    -(x - c)/|x - c| (radial, scalar_wave.cpp:28-33), zeroed on Dirichlet
+    faces (scalar_wave.cpp:70-95).  On u = 1 + 2x - 3y + 0.5z every closure row
+    is exact, so the rhs has a closed form at every point; a sign, component
+    or face mistake changes it."""
+    grid_bc = [O.DIRICHLET, O.FLOATING, O.FLOATING, O.DIRICHLET, O.DIRICHLET, O.FLOATING]
+    cfg, a = _box_wave(kind, grid_bc)
+    s = H.oracle_system(cfg)
+    g = np.array([2.0, -3.0, 0.5])
+    u = _poly_field(s, lambda x, y, z: 1 + g[0] * x + g[1] * y + g[2] * z)
+    rhs = s.rhs(u, 0.37).D
+    n = np.array(cfg["n"])
+    ijk = H.unflat(np.arange(rhs.size), cfg["n"])
+    xyz = ijk / (n - 1.0)
+    if kind == O.SYS_WAVE_LINEAR:
+        want = np.full(rhs.size, -(a @ g))
+    else:
+        r = xyz - np.array(cfg["params"][:3])
+        want = -(r @ g) / np.linalg.norm(r, axis=1)
+    face = (ijk[:, 0] == 0) | (ijk[:, 1] == n[1] - 1) | (ijk[:, 2] == 0)
+    want[face] = 0.0
+    np.testing.assert_allclose(rhs, want, rtol=0, atol=1e-11)
+
+
+def test_rk4_step_is_the_fourth_order_taylor_polynomial():
+    """rk4.cpp:16-47 on an autonomous linear problem (constant-velocity wave,
+    floating faces, no bodies: rhs(u) = L u with no boundary data): one step
+    equals (I + dtL + (dtL)^2/2 + (dtL)^3/6 + (dtL)^4/24) u0, with L u computed
+    by the (separately pinned) rhs.  A wrong stage time or weight fails."""
+    cfg, _ = _box_wave(O.SYS_WAVE_LINEAR, [O.FLOATING] * 6, n=(14, 16, 18))
+    s = H.oracle_system(cfg)
+    u0 = s.field()
+    rng = np.random.default_rng(11)
+    u0.D[:] = rng.standard_normal(u0.D.size)
+    dt = 0.9 * s.dt(0.5, 1.0)
+    terms = [u0.D.copy()]
+    cur = u0
+    for _ in range(4):
+        nxt = s.rhs(cur, 0.0)
+        cur = s.field()
+        cur.D[:] = nxt.D
+        terms.append(cur.D.copy())
+    want = sum(terms[k] * dt ** k / math.factorial(k) for k in range(5))
+    out, rk, srhs = s.field(), s.field(), s.field()
+    s.rk4_step(u0, out, rk, srhs, 0.0, dt)
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(out.D - want)) <= 1e-13 * scale
+    # and the five terms do not collapse (the check has teeth)
+    assert np.max(np.abs(terms[4])) * dt ** 4 / 24 > 1e-6 * scale
+
+
+def test_stats_over_fluid_points():
+    """compute_scalar_stats (scalar_system_utils.hpp:77-181): the L-inf error
+    is taken over fluid points only, the first strict maximum wins, and a NaN
+    point never wins (MaxLoc with `e > val`, :102-113)."""
+    cfg = H.sphere_wave_cfg(24, scheme=O.E4U_1, alpha=(0.1, 0.2))
+    s = H.oracle_system(cfg)
+    u = s.exact(0.1)
+    fl = s.mesh.fluid()
+    solid = np.setdiff1d(np.arange(u.D.size), fl)
+    u.D[solid] = 1e6  # solid points are ignored
+    u.D[fl[7]] += 0.25
+    st = s.stats(u, 0.1)
+    assert abs(st[0] - 0.25) < 1e-15 and int(st[4]) == fl[7]
+    u.D[fl[8]] = u.D[fl[8]] + 0.25  # a tie later in fluid order loses
+    u.D[fl[9]] = np.nan
+    st = s.stats(u, 0.1)
+    assert abs(st[0] - 0.25) < 1e-15 and int(st[4]) == fl[7]
