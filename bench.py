@@ -11,9 +11,9 @@ HBM when the timed region starts.
 
 Forms measured:
   * fused   -- RHS + stage update + boundary update per RK4 stage over three
-               registers: one x-marching kernel for interior, solid and
-               wall-closure points and one kernel for the remaining listed
-               points and the cut points (the `value`);
+               registers, one launch per stage: x-marching tiles for
+               interior, solid and wall-closure points, and workgroups for
+               the remaining listed points and the cut points (the `value`);
   * plain   -- the reference's schedule, one kernel per reference operation
                (zero fills, block + full-length CSR per direction, dot, slot
                ops, boundary update, u0 <- u1 copy).
@@ -170,8 +170,8 @@ def main():
     t_end = t_start + args.steps * dt
     stats = prob.stats(u0, t_end)
 
-    # ---- dominant kernel: the main fused stage kernel, event-timed per launch
-    # on the context's stream, priced over the points it updates
+    # ---- dominant kernel: the fused stage launch, event-timed per launch on
+    # the context's stream, priced over the points it updates
     ms_kernel = prob.time_fused_stage_kernel(u0, t_end, dt, 5)
     main_pts, special_pts = prob.stage_points()
     algo_bytes_launch = FUSED_BYTES_PER_PT_STEP / 4.0 * main_pts
@@ -210,19 +210,23 @@ def main():
         "config": {
             "workload": f"3-D scalar wave u_t + a.grad u = 0, {args.n}^3, Dirichlet sphere r=0.3, "
                         "E4u cut-cell (alpha -0.7733, 0.1624), gaussian pulse, dt = 0.5h, RK4",
-            "form": "fused FP64 over three registers: per RK4 stage one x-marching kernel "
-                    "(interior, solid and wall-closure points) and one special-point kernel "
-                    "(remaining closure rows, faces, cut points)",
+            "form": "fused FP64 over three registers: one launch per RK4 stage (x-marching "
+                    "tiles for interior, solid and wall-closure points; special-point "
+                    "workgroups for the remaining closure rows, faces and cut points)",
             "parallelism": "1 GPU",
-            "tuning": {"variant": ["point-wise 64x4 tiles", "x-marching tiles"][tuning[0]],
+            "tuning": {"variant": ["point-wise 64x4 tiles",
+                                   "x-marching tiles, special points in a second launch",
+                                   "x-marching tiles, special workgroups first",
+                                   "x-marching tiles, special workgroups last",
+                                   "x-marching tiles, special workgroups spread"][tuning[0]],
                        "march_ty": tuning[1], "march_xchunk": tuning[2],
                        "autotune_ms_per_step": tuning[3]},
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
-                     "kernel": ["k_rk4_stage (point-wise tiles)",
-                                "k_stage_march (x-marching tiles)"][tuning[0]],
+                     "kernel": "k_rk4_stage (point-wise tiles)" if tuning[0] == 0 else
+                               "k_stage (x-marching tiles + special-point workgroups)",
                      "algorithmic_bytes_per_launch": algo_bytes_launch,
                      "points_per_launch": main_pts,
                      "special_points_per_stage": special_pts,

@@ -218,18 +218,21 @@ int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt,
 
 /* Fused-stage kernel choice (no reference counterpart; the reference has one
  * Kokkos TeamPolicy shape, src/matrices/block.hpp:185-194): variant 0 is the
- * point-wise kernel on 64 x 4 tiles; variant 1 marches tiles of march_ty x 64
- * points (march_ty = 4, 8 or 16) along x over chunks of march_xchunk planes
- * for interior and solid points, and evaluates the remaining points
- * (closure rows, B terms, Dirichlet faces) from a list built once, in a
- * second kernel that also updates the Dirichlet cut points.  Both variants
- * give the same doubles.  A new system starts on variant 1 (march_ty 8,
- * march_xchunk 16) when its stencil width is 3 or 5 and N < 2^32. */
+ * point-wise kernel on 64 x 4 tiles.  Variants 1-4 march tiles of
+ * march_ty x 64 points (march_ty = 4, 8 or 16) along x over chunks of
+ * march_xchunk planes for interior, solid and wall-closure points, while
+ * special workgroups evaluate the remaining points (closure rows with B
+ * terms, closures out of the tile's reach, Dirichlet faces) from records
+ * built once, and update the Dirichlet cut points: in a second launch (1),
+ * or in the same launch before the tiles (2), after them (3) or spread among
+ * them (4).  All variants give the same doubles.  A new system starts on
+ * variant 3 (march_ty 4, march_xchunk 16) when its stencil width is 3 or 5
+ * and N < 2^32. */
 int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xchunk);
 int shx_system_get_tuning(const shx_system* s, int out[3]);
-/* Points of D updated per stage by the main fused-stage kernel (out[0]) and
- * by the special-point kernel (out[1], 0 for variant 0) of the current
- * variant; builds the point classes of variant 1 if needed. */
+/* Points of D updated per stage by the timed fused-stage launch (out[0]) and,
+ * among them, points evaluated from special-point records (out[1], 0 for
+ * variant 0); builds the point classes of variant 1 if needed. */
 int shx_system_stage_points(shx_system* s, int64_t out[2]);
 /* Autotuner: times every candidate on a copy of u0 (u0 is not modified),
  * keeps the fastest and returns its ms per step. */

@@ -438,8 +438,11 @@ void orc_system_stats(orc_system* s, const orc_field* u, double t, double st[11]
     orc_field sol;
     field_alloc(m, &sol);
     orc_system_exact(s, &sol, t);
+    /* MaxLoc starts from its identity (lowest value, largest int index), as
+     * Kokkos 5.1 MaxLoc::init does; strict `>` keeps the first maximum and
+     * never takes a NaN */
     double umin = DBL_MAX, umax = -DBL_MAX, err_d = -DBL_MAX;
-    long err_idx = 0;
+    long err_idx = 2147483647L;
     for (long q = 0; q < m->nfluid; q++) {
         long i = m->fluid[q];
         if (u->D[i] < umin) umin = u->D[i];
@@ -454,7 +457,7 @@ void orc_system_stats(orc_system* s, const orc_field* u, double t, double st[11]
     double ce[3] = {0, 0, 0}, ci[3] = {0, 0, 0};
     for (int d = 0; d < 3; d++) {
         double best = -DBL_MAX;
-        long bi = 0;
+        long bi = 2147483647L;
         int any = 0;
         for (long i = 0; i < m->nR[d]; i++) {
             if (s->obj_bc[m->R[d][i].shape_id] == ORC_DIRICHLET) continue;

@@ -310,8 +310,10 @@ class Problem:
               "rk4_fused")
         self.ctx.sync()
 
-    def set_tuning(self, variant: int, march_ty: int = 8, march_xchunk: int = 16):
-        """Fused-stage kernel: 0 point-wise tiles, 1 x-marching tiles."""
+    def set_tuning(self, variant: int, march_ty: int = 4, march_xchunk: int = 16):
+        """Fused-stage kernel: 0 point-wise tiles; 1-4 x-marching tiles with the
+        special-point work in a second launch (1) or in the same launch before
+        (2), after (3) or among (4) the tiles."""
         self._need_device()
         check(lib().shx_system_set_tuning(self.sys, variant, march_ty, march_xchunk), self.ctx.p,
               "set_tuning")
@@ -322,8 +324,8 @@ class Problem:
         return tuple(out)
 
     def stage_points(self) -> tuple:
-        """(points of D updated by the main fused-stage kernel, points updated
-        by the special-point kernel) for the current variant."""
+        """(points of D updated by the timed fused-stage launch, points among
+        them evaluated from special-point records) for the current variant."""
         self._need_device()
         out = (C.c_int64 * 2)()
         check(lib().shx_system_stage_points(self.sys, out), self.ctx.p, "stage_points")

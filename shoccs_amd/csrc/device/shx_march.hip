@@ -3,7 +3,7 @@
 // Every point of D carries a class byte, computed once per system:
 //   * interior points (all rows circulant), points with no rows, and points
 //     whose y/z closure rows (no B terms) read only values of the LDS tile
-//     are handled by k_stage_march: a workgroup owns a (y, z) tile of TY x 64
+//     are handled by march tiles: a workgroup owns a (y, z) tile of TY x 64
 //     points and walks a chunk of x planes.  Each thread keeps the W
 //     x-neighbours of its point in registers (a queue shifted by one plane
 //     per step); the current plane is staged in LDS, double-buffered, with
@@ -13,8 +13,8 @@
 //   * the remaining points (closure rows, B terms, Dirichlet faces) are
 //     listed once, each with a record naming its row in every direction
 //     (interior, none, or an entry of a deduplicated row table), and
-//     evaluated by k_stage_special without segment-table scans; the same
-//     launch updates the Dirichlet cut points.
+//     evaluated without segment-table scans, together with the Dirichlet
+//     cut points, by the first workgroups of the same launch (k_stage).
 // Both kernels use the formulas of shx_point.hpp in the same order as the
 // point-wise tiles, so results are bit-identical to them.
 #include <hip/hip_runtime.h>
@@ -86,30 +86,30 @@ __global__ void k_classify(DevSys s, DevOps ops, unsigned char* __restrict__ cls
     cls[idx] = c;
 }
 
-// One y or z row of a pattern point from the LDS tile: the closure row of n
-// columns starting at offset off (weights w, in LDS).  `c0` is the tile slot
-// of the point, `step` the slot distance of one position along the direction.
-__device__ __forceinline__ double pattern_dot(int n, int off, const double* w, const double* c0,
-                                              int step)
+// A y or z closure row from the LDS tile: `c0` is the tile slot of the
+// point, `step` the slot distance of one position along the direction.
+__device__ __forceinline__ double tile_row(const ClosureRow& r, const double* c0, int step)
 {
     double s = 0.0;
 #pragma unroll 1
-    for (int m = 0; m < n; ++m) s += w[m] * c0[(off + m) * step];
+    for (int m = 0; m < r.n; ++m) s += r.w[m] * c0[(r.off + m) * step];
     return s;
 }
 
+// One march tile: logical workgroup L of the tile grid.
 template <int KIND, int STAGE, int W, int TY>
-__global__ __launch_bounds__(64 * TY) void k_stage_march(MarchArgs a)
+__device__ __forceinline__ void march_tile(const MarchArgs& a, unsigned L)
 {
     constexpr int TZ = 64;
     constexpr int hw = W / 2;
     constexpr int LY = TY + 2 * hw, LZ = TZ + 2 * hw;
+    constexpr int HQ = hw + kXReach;  // x queue half width
+    constexpr int QW = 2 * HQ + 1;    // x queue length
     static_assert(TY >= 2 * hw, "y halo rows are loaded by distinct threads");
     __shared__ double tile[2][LY][LZ];
-    __shared__ MarchPattern spat[kMaxPatterns];
+    __shared__ ClosureRow crow[3][kMaxClosureRows];
 
-    // workgroup -> (z tile, y tile, x chunk), contiguous per XCD, z fastest
-    const unsigned L = xcd_remap(blockIdx.x, a.nblocks);
+    // tile -> (z tile, y tile, x chunk), z fastest
     const int bz = (int)(L % (unsigned)a.nbz);
     const unsigned r1 = L / (unsigned)a.nbz;
     const int by = (int)(r1 % (unsigned)a.nby);
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(64 * TY) void k_stage_march(MarchArgs a)
     const long col = (long)jc * n2 + kc;
 
     // halo sources inside a plane (positions clamped to the mesh; clamped
-    // values are never used by an interior row)
+    // values are never used by a row evaluated here)
     int yslot = -1, zslot = -1;
     long yoff = 0, zoff = 0;
     if (ty < hw) {
@@ -145,13 +145,14 @@ __global__ __launch_bounds__(64 * TY) void k_stage_march(MarchArgs a)
         zoff = (long)jc * n2 + kk;
     }
 
-    // closure patterns to LDS (read after the first barrier of the march)
+    // closure-row tables to LDS (read after the first barrier of the march)
     {
-        const int nwords = a.npat * (int)(sizeof(MarchPattern) / sizeof(double));
-        const double* src = reinterpret_cast<const double*>(a.pat);
-        double* dst = reinterpret_cast<double*>(spat);
+        const int nwords = 3 * kMaxClosureRows * (int)(sizeof(ClosureRow) / sizeof(double));
+        const double* src = reinterpret_cast<const double*>(a.rows);
+        double* dst = reinterpret_cast<double*>(&crow[0][0]);
         for (int w = threadIdx.x; w < nwords; w += 64 * TY) dst[w] = src[w];
     }
+    const int ny1 = a.nrows[1] + 1, nz1 = a.nrows[2] + 1;
 
     const bool px = a.ops.d[0].present, py = a.ops.d[1].present, pz = a.ops.d[2].present;
     double cx[W], cy[W], cz[W];
@@ -165,11 +166,11 @@ __global__ __launch_bounds__(64 * TY) void k_stage_march(MarchArgs a)
     const int x0 = bx * a.xchunk;
     const int x1 = min(x0 + a.xchunk, n0);
     const double* __restrict__ us = a.us;
-    // x queue: q[m] = us at plane x - hw + m; qn is the next queue element.
-    double q[W];
+    // x queue: q[m] = us at plane x - HQ + m; qn is the next queue element.
+    double q[QW];
 #pragma unroll
-    for (int m = 0; m < W - 1; ++m) q[m] = us[(long)min(max(x0 - hw + m, 0), n0 - 1) * plane + col];
-    double qn = us[(long)min(x0 + hw, n0 - 1) * plane + col];
+    for (int m = 0; m < QW - 1; ++m) q[m] = us[(long)min(max(x0 - HQ + m, 0), n0 - 1) * plane + col];
+    double qn = us[(long)min(x0 + HQ, n0 - 1) * plane + col];
     // values of the current plane; those of the next plane are loaded one
     // iteration ahead so that their latency overlaps the current plane
     long pcur = (long)x0 * plane;
@@ -181,11 +182,11 @@ __global__ __launch_bounds__(64 * TY) void k_stage_march(MarchArgs a)
 
     for (int x = x0; x < x1; ++x) {
         const long pb = (long)x * plane;
-        q[W - 1] = qn;
+        q[QW - 1] = qn;
         // next plane (the last iteration reloads its own plane; unused)
         const int xn = min(x + 1, x1 - 1);
         const long pn = (long)xn * plane;
-        qn = us[(long)min(xn + hw, n0 - 1) * plane + col];
+        qn = us[(long)min(xn + HQ, n0 - 1) * plane + col];
         const double nyv = yslot >= 0 ? us[pn + yoff] : 0.0;
         const double nzv = zslot >= 0 ? us[pn + zoff] : 0.0;
         const unsigned char nc = a.cls[pn + col];
@@ -193,77 +194,81 @@ __global__ __launch_bounds__(64 * TY) void k_stage_march(MarchArgs a)
         const double nacc = STAGE == 0 ? 0.0 : a.acc[pn + col];
         const long idx = pb + col;
         double(*t)[LZ] = tile[x & 1];
-        t[ty + hw][tz + hw] = q[hw];
+        t[ty + hw][tz + hw] = q[HQ];
         if (yslot >= 0) t[yslot][tz + hw] = yv;
         if (zslot >= 0) t[ty + hw][zslot] = zv;
         __syncthreads();
         if (active && c != kSpecial) {
-            double rhs;
-            if (c != kNoRows) {
-                double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-                if (px) {
-                    double sx = 0.0;
-#pragma unroll
-                    for (int m = 0; m < W; ++m) sx += cx[m] * q[m];
-                    d0 = sx;
-                }
-                if (c == kInterior) {
-                    if (py) {
-                        double sy = 0.0;
-#pragma unroll
-                        for (int m = 0; m < W; ++m) sy += cy[m] * t[ty + m][tz + hw];
-                        d1 = sy;
+            double rhs = 0.0;
+            double v;
+            {
+                if (c != kNoRows) {
+                    int ix = 0, iy = 0, iz = 0;
+                    if (c >= kPattern0) {
+                        int p = c - kPattern0;
+                        iz = p % nz1;
+                        p /= nz1;
+                        iy = p % ny1;
+                        ix = p / ny1;
                     }
-                    if (pz) {
-                        double sz = 0.0;
+                    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+                    if (px) {
+                        double sx = 0.0;
+                        if (ix == 0) {
 #pragma unroll
-                        for (int m = 0; m < W; ++m) sz += cz[m] * t[ty + hw][tz + m];
-                        d2 = sz;
+                            for (int m = 0; m < W; ++m) sx += cx[m] * q[kXReach + m];
+                        } else {
+                            // closure row over the queue: only its own columns,
+                            // in column order
+                            const ClosureRow& r = crow[0][ix - 1];
+#pragma unroll
+                            for (int m = 0; m < QW; ++m) {
+                                const int ci = m - HQ - r.off;
+                                if (ci >= 0 && ci < r.n) sx += r.w[ci] * q[m];
+                            }
+                        }
+                        d0 = sx;
                     }
-                } else {
-                    // y/z rows of a pattern point: interior or a closure row
-                    const MarchPattern& pt = spat[c - kPattern0];
                     if (py) {
-                        const int yn = pt.yn;
-                        if (yn == 0) {
+                        if (iy == 0) {
                             double sy = 0.0;
 #pragma unroll
                             for (int m = 0; m < W; ++m) sy += cy[m] * t[ty + m][tz + hw];
                             d1 = sy;
                         } else {
-                            d1 = pattern_dot(yn, pt.yoff, pt.wy, &t[ty + hw][tz + hw], LZ);
+                            d1 = tile_row(crow[1][iy - 1], &t[ty + hw][tz + hw], LZ);
                         }
                     }
                     if (pz) {
-                        const int zn = pt.zn;
-                        if (zn == 0) {
+                        if (iz == 0) {
                             double sz = 0.0;
 #pragma unroll
                             for (int m = 0; m < W; ++m) sz += cz[m] * t[ty + hw][tz + m];
                             d2 = sz;
                         } else {
-                            d2 = pattern_dot(zn, pt.zoff, pt.wz, &t[ty + hw][tz + hw], 1);
+                            d2 = tile_row(crow[2][iz - 1], &t[ty + hw][tz + hw], 1);
                         }
                     }
-                }
-                if (KIND == SHX_SYS_HEAT) {
-                    // laplacian rows added in direction order (plus_eq)
-                    double lap = 0.0;
-                    if (px) lap += d0;
-                    if (py) lap += d1;
-                    if (pz) lap += d2;
-                    rhs = heat_finish(a.s, false, true, x, j, k, lap, a.t_s);
+                    if (KIND == SHX_SYS_HEAT) {
+                        // laplacian rows added in direction order (plus_eq)
+                        double lap = 0.0;
+                        if (px) lap += d0;
+                        if (py) lap += d1;
+                        if (pz) lap += d2;
+                        rhs = heat_finish(a.s, false, true, x, j, k, lap, a.t_s);
+                    } else {
+                        rhs = wave_combine<KIND>(a.s, false, x, j, k, d0, d1, d2);
+                    }
                 } else {
-                    rhs = wave_combine<KIND>(a.s, false, x, j, k, d0, d1, d2);
+                    rhs = KIND == SHX_SYS_HEAT ? heat_finish(a.s, false, false, x, j, k, 0.0, a.t_s)
+                                               : wave_combine<KIND>(a.s, false, x, j, k, 0.0, 0.0, 0.0);
                 }
-            } else {
-                rhs = KIND == SHX_SYS_HEAT ? heat_finish(a.s, false, false, x, j, k, 0.0, a.t_s)
-                                           : wave_combine<KIND>(a.s, false, x, j, k, 0.0, 0.0, 0.0);
+                v = stage_update<STAGE>(rhs, accv, u0v, a.acc, idx, a.c_acc, a.c_next);
             }
-            a.out[idx] = stage_update<STAGE>(rhs, accv, u0v, a.acc, idx, a.c_acc, a.c_next);
+            a.out[idx] = v;
         }
 #pragma unroll
-        for (int m = 0; m < W - 1; ++m) q[m] = q[m + 1];
+        for (int m = 0; m < QW - 1; ++m) q[m] = q[m + 1];
         yv = nyv;
         zv = nzv;
         c = nc;
@@ -305,10 +310,10 @@ __device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRo
     return r;
 }
 
+// Item q of the special work: a listed point, then the cut points.
 template <int KIND, int STAGE, int W>
-__global__ __launch_bounds__(256) void k_stage_special(SpecialArgs a)
+__device__ __forceinline__ void special_item(const SpecialArgs& a, long q)
 {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < a.nspecial) {
         const SpecialRec rec = a.recs[q];
         const long idx = rec.idx;
@@ -371,44 +376,79 @@ __global__ __launch_bounds__(256) void k_stage_special(SpecialArgs a)
     }
 }
 
+// One fused RK4 stage.  Workgroups go in groups of 8 (one per XCD): `sg`
+// groups evaluate the special items and `mg` groups are march tiles.  The
+// two parts read only the stage input and write disjoint points, so any
+// placement is correct; `mode` picks it: 1 special groups first, 2 last,
+// 3 spread evenly (group g is special when floor((g+1) sg / T) exceeds
+// floor(g sg / T), T = sg + mg).  A launch with sg = 0 or mg = 0 runs one
+// part alone.  Tile workgroups keep their XCD (b mod 8) and map to tiles
+// contiguously per XCD.
+template <int KIND, int STAGE, int W, int TY>
+__global__ __launch_bounds__(64 * TY) void k_stage(MarchArgs a, SpecialArgs sp, unsigned sg,
+                                                   unsigned mg, int mode)
+{
+    const unsigned g = blockIdx.x / 8u, lane8 = blockIdx.x % 8u;
+    bool special;
+    unsigned si, mi;  // group index within its part
+    if (mode == 1) {
+        special = g < sg;
+        si = g;
+        mi = g - sg;
+    } else if (mode == 2) {
+        special = g >= mg;
+        si = g - mg;
+        mi = g;
+    } else {
+        const unsigned long long T = (unsigned long long)sg + mg;
+        const unsigned before = (unsigned)((unsigned long long)g * sg / T);
+        const unsigned upto = (unsigned)((unsigned long long)(g + 1) * sg / T);
+        special = upto > before;
+        si = before;
+        mi = g - before;
+    }
+    if (special) {
+        const long blk = (long)si * 8 + lane8;
+        special_item<KIND, STAGE, W>(sp, blk * (64 * TY) + threadIdx.x);
+        return;
+    }
+    const unsigned m = mi * 8u + lane8;
+    if (m >= a.nblocks) return;
+    march_tile<KIND, STAGE, W, TY>(a, xcd_remap(m, a.nblocks));
+}
+
 template <int KIND, int STAGE, int W>
-int march_ty(hipStream_t st, int ty, const MarchArgs& a)
+int stage_ty(hipStream_t st, int ty, int mode, const MarchArgs& a, const SpecialArgs& sp, long nitems)
 {
-    switch (ty) {
-    case 4:
-        k_stage_march<KIND, STAGE, W, 4><<<a.nblocks, 64 * 4, 0, st>>>(a);
-        return 0;
-    case 8:
-        k_stage_march<KIND, STAGE, W, 8><<<a.nblocks, 64 * 8, 0, st>>>(a);
-        return 0;
-    case 16:
-        k_stage_march<KIND, STAGE, W, 16><<<a.nblocks, 64 * 16, 0, st>>>(a);
-        return 0;
-    default:
-        return -1;
+    const int threads = 64 * ty;
+    const unsigned sg = (unsigned)(((nitems + threads - 1) / threads + 7) / 8);
+    const unsigned mg = (a.nblocks + 7u) / 8u;
+    auto go = [&](unsigned s_groups, unsigned m_groups, int md) {
+        const unsigned grid = 8u * (s_groups + m_groups);
+        if (grid == 0) return 0;
+        switch (ty) {
+        case 4: k_stage<KIND, STAGE, W, 4><<<grid, 64 * 4, 0, st>>>(a, sp, s_groups, m_groups, md); return 0;
+        case 8: k_stage<KIND, STAGE, W, 8><<<grid, 64 * 8, 0, st>>>(a, sp, s_groups, m_groups, md); return 0;
+        case 16: k_stage<KIND, STAGE, W, 16><<<grid, 64 * 16, 0, st>>>(a, sp, s_groups, m_groups, md); return 0;
+        default: return -1;
+        }
+    };
+    if (mode == 0) {  // two launches: tiles, then special items
+        const int rc = go(0, mg, 1);
+        return rc != 0 ? rc : go(sg, 0, 1);
     }
+    return go(sg, mg, mode);
 }
 
 template <int KIND, int W>
-int march_stage(hipStream_t st, int stage, int ty, const MarchArgs& a)
+int stage_of(hipStream_t st, int stage, int ty, int mode, const MarchArgs& a, const SpecialArgs& sp,
+             long nitems)
 {
     switch (stage) {
-    case 0: return march_ty<KIND, 0, W>(st, ty, a);
-    case 1: return march_ty<KIND, 1, W>(st, ty, a);
-    case 2: return march_ty<KIND, 2, W>(st, ty, a);
-    case 3: return march_ty<KIND, 3, W>(st, ty, a);
-    default: return -1;
-    }
-}
-
-template <int KIND, int W>
-int special_stage(hipStream_t st, int stage, unsigned grid, const SpecialArgs& a)
-{
-    switch (stage) {
-    case 0: k_stage_special<KIND, 0, W><<<grid, 256, 0, st>>>(a); return 0;
-    case 1: k_stage_special<KIND, 1, W><<<grid, 256, 0, st>>>(a); return 0;
-    case 2: k_stage_special<KIND, 2, W><<<grid, 256, 0, st>>>(a); return 0;
-    case 3: k_stage_special<KIND, 3, W><<<grid, 256, 0, st>>>(a); return 0;
+    case 0: return stage_ty<KIND, 0, W>(st, ty, mode, a, sp, nitems);
+    case 1: return stage_ty<KIND, 1, W>(st, ty, mode, a, sp, nitems);
+    case 2: return stage_ty<KIND, 2, W>(st, ty, mode, a, sp, nitems);
+    case 3: return stage_ty<KIND, 3, W>(st, ty, mode, a, sp, nitems);
     default: return -1;
     }
 }
@@ -422,32 +462,18 @@ int launch_classify(hipStream_t st, const DevSys& s, const DevOps& ops, unsigned
     return 0;
 }
 
-int launch_stage_march(hipStream_t st, int kind, int stage, int width, int ty, const MarchArgs& a)
+int launch_stage_march(hipStream_t st, int kind, int stage, int width, int ty, int mode,
+                       const MarchArgs& a, const SpecialArgs& sp)
 {
-    if (a.nblocks == 0) return 0;
+    const long nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2;
+    if (a.nblocks == 0 && nitems == 0) return 0;
     switch (kind * 10 + width) {
-    case SHX_SYS_HEAT * 10 + 5: return march_stage<SHX_SYS_HEAT, 5>(st, stage, ty, a);
-    case SHX_SYS_HEAT * 10 + 3: return march_stage<SHX_SYS_HEAT, 3>(st, stage, ty, a);
-    case SHX_SYS_WAVE_LINEAR * 10 + 5: return march_stage<SHX_SYS_WAVE_LINEAR, 5>(st, stage, ty, a);
-    case SHX_SYS_WAVE_LINEAR * 10 + 3: return march_stage<SHX_SYS_WAVE_LINEAR, 3>(st, stage, ty, a);
-    case SHX_SYS_WAVE_RADIAL * 10 + 5: return march_stage<SHX_SYS_WAVE_RADIAL, 5>(st, stage, ty, a);
-    case SHX_SYS_WAVE_RADIAL * 10 + 3: return march_stage<SHX_SYS_WAVE_RADIAL, 3>(st, stage, ty, a);
-    default: return -1;
-    }
-}
-
-int launch_stage_special(hipStream_t st, int kind, int stage, int width, const SpecialArgs& a)
-{
-    const long n = a.nspecial + a.nR0 + a.nR1 + a.nR2;
-    if (n == 0) return 0;
-    const unsigned grid = (unsigned)((n + 255) / 256);
-    switch (kind * 10 + width) {
-    case SHX_SYS_HEAT * 10 + 5: return special_stage<SHX_SYS_HEAT, 5>(st, stage, grid, a);
-    case SHX_SYS_HEAT * 10 + 3: return special_stage<SHX_SYS_HEAT, 3>(st, stage, grid, a);
-    case SHX_SYS_WAVE_LINEAR * 10 + 5: return special_stage<SHX_SYS_WAVE_LINEAR, 5>(st, stage, grid, a);
-    case SHX_SYS_WAVE_LINEAR * 10 + 3: return special_stage<SHX_SYS_WAVE_LINEAR, 3>(st, stage, grid, a);
-    case SHX_SYS_WAVE_RADIAL * 10 + 5: return special_stage<SHX_SYS_WAVE_RADIAL, 5>(st, stage, grid, a);
-    case SHX_SYS_WAVE_RADIAL * 10 + 3: return special_stage<SHX_SYS_WAVE_RADIAL, 3>(st, stage, grid, a);
+    case SHX_SYS_HEAT * 10 + 5: return stage_of<SHX_SYS_HEAT, 5>(st, stage, ty, mode, a, sp, nitems);
+    case SHX_SYS_HEAT * 10 + 3: return stage_of<SHX_SYS_HEAT, 3>(st, stage, ty, mode, a, sp, nitems);
+    case SHX_SYS_WAVE_LINEAR * 10 + 5: return stage_of<SHX_SYS_WAVE_LINEAR, 5>(st, stage, ty, mode, a, sp, nitems);
+    case SHX_SYS_WAVE_LINEAR * 10 + 3: return stage_of<SHX_SYS_WAVE_LINEAR, 3>(st, stage, ty, mode, a, sp, nitems);
+    case SHX_SYS_WAVE_RADIAL * 10 + 5: return stage_of<SHX_SYS_WAVE_RADIAL, 5>(st, stage, ty, mode, a, sp, nitems);
+    case SHX_SYS_WAVE_RADIAL * 10 + 3: return stage_of<SHX_SYS_WAVE_RADIAL, 3>(st, stage, ty, mode, a, sp, nitems);
     default: return -1;
     }
 }

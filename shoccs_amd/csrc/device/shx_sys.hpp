@@ -82,30 +82,38 @@ struct RPtrs {
 //                not on a Dirichlet face, fluid: register/LDS stencils only;
 //   kNoRows   -- no row in any direction, not on a face, not fluid (solid
 //                points): the right-hand side of zero derivatives;
-//   kPattern0 + p -- fluid, not on a face, interior x row, and y/z rows that
-//                are interior or closure rows without B terms whose columns
-//                lie in the workgroup's LDS tile: MarchPattern p;
-//   kSpecial  -- anything else (x closures, B terms, faces): evaluated by the
-//                special-point kernel from its record.
+//   kPattern0 + (ix (ny + 1) + iy) (nz + 1) + iz -- fluid, not on a face,
+//                each direction's row interior (index 0) or closure row
+//                i - 1 of that direction's table: closure rows without B
+//                terms whose columns the tile holds (x: the register queue,
+//                y/z: the LDS plane);
+//   kSpecial  -- anything else (rows with B terms, closures out of reach,
+//                Dirichlet faces): evaluated from its record by the
+//                special-point work.  Face records carry no rows: the
+//                right-hand side there is zero and the value exact(t_next).
 // The classification kernel marks raw special points kRawSpecial | flags;
-// the host turns them into kPattern0 + p or kSpecial.
+// the host turns them into a pattern or kSpecial.
 constexpr unsigned char kInterior = 0;
 constexpr unsigned char kSpecial = 1;
 constexpr unsigned char kNoRows = 2;
 constexpr unsigned char kPattern0 = 3;
-constexpr int kMaxPatterns = 64;     // staged in LDS: 64 x 144 B
-constexpr int kMaxPatternCols = 8;
+constexpr int kMaxPatterns = 256 - kPattern0;
 constexpr unsigned char kRawSpecial = 0x80;
 constexpr unsigned char kRawFace = 0x01;
 constexpr unsigned char kRawFluid = 0x02;
+constexpr int kMaxClosureRows = 16;   // per direction, staged in LDS
+constexpr int kMaxClosureCols = 8;
+// x closure rows are taken in the tiles when their columns lie within
+// hw + kXReach planes of the point (the register queue); a reach of 2 (all
+// 4th-order wall closures) measured no faster at 256^3 than 0 and costs
+// registers, so x-wall closures go to the special work
+constexpr int kXReach = 0;
 
-// y and z rows of a pattern point: n == 0 is the interior circulant row,
-// otherwise the dot of w[c] with the values at offsets off + c along the
-// direction, c < n (w copied from the operator's coefficient pool).
-struct MarchPattern {
-    int yoff, yn, zoff, zn;
-    double wy[kMaxPatternCols];
-    double wz[kMaxPatternCols];
+// A closure row without B terms: the dot of w[c] with the values at offsets
+// off + c along its direction, c < n (w copied from the coefficient pool).
+struct ClosureRow {
+    int off, n;
+    double w[kMaxClosureCols];
 };
 
 // Arguments of the x-marching fused stage kernel.
@@ -117,9 +125,9 @@ struct MarchArgs {
     double* acc;
     double* out;
     const unsigned char* cls;
-    const MarchPattern* pat;
-    int npat;
-    double t_s, c_acc, c_next;
+    const ClosureRow* rows;  // 3 x kMaxClosureRows
+    int nrows[3];
+    double t_s, c_acc, c_next, t_next;
     int nbz, nby, xchunk;
     unsigned nblocks;
 };
@@ -163,9 +171,11 @@ struct SpecialArgs {
 };
 
 // Host launchers (shx_march.hip); each returns 0, or -1 for a template
-// combination that is not built.
+// combination that is not built.  launch_stage_march runs one fused RK4 stage
+// of the x-marching form: tiles and special items in one launch (mode 1:
+// special workgroups first, 2: last, 3: spread evenly) or in two (mode 0).
 int launch_classify(hipStream_t st, const DevSys& s, const DevOps& ops, unsigned char* cls, long N);
-int launch_stage_march(hipStream_t st, int kind, int stage, int width, int ty, const MarchArgs& a);
-int launch_stage_special(hipStream_t st, int kind, int stage, int width, const SpecialArgs& a);
+int launch_stage_march(hipStream_t st, int kind, int stage, int width, int ty, int mode,
+                       const MarchArgs& a, const SpecialArgs& sp);
 
 }  // namespace shx

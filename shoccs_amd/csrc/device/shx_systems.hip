@@ -269,18 +269,20 @@ struct shx_system {
     // statistics partials
     double* part = nullptr;
     long* part_idx = nullptr;
-    // fused-stage kernel choice: 0 point-wise tiles, 1 x-marching tiles
+    // fused-stage kernel choice: 0 point-wise tiles; 1-4 x-marching tiles
+    // with the special work in a second launch (1), or in the same launch
+    // before the tiles (2), after them (3) or spread among them (4)
     int variant = 0;
-    int march_ty = 8;
+    int march_ty = 4;
     int march_xchunk = 16;
     // x-marching form: class byte per point of D, the special-point records
     // and the per-direction special-row tables
     unsigned char* cls = nullptr;
     SpecialRec* special = nullptr;
     SpecialRow* srows[3] = {nullptr, nullptr, nullptr};
-    MarchPattern* mpat = nullptr;
+    ClosureRow* crows = nullptr;  // 3 x kMaxClosureRows
+    int nrows[3] = {0, 0, 0};
     long nspecial = 0;
-    long npatterns = 0;
     bool classes_ready = false;
     int width = 5;  // interior stencil width shared by the three derivatives
 };
@@ -589,19 +591,13 @@ int ensure_classes(shx_system* s)
     }
     if (rc == SHX_OK) rc = sync(s->ctx);
     if (rc != SHX_OK) return rc;
-    // Raw special points become pattern points of the x-marching kernel when
-    // their rows allow it, or special points with a record; rows other than
-    // interior ones go to a per-direction table, deduplicated (grid-wall
+    // Raw special points become face points, closure-pattern points of the
+    // x-marching tiles, or special points with a record.  Rows other than
+    // interior ones go to per-direction tables, deduplicated (grid-wall
     // closures repeat from line to line).
-    std::vector<SpecialRec> recs;
-    std::vector<SpecialRow> rows[3];
-    std::map<std::array<int, 8>, unsigned> seen[3];
-    std::vector<MarchPattern> pats;
-    std::map<std::array<int, 6>, int> pat_of;
-    const std::vector<double>* ypool = s->d[1] ? &s->d[1]->h_pool : nullptr;
-    const std::vector<double>* zpool = s->d[2] ? &s->d[2]->h_pool : nullptr;
     const int n1 = s->n[1], n2 = s->n[2];
     const int hw = s->width / 2;
+    const int hq = hw + kXReach;
     // 0 = no row, 1 = interior row, 2 = other row (e filled); the segment
     // holding row q is found as the device slow path finds it
     auto find_row = [&](int c, int pair, int q, SpecialRow& e) -> int {
@@ -644,64 +640,148 @@ int ensure_classes(shx_system* s)
         }
         return 0;
     };
-    // a closure row without B terms whose columns [q+off, q+off+n) lie in
-    // the LDS tile of every tile height (the 4-row tile is the smallest)
-    auto in_tile = [&](const SpecialRow& e, int q, int tile, int n) {
-        const int lo = std::max(0, (q / tile) * tile - hw);
-        const int hi = std::min(n, (q / tile) * tile + tile + hw);
-        return e.nb == 0 && e.ncols >= 1 && e.ncols <= kMaxPatternCols && q + e.off >= lo &&
-               q + e.off + e.ncols <= hi;
-    };
-    for (long idx = 0; idx < s->N; ++idx) {
-        const unsigned char raw = h[(size_t)idx];
-        if (!(raw & kRawSpecial)) continue;
+    auto rows_of = [&](long idx, int kind[3], SpecialRow e[3]) {
         const int i = (int)(idx / ((long)n1 * n2));
         const int rem = (int)(idx - (long)i * n1 * n2);
         const int j = rem / n2, k = rem - (rem / n2) * n2;
-        int kind[3] = {1, 1, 1};
-        SpecialRow e[3];
         for (int c = 0; c < 3; ++c) {
+            kind[c] = 1;
             if (!s->d[c] || !s->ops.d[c].present) continue;
             const int pair = c == 0 ? j * n2 + k : c == 1 ? i * n2 + k : i * n1 + j;
             const int q = c == 0 ? i : c == 1 ? j : k;
             kind[c] = find_row(c, pair, q, e[c]);
         }
-        const bool eligible = !(raw & kRawFace) && (raw & kRawFluid) && kind[0] == 1 &&
-                              kind[1] != 0 && kind[2] != 0 &&
-                              (kind[1] == 1 || in_tile(e[1], j, 4, n1)) &&
-                              (kind[2] == 1 || in_tile(e[2], k, 64, n2));
-        if (eligible) {
-            MarchPattern pt{};
-            std::array<int, 6> key{0, 0, 0, 0, 0, 0};
-            if (kind[1] == 2) {
-                pt.yoff = e[1].off;
-                pt.yn = e[1].ncols;
-                for (int m = 0; m < pt.yn; ++m) pt.wy[m] = (*ypool)[(size_t)e[1].coef + m];
-                key[0] = pt.yoff;
-                key[1] = pt.yn;
-                key[2] = e[1].coef;
-            }
-            if (kind[2] == 2) {
-                pt.zoff = e[2].off;
-                pt.zn = e[2].ncols;
-                for (int m = 0; m < pt.zn; ++m) pt.wz[m] = (*zpool)[(size_t)e[2].coef + m];
-                key[3] = pt.zoff;
-                key[4] = pt.zn;
-                key[5] = e[2].coef;
-            }
-            auto it = pat_of.find(key);
-            if (it == pat_of.end() && (int)pats.size() < kMaxPatterns) {
-                it = pat_of.emplace(key, (int)pats.size()).first;
-                pats.push_back(pt);
-            }
-            if (it != pat_of.end()) {
-                h[(size_t)idx] = (unsigned char)(kPattern0 + it->second);
-                continue;
-            }
-        }
+    };
+    // a closure row without B terms whose columns [q+off, q+off+n) lie in the
+    // mesh and in what the tile holds: the register queue for x, the LDS
+    // plane of every tile height for y (the 4-row tile is the smallest) and z
+    auto reachable = [&](int c, const SpecialRow& e, int q) {
+        if (e.nb != 0 || e.ncols < 1 || e.ncols > kMaxClosureCols) return false;
+        const int first = q + e.off, last = q + e.off + e.ncols - 1;
+        if (c == 0) return e.off >= -hq && e.off + e.ncols - 1 <= hq && first >= 0 && last < s->n[0];
+        const int tile = c == 1 ? 4 : 64, n = c == 1 ? n1 : n2;
+        const int lo = std::max(0, (q / tile) * tile - hw);
+        const int hi = std::min(n, (q / tile) * tile + tile + hw);
+        return first >= lo && last < hi;
+    };
+    struct Cand {
+        long idx;
+        int rid[3];
+    };
+    std::vector<bool> raw_face((size_t)s->N, false);
+    std::vector<Cand> cands;
+    std::vector<ClosureRow> allrows[3];
+    std::vector<long> freq[3];
+    std::map<std::array<int, 3>, int> rowid[3];
+    for (long idx = 0; idx < s->N; ++idx) {
+        const unsigned char raw = h[(size_t)idx];
+        if (!(raw & kRawSpecial)) continue;
+        raw_face[(size_t)idx] = (raw & kRawFace) != 0;
         h[(size_t)idx] = kSpecial;
+        if ((raw & kRawFace) || !(raw & kRawFluid)) continue;
+        int kind[3];
+        SpecialRow e[3];
+        rows_of(idx, kind, e);
+        const int i = (int)(idx / ((long)n1 * n2));
+        const int rem = (int)(idx - (long)i * n1 * n2);
+        const int qs[3] = {i, rem / n2, rem % n2};
+        bool ok = true;
+        for (int c = 0; c < 3 && ok; ++c)
+            ok = kind[c] == 1 || (kind[c] == 2 && reachable(c, e[c], qs[c]));
+        if (!ok) continue;
+        Cand cd{idx, {-1, -1, -1}};
+        for (int c = 0; c < 3; ++c) {
+            if (kind[c] != 2) continue;
+            const std::array<int, 3> key{e[c].off, e[c].ncols, e[c].coef};
+            auto it = rowid[c].find(key);
+            if (it == rowid[c].end()) {
+                it = rowid[c].emplace(key, (int)allrows[c].size()).first;
+                ClosureRow r{};
+                r.off = e[c].off;
+                r.n = e[c].ncols;
+                for (int m = 0; m < r.n; ++m) r.w[m] = s->d[c]->h_pool[(size_t)e[c].coef + m];
+                allrows[c].push_back(r);
+                freq[c].push_back(0);
+            }
+            cd.rid[c] = it->second;
+            freq[c][it->second]++;
+        }
+        cands.push_back(cd);
+    }
+    // rank the rows of each direction by use; pick table sizes (nx, ny, nz),
+    // (nx+1)(ny+1)(nz+1) <= kMaxPatterns, covering the most points
+    constexpr int R1 = kMaxClosureRows + 1;  // rank R1 = not in any table
+    std::vector<int> rank[3];
+    for (int c = 0; c < 3; ++c) {
+        std::vector<int> order(allrows[c].size());
+        for (size_t r = 0; r < order.size(); ++r) order[r] = (int)r;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return freq[c][x] > freq[c][y]; });
+        rank[c].assign(order.size(), R1);
+        for (size_t r = 0; r < order.size() && (int)r < kMaxClosureRows; ++r) rank[c][order[r]] = (int)r + 1;
+    }
+    auto rk = [&](const Cand& cd, int c) { return cd.rid[c] < 0 ? 0 : rank[c][cd.rid[c]]; };
+    std::vector<long> hist((size_t)(R1 + 1) * (R1 + 1) * (R1 + 1), 0);
+    auto at = [&](int x, int y, int z) -> long& { return hist[((size_t)x * (R1 + 1) + y) * (R1 + 1) + z]; };
+    for (const Cand& cd : cands) at(rk(cd, 0), rk(cd, 1), rk(cd, 2))++;
+    for (int x = 0; x <= R1; ++x)  // inclusive prefix sums
+        for (int y = 0; y <= R1; ++y)
+            for (int z = 0; z <= R1; ++z) {
+                long v = at(x, y, z);
+                if (x) v += at(x - 1, y, z);
+                if (y) v += at(x, y - 1, z);
+                if (z) v += at(x, y, z - 1);
+                if (x && y) v -= at(x - 1, y - 1, z);
+                if (x && z) v -= at(x - 1, y, z - 1);
+                if (y && z) v -= at(x, y - 1, z - 1);
+                if (x && y && z) v += at(x - 1, y - 1, z - 1);
+                at(x, y, z) = v;
+            }
+    int nr[3] = {0, 0, 0};
+    long best = -1;
+    const int lim[3] = {std::min<int>(kMaxClosureRows, (int)allrows[0].size()),
+                        std::min<int>(kMaxClosureRows, (int)allrows[1].size()),
+                        std::min<int>(kMaxClosureRows, (int)allrows[2].size())};
+    for (int x = 0; x <= lim[0]; ++x)
+        for (int y = 0; y <= lim[1]; ++y)
+            for (int z = 0; z <= lim[2]; ++z) {
+                if ((x + 1) * (y + 1) * (z + 1) > kMaxPatterns) continue;
+                const long cov = at(x, y, z);
+                if (cov > best) {
+                    best = cov;
+                    nr[0] = x;
+                    nr[1] = y;
+                    nr[2] = z;
+                }
+            }
+    for (const Cand& cd : cands) {
+        const int rx = rk(cd, 0), ry = rk(cd, 1), rz = rk(cd, 2);
+        if (rx > nr[0] || ry > nr[1] || rz > nr[2]) continue;
+        h[(size_t)cd.idx] = (unsigned char)(kPattern0 + (rx * (nr[1] + 1) + ry) * (nr[2] + 1) + rz);
+    }
+    std::vector<ClosureRow> table((size_t)3 * kMaxClosureRows, ClosureRow{});
+    for (int c = 0; c < 3; ++c) {
+        s->nrows[c] = nr[c];
+        for (size_t r = 0; r < allrows[c].size(); ++r)
+            if (rank[c][r] <= nr[c]) table[(size_t)c * kMaxClosureRows + rank[c][r] - 1] = allrows[c][r];
+    }
+    // records of the remaining special points, in index order; a Dirichlet
+    // face point needs no rows (its right-hand side is zero: gG = 0 or the
+    // heat face rule) and takes exact(t_next)
+    std::vector<SpecialRec> recs;
+    std::vector<SpecialRow> rows[3];
+    std::map<std::array<int, 8>, unsigned> seen[3];
+    for (long idx = 0; idx < s->N; ++idx) {
+        if (h[(size_t)idx] != kSpecial) continue;
         SpecialRec r;
         r.idx = (unsigned)idx;
+        if (raw_face[(size_t)idx]) {
+            r.ref[0] = r.ref[1] = r.ref[2] = kRowNone;
+            recs.push_back(r);
+            continue;
+        }
+        int kind[3];
+        SpecialRow e[3];
+        rows_of(idx, kind, e);
         for (int c = 0; c < 3; ++c) {
             r.ref[c] = kind[c] == 1 ? kRowInterior : kRowNone;
             if (kind[c] != 2) continue;
@@ -720,20 +800,22 @@ int ensure_classes(shx_system* s)
         recs.push_back(r);
     }
     s->nspecial = (long)recs.size();
-    s->npatterns = (long)pats.size();
     e = hipMemcpy(s->cls, h.data(), (size_t)s->N, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(s->ctx, e, "classes h2d");
     int rc2 = upload_vec(s->ctx, &s->special, recs);
-    if (rc2 == SHX_OK) rc2 = upload_vec(s->ctx, &s->mpat, pats);
+    if (rc2 == SHX_OK) rc2 = upload_vec(s->ctx, &s->crows, table);
     for (int c = 0; c < 3 && rc2 == SHX_OK; ++c) rc2 = upload_vec(s->ctx, &s->srows[c], rows[c]);
     if (rc2 != SHX_OK) return rc2;
     s->classes_ready = true;
     return SHX_OK;
 }
 
-// Main kernel of the x-marching form: interior and no-row points.
-int launch_march(shx_system* s, int stage, const double* usD, const double* u0, double* acc,
-                 double* out, double ts, double c_acc, double c_next)
+// One stage of the x-marching form in one launch: march tiles for interior,
+// solid and pattern points; special items for the listed points and the
+// Dirichlet cut points.
+int launch_march(shx_system* s, int stage, const double* usD, RPtrs usR, const double* u0,
+                 double* acc, double* out, double* const outR[3], double ts, double c_acc,
+                 double c_next, double tn)
 {
     MarchArgs a;
     a.s = s->dev;
@@ -743,9 +825,10 @@ int launch_march(shx_system* s, int stage, const double* usD, const double* u0, 
     a.acc = acc;
     a.out = out;
     a.cls = s->cls;
-    a.pat = s->mpat;
-    a.npat = (int)s->npatterns;
+    a.rows = s->crows;
+    for (int c = 0; c < 3; ++c) a.nrows[c] = s->nrows[c];
     a.t_s = ts;
+    a.t_next = tn;
     a.c_acc = c_acc;
     a.c_next = c_next;
     a.xchunk = s->march_xchunk;
@@ -753,58 +836,51 @@ int launch_march(shx_system* s, int stage, const double* usD, const double* u0, 
     a.nby = (s->n[1] + s->march_ty - 1) / s->march_ty;
     const long nbx = (s->n[0] + a.xchunk - 1) / a.xchunk;
     const long nb = (long)a.nbz * a.nby * nbx;
-    if (nb >= (1L << 31)) return fail(s->ctx, SHX_EUNSUPPORTED, "march: too many workgroups");
+    SpecialArgs sp;
+    sp.s = s->dev;
+    sp.ops = s->ops;
+    sp.us = usD;
+    sp.usR = usR;
+    sp.u0 = u0;
+    sp.acc = acc;
+    sp.out = out;
+    sp.recs = s->special;
+    sp.rows0 = s->srows[0];
+    sp.rows1 = s->srows[1];
+    sp.rows2 = s->srows[2];
+    sp.nspecial = s->nspecial;
+    sp.nR0 = s->nR[0];
+    sp.nR1 = s->nR[1];
+    sp.nR2 = s->nR[2];
+    sp.pos0 = s->rpos[0];
+    sp.pos1 = s->rpos[1];
+    sp.pos2 = s->rpos[2];
+    sp.dir0 = s->rdir[0];
+    sp.dir1 = s->rdir[1];
+    sp.dir2 = s->rdir[2];
+    sp.outR0 = outR[0];
+    sp.outR1 = outR[1];
+    sp.outR2 = outR[2];
+    sp.t_s = ts;
+    sp.c_acc = c_acc;
+    sp.c_next = c_next;
+    sp.t_next = tn;
+    const long nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2;
+    if (nb + nitems / 64 + 8 >= (1L << 31)) return fail(s->ctx, SHX_EUNSUPPORTED, "march: too many workgroups");
     a.nblocks = (unsigned)nb;
-    if (launch_stage_march(s->ctx->stream, s->kind, stage, s->width, s->march_ty, a) != 0)
+    if (launch_stage_march(s->ctx->stream, s->kind, stage, s->width, s->march_ty, s->variant - 1, a,
+                           sp) != 0)
         return fail(s->ctx, SHX_EINVAL, "march: unsupported tile height or stencil width");
     return check(s->ctx, "rk4 march stage kernel");
 }
 
-// Second kernel of the x-marching form: listed points and Dirichlet cut points.
-int launch_special(shx_system* s, int stage, const double* usD, RPtrs usR, const double* u0,
-                   double* acc, double* out, double* const outR[3], double ts, double c_acc,
-                   double c_next, double tn)
-{
-    SpecialArgs a;
-    a.s = s->dev;
-    a.ops = s->ops;
-    a.us = usD;
-    a.usR = usR;
-    a.u0 = u0;
-    a.acc = acc;
-    a.out = out;
-    a.recs = s->special;
-    a.rows0 = s->srows[0];
-    a.rows1 = s->srows[1];
-    a.rows2 = s->srows[2];
-    a.nspecial = s->nspecial;
-    a.nR0 = s->nR[0];
-    a.nR1 = s->nR[1];
-    a.nR2 = s->nR[2];
-    a.pos0 = s->rpos[0];
-    a.pos1 = s->rpos[1];
-    a.pos2 = s->rpos[2];
-    a.dir0 = s->rdir[0];
-    a.dir1 = s->rdir[1];
-    a.dir2 = s->rdir[2];
-    a.outR0 = outR[0];
-    a.outR1 = outR[1];
-    a.outR2 = outR[2];
-    a.t_s = ts;
-    a.c_acc = c_acc;
-    a.c_next = c_next;
-    a.t_next = tn;
-    if (launch_stage_special(s->ctx->stream, s->kind, stage, s->width, a) != 0)
-        return fail(s->ctx, SHX_EINVAL, "special points: unsupported stencil width");
-    return check(s->ctx, "rk4 special-point kernel");
-}
-
-// One fused step; when ev is given, event pairs bracket each stage's main
-// kernel (the point-wise tiles, or the x-marching kernel).
+// One fused step; when ev is given, event pairs bracket each stage's kernels
+// (the point-wise tiles, or the x-marching launch or launches; the
+// point-wise form's cut-point update runs outside the brackets).
 int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t* ev)
 {
     int rc = ensure_fused(s);
-    const bool march = s->variant == 1;
+    const bool march = s->variant >= 1;
     if (rc == SHX_OK && march) rc = ensure_classes(s);
     for (int stage = 0; stage < 4 && rc == SHX_OK; ++stage) {
         const double* usD = stage == 0 ? u0->D : s->stageD[(stage - 1) & 1];
@@ -819,7 +895,7 @@ int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t
         const double tn = stage < 3 ? t + dt * kRki[stage + 1] : t + dt;
         if (ev) hipEventRecord(ev[2 * stage], s->ctx->stream);
         if (march)
-            rc = launch_march(s, stage, usD, u0->D, s->accD, outD, ts, c_acc, c_next);
+            rc = launch_march(s, stage, usD, usR, u0->D, s->accD, outD, outR, ts, c_acc, c_next, tn);
         else if (s->kind == SHX_SYS_HEAT)
             rc = launch_stage<SHX_SYS_HEAT>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
         else if (s->kind == SHX_SYS_WAVE_RADIAL)
@@ -827,11 +903,7 @@ int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t
         else
             rc = launch_stage<SHX_SYS_WAVE_LINEAR>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
         if (ev) hipEventRecord(ev[2 * stage + 1], s->ctx->stream);
-        if (rc != SHX_OK) break;
-        if (march)
-            rc = launch_special(s, stage, usD, usR, u0->D, s->accD, outD, outR, ts, c_acc, c_next, tn);
-        else
-            rc = r_exact_all(s, outR, tn);
+        if (rc == SHX_OK && !march) rc = r_exact_all(s, outR, tn);
     }
     return rc;
 }
@@ -955,8 +1027,9 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
         }
     }
     if (s->width == 0) s->width = 5;
-    // the x-marching form is the default where it applies
-    s->variant = march_ok(s) ? 1 : 0;
+    // the x-marching form, special workgroups after the tiles, is the default
+    // where it applies
+    s->variant = march_ok(s) ? 3 : 0;
     *out = s;
     return SHX_OK;
 }
@@ -982,7 +1055,7 @@ int shx_system_destroy(shx_system* s)
     hipFree(s->cls);
     hipFree(s->special);
     for (int c = 0; c < 3; ++c) hipFree(s->srows[c]);
-    hipFree(s->mpat);
+    hipFree(s->crows);
     hipFree(s->srcD);
     hipFree(s->mms);
     hipFree(s->fluid_mask);
@@ -1060,8 +1133,11 @@ int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[1
         if (e != hipSuccess) return hip_fail(s->ctx, e, "stats copy");
         rc = sync(s->ctx);
         if (rc != SHX_OK) return rc;
+        // MaxLoc identity (lowest value, largest index) as Kokkos 5.1's
+        // MaxLoc::init, so that a field without a strict maximum (all NaN)
+        // reports what the reference reports
         double err = -DBL_MAX, mn = DBL_MAX, mx = -DBL_MAX;
-        long ei = 0;
+        long ei = 2147483647L;
         for (unsigned b = 0; b < grid; ++b) {
             if (h[b] > err || (h[b] == err && hi[b] < ei)) {
                 err = h[b];
@@ -1070,7 +1146,9 @@ int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[1
             mn = std::min(mn, h[nparts + b]);
             mx = std::max(mx, h[2 * nparts + b]);
         }
-        stats[0] = err;
+        // err = max(err_d, err_rx, err_ry, err_rz) with the R terms 0 for
+        // Dirichlet bodies (scalar_system_utils.hpp:169)
+        stats[0] = err < 0.0 ? 0.0 : err;
         stats[1] = mn;
         stats[2] = mx;
         stats[3] = err;
@@ -1167,11 +1245,12 @@ int shx_time_fused_stage_kernel(shx_system* s, shx_field* u0, double t, double d
 
 int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xchunk)
 {
-    if (!s || variant < 0 || variant > 1) return SHX_EINVAL;
-    if (variant == 1 && !(march_ty == 4 || march_ty == 8 || march_ty == 16)) return SHX_EINVAL;
-    if (variant == 1 && march_xchunk < 1) return SHX_EINVAL;
+    if (!s || variant < 0 || variant > 4) return SHX_EINVAL;
+    if (variant >= 1 && !(march_ty == 4 || march_ty == 8 || march_ty == 16)) return SHX_EINVAL;
+    if (variant >= 1 && march_xchunk < 1) return SHX_EINVAL;
+    if (variant >= 1 && !march_ok(s)) return fail(s->ctx, SHX_EUNSUPPORTED, "x-marching form: unsupported system");
     s->variant = variant;
-    if (variant == 1) {
+    if (variant >= 1) {
         s->march_ty = march_ty;
         s->march_xchunk = march_xchunk;
     }
@@ -1181,10 +1260,10 @@ int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xc
 int shx_system_stage_points(shx_system* s, int64_t out[2])
 {
     if (!s || !out) return SHX_EINVAL;
-    if (s->variant == 1) {
+    if (s->variant >= 1) {
         const int rc = ensure_classes(s);
         if (rc != SHX_OK) return rc;
-        out[0] = s->N - s->nspecial;
+        out[0] = s->N;
         out[1] = s->nspecial;
     } else {
         out[0] = s->N;
@@ -1224,8 +1303,9 @@ int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt,
     };
     std::vector<cand> cands{{0, 4, 32}};
     if (march_ok(s))
-        for (int ty : {4, 8, 16})
-            for (int xc : {16, 32, 64, 128, 1 << 16}) cands.push_back({1, ty, std::min(xc, s->n[0])});
+        for (int v = 1; v <= 4; ++v)
+            for (int ty : {4, 8, 16})
+                for (int xc : {16, 32, 64, 1 << 16}) cands.push_back({v, ty, std::min(xc, s->n[0])});
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
