@@ -189,9 +189,11 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, unsigned L)
         qn = us[(long)min(xn + HQ, n0 - 1) * plane + col];
         const double nyv = yslot >= 0 ? us[pn + yoff] : 0.0;
         const double nzv = zslot >= 0 ? us[pn + zoff] : 0.0;
-        const unsigned char nc = a.cls[pn + col];
-        const double nu0 = a.u0[pn + col];
-        const double nacc = STAGE == 0 ? 0.0 : a.acc[pn + col];
+        // single-use streams: non-temporal, so that L2 keeps the stage
+        // values that neighbouring tiles and chunks re-read
+        const unsigned char nc = __builtin_nontemporal_load(a.cls + pn + col);
+        const double nu0 = __builtin_nontemporal_load(a.u0 + pn + col);
+        const double nacc = STAGE == 0 ? 0.0 : __builtin_nontemporal_load(a.acc + pn + col);
         const long idx = pb + col;
         double(*t)[LZ] = tile[x & 1];
         t[ty + hw][tz + hw] = q[HQ];
@@ -263,9 +265,16 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, unsigned L)
                     rhs = KIND == SHX_SYS_HEAT ? heat_finish(a.s, false, false, x, j, k, 0.0, a.t_s)
                                                : wave_combine<KIND>(a.s, false, x, j, k, 0.0, 0.0, 0.0);
                 }
-                v = stage_update<STAGE>(rhs, accv, u0v, a.acc, idx, a.c_acc, a.c_next);
+                // stage_update, with non-temporal stores
+                const double aa = (STAGE == 0 ? 0.0 : accv) + a.c_acc * rhs;
+                if (STAGE < 3) {
+                    __builtin_nontemporal_store(aa, a.acc + idx);
+                    v = u0v + a.c_next * rhs;
+                } else {
+                    v = u0v + 1.0 * aa;
+                }
             }
-            a.out[idx] = v;
+            __builtin_nontemporal_store(v, a.out + idx);
         }
 #pragma unroll
         for (int m = 0; m < QW - 1; ++m) q[m] = q[m + 1];
