@@ -41,6 +41,9 @@ FUSED_BYTES_PER_PT_STEP = 136.0
 # the reference's schedule (init 32 + assign 72 + 4 x RHS 152 + accumulate 96
 # + final 24) plus the driver's u0 <- u1 copy (16)
 PLAIN_BYTES_PER_PT_STEP = 848.0
+# mixed form: stage register FP32, u0 and the accumulator FP64:
+# stage 1 R u0 W k W u1 (20), stages 2-3 (32 each), stage 4 (28)
+MIXED_BYTES_PER_PT_STEP = 112.0
 # boundary-closure free parameters of the reference's scalar-wave configs
 E4U_ALPHA = (-0.7733323791884821, 0.1623961700641681)
 
@@ -134,12 +137,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default fused-stage kernel shape (short profiling runs)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
     ap.add_argument("--n", type=int, default=N_SIDE)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box here has one "
               "GPU and the multi-GPU slab decomposition is design-only.", file=sys.stderr)
         sys.exit(2)
+
+    import numpy as np
 
     import shoccs_amd as S
 
@@ -169,6 +175,7 @@ def main():
     wall_ms_step = (time.perf_counter() - w0) * 1e3 / args.steps
     t_end = t_start + args.steps * dt
     stats = prob.stats(u0, t_end)
+    f64_final = u0.download() if not args.no_mixed else None
 
     # ---- dominant kernel: the fused stage launch, event-timed per launch on
     # the context's stream, priced over the points it updates
@@ -192,6 +199,33 @@ def main():
         plain = {"mpts": N / (ms_plain * 1e-3) / 1e6, "ms_per_step": ms_plain,
                  "algorithmic_bytes_per_pt_step": PLAIN_BYTES_PER_PT_STEP,
                  "hbm_frac": PLAIN_BYTES_PER_PT_STEP * N / (ms_plain * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+    # ---- mixed form (FP32 stage register): same workload and step count,
+    # compared with the FP64 form's final state
+    mixed = None
+    if not args.no_mixed:
+        prob.set_precision(S.F32_COMP)
+        um = prob.field()
+        prob.initialize(um, 0.0)
+        prob.update_boundary(um, 0.0)
+        m_tuning = (*prob.tuning(), None) if args.no_autotune else prob.autotune(um, 0.0, dt)
+        prob.rk4_fused(um, 0.0, dt, args.warmup)
+        ms_mixed = prob.time_fused(um, t_start, dt, args.steps, 1)
+        mixed_final = um.download()
+        m_stats = prob.stats(um, t_end)
+        ms_mk = prob.time_fused_stage_kernel(um, t_end, dt, 5)
+        m_achieved = MIXED_BYTES_PER_PT_STEP / 4.0 * main_pts / (ms_mk * 1e-3) / 1e9
+        scale = float(np.max(np.abs(f64_final)))
+        mixed = {"mpts": N / (ms_mixed * 1e-3) / 1e6, "ms_per_step": ms_mixed,
+                 "stage_kernel_ms": ms_mk, "achieved_gbs": m_achieved,
+                 "hbm_frac": m_achieved / HBM_PEAK_GBS,
+                 "algorithmic_bytes_per_pt_step": MIXED_BYTES_PER_PT_STEP,
+                 "max_abs_diff_vs_fp64": float(np.max(np.abs(mixed_final - f64_final))),
+                 "rel_diff_vs_fp64": float(np.max(np.abs(mixed_final - f64_final))) / scale,
+                 "linf_error_vs_exact": m_stats[0],
+                 "tuning": list(m_tuning[:3])}
+        um.free()
+        prob.set_precision(S.F64)
 
     mpts = N / (ms_step * 1e-3) / 1e6
     result = {
@@ -237,6 +271,8 @@ def main():
                        "algorithmic_bytes_per_pt_step": FUSED_BYTES_PER_PT_STEP},
         "plain_fp64": plain,
         "fused_over_plain": plain["ms_per_step"] / ms_step if plain else None,
+        "mixed_f32_comp": mixed,
+        "mixed_over_plain": plain["ms_per_step"] / mixed["ms_per_step"] if plain and mixed else None,
         "linf_error_vs_exact": stats[0],
         "final_time": t_end,
     }

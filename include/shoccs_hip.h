@@ -229,6 +229,17 @@ int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt,
  * variant 3 (march_ty 4, march_xchunk 16) when its stencil width is 3 or 5
  * and N < 2^32. */
 int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xchunk);
+/* Precision of the fused form (no reference counterpart; the reference is
+ * FP64 throughout).  SHX_F64: every register FP64, results bit-identical to
+ * the reference's sums.  SHX_F32_COMP (the mixed form): the RK stage
+ * register is stored in FP32 and interior stencils on it are summed in FP32
+ * with Kahan compensation; u0, the RK accumulator, closure rows, B terms and
+ * cut points stay FP64 (112 instead of 136 algorithmic bytes per point and
+ * step).  Needs variants 1-4; selecting it on variant 0 switches to 3. */
+#define SHX_F64 0
+#define SHX_F32_COMP 1
+int shx_system_set_precision(shx_system* s, int precision);
+int shx_system_get_precision(const shx_system* s, int* precision);
 int shx_system_get_tuning(const shx_system* s, int out[3]);
 /* Points of D updated per stage by the timed fused-stage launch (out[0]) and,
  * among them, points evaluated from special-point records (out[1], 0 for

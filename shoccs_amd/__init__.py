@@ -24,14 +24,14 @@ import weakref
 
 import numpy as np
 
-from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, E2_2, E4_1, E4_2, E4U_1, MMS_GAUSS,
-                   MMS_SINPROD, SYS_HEAT, SYS_WAVE_LINEAR, SYS_WAVE_RADIAL, ProblemDesc,
-                   ShoccsError, ShxField, ShxLineBlock, check, lib)
+from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, E2_2, E4_1, E4_2, E4U_1, F32_COMP, F64,
+                   MMS_GAUSS, MMS_SINPROD, SYS_HEAT, SYS_WAVE_LINEAR, SYS_WAVE_RADIAL,
+                   ProblemDesc, ShoccsError, ShxField, ShxLineBlock, check, lib)
 
 __all__ = [
     "Context", "Problem", "Field", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
     "E2_2", "E4_2", "E4_1", "E4U_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
-    "MMS_GAUSS", "MMS_SINPROD", "eq", "plus_eq",
+    "MMS_GAUSS", "MMS_SINPROD", "F64", "F32_COMP", "eq", "plus_eq",
 ]
 
 eq = 0
@@ -317,6 +317,18 @@ class Problem:
         self._need_device()
         check(lib().shx_system_set_tuning(self.sys, variant, march_ty, march_xchunk), self.ctx.p,
               "set_tuning")
+
+    def set_precision(self, precision: int):
+        """Fused-form precision: F64 (bit-identical to the reference's sums) or
+        F32_COMP (FP32 stage register with compensated interior sums; u0, the
+        accumulator, closures and B terms in FP64)."""
+        self._need_device()
+        check(lib().shx_system_set_precision(self.sys, precision), self.ctx.p, "set_precision")
+
+    def precision(self) -> int:
+        out = C.c_int()
+        check(lib().shx_system_get_precision(self.sys, C.byref(out)), self.ctx.p, "get_precision")
+        return out.value
 
     def tuning(self) -> tuple:
         out = (C.c_int * 3)()

@@ -17,6 +17,7 @@ BC_DIRICHLET, BC_FLOATING, BC_NEUMANN = 0, 1, 2
 SYS_WAVE_RADIAL, SYS_WAVE_LINEAR, SYS_HEAT = 0, 1, 2
 MMS_GAUSS, MMS_SINPROD = 0, 1
 E2_2, E4_2, E4_1, E4U_1 = 0, 1, 2, 3
+F64, F32_COMP = 0, 1
 
 dp = C.POINTER(C.c_double)
 i64p = C.POINTER(C.c_int64)
@@ -101,6 +102,8 @@ _SIGS = [
     ("shx_system_set_tuning", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     ("shx_system_get_tuning", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("shx_system_stage_points", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("shx_system_set_precision", C.c_int, [C.c_void_p, C.c_int]),
+    ("shx_system_get_precision", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("shx_system_autotune", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double, dp]),
     ("shoccs_problem_create", C.c_int, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]),
     ("shoccs_problem_destroy", C.c_int, [C.c_void_p]),

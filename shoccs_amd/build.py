@@ -21,7 +21,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
-DEVICE_SOURCES = ["device/shx_core.hip", "device/shx_systems.hip", "device/shx_march.hip"]
+DEVICE_SOURCES = ["device/shx_core.hip", "device/shx_systems.hip", "device/shx_march.hip",
+                  "device/shx_mixed.hip"]
 HOST_SOURCES = ["host/mesh.cpp", "host/stencils.cpp", "host/operators.cpp", "host/problem.cpp"]
 
 COMMON = ["-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-fno-fast-math",

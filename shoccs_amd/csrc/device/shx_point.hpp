@@ -195,17 +195,17 @@ __device__ __forceinline__ double stage_update(double rhs, double accv, double u
 }
 
 // Stage update and store of one point from its right-hand side; Dirichlet
-// faces take exact(t_next).
-template <int KIND, int STAGE>
+// faces take exact(t_next).  The output register may be FP32 (mixed form).
+template <int KIND, int STAGE, typename TO>
 __device__ __forceinline__ void stage_store(const DevSys& s, double rhs, const double* u0,
-                                            double* __restrict__ acc, double* out, long idx,
-                                            int i, int j, int k, double c_acc, double c_next,
+                                            double* __restrict__ acc, TO* out, long idx, int i,
+                                            int j, int k, double c_acc, double c_next,
                                             double t_next)
 {
     double v = stage_update<STAGE>(rhs, STAGE == 0 ? 0.0 : acc[idx], u0[idx], acc, idx, c_acc,
                                    c_next);
     if (on_dirichlet_face(s, i, j, k)) v = exact_of<KIND>(s, s.x[i], s.y[j], s.z[k], t_next);
-    out[idx] = v;
+    out[idx] = (TO)v;
 }
 
 // One point of one fused RK4 stage.

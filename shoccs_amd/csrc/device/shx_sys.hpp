@@ -116,14 +116,15 @@ struct ClosureRow {
     double w[kMaxClosureCols];
 };
 
-// Arguments of the x-marching fused stage kernel.
+// Arguments of the x-marching fused stage kernel.  The stage input `us` and
+// output `out` are FP64, or FP32 for the mixed form's stage register.
 struct MarchArgs {
     DevSys s;
     DevOps ops;
-    const double* us;
+    const void* us;
     const double* u0;
     double* acc;
-    double* out;
+    void* out;
     const unsigned char* cls;
     const ClosureRow* rows;  // 3 x kMaxClosureRows
     int nrows[3];
@@ -155,11 +156,11 @@ struct SpecialRec {
 struct SpecialArgs {
     DevSys s;
     DevOps ops;
-    const double* us;
+    const void* us;
     RPtrs usR;
     const double* u0;
     double* acc;
-    double* out;
+    void* out;
     const SpecialRec* recs;
     const SpecialRow *rows0, *rows1, *rows2;
     long nspecial;
@@ -177,5 +178,10 @@ struct SpecialArgs {
 int launch_classify(hipStream_t st, const DevSys& s, const DevOps& ops, unsigned char* cls, long N);
 int launch_stage_march(hipStream_t st, int kind, int stage, int width, int ty, int mode,
                        const MarchArgs& a, const SpecialArgs& sp);
+// The mixed form (shx_mixed.hip): stage register FP32, interior stencils on it
+// in FP32 with compensated sums; u0, the accumulator, closure rows, B terms
+// and cut points in FP64.
+int launch_stage_march_mixed(hipStream_t st, int kind, int stage, int width, int ty, int mode,
+                             const MarchArgs& a, const SpecialArgs& sp);
 
 }  // namespace shx

@@ -258,6 +258,8 @@ struct shx_system {
     signed char* rdir[3] = {nullptr, nullptr, nullptr};
     // fused RK4 registers (stage ping-pong and accumulator), allocated lazily
     double* stageD[2] = {nullptr, nullptr};
+    float* stageF[2] = {nullptr, nullptr};  // mixed form: FP32 stage register
+    int precision = SHX_F64;
     double* stageR[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     double* accD = nullptr;
     // reference-schedule buffers, allocated lazily
@@ -325,13 +327,22 @@ int face_exact(shx_system* s, double* D, double t)
     return check(s->ctx, "face boundary kernel");
 }
 
+// Fused-form registers: the accumulator, the stage ping-pong (FP64, or FP32
+// for the mixed form) and the stage's cut-point values (FP64).
 int ensure_fused(shx_system* s)
 {
-    if (s->accD) return SHX_OK;
-    int rc = alloc_d(s->ctx, &s->accD, s->N);
+    int rc = SHX_OK;
+    if (!s->accD) {
+        rc = alloc_d(s->ctx, &s->accD, s->N);
+        for (int b = 0; b < 2 && rc == SHX_OK; ++b)
+            for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = alloc_d(s->ctx, &s->stageR[b][c], s->nR[c]);
+    }
     for (int b = 0; b < 2 && rc == SHX_OK; ++b) {
-        rc = alloc_d(s->ctx, &s->stageD[b], s->N);
-        for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = alloc_d(s->ctx, &s->stageR[b][c], s->nR[c]);
+        if (s->precision == SHX_F64 && !s->stageD[b]) rc = alloc_d(s->ctx, &s->stageD[b], s->N);
+        if (s->precision == SHX_F32_COMP && !s->stageF[b]) {
+            const hipError_t e = hipMalloc((void**)&s->stageF[b], sizeof(float) * (size_t)(s->N > 0 ? s->N : 1));
+            if (e != hipSuccess) rc = hip_fail(s->ctx, e, "hipMalloc");
+        }
     }
     return rc;
 }
@@ -813,8 +824,8 @@ int ensure_classes(shx_system* s)
 // One stage of the x-marching form in one launch: march tiles for interior,
 // solid and pattern points; special items for the listed points and the
 // Dirichlet cut points.
-int launch_march(shx_system* s, int stage, const double* usD, RPtrs usR, const double* u0,
-                 double* acc, double* out, double* const outR[3], double ts, double c_acc,
+int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const double* u0,
+                 double* acc, void* out, double* const outR[3], double ts, double c_acc,
                  double c_next, double tn)
 {
     MarchArgs a;
@@ -868,8 +879,12 @@ int launch_march(shx_system* s, int stage, const double* usD, RPtrs usR, const d
     const long nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2;
     if (nb + nitems / 64 + 8 >= (1L << 31)) return fail(s->ctx, SHX_EUNSUPPORTED, "march: too many workgroups");
     a.nblocks = (unsigned)nb;
-    if (launch_stage_march(s->ctx->stream, s->kind, stage, s->width, s->march_ty, s->variant - 1, a,
-                           sp) != 0)
+    const int lrc = s->precision == SHX_F32_COMP
+                        ? launch_stage_march_mixed(s->ctx->stream, s->kind, stage, s->width,
+                                                   s->march_ty, s->variant - 1, a, sp)
+                        : launch_stage_march(s->ctx->stream, s->kind, stage, s->width, s->march_ty,
+                                             s->variant - 1, a, sp);
+    if (lrc != 0)
         return fail(s->ctx, SHX_EINVAL, "march: unsupported tile height or stencil width");
     return check(s->ctx, "rk4 march stage kernel");
 }
@@ -879,14 +894,22 @@ int launch_march(shx_system* s, int stage, const double* usD, RPtrs usR, const d
 // point-wise form's cut-point update runs outside the brackets).
 int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t* ev)
 {
-    int rc = ensure_fused(s);
     const bool march = s->variant >= 1;
+    const bool mixed = s->precision == SHX_F32_COMP;
+    if (mixed && !march)
+        return fail(s->ctx, SHX_EUNSUPPORTED, "mixed precision needs the x-marching form (variants 1-4)");
+    int rc = ensure_fused(s);
     if (rc == SHX_OK && march) rc = ensure_classes(s);
     for (int stage = 0; stage < 4 && rc == SHX_OK; ++stage) {
-        const double* usD = stage == 0 ? u0->D : s->stageD[(stage - 1) & 1];
+        const void* usV = stage == 0 ? (const void*)u0->D
+                                     : mixed ? (const void*)s->stageF[(stage - 1) & 1]
+                                             : (const void*)s->stageD[(stage - 1) & 1];
+        void* outV = stage == 3 ? (void*)u0->D
+                                : mixed ? (void*)s->stageF[stage & 1] : (void*)s->stageD[stage & 1];
+        const double* usD = static_cast<const double*>(usV);
+        double* outD = static_cast<double*>(outV);
         RPtrs usR;
         for (int c = 0; c < 3; ++c) usR.r[c] = stage == 0 ? u0->R[c] : s->stageR[(stage - 1) & 1][c];
-        double* outD = stage == 3 ? u0->D : s->stageD[stage & 1];
         double* outR[3];
         for (int c = 0; c < 3; ++c) outR[c] = stage == 3 ? u0->R[c] : s->stageR[stage & 1][c];
         const double ts = t + dt * kRki[stage];
@@ -895,7 +918,7 @@ int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t
         const double tn = stage < 3 ? t + dt * kRki[stage + 1] : t + dt;
         if (ev) hipEventRecord(ev[2 * stage], s->ctx->stream);
         if (march)
-            rc = launch_march(s, stage, usD, usR, u0->D, s->accD, outD, outR, ts, c_acc, c_next, tn);
+            rc = launch_march(s, stage, usV, usR, u0->D, s->accD, outV, outR, ts, c_acc, c_next, tn);
         else if (s->kind == SHX_SYS_HEAT)
             rc = launch_stage<SHX_SYS_HEAT>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
         else if (s->kind == SHX_SYS_WAVE_RADIAL)
@@ -1051,6 +1074,8 @@ int shx_system_destroy(shx_system* s)
     }
     hipFree(s->stageD[0]);
     hipFree(s->stageD[1]);
+    hipFree(s->stageF[0]);
+    hipFree(s->stageF[1]);
     hipFree(s->accD);
     hipFree(s->cls);
     hipFree(s->special);
@@ -1249,11 +1274,30 @@ int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xc
     if (variant >= 1 && !(march_ty == 4 || march_ty == 8 || march_ty == 16)) return SHX_EINVAL;
     if (variant >= 1 && march_xchunk < 1) return SHX_EINVAL;
     if (variant >= 1 && !march_ok(s)) return fail(s->ctx, SHX_EUNSUPPORTED, "x-marching form: unsupported system");
+    if (variant == 0 && s->precision == SHX_F32_COMP)
+        return fail(s->ctx, SHX_EUNSUPPORTED, "mixed precision needs the x-marching form (variants 1-4)");
     s->variant = variant;
     if (variant >= 1) {
         s->march_ty = march_ty;
         s->march_xchunk = march_xchunk;
     }
+    return SHX_OK;
+}
+
+int shx_system_set_precision(shx_system* s, int precision)
+{
+    if (!s || (precision != SHX_F64 && precision != SHX_F32_COMP)) return SHX_EINVAL;
+    if (precision == SHX_F32_COMP && !march_ok(s))
+        return fail(s->ctx, SHX_EUNSUPPORTED, "mixed precision: unsupported system");
+    if (precision == SHX_F32_COMP && s->variant == 0) s->variant = 3;
+    s->precision = precision;
+    return SHX_OK;
+}
+
+int shx_system_get_precision(const shx_system* s, int* precision)
+{
+    if (!s || !precision) return SHX_EINVAL;
+    *precision = s->precision;
     return SHX_OK;
 }
 
@@ -1301,7 +1345,8 @@ int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt,
     struct cand {
         int v, ty, xc;
     };
-    std::vector<cand> cands{{0, 4, 32}};
+    std::vector<cand> cands;
+    if (s->precision == SHX_F64) cands.push_back({0, 4, 32});
     if (march_ok(s))
         for (int v = 1; v <= 4; ++v)
             for (int ty : {4, 8, 16})

@@ -274,3 +274,85 @@ def test_context_close_releases_problems():
     c.close()
     assert prob.p is None and f.c is None
     del f, prob
+
+
+# ---------------------------------------------------------------- mixed form
+# No reference counterpart: the mixed form (FP32 stage register, compensated
+# FP32 interior sums, FP64 u0 / accumulator / closures / B terms) is checked
+# against the FP64 form and the exact solution.
+MIXED_CFGS = {
+    "heat3d_E4_2": HEAT_CFGS["heat3d_E4_2"],
+    "wave_box_E4u": dict(H.sphere_wave_cfg(40, scheme=O.E4U_1, alpha=(-0.7733323791884821, 0.1623961700641681)),
+                         spheres=[], obj_bc=[]),
+    "wave30_radial_E4u": WAVE_CFGS["wave30_radial_E4u"],
+}
+
+
+def _run(prob, u0flat, dt, nsteps, precision, variant=3, ty=4, xc=16):
+    import shoccs_amd as S
+
+    prob.set_tuning(variant, ty, xc)
+    prob.set_precision(precision)
+    g = prob.field()
+    g.upload(u0flat)
+    prob.rk4_fused(g, 0.0, dt, nsteps)
+    out = g.download()
+    prob.set_precision(S.F64)
+    return out
+
+
+@pytest.mark.parametrize("name", list(MIXED_CFGS))
+def test_mixed_form_close_to_fp64(ctx, name):
+    """After 5 steps the mixed form differs from the FP64 form by at most
+    1e-5 of the solution's max-norm, and its error against the exact
+    solution is within 1.5x of the FP64 form's (plus 1e-7 of the max-norm)."""
+    import shoccs_amd as S
+
+    cfg = MIXED_CFGS[name]
+    osys, prob = _pair(ctx, cfg)
+    u0 = osys.field()
+    osys.initialize(u0, 0.0)
+    osys.update_boundary(u0, 0.0)
+    dt = osys.dt(0.5, 0.8)
+    n = 5
+    f64 = _run(prob, u0.flat(), dt, n, S.F64)
+    mix = _run(prob, u0.flat(), dt, n, S.F32_COMP)
+    scale = np.max(np.abs(f64))
+    assert np.max(np.abs(mix - f64)) <= 1e-5 * scale
+    exact = osys.exact(n * dt).flat()
+    fl = osys.mesh.fluid()
+    e64 = np.max(np.abs(f64[fl] - exact[fl]))
+    emx = np.max(np.abs(mix[fl] - exact[fl]))
+    assert emx <= 1.5 * e64 + 1e-7 * scale, (emx, e64)
+
+
+@pytest.mark.parametrize("name", ["heat3d_E4_2", "wave_box_E4u"])
+def test_mixed_form_same_doubles_for_every_tiling(ctx, name):
+    """Tile shape and placement of the special work do not change the mixed
+    form's results (the same arithmetic per point).  Bounded solutions only:
+    a solution that outgrows FP32's range (E4_1's singular grid-wall
+    closures) overflows the FP32 stage register."""
+    import shoccs_amd as S
+
+    osys, prob = _pair(ctx, MIXED_CFGS[name])
+    u0 = osys.field()
+    osys.initialize(u0, 0.0)
+    osys.update_boundary(u0, 0.0)
+    dt = osys.dt(0.5, 0.8)
+    ref = _run(prob, u0.flat(), dt, 3, S.F32_COMP, variant=3, ty=4, xc=16)
+    assert np.all(np.isfinite(ref))
+    for v, ty, xc in [(1, 8, 7), (2, 16, 64), (4, 4, 1000)]:
+        assert np.array_equal(_run(prob, u0.flat(), dt, 3, S.F32_COMP, v, ty, xc), ref)
+
+
+def test_mixed_form_needs_the_marching_tiles(ctx):
+    import shoccs_amd as S
+
+    osys, prob = _pair(ctx, HEAT_CFGS["heat3d_E4_2"])
+    prob.set_tuning(0)
+    prob.set_precision(S.F32_COMP)
+    assert prob.tuning()[0] == 3 and prob.precision() == S.F32_COMP
+    with pytest.raises(S.ShoccsError):
+        prob.set_tuning(0)
+    prob.set_precision(S.F64)
+    prob.set_tuning(0)
