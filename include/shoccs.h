@@ -71,6 +71,41 @@ int64_t shoccs_deriv_npool(const shoccs_problem* p, int dir);
 int shoccs_deriv_pool(const shoccs_problem* p, int dir, double* out);
 int shoccs_deriv_interior(const shoccs_problem* p, int dir, double* out9);
 
+/* ---------------------------------------------------------------- driver */
+/* integrators of shoccs_run */
+#define SHOCCS_RK4 0       /* rk4, the reference's slot schedule (shx_rk4_step) */
+#define SHOCCS_RK4_FUSED 1 /* rk4, fused form at the system's precision and tuning */
+#define SHOCCS_EULER 2     /* euler (shx_euler_step) */
+
+typedef struct shoccs_run_desc {
+    int32_t integrator;
+    /* step_controller (src/temporal/step_controller.cpp:13-27): run while
+     * step < max_step and time < max_time; both left at INT32_MAX / DBL_MAX
+     * means zero steps.  dt = h_cfl h_min (wave) or p_cfl h_min^2 / 4k (heat);
+     * a dt below min_dt ends the run prematurely. */
+    int32_t max_step;
+    double max_time;
+    double h_cfl, p_cfl, min_dt;
+    double* stats;     /* optional: up to stats_cap rows of the 11 statistics, */
+    int64_t stats_cap; /* row 0 at t = 0, then one row per step */
+} shoccs_run_desc;
+
+typedef struct shoccs_run_result {
+    double time;       /* controller time at the end */
+    int32_t step;      /* steps taken */
+    int32_t completed; /* 1: the controller ran out; 0: ended prematurely (invalid
+                        * statistics or dt < min_dt) */
+    double linf;       /* the last L-inf error when completed, NaN otherwise */
+    int64_t nstats;    /* rows written to stats */
+} shoccs_run_result;
+
+/* simulation_cycle::run (src/simulation/simulation_cycle.cpp:31-143) without
+ * I/O: initialize u at t = 0, update its boundary, then step while the
+ * controller runs and system::valid(stats) holds (scalar_wave.cpp:108-112:
+ * finite and |L-inf| <= max_error; heat.cpp:54-58: <= 1e6).  u holds the
+ * final solution.  Synchronous. */
+int shoccs_run(shoccs_problem* p, const shoccs_run_desc* d, shx_field* u, shoccs_run_result* r);
+
 /* device fields of the problem's shape: D then Rx, Ry, Rz */
 int shoccs_field_alloc(shoccs_problem* p, shx_field* f);
 int shoccs_field_free(shoccs_problem* p, shx_field* f);

@@ -210,6 +210,11 @@ int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[1
 int shx_rk4_step(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk,
                  shx_field* srhs, double t, double dt);
 
+/* integrators::euler (src/temporal/euler.cpp:11-27): out <- u0, zero srhs,
+ * srhs = RHS(out) at t, out = u0 + dt srhs, update_boundary(out, t + dt). */
+int shx_euler_step(shx_system* s, shx_field* u0, shx_field* out, shx_field* srhs, double t,
+                   double dt);
+
 /* The same step with the RHS, the stage update and the boundary update fused
  * into one kernel per stage over three registers (u0, stage, accumulator):
  * u0 is advanced in place by nsteps steps of size dt from time t.  Scratch

@@ -21,7 +21,7 @@
  *              src/operators/gradient.cpp:34-46, laplacian.cpp:37-58
  *   systems    src/systems/scalar_wave.cpp:28-362, heat.cpp:111-361,
  *              src/systems/detail/scalar_system_utils.hpp:16-219
- *   temporal   src/temporal/rk4.cpp:11-47, slot_ops.hpp:11-59
+ *   temporal   src/temporal/rk4.cpp:11-47, euler.cpp:11-27, slot_ops.hpp:11-59
  *
  * Arithmetic is compiled with -ffp-contract=off so that every sum and product
  * rounds exactly as the reference's x86-64 build does.
@@ -186,6 +186,8 @@ void orc_system_initialize(orc_system*, orc_field* u, double t);
 void orc_system_exact(orc_system*, orc_field* out, double t);
 void orc_system_stats(orc_system*, const orc_field* u, double t, double stats[11]);
 double orc_system_dt(orc_system*, double h_cfl, double p_cfl);
+void orc_euler_step(orc_system*, orc_field* u0, orc_field* out, orc_field* srhs, double t,
+                    double dt);
 void orc_rk4_step(orc_system*, orc_field* u0, orc_field* out, orc_field* rk,
                   orc_field* srhs, double t, double dt);
 

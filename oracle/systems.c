@@ -502,6 +502,7 @@ double orc_system_dt(orc_system* s, double h_cfl, double p_cfl)
     return h_cfl * hmin;
 }
 
+
 /* ----------------------------------------------------------------- rk4 */
 static void slot_assign_lc(const orc_mesh* m, orc_field* dst, const orc_field* src, double c,
                            const orc_field* r)
@@ -521,6 +522,21 @@ static void slot_accumulate(const orc_mesh* m, orc_field* dst, double c, const o
     for (int d = 0; d < 3; d++)
         for (long i = 0; i < m->nR[d]; i++) dst->R[d][i] += c * r->R[d][i];
 }
+
+/* euler::operator() (src/temporal/euler.cpp:11-27): out <- u0, zero srhs,
+ * srhs = RHS(out) at t, out = u0 + dt * srhs, boundary at t + dt */
+void orc_euler_step(orc_system* s, orc_field* u0, orc_field* out, orc_field* srhs, double t,
+                    double dt)
+{
+    const orc_mesh* m = s->m;
+    memcpy(out->D, u0->D, (size_t)orc_mesh_size(m) * sizeof(double));
+    for (int d = 0; d < 3; d++) memcpy(out->R[d], u0->R[d], (size_t)m->nR[d] * sizeof(double));
+    field_zero(m, srhs);
+    orc_system_rhs(s, out, srhs, t);
+    slot_assign_lc(m, out, u0, dt, srhs);
+    orc_system_update_boundary(s, out, t + dt);
+}
+
 
 void orc_rk4_step(orc_system* s, orc_field* u0, orc_field* out, orc_field* rk,
                   orc_field* srhs, double t, double dt)

@@ -24,14 +24,15 @@ import weakref
 
 import numpy as np
 
-from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, E2_2, E4_1, E4_2, E4U_1, F32_COMP, F64,
-                   MMS_GAUSS, MMS_SINPROD, SYS_HEAT, SYS_WAVE_LINEAR, SYS_WAVE_RADIAL,
-                   ProblemDesc, ShoccsError, ShxField, ShxLineBlock, check, lib)
+from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, E2_2, E4_1, E4_2, E4U_1, EULER,
+                   F32_COMP, F64, MMS_GAUSS, MMS_SINPROD, RK4, RK4_FUSED, SYS_HEAT,
+                   SYS_WAVE_LINEAR, SYS_WAVE_RADIAL, ProblemDesc, RunDesc, RunResult, ShoccsError,
+                   ShxField, ShxLineBlock, check, lib)
 
 __all__ = [
     "Context", "Problem", "Field", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
     "E2_2", "E4_2", "E4_1", "E4U_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
-    "MMS_GAUSS", "MMS_SINPROD", "F64", "F32_COMP", "eq", "plus_eq",
+    "MMS_GAUSS", "MMS_SINPROD", "F64", "F32_COMP", "RK4", "RK4_FUSED", "EULER", "eq", "plus_eq",
 ]
 
 eq = 0
@@ -303,6 +304,36 @@ class Problem:
         self._need_device()
         check(lib().shx_rk4_step(self.sys, u0.ref(), out.ref(), rk.ref(), srhs.ref(), t, dt),
               self.ctx.p, "rk4")
+
+    def euler(self, u0: Field, out: Field, srhs: Field, t: float, dt: float):
+        """integrators::euler, the reference's slot schedule (shx_euler_step)."""
+        self._need_device()
+        check(lib().shx_euler_step(self.sys, u0.ref(), out.ref(), srhs.ref(), t, dt), self.ctx.p,
+              "euler")
+
+    def run(self, u: Field, integrator: int = RK4, max_step: int | None = None,
+            max_time: float | None = None, h_cfl: float = 1.0, p_cfl: float = 1.0,
+            min_dt: float = 1e-6, keep_stats: bool = True) -> dict:
+        """simulation_cycle::run without I/O (shoccs_run): returns the final
+        time, step count, whether the controller ran out (`completed`), the
+        final L-inf error (NaN when ended prematurely) and the statistics of
+        every step (row 0 at t = 0)."""
+        self._need_device()
+        d = RunDesc()
+        d.integrator = integrator
+        d.max_step = 2 ** 31 - 1 if max_step is None else max_step
+        d.max_time = float(np.finfo(np.float64).max) if max_time is None else max_time
+        d.h_cfl, d.p_cfl, d.min_dt = h_cfl, p_cfl, min_dt
+        cap = 0
+        if keep_stats:
+            cap = (max_step if max_step is not None else 100000) + 1
+        buf = np.zeros((max(cap, 1), 11))
+        d.stats = buf.ctypes.data_as(C.POINTER(C.c_double)) if cap else None
+        d.stats_cap = cap
+        r = RunResult()
+        check(lib().shoccs_run(self.p, C.byref(d), u.ref(), C.byref(r)), self.ctx.p, "run")
+        return dict(time=r.time, step=r.step, completed=bool(r.completed), linf=r.linf,
+                    stats=buf[: r.nstats].copy())
 
     def rk4_fused(self, u0: Field, t: float, dt: float, nsteps: int = 1):
         self._need_device()

@@ -51,6 +51,32 @@ class ProblemDesc(C.Structure):
     ]
 
 
+class RunDesc(C.Structure):
+    _fields_ = [
+        ("integrator", C.c_int32),
+        ("max_step", C.c_int32),
+        ("max_time", C.c_double),
+        ("h_cfl", C.c_double),
+        ("p_cfl", C.c_double),
+        ("min_dt", C.c_double),
+        ("stats", dp),
+        ("stats_cap", C.c_int64),
+    ]
+
+
+class RunResult(C.Structure):
+    _fields_ = [
+        ("time", C.c_double),
+        ("step", C.c_int32),
+        ("completed", C.c_int32),
+        ("linf", C.c_double),
+        ("nstats", C.c_int64),
+    ]
+
+
+RK4, RK4_FUSED, EULER = 0, 1, 2
+
+
 class ShoccsError(RuntimeError):
     pass
 
@@ -90,6 +116,8 @@ _SIGS = [
     ("shx_system_initialize", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double]),
     ("shx_system_exact", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double]),
     ("shx_system_stats", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, dp]),
+    ("shx_euler_step", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
+                                 C.POINTER(ShxField), C.c_double, C.c_double]),
     ("shx_rk4_step", C.c_int, [C.c_void_p] + [C.POINTER(ShxField)] * 4 + [C.c_double, C.c_double]),
     ("shx_rk4_fused_steps_async", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double,
                                             C.c_int]),
@@ -124,6 +152,8 @@ _SIGS = [
     ("shoccs_field_free", C.c_int, [C.c_void_p, C.POINTER(ShxField)]),
     ("shoccs_field_upload", C.c_int, [C.c_void_p, C.POINTER(ShxField), dp]),
     ("shoccs_field_download", C.c_int, [C.c_void_p, C.POINTER(ShxField), dp]),
+    ("shoccs_run", C.c_int, [C.c_void_p, C.POINTER(RunDesc), C.POINTER(ShxField),
+                             C.POINTER(RunResult)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -1182,6 +1182,18 @@ int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[1
     return SHX_OK;
 }
 
+int shx_euler_step(shx_system* s, shx_field* u0, shx_field* out, shx_field* srhs, double t,
+                   double dt)
+{
+    if (!s || !u0 || !out || !srhs) return SHX_EINVAL;
+    int rc = slot_copy(s, out, u0);
+    if (rc == SHX_OK) rc = zero_slot(s, srhs);
+    if (rc == SHX_OK) rc = rhs_unfused_launch(s, out, srhs, t);
+    if (rc == SHX_OK) rc = slot_assign_lc(s, out, u0, dt, srhs);
+    if (rc == SHX_OK) rc = update_boundary_launch(s, out, t + dt);
+    return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
 int shx_rk4_step(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk, shx_field* srhs,
                  double t, double dt)
 {

@@ -1,7 +1,11 @@
 // problem.cpp -- C entry points of the host library (include/shoccs.h): build
 // a mesh, its boundary conditions, stencil and system, upload them through
 // shoccs_hip.h, and expose the assembled structures.
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <new>
 #include <vector>
@@ -20,6 +24,7 @@ struct problem {
     std::unique_ptr<derivative> d[3];
     shx_system* sys = nullptr;
     double diffusivity = 1.0;
+    double max_error = 100.0;  // system::valid bound
 };
 
 }  // namespace ccs
@@ -94,6 +99,11 @@ int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_
     sd.fluid = m.fluid().data();
     sd.nfluid = (int64_t)m.fluid().size();
     std::vector<double> mms;
+    // valid() bounds: scalar_wave max_error (default 100, scalar_wave.cpp:137),
+    // heat 1e6 (heat.cpp:54-58)
+    if (desc->kind == SHX_SYS_WAVE_RADIAL && np > 4) p->max_error = pr[4];
+    if (desc->kind == SHX_SYS_WAVE_LINEAR && np > 8) p->max_error = pr[8];
+    if (desc->kind == SHX_SYS_HEAT) p->max_error = 1e6;
     if (desc->kind == SHX_SYS_WAVE_RADIAL) {
         for (int c = 0; c < 3; ++c) sd.center[c] = pr[c];
         sd.radius = pr[3];
@@ -282,6 +292,92 @@ int shoccs_field_download(shoccs_problem* p, const shx_field* f, double* flat)
         off += p->m->R(a).size();
     }
     return rc;
+}
+
+namespace {
+
+int copy_field(shx_ctx* ctx, const shoccs_problem* p, shx_field* dst, const shx_field* src)
+{
+    int rc = shx_copy(ctx, dst->D, src->D, shoccs_problem_size(p, 0));
+    for (int a = 0; a < 3 && rc == SHX_OK; ++a) {
+        const int64_t n = shoccs_problem_size(p, a + 1);
+        if (n > 0) rc = shx_copy(ctx, dst->R[a], src->R[a], n);
+    }
+    return rc;
+}
+
+}  // namespace
+
+int shoccs_run(shoccs_problem* p, const shoccs_run_desc* d, shx_field* u, shoccs_run_result* r)
+{
+    if (!p || !d || !u || !r || !p->sys || !p->ctx) return SHX_EINVAL;
+    if (d->integrator < SHOCCS_RK4 || d->integrator > SHOCCS_EULER) return SHX_EINVAL;
+    std::memset(r, 0, sizeof *r);
+    r->linf = std::numeric_limits<double>::quiet_NaN();
+    int max_step = d->max_step;
+    const double max_time = d->max_time;
+    if (max_step == INT32_MAX && max_time == DBL_MAX) max_step = 0;
+    shx_system* sys = p->sys;
+    shx_ctx* ctx = p->ctx;
+    const bool fused = d->integrator == SHOCCS_RK4_FUSED;
+
+    // u1 (the step output), rk and srhs registers of the reference schedule
+    shx_field u1{}, rk{}, srhs{};
+    int rc = shoccs_field_alloc(p, &u1);
+    if (rc == SHX_OK && d->integrator == SHOCCS_RK4) rc = shoccs_field_alloc(p, &rk);
+    if (rc == SHX_OK && !fused) rc = shoccs_field_alloc(p, &srhs);
+
+    double stats[11] = {0};
+    double t = 0.0;
+    int step = 0;
+    auto record = [&]() {
+        if (d->stats && r->nstats < d->stats_cap) {
+            std::memcpy(d->stats + 11 * r->nstats, stats, sizeof stats);
+            ++r->nstats;
+        }
+    };
+    auto running = [&]() { return 0.0 <= t && t < max_time && 0 <= step && step < max_step; };
+    auto valid = [&]() { return std::isfinite(stats[0]) && std::abs(stats[0]) <= p->max_error; };
+
+    // simulation_cycle.cpp:59-64: initialize u0, u1 <- u0, update u0's
+    // boundary, statistics of u1
+    if (rc == SHX_OK) rc = shx_system_initialize(sys, u, 0.0);
+    if (rc == SHX_OK) rc = copy_field(ctx, p, &u1, u);
+    if (rc == SHX_OK) rc = shx_system_update_boundary(sys, u, 0.0);
+    if (rc == SHX_OK) rc = shx_system_stats(sys, &u1, 0.0, stats);
+    if (rc == SHX_OK) record();
+    bool dt_ok = true;
+    while (rc == SHX_OK && running() && valid()) {
+        const double dt = shoccs_problem_dt(p, d->h_cfl, d->p_cfl);
+        if (!(dt >= d->min_dt)) {  // step_controller::check_timestep_size
+            dt_ok = false;
+            break;
+        }
+        if (d->integrator == SHOCCS_RK4) {
+            rc = shx_rk4_step(sys, u, &u1, &rk, &srhs, t, dt);
+        } else if (d->integrator == SHOCCS_EULER) {
+            rc = shx_euler_step(sys, u, &u1, &srhs, t, dt);
+        } else {
+            rc = shx_rk4_fused_steps_async(sys, u, t, dt, 1);
+            if (rc == SHX_OK) rc = shx_sync(ctx);
+        }
+        if (rc != SHX_OK) break;
+        t += dt;  // controller.advance
+        step += 1;
+        rc = shx_system_stats(sys, fused ? u : &u1, t, stats);
+        if (rc == SHX_OK) record();
+        if (rc == SHX_OK && !fused) rc = copy_field(ctx, p, u, &u1);  // u0 <- u1
+    }
+    if (rc == SHX_OK) rc = shx_sync(ctx);
+    shoccs_field_free(p, &u1);
+    if (rk.D) shoccs_field_free(p, &rk);
+    if (srhs.D) shoccs_field_free(p, &srhs);
+    if (rc != SHX_OK) return rc;
+    r->time = t;
+    r->step = step;
+    r->completed = dt_ok && !running();
+    if (r->completed) r->linf = stats[0];
+    return SHX_OK;
 }
 
 }  // extern "C"

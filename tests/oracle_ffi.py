@@ -160,6 +160,7 @@ def lib():
     L.orc_system_dt.restype = C.c_double
     L.orc_system_dt.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.orc_rk4_step.argtypes = [C.c_void_p] + [C.POINTER(Field)] * 4 + [C.c_double, C.c_double]
+    L.orc_euler_step.argtypes = [C.c_void_p] + [C.POINTER(Field)] * 3 + [C.c_double, C.c_double]
     L.orc_gradient_apply.argtypes = [C.c_void_p] + [C.POINTER(Field)] * 4
     L.orc_laplacian_apply.argtypes = [C.c_void_p] + [C.POINTER(Field)] * 3
     L.orc_set_threads.argtypes = [C.c_int]
@@ -394,6 +395,9 @@ class OracleSystem:
 
     def rk4_step(self, u0, out, rk, srhs, t, dt):
         lib().orc_rk4_step(self.p, u0.ref(), out.ref(), rk.ref(), srhs.ref(), t, dt)
+
+    def euler_step(self, u0, out, srhs, t, dt):
+        lib().orc_euler_step(self.p, u0.ref(), out.ref(), srhs.ref(), t, dt)
 
     def gradient(self, u: OracleField):
         outs = [self.field() for _ in range(3)]

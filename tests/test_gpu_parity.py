@@ -356,3 +356,115 @@ def test_mixed_form_needs_the_marching_tiles(ctx):
         prob.set_tuning(0)
     prob.set_precision(S.F64)
     prob.set_tuning(0)
+
+
+# ------------------------------------------------- euler and the driver
+@pytest.mark.parametrize("name", ["wave48_E4_1", "heat3d_E4_2", "heat2d_21"])
+def test_euler_steps(ctx, name):
+    """integrators::euler on the device against the oracle's step (1e-11
+    relative max-norm over 3 steps, as the RK4 steps)."""
+    cfg = ALL[name]
+    osys, prob = _pair(ctx, cfg)
+    u0 = osys.field()
+    osys.initialize(u0, 0.0)
+    osys.update_boundary(u0, 0.0)
+    dt = 0.25 * osys.dt(0.5, 0.8)
+    g = prob.field()
+    g.upload(u0.flat())
+    go, gs = prob.field(), prob.field()
+    out, srhs = osys.field(), osys.field()
+    t = 0.0
+    for _ in range(3):
+        osys.euler_step(u0, out, srhs, t, dt)
+        u0.D[:] = out.D
+        for d in range(3):
+            u0.R[d][:] = out.R[d]
+        prob.euler(g, go, gs, t, dt)
+        g.upload(go.download())
+        t += dt
+    got, want = g.download(), u0.flat()
+    assert np.max(np.abs(got - want)) <= 1e-11 * np.max(np.abs(want))
+
+
+def _oracle_cycle(osys, integrator, max_step, max_time, h_cfl, p_cfl, max_error):
+    """simulation_cycle::run restated over the oracle (stats on u1, the
+    controller bounds, valid())."""
+    import shoccs_amd as S
+
+    u0 = osys.field()
+    osys.initialize(u0, 0.0)
+    u1 = osys.field()
+    u1.D[:] = u0.D
+    for d in range(3):
+        u1.R[d][:] = u0.R[d]
+    osys.update_boundary(u0, 0.0)
+    st = osys.stats(u1, 0.0)
+    rows = [st]
+    t, step = 0.0, 0
+    rk, srhs = osys.field(), osys.field()
+    valid = lambda v: np.isfinite(v) and abs(v) <= max_error
+    while t < max_time and step < max_step and valid(st[0]):
+        dt = osys.dt(h_cfl, p_cfl)
+        if integrator == S.EULER:
+            osys.euler_step(u0, u1, srhs, t, dt)
+        else:
+            osys.rk4_step(u0, u1, rk, srhs, t, dt)
+        t += dt
+        step += 1
+        st = osys.stats(u1, t)
+        rows.append(st)
+        u0.D[:] = u1.D
+        for d in range(3):
+            u0.R[d][:] = u1.R[d]
+    completed = not (t < max_time and step < max_step)
+    return dict(time=t, step=step, completed=completed, u=u0.flat(), stats=np.array(rows))
+
+
+@pytest.mark.parametrize("integrator", ["RK4", "RK4_FUSED", "EULER"])
+def test_driver_matches_the_cycle(ctx, integrator):
+    """shoccs_run follows simulation_cycle::run: same number of steps, end
+    time (max_time is not a multiple of dt: the last step overshoots), final
+    state and per-step statistics as the loop over the oracle."""
+    import shoccs_amd as S
+
+    cfg = HEAT_CFGS["heat2d_21"]
+    osys, prob = _pair(ctx, cfg)
+    integ = getattr(S, integrator)
+    dt = osys.dt(1.0, 0.8)
+    max_time = 6.5 * dt
+    want = _oracle_cycle(osys, S.RK4 if integ == S.RK4_FUSED else integ, 10 ** 6, max_time, 1.0,
+                         0.8, 1e6)
+    g = prob.field()
+    got = prob.run(g, integ, max_time=max_time, h_cfl=1.0, p_cfl=0.8)
+    assert got["step"] == want["step"] == 7 and got["time"] == want["time"]
+    assert got["completed"] and want["completed"]
+    scale = np.max(np.abs(want["u"]))
+    assert np.max(np.abs(g.download() - want["u"])) <= 1e-11 * scale
+    assert got["stats"].shape == want["stats"].shape
+    np.testing.assert_allclose(got["stats"][:, :4], want["stats"][:, :4], rtol=1e-9, atol=1e-15)
+    assert got["linf"] == got["stats"][-1, 0]
+
+
+def test_driver_stops_on_invalid_stats(ctx):
+    """valid() false ends the run prematurely: the L-inf error is reported
+    as NaN, and the step that broke the bound is the last one."""
+    import shoccs_amd as S
+
+    cfg = dict(WAVE_CFGS["wave30_radial_E4u"])
+    cfg["params"] = list(cfg["params"][:4]) + [1e-3]  # max_error
+    osys, prob = _pair(ctx, cfg)
+    g = prob.field()
+    got = prob.run(g, S.RK4_FUSED, max_step=50, h_cfl=0.5)
+    assert not got["completed"] and np.isnan(got["linf"])
+    assert 0 < got["step"] < 50
+    assert got["stats"][-1, 0] > 1e-3 and np.all(got["stats"][:-1, 0] <= 1e-3)
+
+
+def test_driver_zero_steps_without_bounds(ctx):
+    """step_controller with neither max_step nor max_time runs no step."""
+    import shoccs_amd as S
+
+    osys, prob = _pair(ctx, HEAT_CFGS["heat2d_21"])
+    g = prob.field()
+    got = prob.run(g, S.RK4)
+    assert got["step"] == 0 and got["completed"] and got["stats"].shape[0] == 1

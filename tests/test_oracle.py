@@ -411,3 +411,17 @@ def test_stats_over_fluid_points():
     u.D[fl[9]] = np.nan
     st = s.stats(u, 0.1)
     assert abs(st[0] - 0.25) < 1e-15 and int(st[4]) == fl[7]
+
+
+def test_euler_step_is_the_first_order_taylor_polynomial():
+    """euler.cpp:11-27 on an autonomous linear problem (no boundary data):
+    one step is u0 + dt L u0, with L u from the (pinned) rhs."""
+    cfg, _ = _box_wave(O.SYS_WAVE_LINEAR, [O.FLOATING] * 6, n=(14, 16, 18))
+    s = H.oracle_system(cfg)
+    u0 = s.field()
+    u0.D[:] = np.random.default_rng(12).standard_normal(u0.D.size)
+    dt = 0.7 * s.dt(0.5, 1.0)
+    want = u0.D + dt * s.rhs(u0, 0.0).D
+    out, srhs = s.field(), s.field()
+    s.euler_step(u0, out, srhs, 0.0, dt)
+    assert np.array_equal(out.D, want)
