@@ -254,6 +254,28 @@ int shx_system_stage_points(shx_system* s, int64_t out[2]);
  * keeps the fastest and returns its ms per step. */
 int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt, double* best_ms);
 
+/* ------------------------------------------------------------------- batch */
+/* Many independent problems stepped together (no reference counterpart; the
+ * reference steps one problem per process): every fused RK4 stage of all of
+ * them is one launch, their tile workgroups side by side, with per-problem
+ * arguments and clocks (t = t0 + n dt) in device memory so that a captured
+ * hipGraph stays valid at any time.  The systems must share the context,
+ * system family, stencil width and precision, and take the x-marching form
+ * (tile height march_ty for all; each keeps its x chunk).  u0[p] is advanced
+ * in place; the batch keeps the field descriptors, not the memory. */
+typedef struct shx_batch shx_batch;
+int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* u0,
+                     const double* t0, const double* dt, int n, int march_ty, shx_batch** out);
+int shx_batch_destroy(shx_batch* b);
+/* nsteps steps of every problem (4 stage launches + 1 clock launch each) */
+int shx_batch_steps_async(shx_batch* b, int nsteps);
+/* capture nsteps steps as one hipGraph owned by the batch; each
+ * shx_batch_launch_async replays it (advancing every problem nsteps steps) */
+int shx_batch_capture(shx_batch* b, int nsteps);
+int shx_batch_launch_async(shx_batch* b);
+/* current time of every problem */
+int shx_batch_times(shx_batch* b, double* t);
+
 /* ------------------------------------------------------------- measurement */
 /* Run `nsteps` fused steps `reps` times and report the mean device time per
  * step in milliseconds, measured with HIP events on the context's stream. */
@@ -267,6 +289,9 @@ int shx_time_unfused_steps(shx_system* s, shx_field* u0, shx_field* out, shx_fie
  * by a HIP event pair on the context's stream. */
 int shx_time_fused_stage_kernel(shx_system* s, shx_field* u0, double t, double dt, int nsteps,
                                 double* ms_per_kernel);
+/* Mean device time per step of `reps` batch steps (use_graph = 0) or graph
+ * replays (use_graph = 1, per captured step); points = total points of D. */
+int shx_time_batch(shx_batch* b, int use_graph, int reps, double* ms_per_step, int64_t* points);
 
 #ifdef __cplusplus
 }

@@ -30,7 +30,7 @@ from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, E2_2, E4_1, E4_2, E4U_
                    ShxField, ShxLineBlock, check, lib)
 
 __all__ = [
-    "Context", "Problem", "Field", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
+    "Context", "Problem", "Field", "Batch", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
     "E2_2", "E4_2", "E4_1", "E4U_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
     "MMS_GAUSS", "MMS_SINPROD", "F64", "F32_COMP", "RK4", "RK4_FUSED", "EULER", "eq", "plus_eq",
 ]
@@ -50,6 +50,7 @@ class Context:
             raise ShoccsError(f"shx_init({device}) failed with code {rc}: no usable GPU")
         self.p = p
         self._problems = weakref.WeakSet()
+        self._batches = weakref.WeakSet()
 
     def sync(self):
         check(lib().shx_sync(self.p), self.p, "shx_sync")
@@ -75,6 +76,8 @@ class Context:
     def close(self):
         """Release every problem made on this context, then the context."""
         if getattr(self, "p", None):
+            for bat in list(getattr(self, "_batches", ())):
+                bat.close()
             for prob in list(getattr(self, "_problems", ())):
                 prob.close()
             lib().shx_finalize(self.p)
@@ -171,8 +174,13 @@ class Problem:
 
     def close(self):
         """Free the problem's live fields, then the problem (device state
-        included); safe to call more than once."""
+        included); safe to call more than once.  Batches that step this
+        problem are closed first."""
         if getattr(self, "p", None):
+            if self.ctx is not None:
+                for bat in list(getattr(self.ctx, "_batches", ())):
+                    if any(p is self for p in bat.problems):
+                        bat.close()
             for f in list(getattr(self, "_fields", ())):
                 f.free()
             lib().shoccs_problem_destroy(self.p)
@@ -400,3 +408,68 @@ class Problem:
         check(lib().shx_time_unfused_steps(self.sys, u0.ref(), out.ref(), rk.ref(), srhs.ref(), t, dt,
                                            nsteps, C.byref(ms)), self.ctx.p, "time_unfused")
         return ms.value
+
+
+class Batch:
+    """Problems stepped together, one launch per RK4 stage (shx_batch_*).
+
+    ``problems`` share the context, system family, stencil width and
+    precision; ``fields[p]`` (one per problem) is advanced in place from time
+    ``t0[p]`` with step ``dt[p]``.  ``capture(n)`` records n steps as a
+    hipGraph that ``launch()`` replays.
+    """
+
+    def __init__(self, ctx: Context, problems, fields, t0, dt, march_ty: int = 4):
+        n = len(problems)
+        assert n == len(fields) == len(t0) == len(dt) and n > 0
+        self.ctx = ctx
+        self.problems = list(problems)
+        self.fields = list(fields)  # the batch refers to their device memory
+        syss = (C.c_void_p * n)(*[p.sys for p in problems])
+        flds = (ShxField * n)(*[f.c for f in fields])
+        t0a = np.ascontiguousarray(t0, dtype=np.float64)
+        dta = np.ascontiguousarray(dt, dtype=np.float64)
+        h = C.c_void_p()
+        check(lib().shx_batch_create(ctx.p, syss, flds, t0a.ctypes.data_as(C.POINTER(C.c_double)),
+                                     dta.ctypes.data_as(C.POINTER(C.c_double)), n, march_ty,
+                                     C.byref(h)), ctx.p, "batch create")
+        self.h = h
+        self.n = n
+        ctx._batches.add(self)
+
+    def steps(self, nsteps: int):
+        check(lib().shx_batch_steps_async(self.h, nsteps), self.ctx.p, "batch steps")
+        self.ctx.sync()
+
+    def capture(self, nsteps: int):
+        check(lib().shx_batch_capture(self.h, nsteps), self.ctx.p, "batch capture")
+
+    def launch(self):
+        check(lib().shx_batch_launch_async(self.h), self.ctx.p, "batch launch")
+        self.ctx.sync()
+
+    def times(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        check(lib().shx_batch_times(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), self.ctx.p,
+              "batch times")
+        return out
+
+    def time_steps(self, use_graph: bool, reps: int) -> tuple:
+        """(ms per step, total points) over `reps` steps or graph replays."""
+        ms = C.c_double()
+        pts = C.c_int64()
+        check(lib().shx_time_batch(self.h, int(use_graph), reps, C.byref(ms), C.byref(pts)),
+              self.ctx.p, "batch time")
+        return ms.value, pts.value
+
+    def close(self):
+        if getattr(self, "h", None) and getattr(self.ctx, "p", None):
+            lib().shx_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -152,6 +152,14 @@ _SIGS = [
     ("shoccs_field_free", C.c_int, [C.c_void_p, C.POINTER(ShxField)]),
     ("shoccs_field_upload", C.c_int, [C.c_void_p, C.POINTER(ShxField), dp]),
     ("shoccs_field_download", C.c_int, [C.c_void_p, C.POINTER(ShxField), dp]),
+    ("shx_batch_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(ShxField), dp, dp,
+                                   C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    ("shx_batch_destroy", C.c_int, [C.c_void_p]),
+    ("shx_batch_steps_async", C.c_int, [C.c_void_p, C.c_int]),
+    ("shx_batch_capture", C.c_int, [C.c_void_p, C.c_int]),
+    ("shx_batch_launch_async", C.c_int, [C.c_void_p]),
+    ("shx_batch_times", C.c_int, [C.c_void_p, dp]),
+    ("shx_time_batch", C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, C.POINTER(C.c_int64)]),
     ("shoccs_run", C.c_int, [C.c_void_p, C.POINTER(RunDesc), C.POINTER(ShxField),
                              C.POINTER(RunResult)]),
 ]

@@ -76,6 +76,12 @@ __global__ void k_classify(DevSys s, DevOps ops, unsigned char* __restrict__ cls
     cls[idx] = c;
 }
 
+__global__ void k_batch_advance(BatchClock* __restrict__ clk, int nprob)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nprob) clk[p].n += 1;
+}
+
 }  // namespace
 
 int launch_classify(hipStream_t st, const DevSys& s, const DevOps& ops, unsigned char* cls, long N)
@@ -86,9 +92,21 @@ int launch_classify(hipStream_t st, const DevSys& s, const DevOps& ops, unsigned
 }
 
 int launch_stage_march(hipStream_t st, int kind, int stage, int width, int ty, int mode,
-                       const MarchArgs& a, const SpecialArgs& sp)
+                       const MarchArgs& a, const SpecialArgs& sp, const StageScalars& ss)
 {
-    return launch_stage_form<false>(st, kind, stage, width, ty, mode, a, sp);
+    return launch_stage_form<false>(st, kind, stage, width, ty, mode, a, sp, ss);
+}
+
+int launch_batch_stage(hipStream_t st, int kind, int stage, int width, int ty, const BatchLaunch& bl)
+{
+    return launch_batch_form<false>(st, kind, stage, width, ty, bl);
+}
+
+int launch_batch_advance(hipStream_t st, BatchClock* clk, int nprob)
+{
+    if (nprob <= 0) return 0;
+    k_batch_advance<<<(unsigned)((nprob + 255) / 256), 256, 0, st>>>(clk, nprob);
+    return 0;
 }
 
 }  // namespace shx

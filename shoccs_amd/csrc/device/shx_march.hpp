@@ -65,7 +65,7 @@ struct StageTypes {
 
 // One march tile: logical workgroup L of the tile grid.
 template <int KIND, int STAGE, int W, int TY, bool MIXED>
-__device__ __forceinline__ void march_tile(const MarchArgs& a, unsigned L)
+__device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalars& ss, unsigned L)
 {
     using TI = typename StageTypes<MIXED, STAGE>::in;
     using TO = typename StageTypes<MIXED, STAGE>::out;
@@ -248,19 +248,19 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, unsigned L)
                         if (px) lap += d0;
                         if (py) lap += d1;
                         if (pz) lap += d2;
-                        rhs = heat_finish(a.s, false, true, x, j, k, lap, a.t_s);
+                        rhs = heat_finish(a.s, false, true, x, j, k, lap, ss.t_s);
                     } else {
                         rhs = wave_combine<KIND>(a.s, false, x, j, k, d0, d1, d2);
                     }
                 } else {
-                    rhs = KIND == SHX_SYS_HEAT ? heat_finish(a.s, false, false, x, j, k, 0.0, a.t_s)
+                    rhs = KIND == SHX_SYS_HEAT ? heat_finish(a.s, false, false, x, j, k, 0.0, ss.t_s)
                                                : wave_combine<KIND>(a.s, false, x, j, k, 0.0, 0.0, 0.0);
                 }
                 // stage_update, with non-temporal stores
-                const double aa = (STAGE == 0 ? 0.0 : accv) + a.c_acc * rhs;
+                const double aa = (STAGE == 0 ? 0.0 : accv) + ss.c_acc * rhs;
                 if (STAGE < 3) {
                     __builtin_nontemporal_store(aa, a.acc + idx);
-                    v = u0v + a.c_next * rhs;
+                    v = u0v + ss.c_next * rhs;
                 } else {
                     v = u0v + 1.0 * aa;
                 }
@@ -313,7 +313,7 @@ __device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRo
 // Item q of the special work: a listed point, then the cut points.  Rows are
 // evaluated in FP64 in both forms.
 template <int KIND, int STAGE, int W, bool MIXED>
-__device__ __forceinline__ void special_item(const SpecialArgs& a, long q)
+__device__ __forceinline__ void special_item(const SpecialArgs& a, const StageScalars& ss, long q)
 {
     using TI = typename StageTypes<MIXED, STAGE>::in;
     using TO = typename StageTypes<MIXED, STAGE>::out;
@@ -340,7 +340,7 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, long q)
                 if (r.row) lap = row_plus(lap, r);
             }
             rhs = heat_finish(a.s, on_dirichlet_face(a.s, i, j, k), is_fluid(a.s, idx), i, j, k, lap,
-                              a.t_s);
+                              ss.t_s);
         } else {
             double du0 = 0.0, du1 = 0.0, du2 = 0.0;
             if (ops.d[0].present) {
@@ -357,26 +357,26 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, long q)
             }
             rhs = wave_combine<KIND>(a.s, on_dirichlet_face(a.s, i, j, k), i, j, k, du0, du1, du2);
         }
-        stage_store<KIND, STAGE>(a.s, rhs, a.u0, a.acc, static_cast<TO*>(a.out), idx, i, j, k, a.c_acc,
-                                 a.c_next, a.t_next);
+        stage_store<KIND, STAGE>(a.s, rhs, a.u0, a.acc, static_cast<TO*>(a.out), idx, i, j, k, ss.c_acc,
+                                 ss.c_next, ss.t_next);
         return;
     }
     q -= a.nspecial;
     if (q < a.nR0) {
         if (a.dir0[q])
-            a.outR0[q] = exact_of<KIND>(a.s, a.pos0[3 * q], a.pos0[3 * q + 1], a.pos0[3 * q + 2], a.t_next);
+            a.outR0[q] = exact_of<KIND>(a.s, a.pos0[3 * q], a.pos0[3 * q + 1], a.pos0[3 * q + 2], ss.t_next);
         return;
     }
     q -= a.nR0;
     if (q < a.nR1) {
         if (a.dir1[q])
-            a.outR1[q] = exact_of<KIND>(a.s, a.pos1[3 * q], a.pos1[3 * q + 1], a.pos1[3 * q + 2], a.t_next);
+            a.outR1[q] = exact_of<KIND>(a.s, a.pos1[3 * q], a.pos1[3 * q + 1], a.pos1[3 * q + 2], ss.t_next);
         return;
     }
     q -= a.nR1;
     if (q < a.nR2) {
         if (a.dir2[q])
-            a.outR2[q] = exact_of<KIND>(a.s, a.pos2[3 * q], a.pos2[3 * q + 1], a.pos2[3 * q + 2], a.t_next);
+            a.outR2[q] = exact_of<KIND>(a.s, a.pos2[3 * q], a.pos2[3 * q + 1], a.pos2[3 * q + 2], ss.t_next);
     }
 }
 
@@ -389,10 +389,11 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, long q)
 // part alone.  Tile workgroups keep their XCD (b mod 8) and map to tiles
 // contiguously per XCD.
 template <int KIND, int STAGE, int W, int TY, bool MIXED>
-__global__ __launch_bounds__(64 * TY) void k_stage(MarchArgs a, SpecialArgs sp, unsigned sg,
-                                                   unsigned mg, int mode)
+__device__ __forceinline__ void stage_block(const MarchArgs& a, const SpecialArgs& sp,
+                                            const StageScalars& ss, unsigned b, unsigned sg,
+                                            unsigned mg, int mode)
 {
-    const unsigned g = blockIdx.x / 8u, lane8 = blockIdx.x % 8u;
+    const unsigned g = b / 8u, lane8 = b % 8u;
     bool special;
     unsigned si, mi;  // group index within its part
     if (mode == 1) {
@@ -413,16 +414,68 @@ __global__ __launch_bounds__(64 * TY) void k_stage(MarchArgs a, SpecialArgs sp, 
     }
     if (special) {
         const long blk = (long)si * 8 + lane8;
-        special_item<KIND, STAGE, W, MIXED>(sp, blk * (64 * TY) + threadIdx.x);
+        special_item<KIND, STAGE, W, MIXED>(sp, ss, blk * (64 * TY) + threadIdx.x);
         return;
     }
     const unsigned m = mi * 8u + lane8;
     if (m >= a.nblocks) return;
-    march_tile<KIND, STAGE, W, TY, MIXED>(a, xcd_remap(m, a.nblocks));
+    march_tile<KIND, STAGE, W, TY, MIXED>(a, ss, xcd_remap(m, a.nblocks));
+}
+
+template <int KIND, int STAGE, int W, int TY, bool MIXED>
+__global__ __launch_bounds__(64 * TY) void k_stage(MarchArgs a, SpecialArgs sp, StageScalars ss,
+                                                   unsigned sg, unsigned mg, int mode)
+{
+    stage_block<KIND, STAGE, W, TY, MIXED>(a, sp, ss, blockIdx.x, sg, mg, mode);
+}
+
+// RK4 stage constants (rk4.cpp:13-14), as the host's
+constexpr double kStageTime[4] = {0.0, 0.5, 0.5, 1.0};
+constexpr double kStageWeight[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+// One fused RK4 stage of a batch of problems in one launch: problem p owns
+// workgroups [start[p], start[p+1]) (multiples of 8, so that tile workgroups
+// keep their XCD); its arguments and clock live in device memory, so that a
+// captured launch stays valid at any time.  The stage scalars are formed as
+// the single-problem host path forms them (t = t0 + n dt).
+template <int KIND, int STAGE, int W, int TY, bool MIXED>
+__global__ __launch_bounds__(64 * TY) void k_batch_stage(const BatchEntry* __restrict__ ent,
+                                                         const BatchClock* __restrict__ clk,
+                                                         const unsigned* __restrict__ start,
+                                                         int nprob)
+{
+    int lo = 0, hi = nprob;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (start[mid] <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    // the problem's arguments: read-only for the kernel's lifetime, read
+    // through the constant address space so that their uniform loads go to
+    // scalar registers and may be repeated instead of kept live
+#if defined(__HIP_DEVICE_COMPILE__)
+    using CEntry = const __attribute__((address_space(4))) BatchEntry;
+    const CEntry* ce = (const CEntry*)ent;
+#else  // the host pass only parses the kernel
+    const BatchEntry* ce = ent;
+#endif
+    const MarchArgs m = ce[lo].m;
+    const SpecialArgs sp = ce[lo].sp;
+    const unsigned sg = ce[lo].sg, mg = ce[lo].mg;
+    const int mode = ce[lo].mode;
+    const double dt = clk[lo].dt;
+    const double t = clk[lo].t0 + (double)clk[lo].n * dt;
+    StageScalars ss;
+    ss.t_s = t + dt * kStageTime[STAGE];
+    ss.c_acc = dt * kStageWeight[STAGE];
+    ss.c_next = STAGE < 3 ? dt * kStageTime[STAGE + 1] : 1.0;
+    ss.t_next = STAGE < 3 ? t + dt * kStageTime[STAGE + 1] : t + dt;
+    stage_block<KIND, STAGE, W, TY, MIXED>(m, sp, ss, blockIdx.x - start[lo], sg, mg, mode);
 }
 
 template <int KIND, int STAGE, int W, bool MIXED>
-int stage_ty(hipStream_t st, int ty, int mode, const MarchArgs& a, const SpecialArgs& sp, long nitems)
+int stage_ty(hipStream_t st, int ty, int mode, const MarchArgs& a, const SpecialArgs& sp,
+             const StageScalars& ss, long nitems)
 {
     const int threads = 64 * ty;
     const unsigned sg = (unsigned)(((nitems + threads - 1) / threads + 7) / 8);
@@ -431,9 +484,9 @@ int stage_ty(hipStream_t st, int ty, int mode, const MarchArgs& a, const Special
         const unsigned grid = 8u * (s_groups + m_groups);
         if (grid == 0) return 0;
         switch (ty) {
-        case 4: k_stage<KIND, STAGE, W, 4, MIXED><<<grid, 64 * 4, 0, st>>>(a, sp, s_groups, m_groups, md); return 0;
-        case 8: k_stage<KIND, STAGE, W, 8, MIXED><<<grid, 64 * 8, 0, st>>>(a, sp, s_groups, m_groups, md); return 0;
-        case 16: k_stage<KIND, STAGE, W, 16, MIXED><<<grid, 64 * 16, 0, st>>>(a, sp, s_groups, m_groups, md); return 0;
+        case 4: k_stage<KIND, STAGE, W, 4, MIXED><<<grid, 64 * 4, 0, st>>>(a, sp, ss, s_groups, m_groups, md); return 0;
+        case 8: k_stage<KIND, STAGE, W, 8, MIXED><<<grid, 64 * 8, 0, st>>>(a, sp, ss, s_groups, m_groups, md); return 0;
+        case 16: k_stage<KIND, STAGE, W, 16, MIXED><<<grid, 64 * 16, 0, st>>>(a, sp, ss, s_groups, m_groups, md); return 0;
         default: return -1;
         }
     };
@@ -446,13 +499,13 @@ int stage_ty(hipStream_t st, int ty, int mode, const MarchArgs& a, const Special
 
 template <int KIND, int W, bool MIXED>
 int stage_of(hipStream_t st, int stage, int ty, int mode, const MarchArgs& a, const SpecialArgs& sp,
-             long nitems)
+             const StageScalars& ss, long nitems)
 {
     switch (stage) {
-    case 0: return stage_ty<KIND, 0, W, MIXED>(st, ty, mode, a, sp, nitems);
-    case 1: return stage_ty<KIND, 1, W, MIXED>(st, ty, mode, a, sp, nitems);
-    case 2: return stage_ty<KIND, 2, W, MIXED>(st, ty, mode, a, sp, nitems);
-    case 3: return stage_ty<KIND, 3, W, MIXED>(st, ty, mode, a, sp, nitems);
+    case 0: return stage_ty<KIND, 0, W, MIXED>(st, ty, mode, a, sp, ss, nitems);
+    case 1: return stage_ty<KIND, 1, W, MIXED>(st, ty, mode, a, sp, ss, nitems);
+    case 2: return stage_ty<KIND, 2, W, MIXED>(st, ty, mode, a, sp, ss, nitems);
+    case 3: return stage_ty<KIND, 3, W, MIXED>(st, ty, mode, a, sp, ss, nitems);
     default: return -1;
     }
 }
@@ -460,25 +513,71 @@ int stage_of(hipStream_t st, int stage, int ty, int mode, const MarchArgs& a, co
 // One fused RK4 stage of the x-marching form (FP64 or mixed).
 template <bool MIXED>
 int launch_stage_form(hipStream_t st, int kind, int stage, int width, int ty, int mode,
-                      const MarchArgs& a, const SpecialArgs& sp)
+                      const MarchArgs& a, const SpecialArgs& sp, const StageScalars& ss)
 {
     const long nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2;
     if (a.nblocks == 0 && nitems == 0) return 0;
     switch (kind * 10 + width) {
-    case SHX_SYS_HEAT * 10 + 5: return stage_of<SHX_SYS_HEAT, 5, MIXED>(st, stage, ty, mode, a, sp, nitems);
-    case SHX_SYS_HEAT * 10 + 3: return stage_of<SHX_SYS_HEAT, 3, MIXED>(st, stage, ty, mode, a, sp, nitems);
+    case SHX_SYS_HEAT * 10 + 5: return stage_of<SHX_SYS_HEAT, 5, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
+    case SHX_SYS_HEAT * 10 + 3: return stage_of<SHX_SYS_HEAT, 3, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
     case SHX_SYS_WAVE_LINEAR * 10 + 5:
-        return stage_of<SHX_SYS_WAVE_LINEAR, 5, MIXED>(st, stage, ty, mode, a, sp, nitems);
+        return stage_of<SHX_SYS_WAVE_LINEAR, 5, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
     case SHX_SYS_WAVE_LINEAR * 10 + 3:
-        return stage_of<SHX_SYS_WAVE_LINEAR, 3, MIXED>(st, stage, ty, mode, a, sp, nitems);
+        return stage_of<SHX_SYS_WAVE_LINEAR, 3, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
     case SHX_SYS_WAVE_RADIAL * 10 + 5:
-        return stage_of<SHX_SYS_WAVE_RADIAL, 5, MIXED>(st, stage, ty, mode, a, sp, nitems);
+        return stage_of<SHX_SYS_WAVE_RADIAL, 5, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
     case SHX_SYS_WAVE_RADIAL * 10 + 3:
-        return stage_of<SHX_SYS_WAVE_RADIAL, 3, MIXED>(st, stage, ty, mode, a, sp, nitems);
+        return stage_of<SHX_SYS_WAVE_RADIAL, 3, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
     default: return -1;
     }
 }
 
+
+template <int KIND, int STAGE, int W, bool MIXED>
+int batch_ty(hipStream_t st, int ty, const BatchLaunch& bl)
+{
+    switch (ty) {
+    case 4:
+        k_batch_stage<KIND, STAGE, W, 4, MIXED><<<bl.grid, 64 * 4, 0, st>>>(bl.ent, bl.clk, bl.start, bl.nprob);
+        return 0;
+    case 8:
+        k_batch_stage<KIND, STAGE, W, 8, MIXED><<<bl.grid, 64 * 8, 0, st>>>(bl.ent, bl.clk, bl.start, bl.nprob);
+        return 0;
+    case 16:
+        k_batch_stage<KIND, STAGE, W, 16, MIXED><<<bl.grid, 64 * 16, 0, st>>>(bl.ent, bl.clk, bl.start, bl.nprob);
+        return 0;
+    default:
+        return -1;
+    }
+}
+
+template <int KIND, int W, bool MIXED>
+int batch_stage_of(hipStream_t st, int stage, int ty, const BatchLaunch& bl)
+{
+    switch (stage) {
+    case 0: return batch_ty<KIND, 0, W, MIXED>(st, ty, bl);
+    case 1: return batch_ty<KIND, 1, W, MIXED>(st, ty, bl);
+    case 2: return batch_ty<KIND, 2, W, MIXED>(st, ty, bl);
+    case 3: return batch_ty<KIND, 3, W, MIXED>(st, ty, bl);
+    default: return -1;
+    }
+}
+
+// One fused RK4 stage of a batch (FP64 or mixed).
+template <bool MIXED>
+int launch_batch_form(hipStream_t st, int kind, int stage, int width, int ty, const BatchLaunch& bl)
+{
+    if (bl.grid == 0) return 0;
+    switch (kind * 10 + width) {
+    case SHX_SYS_HEAT * 10 + 5: return batch_stage_of<SHX_SYS_HEAT, 5, MIXED>(st, stage, ty, bl);
+    case SHX_SYS_HEAT * 10 + 3: return batch_stage_of<SHX_SYS_HEAT, 3, MIXED>(st, stage, ty, bl);
+    case SHX_SYS_WAVE_LINEAR * 10 + 5: return batch_stage_of<SHX_SYS_WAVE_LINEAR, 5, MIXED>(st, stage, ty, bl);
+    case SHX_SYS_WAVE_LINEAR * 10 + 3: return batch_stage_of<SHX_SYS_WAVE_LINEAR, 3, MIXED>(st, stage, ty, bl);
+    case SHX_SYS_WAVE_RADIAL * 10 + 5: return batch_stage_of<SHX_SYS_WAVE_RADIAL, 5, MIXED>(st, stage, ty, bl);
+    case SHX_SYS_WAVE_RADIAL * 10 + 3: return batch_stage_of<SHX_SYS_WAVE_RADIAL, 3, MIXED>(st, stage, ty, bl);
+    default: return -1;
+    }
+}
 
 }  // namespace
 

@@ -12,9 +12,15 @@
 namespace shx {
 
 int launch_stage_march_mixed(hipStream_t st, int kind, int stage, int width, int ty, int mode,
-                             const MarchArgs& a, const SpecialArgs& sp)
+                             const MarchArgs& a, const SpecialArgs& sp, const StageScalars& ss)
 {
-    return launch_stage_form<true>(st, kind, stage, width, ty, mode, a, sp);
+    return launch_stage_form<true>(st, kind, stage, width, ty, mode, a, sp, ss);
+}
+
+int launch_batch_stage_mixed(hipStream_t st, int kind, int stage, int width, int ty,
+                             const BatchLaunch& bl)
+{
+    return launch_batch_form<true>(st, kind, stage, width, ty, bl);
 }
 
 }  // namespace shx

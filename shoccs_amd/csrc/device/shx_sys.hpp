@@ -128,7 +128,6 @@ struct MarchArgs {
     const unsigned char* cls;
     const ClosureRow* rows;  // 3 x kMaxClosureRows
     int nrows[3];
-    double t_s, c_acc, c_next, t_next;
     int nbz, nby, xchunk;
     unsigned nblocks;
 };
@@ -168,7 +167,38 @@ struct SpecialArgs {
     const double *pos0, *pos1, *pos2;
     const signed char *dir0, *dir1, *dir2;
     double *outR0, *outR1, *outR2;
+};
+
+// Per-stage scalars of one fused RK4 stage (rk4.cpp:27-46): the stage time,
+// the accumulator and next-stage coefficients, and the time of the values
+// the stage writes.
+struct StageScalars {
     double t_s, c_acc, c_next, t_next;
+};
+
+// One problem of a batch, for one stage: its tile and special-point
+// arguments and its workgroup groups (mode as launch_stage_march's).
+struct BatchEntry {
+    MarchArgs m;
+    SpecialArgs sp;
+    unsigned sg, mg;
+    int mode;
+};
+
+// Time of one problem of a batch: t = t0 + n dt at the start of a step.
+struct BatchClock {
+    double t0, dt;
+    long long n;
+};
+
+// One batched stage launch: per-problem entries of that stage, clocks, and
+// the first workgroup of each problem (nprob entries; grid in total).
+struct BatchLaunch {
+    const BatchEntry* ent;
+    const BatchClock* clk;
+    const unsigned* start;
+    int nprob;
+    unsigned grid;
 };
 
 // Host launchers (shx_march.hip); each returns 0, or -1 for a template
@@ -177,11 +207,17 @@ struct SpecialArgs {
 // special workgroups first, 2: last, 3: spread evenly) or in two (mode 0).
 int launch_classify(hipStream_t st, const DevSys& s, const DevOps& ops, unsigned char* cls, long N);
 int launch_stage_march(hipStream_t st, int kind, int stage, int width, int ty, int mode,
-                       const MarchArgs& a, const SpecialArgs& sp);
+                       const MarchArgs& a, const SpecialArgs& sp, const StageScalars& ss);
 // The mixed form (shx_mixed.hip): stage register FP32, interior stencils on it
 // in FP32 with compensated sums; u0, the accumulator, closure rows, B terms
 // and cut points in FP64.
 int launch_stage_march_mixed(hipStream_t st, int kind, int stage, int width, int ty, int mode,
-                             const MarchArgs& a, const SpecialArgs& sp);
+                             const MarchArgs& a, const SpecialArgs& sp, const StageScalars& ss);
+// One fused RK4 stage of every problem of a batch in one launch, and the
+// clock advance after a step (n += 1 for every problem).
+int launch_batch_stage(hipStream_t st, int kind, int stage, int width, int ty, const BatchLaunch& bl);
+int launch_batch_stage_mixed(hipStream_t st, int kind, int stage, int width, int ty,
+                             const BatchLaunch& bl);
+int launch_batch_advance(hipStream_t st, BatchClock* clk, int nprob);
 
 }  // namespace shx

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -821,14 +822,10 @@ int ensure_classes(shx_system* s)
     return SHX_OK;
 }
 
-// One stage of the x-marching form in one launch: march tiles for interior,
-// solid and pattern points; special items for the listed points and the
-// Dirichlet cut points.
-int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const double* u0,
-                 double* acc, void* out, double* const outR[3], double ts, double c_acc,
-                 double c_next, double tn)
+// Tile and special-point arguments of one stage of the x-marching form.
+int march_args(shx_system* s, const void* usD, RPtrs usR, const double* u0, double* acc, void* out,
+               double* const outR[3], MarchArgs& a, SpecialArgs& sp, long& nitems)
 {
-    MarchArgs a;
     a.s = s->dev;
     a.ops = s->ops;
     a.us = usD;
@@ -838,16 +835,11 @@ int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const dou
     a.cls = s->cls;
     a.rows = s->crows;
     for (int c = 0; c < 3; ++c) a.nrows[c] = s->nrows[c];
-    a.t_s = ts;
-    a.t_next = tn;
-    a.c_acc = c_acc;
-    a.c_next = c_next;
     a.xchunk = s->march_xchunk;
     a.nbz = (s->n[2] + 63) / 64;
     a.nby = (s->n[1] + s->march_ty - 1) / s->march_ty;
     const long nbx = (s->n[0] + a.xchunk - 1) / a.xchunk;
     const long nb = (long)a.nbz * a.nby * nbx;
-    SpecialArgs sp;
     sp.s = s->dev;
     sp.ops = s->ops;
     sp.us = usD;
@@ -872,18 +864,45 @@ int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const dou
     sp.outR0 = outR[0];
     sp.outR1 = outR[1];
     sp.outR2 = outR[2];
-    sp.t_s = ts;
-    sp.c_acc = c_acc;
-    sp.c_next = c_next;
-    sp.t_next = tn;
-    const long nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2;
+    nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2;
     if (nb + nitems / 64 + 8 >= (1L << 31)) return fail(s->ctx, SHX_EUNSUPPORTED, "march: too many workgroups");
     a.nblocks = (unsigned)nb;
+    return SHX_OK;
+}
+
+// Stage register pointers of stage `stage` of a fused step on u0: input,
+// output (FP64 or FP32 stage buffers, u0 at the ends) and cut points.
+void stage_registers(shx_system* s, shx_field* u0, int stage, const void*& usV, void*& outV,
+                     RPtrs& usR, double* outR[3])
+{
+    const bool mixed = s->precision == SHX_F32_COMP;
+    usV = stage == 0 ? (const void*)u0->D
+                     : mixed ? (const void*)s->stageF[(stage - 1) & 1] : (const void*)s->stageD[(stage - 1) & 1];
+    outV = stage == 3 ? (void*)u0->D : mixed ? (void*)s->stageF[stage & 1] : (void*)s->stageD[stage & 1];
+    for (int c = 0; c < 3; ++c) {
+        usR.r[c] = stage == 0 ? u0->R[c] : s->stageR[(stage - 1) & 1][c];
+        outR[c] = stage == 3 ? u0->R[c] : s->stageR[stage & 1][c];
+    }
+}
+
+// One stage of the x-marching form in one launch: march tiles for interior,
+// solid and pattern points; special items for the listed points and the
+// Dirichlet cut points.
+int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const double* u0,
+                 double* acc, void* out, double* const outR[3], double ts, double c_acc,
+                 double c_next, double tn)
+{
+    MarchArgs a;
+    SpecialArgs sp;
+    long nitems = 0;
+    int rc = march_args(s, usD, usR, u0, acc, out, outR, a, sp, nitems);
+    if (rc != SHX_OK) return rc;
+    const StageScalars ss{ts, c_acc, c_next, tn};
     const int lrc = s->precision == SHX_F32_COMP
                         ? launch_stage_march_mixed(s->ctx->stream, s->kind, stage, s->width,
-                                                   s->march_ty, s->variant - 1, a, sp)
+                                                   s->march_ty, s->variant - 1, a, sp, ss)
                         : launch_stage_march(s->ctx->stream, s->kind, stage, s->width, s->march_ty,
-                                             s->variant - 1, a, sp);
+                                             s->variant - 1, a, sp, ss);
     if (lrc != 0)
         return fail(s->ctx, SHX_EINVAL, "march: unsupported tile height or stencil width");
     return check(s->ctx, "rk4 march stage kernel");
@@ -901,17 +920,13 @@ int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t
     int rc = ensure_fused(s);
     if (rc == SHX_OK && march) rc = ensure_classes(s);
     for (int stage = 0; stage < 4 && rc == SHX_OK; ++stage) {
-        const void* usV = stage == 0 ? (const void*)u0->D
-                                     : mixed ? (const void*)s->stageF[(stage - 1) & 1]
-                                             : (const void*)s->stageD[(stage - 1) & 1];
-        void* outV = stage == 3 ? (void*)u0->D
-                                : mixed ? (void*)s->stageF[stage & 1] : (void*)s->stageD[stage & 1];
+        const void* usV;
+        void* outV;
+        RPtrs usR;
+        double* outR[3];
+        stage_registers(s, u0, stage, usV, outV, usR, outR);
         const double* usD = static_cast<const double*>(usV);
         double* outD = static_cast<double*>(outV);
-        RPtrs usR;
-        for (int c = 0; c < 3; ++c) usR.r[c] = stage == 0 ? u0->R[c] : s->stageR[(stage - 1) & 1][c];
-        double* outR[3];
-        for (int c = 0; c < 3; ++c) outR[c] = stage == 3 ? u0->R[c] : s->stageR[stage & 1][c];
         const double ts = t + dt * kRki[stage];
         const double c_acc = dt * kRkf[stage];
         const double c_next = stage < 3 ? dt * kRki[stage + 1] : 1.0;
@@ -1395,6 +1410,227 @@ int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt,
     hipFree(tmp.D);
     for (int c = 0; c < 3; ++c) hipFree(tmp.R[c]);
     if (best_ms) *best_ms = best;
+    return rc;
+}
+
+// ------------------------------------------------------------------- batch
+// Many problems stepped together: one launch per RK4 stage for all of them,
+// the per-problem arguments and clocks in device memory.  No reference
+// counterpart (the reference steps one problem per process).
+
+}  // extern "C"
+
+struct shx_batch {
+    shx_ctx* ctx = nullptr;
+    int n = 0;
+    int kind = 0, width = 0, precision = SHX_F64, ty = 4;
+    std::vector<shx_system*> sys;
+    std::vector<shx_field> u;
+    BatchEntry* ent[4] = {nullptr, nullptr, nullptr, nullptr};
+    BatchClock* clk = nullptr;
+    unsigned* start = nullptr;
+    unsigned grid = 0;
+    long points = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_steps = 0;
+};
+
+namespace {
+
+int batch_step(shx_batch* b)
+{
+    hipStream_t st = b->ctx->stream;
+    for (int stage = 0; stage < 4; ++stage) {
+        BatchLaunch bl{b->ent[stage], b->clk, b->start, b->n, b->grid};
+        const int lrc = b->precision == SHX_F32_COMP
+                            ? launch_batch_stage_mixed(st, b->kind, stage, b->width, b->ty, bl)
+                            : launch_batch_stage(st, b->kind, stage, b->width, b->ty, bl);
+        if (lrc != 0) return fail(b->ctx, SHX_EINVAL, "batch: unsupported tile height or stencil width");
+    }
+    launch_batch_advance(st, b->clk, b->n);
+    return check(b->ctx, "batch step");
+}
+
+void batch_free_graph(shx_batch* b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_steps = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* u0, const double* t0,
+                     const double* dt, int n, int march_ty, shx_batch** out)
+{
+    if (!ctx || !systems || !u0 || !t0 || !dt || n < 1 || !out) return SHX_EINVAL;
+    if (!(march_ty == 4 || march_ty == 8 || march_ty == 16)) return SHX_EINVAL;
+    *out = nullptr;
+    auto b = std::unique_ptr<shx_batch>(new (std::nothrow) shx_batch());
+    if (!b) return SHX_ENOMEM;
+    b->ctx = ctx;
+    b->n = n;
+    b->ty = march_ty;
+    b->kind = systems[0]->kind;
+    b->width = systems[0]->width;
+    b->precision = systems[0]->precision;
+    std::vector<BatchEntry> ent[4];
+    std::vector<BatchClock> clk;
+    std::vector<unsigned> start;
+    unsigned long long next = 0;
+    for (int p = 0; p < n; ++p) {
+        shx_system* s = systems[p];
+        if (!s || s->ctx != ctx || s->kind != b->kind || s->width != b->width || s->precision != b->precision)
+            return fail(ctx, SHX_EINVAL, "batch: problems differ in context, system, stencil width or precision");
+        if (!march_ok(s)) return fail(ctx, SHX_EUNSUPPORTED, "batch: the x-marching form does not apply");
+        const int saved_ty = s->march_ty;
+        s->march_ty = march_ty;  // tile height shared by the batch
+        if (s->variant == 0) s->variant = 3;
+        int rc = ensure_fused(s);
+        if (rc == SHX_OK) rc = ensure_classes(s);
+        if (rc != SHX_OK) {
+            s->march_ty = saved_ty;
+            return rc;
+        }
+        b->sys.push_back(s);
+        b->u.push_back(u0[p]);
+        shx_field* up = &b->u.back();
+        start.push_back((unsigned)next);
+        unsigned sg = 0, mg = 0;
+        for (int stage = 0; stage < 4; ++stage) {
+            BatchEntry e{};
+            const void* usV;
+            void* outV;
+            RPtrs usR;
+            double* outR[3];
+            stage_registers(s, up, stage, usV, outV, usR, outR);
+            long nitems = 0;
+            rc = march_args(s, usV, usR, up->D, s->accD, outV, outR, e.m, e.sp, nitems);
+            if (rc != SHX_OK) {
+                s->march_ty = saved_ty;
+                return rc;
+            }
+            const long threads = 64L * march_ty;
+            e.sg = sg = (unsigned)(((nitems + threads - 1) / threads + 7) / 8);
+            e.mg = mg = (e.m.nblocks + 7u) / 8u;
+            e.mode = 2;  // special work after the tiles
+            ent[stage].push_back(e);
+        }
+        s->march_ty = saved_ty;
+        next += 8ull * (sg + mg);
+        clk.push_back(BatchClock{t0[p], dt[p], 0});
+        b->points += s->N;
+    }
+    if (next >= (1ull << 31)) return fail(ctx, SHX_EUNSUPPORTED, "batch: too many workgroups");
+    b->grid = (unsigned)next;
+    int rc = SHX_OK;
+    for (int stage = 0; stage < 4 && rc == SHX_OK; ++stage) rc = upload_vec(ctx, &b->ent[stage], ent[stage]);
+    if (rc == SHX_OK) rc = upload_vec(ctx, &b->clk, clk);
+    if (rc == SHX_OK) rc = upload_vec(ctx, &b->start, start);
+    if (rc != SHX_OK) {
+        shx_batch_destroy(b.release());
+        return rc;
+    }
+    *out = b.release();
+    return SHX_OK;
+}
+
+int shx_batch_destroy(shx_batch* b)
+{
+    if (!b) return SHX_EINVAL;
+    hipStreamSynchronize(b->ctx->stream);
+    batch_free_graph(b);
+    for (auto& e : b->ent) hipFree(e);
+    hipFree(b->clk);
+    hipFree(b->start);
+    delete b;
+    return SHX_OK;
+}
+
+int shx_batch_steps_async(shx_batch* b, int nsteps)
+{
+    if (!b || nsteps < 0) return SHX_EINVAL;
+    int rc = SHX_OK;
+    for (int k = 0; k < nsteps && rc == SHX_OK; ++k) rc = batch_step(b);
+    return rc;
+}
+
+int shx_batch_capture(shx_batch* b, int nsteps)
+{
+    if (!b || nsteps < 1) return SHX_EINVAL;
+    batch_free_graph(b);
+    hipStream_t st = b->ctx->stream;
+    int rc = sync(b->ctx);
+    if (rc != SHX_OK) return rc;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(b->ctx, e, "batch capture");
+    for (int k = 0; k < nsteps && rc == SHX_OK; ++k) rc = batch_step(b);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    if (rc != SHX_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(b->ctx, e, "batch capture");
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        batch_free_graph(b);
+        return hip_fail(b->ctx, e, "batch graph instantiate");
+    }
+    b->graph_steps = nsteps;
+    return SHX_OK;
+}
+
+int shx_batch_launch_async(shx_batch* b)
+{
+    if (!b) return SHX_EINVAL;
+    if (!b->exec) return fail(b->ctx, SHX_EINVAL, "batch: no captured graph");
+    const hipError_t e = hipGraphLaunch(b->exec, b->ctx->stream);
+    return e == hipSuccess ? SHX_OK : hip_fail(b->ctx, e, "batch graph launch");
+}
+
+int shx_batch_times(shx_batch* b, double* t)
+{
+    if (!b || !t) return SHX_EINVAL;
+    std::vector<BatchClock> clk((size_t)b->n);
+    hipError_t e = hipMemcpyAsync(clk.data(), b->clk, sizeof(BatchClock) * clk.size(), hipMemcpyDeviceToHost,
+                                  b->ctx->stream);
+    if (e != hipSuccess) return hip_fail(b->ctx, e, "batch clocks");
+    const int rc = sync(b->ctx);
+    if (rc != SHX_OK) return rc;
+    for (int p = 0; p < b->n; ++p) t[p] = clk[(size_t)p].t0 + (double)clk[(size_t)p].n * clk[(size_t)p].dt;
+    return SHX_OK;
+}
+
+int shx_time_batch(shx_batch* b, int use_graph, int reps, double* ms_per_step, int64_t* points)
+{
+    if (!b || reps < 1 || !ms_per_step) return SHX_EINVAL;
+    if (use_graph && !b->exec) return fail(b->ctx, SHX_EINVAL, "batch: no captured graph");
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return hip_fail(b->ctx, hipGetLastError(), "event");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return hip_fail(b->ctx, hipGetLastError(), "event");
+    }
+    int rc = sync(b->ctx);
+    hipEventRecord(e0, b->ctx->stream);
+    for (int r = 0; r < reps && rc == SHX_OK; ++r)
+        rc = use_graph ? shx_batch_launch_async(b) : batch_step(b);
+    hipEventRecord(e1, b->ctx->stream);
+    if (rc == SHX_OK) rc = sync(b->ctx);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    const int steps = use_graph ? b->graph_steps : 1;
+    *ms_per_step = (double)ms / ((double)reps * steps);
+    if (points) *points = b->points;
     return rc;
 }
 

@@ -468,3 +468,67 @@ def test_driver_zero_steps_without_bounds(ctx):
     g = prob.field()
     got = prob.run(g, S.RK4)
     assert got["step"] == 0 and got["completed"] and got["stats"].shape[0] == 1
+
+
+# ---------------------------------------------------------------- batches
+BATCH_CFGS = [H.heat3d_cfg(n=(20, 24, 28)),
+              H.heat3d_cfg(n=(26, 22, 30), center=(0.45, 0.5, 0.52), radius=0.25),
+              H.heat3d_cfg(n=(18, 34, 24), center=(0.55, 0.52, 0.47), radius=0.18)]
+
+
+def _batch_setup(ctx, precision):
+    import shoccs_amd as S
+
+    pairs = [_pair(ctx, c) for c in BATCH_CFGS]
+    init, dts = [], []
+    for osys, prob in pairs:
+        u0 = osys.field()
+        osys.initialize(u0, 0.0)
+        osys.update_boundary(u0, 0.0)
+        init.append(u0.flat())
+        dts.append(osys.dt(1.0, 0.8))
+        prob.set_tuning(3, 4, 16)
+        prob.set_precision(precision)
+    return pairs, init, dts
+
+
+@pytest.mark.parametrize("precision", ["F64", "F32_COMP"])
+def test_batch_matches_single_problems(ctx, precision):
+    """A batch of heat problems of different shapes steps every problem to
+    the same doubles as stepping it alone (same kernels, same stage scalars),
+    eagerly and by graph replay, and keeps their clocks."""
+    import shoccs_amd as S
+
+    prec = getattr(S, precision)
+    pairs, init, dts = _batch_setup(ctx, prec)
+    want = []
+    for (osys, prob), u, dt in zip(pairs, init, dts):
+        g = prob.field()
+        g.upload(u)
+        prob.rk4_fused(g, 0.0, dt, 4)
+        want.append(g.download())
+        g.free()
+    fields = []
+    for (osys, prob), u in zip(pairs, init):
+        f = prob.field()
+        f.upload(u)
+        fields.append(f)
+    b = S.Batch(ctx, [p for _, p in pairs], fields, [0.0] * 3, dts)
+    b.steps(2)
+    b.capture(1)
+    b.launch()
+    b.launch()
+    for f, w in zip(fields, want):
+        assert np.array_equal(f.download(), w)
+    np.testing.assert_array_equal(b.times(), [4 * dt for dt in dts])
+    b.close()
+    for _, prob in pairs:
+        prob.set_precision(S.F64)
+
+
+def test_batch_rejects_mixed_families(ctx):
+    import shoccs_amd as S
+
+    (o1, p1), (o2, p2) = _pair(ctx, BATCH_CFGS[0]), _pair(ctx, WAVE_CFGS["wave30_radial_E4u"])
+    with pytest.raises(S.ShoccsError):
+        S.Batch(ctx, [p1, p2], [p1.field(), p2.field()], [0.0, 0.0], [1e-3, 1e-3])
