@@ -289,6 +289,11 @@ int shx_time_unfused_steps(shx_system* s, shx_field* u0, shx_field* out, shx_fie
  * by a HIP event pair on the context's stream. */
 int shx_time_fused_stage_kernel(shx_system* s, shx_field* u0, double t, double dt, int nsteps,
                                 double* ms_per_kernel);
+/* Mean device time of one derivative apply (O and B, no Neumann term) over
+ * `reps` applies after one warm-up, as the reference's derivative benchmark
+ * times derivative::operator() (benchmarks/bench_derivative.cpp:56-116). */
+int shx_time_derivative_apply(shx_derivative* d, const shx_field* u, shx_field* du, int accumulate,
+                              int reps, double* ms_per_apply);
 /* Mean device time per step of `reps` batch steps (use_graph = 0) or graph
  * replays (use_graph = 1, per captured step); points = total points of D. */
 int shx_time_batch(shx_batch* b, int use_graph, int reps, double* ms_per_step, int64_t* points);

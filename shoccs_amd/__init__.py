@@ -260,6 +260,15 @@ class Problem:
 
         return apply
 
+    def time_derivative(self, d: int, u: Field, du: Field, reps: int = 20, op: int = eq) -> float:
+        """Mean device milliseconds of one direction-d derivative apply."""
+        self._need_device()
+        dev = lib().shoccs_problem_derivative(self.p, d)
+        ms = C.c_double()
+        check(lib().shx_time_derivative_apply(dev, u.ref(), du.ref(), op, reps, C.byref(ms)),
+              self.ctx.p, "time_derivative")
+        return ms.value
+
     def _dev_ops(self):
         arr = (C.c_void_p * 3)()
         for a in range(3):

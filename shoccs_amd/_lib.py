@@ -159,6 +159,8 @@ _SIGS = [
     ("shx_batch_capture", C.c_int, [C.c_void_p, C.c_int]),
     ("shx_batch_launch_async", C.c_int, [C.c_void_p]),
     ("shx_batch_times", C.c_int, [C.c_void_p, dp]),
+    ("shx_time_derivative_apply", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
+                                            C.c_int, C.c_int, dp]),
     ("shx_time_batch", C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, C.POINTER(C.c_int64)]),
     ("shoccs_run", C.c_int, [C.c_void_p, C.POINTER(RunDesc), C.POINTER(ShxField),
                              C.POINTER(RunResult)]),

@@ -480,6 +480,34 @@ int shx_derivative_apply(shx_derivative* d, const shx_field* u, const shx_field*
     return rc;
 }
 
+int shx_time_derivative_apply(shx_derivative* d, const shx_field* u, shx_field* du, int accumulate,
+                              int reps, double* ms_per_apply)
+{
+    if (!d || !u || !du || reps < 1 || !ms_per_apply) return SHX_EINVAL;
+    int rc = launch_derivative(d, u->D, u->R[d->dir], du->D, accumulate != 0, true);  // warm-up
+    if (rc == SHX_OK) rc = sync(d->ctx);
+    if (rc != SHX_OK) return rc;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return hip_fail(d->ctx, hipGetLastError(), "event");
+    if (hipEventCreate(&b) != hipSuccess) {
+        hipEventDestroy(a);
+        return hip_fail(d->ctx, hipGetLastError(), "event");
+    }
+    hipEventRecord(a, d->ctx->stream);
+    for (int r = 0; r < reps && rc == SHX_OK; ++r)
+        rc = launch_derivative(d, u->D, u->R[d->dir], du->D, accumulate != 0, true);
+    hipEventRecord(b, d->ctx->stream);
+    hipError_t e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (rc != SHX_OK) return rc;
+    if (e != hipSuccess) return hip_fail(d->ctx, e, "time derivative");
+    *ms_per_apply = ms / reps;
+    return SHX_OK;
+}
+
 int shx_block_apply(shx_derivative* d, const double* x, double* b, int accumulate)
 {
     if (!d || !x || !b) return SHX_EINVAL;

@@ -242,18 +242,18 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
                             d2 = tile_row(crow[2][iz - 1], &t[ty + hw][tz + hw], 1);
                         }
                     }
-                    if (KIND == SHX_SYS_HEAT) {
+                    if (kIsHeat<KIND>) {
                         // laplacian rows added in direction order (plus_eq)
                         double lap = 0.0;
                         if (px) lap += d0;
                         if (py) lap += d1;
                         if (pz) lap += d2;
-                        rhs = heat_finish(a.s, false, true, x, j, k, lap, ss.t_s);
+                        rhs = heat_finish<KIND>(a.s, false, true, x, j, k, lap, ss.t_s);
                     } else {
                         rhs = wave_combine<KIND>(a.s, false, x, j, k, d0, d1, d2);
                     }
                 } else {
-                    rhs = KIND == SHX_SYS_HEAT ? heat_finish(a.s, false, false, x, j, k, 0.0, ss.t_s)
+                    rhs = kIsHeat<KIND> ? heat_finish<KIND>(a.s, false, false, x, j, k, 0.0, ss.t_s)
                                                : wave_combine<KIND>(a.s, false, x, j, k, 0.0, 0.0, 0.0);
                 }
                 // stage_update, with non-temporal stores
@@ -325,7 +325,7 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
         point_coords(idx, a.s.n1, a.s.n2, i, j, k);
         const DevOps& ops = a.ops;
         double rhs;
-        if (KIND == SHX_SYS_HEAT) {
+        if (kIsHeat<KIND>) {
             double lap = 0.0;
             if (ops.d[0].present) {
                 const RowVal r = special_row<W>(ops.d[0], a.rows0, rec.ref[0], us, a.usR.r[0], idx);
@@ -339,7 +339,7 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
                 const RowVal r = special_row<W>(ops.d[2], a.rows2, rec.ref[2], us, a.usR.r[2], idx);
                 if (r.row) lap = row_plus(lap, r);
             }
-            rhs = heat_finish(a.s, on_dirichlet_face(a.s, i, j, k), is_fluid(a.s, idx), i, j, k, lap,
+            rhs = heat_finish<KIND>(a.s, on_dirichlet_face(a.s, i, j, k), is_fluid(a.s, idx), i, j, k, lap,
                               ss.t_s);
         } else {
             double du0 = 0.0, du1 = 0.0, du2 = 0.0;
@@ -520,6 +520,8 @@ int launch_stage_form(hipStream_t st, int kind, int stage, int width, int ty, in
     switch (kind * 10 + width) {
     case SHX_SYS_HEAT * 10 + 5: return stage_of<SHX_SYS_HEAT, 5, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
     case SHX_SYS_HEAT * 10 + 3: return stage_of<SHX_SYS_HEAT, 3, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
+    case kHeatGauss * 10 + 5: return stage_of<kHeatGauss, 5, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
+    case kHeatGauss * 10 + 3: return stage_of<kHeatGauss, 3, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
     case SHX_SYS_WAVE_LINEAR * 10 + 5:
         return stage_of<SHX_SYS_WAVE_LINEAR, 5, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);
     case SHX_SYS_WAVE_LINEAR * 10 + 3:
@@ -571,6 +573,8 @@ int launch_batch_form(hipStream_t st, int kind, int stage, int width, int ty, co
     switch (kind * 10 + width) {
     case SHX_SYS_HEAT * 10 + 5: return batch_stage_of<SHX_SYS_HEAT, 5, MIXED>(st, stage, ty, bl);
     case SHX_SYS_HEAT * 10 + 3: return batch_stage_of<SHX_SYS_HEAT, 3, MIXED>(st, stage, ty, bl);
+    case kHeatGauss * 10 + 5: return batch_stage_of<kHeatGauss, 5, MIXED>(st, stage, ty, bl);
+    case kHeatGauss * 10 + 3: return batch_stage_of<kHeatGauss, 3, MIXED>(st, stage, ty, bl);
     case SHX_SYS_WAVE_LINEAR * 10 + 5: return batch_stage_of<SHX_SYS_WAVE_LINEAR, 5, MIXED>(st, stage, ty, bl);
     case SHX_SYS_WAVE_LINEAR * 10 + 3: return batch_stage_of<SHX_SYS_WAVE_LINEAR, 3, MIXED>(st, stage, ty, bl);
     case SHX_SYS_WAVE_RADIAL * 10 + 5: return batch_stage_of<SHX_SYS_WAVE_RADIAL, 5, MIXED>(st, stage, ty, bl);

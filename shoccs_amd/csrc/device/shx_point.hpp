@@ -97,8 +97,42 @@ __device__ __forceinline__ double exact_at(const DevSys& s, double x, double y, 
 template <int KIND>
 __device__ __forceinline__ double exact_of(const DevSys& s, double x, double y, double z, double t)
 {
-    if (KIND == SHX_SYS_HEAT) return mms_call(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z);
+    if (KIND == SHX_SYS_HEAT) {  // sin product, inline (mms_call's expression)
+        double v = sin(kTwoPi * x) * sin(kTwoPi * y);
+        if (s.mms_dims == 3) v = v * sin(kTwoPi * z);
+        return v * exp(-t);
+    }
+    if (KIND == kHeatGauss) return mms_call(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, x, y, z);
     return wave_exact(s, x, y, z, t);
+}
+
+// Sin-product MMS at mesh point (i, j, k) from the per-axis tables: the same
+// operations, in the same order, as mms_call on the point's coordinates.
+__device__ __forceinline__ double sinprod_at(const DevSys& s, int i, int j, int k, double t)
+{
+    double v = s.sinx[i] * s.siny[j];
+    if (s.mms_dims == 3) v = v * s.sinz[k];
+    return v * exp(-t);
+}
+
+// Exact solution at mesh point (i, j, k).
+template <int KIND>
+__device__ __forceinline__ double exact_ijk(const DevSys& s, int i, int j, int k, double t)
+{
+    if (KIND == SHX_SYS_HEAT) return sinprod_at(s, i, j, k, t);
+    return exact_of<KIND>(s, s.x[i], s.y[j], s.z[k], t);
+}
+
+// Heat source S_t - k lap S at mesh point (i, j, k) (heat.cpp:111-117); the
+// sin-product terms as mms_ddt and mms_lap form them.
+template <int KIND>
+__device__ __forceinline__ double heat_source_ijk(const DevSys& s, double t, int i, int j, int k)
+{
+    if (KIND == SHX_SYS_HEAT) {
+        const double v = sinprod_at(s, i, j, k, t);
+        return -v - s.k * (-(s.mms_dims * kTwoPi * kTwoPi) * v);
+    }
+    return heat_source(s, t, s.x[i], s.y[j], s.z[k]);
 }
 
 // Scalar-wave combination gG . du at one point of D (scalar_wave.cpp:208-238:
@@ -126,12 +160,13 @@ __device__ __forceinline__ double wave_combine(const DevSys& s, bool face, int i
 
 // Heat right-hand side from the accumulated laplacian rows (heat.cpp:96-128):
 // scale by the diffusivity, zero on Dirichlet faces, add the source on fluid.
+template <int KIND>
 __device__ __forceinline__ double heat_finish(const DevSys& s, bool face, bool fluid, int i, int j,
                                               int k, double lap, double t)
 {
     double acc = lap * s.k;
     if (face) return 0.0;
-    if (fluid) acc += heat_source(s, t, s.x[i], s.y[j], s.z[k]);
+    if (fluid) acc += heat_source_ijk<KIND>(s, t, i, j, k);
     return acc;
 }
 
@@ -142,7 +177,7 @@ __device__ __forceinline__ double rhs_point(const DevSys& s, const DevOps& ops,
                                             const double* __restrict__ u, const RPtrs& uR,
                                             long idx, int i, int j, int k, double t)
 {
-    if (KIND == SHX_SYS_HEAT) {
+    if (kIsHeat<KIND>) {
         // laplacian: du = 0; du += O_x u; du += B_x u_Rx; then y, z (laplacian.cpp:37-46)
         double acc = 0.0;
         if (ops.d[0].present) {
@@ -157,7 +192,7 @@ __device__ __forceinline__ double rhs_point(const DevSys& s, const DevOps& ops,
             const RowVal r = row_at<2, W, true>(ops.d[2], u, uR.r[2], idx, i, j, k, s.n1, s.n2);
             if (r.row) acc = row_plus(acc, r);
         }
-        return heat_finish(s, on_dirichlet_face(s, i, j, k), is_fluid(s, idx), i, j, k, acc, t);
+        return heat_finish<KIND>(s, on_dirichlet_face(s, i, j, k), is_fluid(s, idx), i, j, k, acc, t);
     } else {
         double du0 = 0.0, du1 = 0.0, du2 = 0.0;
         if (ops.d[0].present) {
@@ -204,7 +239,7 @@ __device__ __forceinline__ void stage_store(const DevSys& s, double rhs, const d
 {
     double v = stage_update<STAGE>(rhs, STAGE == 0 ? 0.0 : acc[idx], u0[idx], acc, idx, c_acc,
                                    c_next);
-    if (on_dirichlet_face(s, i, j, k)) v = exact_of<KIND>(s, s.x[i], s.y[j], s.z[k], t_next);
+    if (on_dirichlet_face(s, i, j, k)) v = exact_ijk<KIND>(s, i, j, k, t_next);
     out[idx] = (TO)v;
 }
 

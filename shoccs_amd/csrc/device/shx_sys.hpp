@@ -21,10 +21,22 @@ struct DevSys {
     double k;
     int mms_kind, mms_dims, mms_n;
     const double* mms;
+    // sin-product MMS: sin(2 pi x_i), sin(2 pi y_j), sin(2 pi z_k) per mesh
+    // coordinate (computed on the device with the device's sin)
+    const double* sinx;
+    const double* siny;
+    const double* sinz;
     const unsigned* fluid_mask;
 };
 
 constexpr double kTwoPi = 6.283185307179586;  // 2 * pi, as 2 * std::numbers::pi
+
+// Kernel family of a system: SHX_SYS_* for the public kinds, with heat split
+// by manufactured solution (the sin product reads per-axis tables; the
+// gaussian is evaluated per point out of line).
+constexpr int kHeatGauss = 3;
+template <int KIND>
+constexpr bool kIsHeat = KIND == SHX_SYS_HEAT || KIND == kHeatGauss;
 
 __device__ __forceinline__ double dot3(const double* a, const double* b)
 {

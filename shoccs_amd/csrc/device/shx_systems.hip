@@ -89,6 +89,13 @@ __global__ void k_exact_R(DevSys s, long n, const double* __restrict__ pos, doub
     out[i] = exact_at(s, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], t);
 }
 
+// sin(2 pi x) per mesh coordinate, for the sin-product MMS
+__global__ void k_sin_table(const double* __restrict__ x, long n, double* __restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = sin(kTwoPi * x[i]);
+}
+
 // initialize_scalar_field on the fluid gather
 __global__ void k_init_fluid(DevSys s, long n, const int* __restrict__ fluid, double* __restrict__ u,
                              double t)
@@ -240,6 +247,7 @@ inline unsigned grid_for(long n) { return (unsigned)((n + 255) / 256); }
 struct shx_system {
     shx_ctx* ctx = nullptr;
     int kind = 0;
+    int family = 0;  // kernel family: kind, or kHeatGauss for heat with the gaussian MMS
     int n[3] = {1, 1, 1};
     long N = 0;
     long nR[3] = {0, 0, 0};
@@ -249,6 +257,7 @@ struct shx_system {
     Tiling tiling;
     // device buffers
     double* xyz[3] = {nullptr, nullptr, nullptr};
+    double* sintab[3] = {nullptr, nullptr, nullptr};  // sin-product MMS tables
     double* mms = nullptr;
     unsigned* fluid_mask = nullptr;
     int* fluid = nullptr;
@@ -424,12 +433,13 @@ int rhs_fused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double t
 {
     const Tiling& tl = s->tiling;
     hipStream_t st = s->ctx->stream;
-    switch (s->kind * 10 + s->width) {
+    switch (s->family * 10 + s->width) {
 #define SHX_RHS_CASE(K, W)                                                                      \
     case K * 10 + W:                                                                            \
         k_rhs<K, W><<<tl.nblocks, kBlock, 0, st>>>(tl, s->dev, s->ops, u->D, rptrs(u), rhs->D, t); \
         break;
         SHX_RHS_CASE(SHX_SYS_HEAT, 3) SHX_RHS_CASE(SHX_SYS_HEAT, 5)
+        SHX_RHS_CASE(kHeatGauss, 3) SHX_RHS_CASE(kHeatGauss, 5)
         SHX_RHS_CASE(SHX_SYS_WAVE_LINEAR, 3) SHX_RHS_CASE(SHX_SYS_WAVE_LINEAR, 5)
         SHX_RHS_CASE(SHX_SYS_WAVE_RADIAL, 3) SHX_RHS_CASE(SHX_SYS_WAVE_RADIAL, 5)
 #undef SHX_RHS_CASE
@@ -899,9 +909,9 @@ int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const dou
     if (rc != SHX_OK) return rc;
     const StageScalars ss{ts, c_acc, c_next, tn};
     const int lrc = s->precision == SHX_F32_COMP
-                        ? launch_stage_march_mixed(s->ctx->stream, s->kind, stage, s->width,
+                        ? launch_stage_march_mixed(s->ctx->stream, s->family, stage, s->width,
                                                    s->march_ty, s->variant - 1, a, sp, ss)
-                        : launch_stage_march(s->ctx->stream, s->kind, stage, s->width, s->march_ty,
+                        : launch_stage_march(s->ctx->stream, s->family, stage, s->width, s->march_ty,
                                              s->variant - 1, a, sp, ss);
     if (lrc != 0)
         return fail(s->ctx, SHX_EINVAL, "march: unsupported tile height or stencil width");
@@ -934,8 +944,10 @@ int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t
         if (ev) hipEventRecord(ev[2 * stage], s->ctx->stream);
         if (march)
             rc = launch_march(s, stage, usV, usR, u0->D, s->accD, outV, outR, ts, c_acc, c_next, tn);
-        else if (s->kind == SHX_SYS_HEAT)
+        else if (s->family == SHX_SYS_HEAT)
             rc = launch_stage<SHX_SYS_HEAT>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
+        else if (s->family == kHeatGauss)
+            rc = launch_stage<kHeatGauss>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
         else if (s->kind == SHX_SYS_WAVE_RADIAL)
             rc = launch_stage<SHX_SYS_WAVE_RADIAL>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
         else
@@ -969,6 +981,7 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
     if (!s) return SHX_ENOMEM;
     s->ctx = ctx;
     s->kind = desc->kind;
+    s->family = desc->kind == SHX_SYS_HEAT && desc->mms_kind == SHX_MMS_GAUSS ? kHeatGauss : desc->kind;
     for (int c = 0; c < 3; ++c) {
         s->n[c] = desc->n[c];
         s->nR[c] = desc->nR[c];
@@ -1001,6 +1014,15 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
     if (rc == SHX_OK) rc = upload(ctx, &s->xyz[1], desc->y, (size_t)s->n[1]);
     if (rc == SHX_OK) rc = upload(ctx, &s->xyz[2], desc->z, (size_t)s->n[2]);
     if (rc == SHX_OK && desc->mms_n > 0) rc = upload(ctx, &s->mms, desc->mms, (size_t)8 * desc->mms_n);
+    if (desc->kind == SHX_SYS_HEAT && desc->mms_kind == SHX_MMS_SINPROD)
+        for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+            rc = alloc_d(ctx, &s->sintab[c], s->n[c]);
+            if (rc == SHX_OK) {
+                k_sin_table<<<(unsigned)((s->n[c] + 255) / 256), 256, 0, ctx->stream>>>(s->xyz[c], s->n[c],
+                                                                                      s->sintab[c]);
+                rc = check(ctx, "sin table kernel");
+            }
+        }
     // fluid gather and bit mask
     std::vector<unsigned> mask((s->N + 31) / 32, 0u);
     std::vector<int> fl(desc->nfluid);
@@ -1050,6 +1072,9 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
     dv.x = s->xyz[0];
     dv.y = s->xyz[1];
     dv.z = s->xyz[2];
+    dv.sinx = s->sintab[0];
+    dv.siny = s->sintab[1];
+    dv.sinz = s->sintab[2];
     dv.mms = s->mms;
     dv.fluid_mask = s->fluid_mask;
     s->width = 0;
@@ -1078,6 +1103,7 @@ int shx_system_destroy(shx_system* s)
     hipStreamSynchronize(s->ctx->stream);
     for (int c = 0; c < 3; ++c) {
         hipFree(s->xyz[c]);
+        hipFree(s->sintab[c]);
         hipFree(s->rpos[c]);
         hipFree(s->rdir[c]);
         for (int b = 0; b < 2; ++b) hipFree(s->stageR[b][c]);
@@ -1476,7 +1502,7 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
     b->ctx = ctx;
     b->n = n;
     b->ty = march_ty;
-    b->kind = systems[0]->kind;
+    b->kind = systems[0]->family;
     b->width = systems[0]->width;
     b->precision = systems[0]->precision;
     std::vector<BatchEntry> ent[4];
@@ -1485,7 +1511,7 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
     unsigned long long next = 0;
     for (int p = 0; p < n; ++p) {
         shx_system* s = systems[p];
-        if (!s || s->ctx != ctx || s->kind != b->kind || s->width != b->width || s->precision != b->precision)
+        if (!s || s->ctx != ctx || s->family != b->kind || s->width != b->width || s->precision != b->precision)
             return fail(ctx, SHX_EINVAL, "batch: problems differ in context, system, stencil width or precision");
         if (!march_ok(s)) return fail(ctx, SHX_EUNSUPPORTED, "batch: the x-marching form does not apply");
         const int saved_ty = s->march_ty;
