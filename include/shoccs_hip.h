@@ -224,7 +224,8 @@ int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt,
 /* Fused-stage kernel choice (no reference counterpart; the reference has one
  * Kokkos TeamPolicy shape, src/matrices/block.hpp:185-194): variant 0 is the
  * point-wise kernel on 64 x 4 tiles.  Variants 1-4 march tiles of
- * march_ty x 64 points (march_ty = 4, 8 or 16) along x over chunks of
+ * march_ty x 128 points (march_ty = 4 or 8; two points per lane, or
+ * march_ty x 64 when n2 is odd) along x over chunks of
  * march_xchunk planes for interior, solid and wall-closure points, while
  * special workgroups evaluate the remaining points (closure rows with B
  * terms, closures out of the tile's reach, Dirichlet faces) from records

@@ -63,13 +63,90 @@ struct StageTypes {
     using out = typename std::conditional<MIXED && STAGE != 3, float, double>::type;
 };
 
-// One march tile: logical workgroup L of the tile grid.
-template <int KIND, int STAGE, int W, int TY, bool MIXED>
+// Planes of loads a march tile keeps in flight ahead of the plane it
+// evaluates (a ring of per-plane load slots in registers).
+#ifndef SHX_MARCH_PF
+#define SHX_MARCH_PF 2
+#endif
+constexpr int kMarchAhead = SHX_MARCH_PF;
+
+// Vector of V elements of T (one 8- or 16-byte access for V = 2).
+template <typename T, int V>
+struct VecOf {
+    typedef T type __attribute__((ext_vector_type(V)));
+};
+
+// VEC consecutive elements at p (p aligned to VEC elements): plain and
+// non-temporal loads, non-temporal store.
+template <int VEC, typename T>
+__device__ __forceinline__ void vload(const T* p, T (&o)[VEC])
+{
+    if constexpr (VEC == 1) {
+        o[0] = *p;
+    } else {
+        const auto v = *reinterpret_cast<const typename VecOf<T, VEC>::type*>(p);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) o[e] = v[e];
+    }
+}
+template <int VEC, typename T>
+__device__ __forceinline__ void vload_nt(const T* p, T (&o)[VEC])
+{
+    if constexpr (VEC == 1) {
+        o[0] = __builtin_nontemporal_load(p);
+    } else {
+        const auto v = __builtin_nontemporal_load(reinterpret_cast<const typename VecOf<T, VEC>::type*>(p));
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) o[e] = v[e];
+    }
+}
+template <int VEC, typename T>
+__device__ __forceinline__ void vstore_nt(T* p, const T (&x)[VEC])
+{
+    if constexpr (VEC == 1) {
+        __builtin_nontemporal_store(x[0], p);
+    } else {
+        typename VecOf<T, VEC>::type v;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) v[e] = x[e];
+        __builtin_nontemporal_store(v, reinterpret_cast<typename VecOf<T, VEC>::type*>(p));
+    }
+}
+// VEC class bytes (VEC = 1 or 2) at p, non-temporal.
+template <int VEC>
+__device__ __forceinline__ void vload_cls(const unsigned char* p, unsigned char (&o)[VEC])
+{
+    static_assert(VEC == 1 || VEC == 2, "class bytes come one or two per lane");
+    if constexpr (VEC == 1) {
+        o[0] = __builtin_nontemporal_load(p);
+    } else {
+        const unsigned short w = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p));
+        o[0] = (unsigned char)(w & 0xffu);
+        o[1] = (unsigned char)(w >> 8);
+    }
+}
+
+// What a lane loads for one plane of its march: the x-queue entries (stage
+// values HQ planes ahead), its y halo values (VEC) and z halo value, and the
+// class bytes, u0 and accumulator of its points.
+template <typename TI, int VEC>
+struct PlaneLoad {
+    TI q[VEC], y[VEC];
+    TI z;
+    double u0[VEC], acc[VEC];
+    unsigned char c[VEC];
+};
+
+// One march tile: logical workgroup L of the tile grid.  A lane owns VEC
+// consecutive points along z (VEC = 2 needs an even z extent, so that the
+// pairs are aligned and never straddle a row end); a tile row holds 64 VEC
+// points.
+template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
 __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalars& ss, unsigned L)
 {
     using TI = typename StageTypes<MIXED, STAGE>::in;
     using TO = typename StageTypes<MIXED, STAGE>::out;
-    constexpr int TZ = 64;
+    constexpr int TZ = 64 * VEC;
     constexpr int hw = W / 2;
     constexpr int LY = TY + 2 * hw, LZ = TZ + 2 * hw;
     constexpr int HQ = hw + kXReach;  // x queue half width
@@ -77,22 +154,26 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
     static_assert(TY >= 2 * hw, "y halo rows are loaded by distinct threads");
     __shared__ TI tile[2][LY][LZ];
     __shared__ ClosureRow crow[3][kMaxClosureRows];
+    __shared__ unsigned patdec[kMaxPatterns];
 
     // tile -> (z tile, y tile, x chunk), z fastest
     const int bz = (int)(L % (unsigned)a.nbz);
     const unsigned r1 = L / (unsigned)a.nbz;
     const int by = (int)(r1 % (unsigned)a.nby);
     const int bx = (int)(r1 / (unsigned)a.nby);
-    const int tz = threadIdx.x % TZ, ty = threadIdx.x / TZ;
+    const int tz = threadIdx.x % 64, ty = threadIdx.x / 64;
     const int n0 = a.s.n0, n1 = a.s.n1, n2 = a.s.n2;
-    const int k = bz * TZ + tz, j = by * TY + ty;
+    const int k = bz * TZ + tz * VEC, j = by * TY + ty;  // the lane's first point
     const bool active = k < n2 && j < n1;
-    const int kc = min(k, n2 - 1), jc = min(j, n1 - 1);
+    const int kc = min(k, n2 - VEC), jc = min(j, n1 - 1);
     const long plane = (long)n1 * n2;
     const long col = (long)jc * n2 + kc;
+    const int zc = tz * VEC + hw;  // tile column of the lane's first point
 
     // halo sources inside a plane (positions clamped to the mesh; clamped
-    // values are never used by a row evaluated here)
+    // values are never used by a row evaluated here): rows ty < hw and
+    // ty >= TY - hw load the VEC values of a row outside the tile; lanes
+    // tz < hw and tz >= 64 - hw one value beyond either end of the tile row
     int yslot = -1, zslot = -1;
     long yoff = 0, zoff = 0;
     if (ty < hw) {
@@ -108,9 +189,9 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
         const int kk = min(max(bz * TZ - hw + tz, 0), n2 - 1);
         zslot = tz;
         zoff = (long)jc * n2 + kk;
-    } else if (tz >= TZ - hw) {
-        const int kk = min(bz * TZ + TZ + tz - (TZ - hw), n2 - 1);
-        zslot = tz + 2 * hw;
+    } else if (tz >= 64 - hw) {
+        const int kk = min(bz * TZ + TZ + tz - (64 - hw), n2 - 1);
+        zslot = TZ + hw + tz - (64 - hw);
         zoff = (long)jc * n2 + kk;
     }
 
@@ -121,7 +202,15 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
         double* dst = reinterpret_cast<double*>(&crow[0][0]);
         for (int w = threadIdx.x; w < nwords; w += 64 * TY) dst[w] = src[w];
     }
-    const int ny1 = a.nrows[1] + 1, nz1 = a.nrows[2] + 1;
+    // pattern decode table: class kPattern0 + p -> table rows (ix, iy, iz),
+    // one byte each (read after the first barrier of the march)
+    {
+        const int ny1 = a.nrows[1] + 1, nz1 = a.nrows[2] + 1;
+        for (int p = threadIdx.x; p < kMaxPatterns; p += 64 * TY) {
+            const int iz = p % nz1, r = p / nz1;
+            patdec[p] = (unsigned)(r / ny1) | ((unsigned)(r % ny1) << 8) | ((unsigned)iz << 16);
+        }
+    }
 
     const bool px = a.ops.d[0].present, py = a.ops.d[1].present, pz = a.ops.d[2].present;
     TI cx[W], cy[W], cz[W];
@@ -136,144 +225,165 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
     const int x1 = min(x0 + a.xchunk, n0);
     const TI* __restrict__ us = static_cast<const TI*>(a.us);
     TO* out = static_cast<TO*>(a.out);
-    // x queue: q[m] = us at plane x - HQ + m; qn is the next queue element.
-    TI q[QW];
+    // Loads of plane xp into a ring slot.  u0, the accumulator and the class
+    // bytes are single-use streams: non-temporal, so that L2 keeps the stage
+    // values that neighbouring tiles and chunks re-read.  Stage 0 reads u0
+    // as its stage value, so its u0 comes from the queue instead.  Past the
+    // chunk every lane loads the plane's first points instead (one line per
+    // wave), so that the loads stay unconditional.
+    auto load_plane = [&](PlaneLoad<TI, VEC>& ld, int xp) {
+        const bool in = xp < x1;
+        const long pp = (long)min(xp, x1 - 1) * plane;
+        const long cc = in ? col : 0;
+        vload<VEC>(us + (long)min(xp + HQ, n0 - 1) * plane + cc, ld.q);
+        if (yslot >= 0) {
+            vload<VEC>(us + pp + (in ? yoff : 0), ld.y);
+        } else {
 #pragma unroll
-    for (int m = 0; m < QW - 1; ++m) q[m] = us[(long)min(max(x0 - HQ + m, 0), n0 - 1) * plane + col];
-    TI qn = us[(long)min(x0 + HQ, n0 - 1) * plane + col];
-    // values of the current plane; those of the next plane are loaded one
-    // iteration ahead so that their latency overlaps the current plane
-    long pcur = (long)x0 * plane;
-    TI yv = yslot >= 0 ? us[pcur + yoff] : (TI)0;
-    TI zv = zslot >= 0 ? us[pcur + zoff] : (TI)0;
-    unsigned char c = a.cls[pcur + col];
-    double u0v = a.u0[pcur + col];
-    double accv = STAGE == 0 ? 0.0 : a.acc[pcur + col];
+            for (int e = 0; e < VEC; ++e) ld.y[e] = (TI)0;
+        }
+        ld.z = zslot >= 0 ? us[pp + (in ? zoff : 0)] : (TI)0;
+        vload_cls<VEC>(a.cls + pp + cc, ld.c);
+        if (STAGE != 0) {
+            vload_nt<VEC>(a.u0 + pp + cc, ld.u0);
+            vload_nt<VEC>(a.acc + pp + cc, ld.acc);
+        } else {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) ld.u0[e] = ld.acc[e] = 0.0;
+        }
+    };
+    // x queue: q[m] = us at plane x - HQ + m for m < QW - 1; the last entry
+    // (plane x + HQ) is read from the ring slot of plane x
+    TI q[QW - 1][VEC];
+#pragma unroll
+    for (int m = 0; m < QW - 1; ++m) vload<VEC>(us + (long)min(max(x0 - HQ + m, 0), n0 - 1) * plane + col, q[m]);
+    // the ring: slot r holds the loads of plane xb + r; they are used in
+    // place while that plane is evaluated, then the slot is reloaded with
+    // plane xb + r + kMarchAhead, so that no in-flight value is ever copied
+    // (the chunk bound is uniform over the workgroup)
+    PlaneLoad<TI, VEC> ring[kMarchAhead];
+#pragma unroll
+    for (int r = 0; r < kMarchAhead; ++r) load_plane(ring[r], x0 + r);
 
-    for (int x = x0; x < x1; ++x) {
-        const long pb = (long)x * plane;
-        q[QW - 1] = qn;
-        // next plane (the last iteration reloads its own plane; unused)
-        const int xn = min(x + 1, x1 - 1);
-        const long pn = (long)xn * plane;
-        qn = us[(long)min(xn + HQ, n0 - 1) * plane + col];
-        const TI nyv = yslot >= 0 ? us[pn + yoff] : (TI)0;
-        const TI nzv = zslot >= 0 ? us[pn + zoff] : (TI)0;
-        // single-use streams: non-temporal, so that L2 keeps the stage
-        // values that neighbouring tiles and chunks re-read
-        const unsigned char nc = __builtin_nontemporal_load(a.cls + pn + col);
-        const double nu0 = __builtin_nontemporal_load(a.u0 + pn + col);
-        const double nacc = STAGE == 0 ? 0.0 : __builtin_nontemporal_load(a.acc + pn + col);
-        const long idx = pb + col;
-        TI(*t)[LZ] = tile[x & 1];
-        t[ty + hw][tz + hw] = q[HQ];
-        if (yslot >= 0) t[yslot][tz + hw] = yv;
-        if (zslot >= 0) t[ty + hw][zslot] = zv;
-        __syncthreads();
-        if (active && c != kSpecial) {
-            double rhs = 0.0;
-            double v;
-            {
-                if (c != kNoRows) {
-                    int ix = 0, iy = 0, iz = 0;
-                    if (c >= kPattern0) {
-                        int p = c - kPattern0;
-                        iz = p % nz1;
-                        p /= nz1;
-                        iy = p % ny1;
-                        ix = p / ny1;
-                    }
-                    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-                    if (px) {
-                        double sx = 0.0;
-                        if (ix == 0) {
-                            if constexpr (std::is_same<TI, double>::value) {
+    for (int xb = x0; xb < x1; xb += kMarchAhead)
 #pragma unroll
-                                for (int m = 0; m < W; ++m) sx += cx[m] * q[kXReach + m];
-                            } else {
-                                TI v[W];
+    for (int r = 0; r < kMarchAhead; ++r) {
+        const int x = xb + r;
+        PlaneLoad<TI, VEC>& ld = ring[r];
+        // steps past the chunk (its last group) only keep the ring's loads
+        // unconditional
+        if (x < x1) {
+            const long idx = (long)x * plane + col;
+            TI(*t)[LZ] = tile[x & 1];
 #pragma unroll
-                                for (int m = 0; m < W; ++m) v[m] = q[kXReach + m];
-                                sx = idot<W>(cx, v);
-                            }
-                        } else {
-                            // closure row over the queue: only its own columns,
-                            // in column order
-                            const ClosureRow& r = crow[0][ix - 1];
+            for (int e = 0; e < VEC; ++e) t[ty + hw][zc + e] = q[HQ][e];
+            if (yslot >= 0) {
 #pragma unroll
-                            for (int m = 0; m < QW; ++m) {
-                                const int ci = m - HQ - r.off;
-                                if (ci >= 0 && ci < r.n) sx += r.w[ci] * (double)q[m];
-                            }
-                        }
-                        d0 = sx;
-                    }
-                    if (py) {
-                        if (iy == 0) {
-                            if constexpr (std::is_same<TI, double>::value) {
-                                double sy = 0.0;
+                for (int e = 0; e < VEC; ++e) t[yslot][zc + e] = ld.y[e];
+            }
+            if (zslot >= 0) t[ty + hw][zslot] = ld.z;
+            __syncthreads();
+            double vout[VEC], vacc[VEC];
+            bool ok[VEC];
+            bool all = true;
 #pragma unroll
-                                for (int m = 0; m < W; ++m) sy += cy[m] * t[ty + m][tz + hw];
-                                d1 = sy;
-                            } else {
-                                TI v[W];
+            for (int e = 0; e < VEC; ++e) {
+                const unsigned char c = ld.c[e];
+                ok[e] = active && c != kSpecial;
+                all = all && ok[e];
+                const int ke = k + e, ze = zc + e;
+                auto qv = [&](int m) -> TI { return m < QW - 1 ? q[m][e] : ld.q[e]; };
+                // interior circulant rows of every present direction, for
+                // every lane (values of lanes that do not use them are
+                // discarded): dots from 0.0 in tap order
+                double sx = 0.0, sy = 0.0, sz = 0.0;
+                if constexpr (std::is_same<TI, double>::value) {
 #pragma unroll
-                                for (int m = 0; m < W; ++m) v[m] = t[ty + m][tz + hw];
-                                d1 = idot<W>(cy, v);
-                            }
-                        } else {
-                            d1 = tile_row(crow[1][iy - 1], &t[ty + hw][tz + hw], LZ);
-                        }
-                    }
-                    if (pz) {
-                        if (iz == 0) {
-                            if constexpr (std::is_same<TI, double>::value) {
-                                double sz = 0.0;
+                    for (int m = 0; m < W; ++m) sx += cx[m] * qv(kXReach + m);
 #pragma unroll
-                                for (int m = 0; m < W; ++m) sz += cz[m] * t[ty + hw][tz + m];
-                                d2 = sz;
-                            } else {
-                                TI v[W];
+                    for (int m = 0; m < W; ++m) sy += cy[m] * t[ty + m][ze];
 #pragma unroll
-                                for (int m = 0; m < W; ++m) v[m] = t[ty + hw][tz + m];
-                                d2 = idot<W>(cz, v);
-                            }
-                        } else {
-                            d2 = tile_row(crow[2][iz - 1], &t[ty + hw][tz + hw], 1);
-                        }
-                    }
-                    if (kIsHeat<KIND>) {
-                        // laplacian rows added in direction order (plus_eq)
-                        double lap = 0.0;
-                        if (px) lap += d0;
-                        if (py) lap += d1;
-                        if (pz) lap += d2;
-                        rhs = heat_finish<KIND>(a.s, false, true, x, j, k, lap, ss.t_s);
-                    } else {
-                        rhs = wave_combine<KIND>(a.s, false, x, j, k, d0, d1, d2);
-                    }
+                    for (int m = 0; m < W; ++m) sz += cz[m] * t[ty + hw][ze - hw + m];
                 } else {
-                    rhs = kIsHeat<KIND> ? heat_finish<KIND>(a.s, false, false, x, j, k, 0.0, ss.t_s)
-                                               : wave_combine<KIND>(a.s, false, x, j, k, 0.0, 0.0, 0.0);
+                    TI vx[W], vy[W], vz[W];
+#pragma unroll
+                    for (int m = 0; m < W; ++m) {
+                        vx[m] = qv(kXReach + m);
+                        vy[m] = t[ty + m][ze];
+                        vz[m] = t[ty + hw][ze - hw + m];
+                    }
+                    sx = idot<W>(cx, vx);
+                    sy = idot<W>(cy, vy);
+                    sz = idot<W>(cz, vz);
                 }
-                // stage_update, with non-temporal stores
-                const double aa = (STAGE == 0 ? 0.0 : accv) + ss.c_acc * rhs;
-                if (STAGE < 3) {
-                    __builtin_nontemporal_store(aa, a.acc + idx);
-                    v = u0v + ss.c_next * rhs;
+                // closure rows of a pattern point replace the interior rows
+                // of their directions
+                if (ok[e] && c >= kPattern0) {
+                    const unsigned pd = patdec[c - kPattern0];
+                    const int ix = (int)(pd & 0xffu), iy = (int)((pd >> 8) & 0xffu), iz = (int)(pd >> 16);
+                    if (ix != 0) {
+                        // over the queue: only the row's own columns, in
+                        // column order
+                        const ClosureRow& cr = crow[0][ix - 1];
+                        double s = 0.0;
+#pragma unroll
+                        for (int m = 0; m < QW; ++m) {
+                            const int ci = m - HQ - cr.off;
+                            if (ci >= 0 && ci < cr.n) s += cr.w[ci] * (double)qv(m);
+                        }
+                        sx = s;
+                    }
+                    if (iy != 0) sy = tile_row(crow[1][iy - 1], &t[ty + hw][ze], LZ);
+                    if (iz != 0) sz = tile_row(crow[2][iz - 1], &t[ty + hw][ze], 1);
+                }
+                // a solid point (kNoRows) has no rows: zero derivatives, no
+                // source
+                const bool rows = c != kNoRows;
+                const double d0 = px && rows ? sx : 0.0;
+                const double d1 = py && rows ? sy : 0.0;
+                const double d2 = pz && rows ? sz : 0.0;
+                double rhs;
+                if (kIsHeat<KIND>) {
+                    // laplacian rows added in direction order (plus_eq)
+                    double lap = 0.0;
+                    if (px) lap += d0;
+                    if (py) lap += d1;
+                    if (pz) lap += d2;
+                    rhs = heat_finish<KIND>(a.s, false, rows, x, j, ke, rows ? lap : 0.0, ss.t_s);
                 } else {
-                    v = u0v + 1.0 * aa;
+                    rhs = wave_combine<KIND>(a.s, false, x, j, ke, d0, d1, d2);
+                }
+                // stage_update (rk4.cpp:27-46 for one point)
+                const double u0v = STAGE == 0 ? (double)q[HQ][e] : ld.u0[e];
+                const double aa = (STAGE == 0 ? 0.0 : ld.acc[e]) + ss.c_acc * rhs;
+                vacc[e] = aa;
+                vout[e] = STAGE < 3 ? u0v + ss.c_next * rhs : u0v + 1.0 * aa;
+            }
+            // non-temporal stores: the lane's whole vector when the tile
+            // evaluated all its points, else point by point (a special point
+            // is written by the special work)
+            if (all) {
+                TO o[VEC];
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) o[e] = (TO)vout[e];
+                if (STAGE < 3) vstore_nt<VEC>(a.acc + idx, vacc);
+                vstore_nt<VEC>(out + idx, o);
+            } else {
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    if (!ok[e]) continue;
+                    if (STAGE < 3) __builtin_nontemporal_store(vacc[e], a.acc + idx + e);
+                    __builtin_nontemporal_store((TO)vout[e], out + idx + e);
                 }
             }
-            __builtin_nontemporal_store((TO)v, out + idx);
-        }
 #pragma unroll
-        for (int m = 0; m < QW - 1; ++m) q[m] = q[m + 1];
-        yv = nyv;
-        zv = nzv;
-        c = nc;
-        u0v = nu0;
-        accv = nacc;
+            for (int m = 0; m < QW - 2; ++m)
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) q[m][e] = q[m + 1][e];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) q[QW - 2][e] = ld.q[e];
+        }
+        load_plane(ld, x + kMarchAhead);
     }
 }
 
@@ -388,7 +498,7 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
 // floor(g sg / T), T = sg + mg).  A launch with sg = 0 or mg = 0 runs one
 // part alone.  Tile workgroups keep their XCD (b mod 8) and map to tiles
 // contiguously per XCD.
-template <int KIND, int STAGE, int W, int TY, bool MIXED>
+template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
 __device__ __forceinline__ void stage_block(const MarchArgs& a, const SpecialArgs& sp,
                                             const StageScalars& ss, unsigned b, unsigned sg,
                                             unsigned mg, int mode)
@@ -419,14 +529,14 @@ __device__ __forceinline__ void stage_block(const MarchArgs& a, const SpecialArg
     }
     const unsigned m = mi * 8u + lane8;
     if (m >= a.nblocks) return;
-    march_tile<KIND, STAGE, W, TY, MIXED>(a, ss, xcd_remap(m, a.nblocks));
+    march_tile<KIND, STAGE, W, TY, VEC, MIXED>(a, ss, xcd_remap(m, a.nblocks));
 }
 
-template <int KIND, int STAGE, int W, int TY, bool MIXED>
+template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
 __global__ __launch_bounds__(64 * TY) void k_stage(MarchArgs a, SpecialArgs sp, StageScalars ss,
                                                    unsigned sg, unsigned mg, int mode)
 {
-    stage_block<KIND, STAGE, W, TY, MIXED>(a, sp, ss, blockIdx.x, sg, mg, mode);
+    stage_block<KIND, STAGE, W, TY, VEC, MIXED>(a, sp, ss, blockIdx.x, sg, mg, mode);
 }
 
 // RK4 stage constants (rk4.cpp:13-14), as the host's
@@ -438,7 +548,7 @@ constexpr double kStageWeight[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 // keep their XCD); its arguments and clock live in device memory, so that a
 // captured launch stays valid at any time.  The stage scalars are formed as
 // the single-problem host path forms them (t = t0 + n dt).
-template <int KIND, int STAGE, int W, int TY, bool MIXED>
+template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
 __global__ __launch_bounds__(64 * TY) void k_batch_stage(const BatchEntry* __restrict__ ent,
                                                          const BatchClock* __restrict__ clk,
                                                          const unsigned* __restrict__ start,
@@ -470,7 +580,7 @@ __global__ __launch_bounds__(64 * TY) void k_batch_stage(const BatchEntry* __res
     ss.c_acc = dt * kStageWeight[STAGE];
     ss.c_next = STAGE < 3 ? dt * kStageTime[STAGE + 1] : 1.0;
     ss.t_next = STAGE < 3 ? t + dt * kStageTime[STAGE + 1] : t + dt;
-    stage_block<KIND, STAGE, W, TY, MIXED>(m, sp, ss, blockIdx.x - start[lo], sg, mg, mode);
+    stage_block<KIND, STAGE, W, TY, VEC, MIXED>(m, sp, ss, blockIdx.x - start[lo], sg, mg, mode);
 }
 
 template <int KIND, int STAGE, int W, bool MIXED>
@@ -483,12 +593,18 @@ int stage_ty(hipStream_t st, int ty, int mode, const MarchArgs& a, const Special
     auto go = [&](unsigned s_groups, unsigned m_groups, int md) {
         const unsigned grid = 8u * (s_groups + m_groups);
         if (grid == 0) return 0;
-        switch (ty) {
-        case 4: k_stage<KIND, STAGE, W, 4, MIXED><<<grid, 64 * 4, 0, st>>>(a, sp, ss, s_groups, m_groups, md); return 0;
-        case 8: k_stage<KIND, STAGE, W, 8, MIXED><<<grid, 64 * 8, 0, st>>>(a, sp, ss, s_groups, m_groups, md); return 0;
-        case 16: k_stage<KIND, STAGE, W, 16, MIXED><<<grid, 64 * 16, 0, st>>>(a, sp, ss, s_groups, m_groups, md); return 0;
+#define SHX_STAGE_CASE(TY, VEC)                                                                      \
+    case TY * 10 + VEC:                                                                              \
+        k_stage<KIND, STAGE, W, TY, VEC, MIXED><<<grid, 64 * TY, 0, st>>>(a, sp, ss, s_groups, m_groups, md); \
+        return 0;
+        switch (ty * 10 + a.vec) {
+            SHX_STAGE_CASE(4, 1)
+            SHX_STAGE_CASE(8, 1)
+            SHX_STAGE_CASE(4, 2)
+            SHX_STAGE_CASE(8, 2)
         default: return -1;
         }
+#undef SHX_STAGE_CASE
     };
     if (mode == 0) {  // two launches: tiles, then special items
         const int rc = go(0, mg, 1);
@@ -538,19 +654,19 @@ int launch_stage_form(hipStream_t st, int kind, int stage, int width, int ty, in
 template <int KIND, int STAGE, int W, bool MIXED>
 int batch_ty(hipStream_t st, int ty, const BatchLaunch& bl)
 {
-    switch (ty) {
-    case 4:
-        k_batch_stage<KIND, STAGE, W, 4, MIXED><<<bl.grid, 64 * 4, 0, st>>>(bl.ent, bl.clk, bl.start, bl.nprob);
+#define SHX_BATCH_CASE(TY, VEC)                                                                      \
+    case TY * 10 + VEC:                                                                              \
+        k_batch_stage<KIND, STAGE, W, TY, VEC, MIXED><<<bl.grid, 64 * TY, 0, st>>>(bl.ent, bl.clk, bl.start, \
+                                                                                 bl.nprob);          \
         return 0;
-    case 8:
-        k_batch_stage<KIND, STAGE, W, 8, MIXED><<<bl.grid, 64 * 8, 0, st>>>(bl.ent, bl.clk, bl.start, bl.nprob);
-        return 0;
-    case 16:
-        k_batch_stage<KIND, STAGE, W, 16, MIXED><<<bl.grid, 64 * 16, 0, st>>>(bl.ent, bl.clk, bl.start, bl.nprob);
-        return 0;
-    default:
-        return -1;
+    switch (ty * 10 + bl.vec) {
+        SHX_BATCH_CASE(4, 1)
+        SHX_BATCH_CASE(8, 1)
+        SHX_BATCH_CASE(4, 2)
+        SHX_BATCH_CASE(8, 2)
+    default: return -1;
     }
+#undef SHX_BATCH_CASE
 }
 
 template <int KIND, int W, bool MIXED>

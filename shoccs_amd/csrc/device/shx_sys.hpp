@@ -141,6 +141,7 @@ struct MarchArgs {
     const ClosureRow* rows;  // 3 x kMaxClosureRows
     int nrows[3];
     int nbz, nby, xchunk;
+    int vec;  // points per lane along z: 2 (even z extent) or 1
     unsigned nblocks;
 };
 
@@ -210,6 +211,7 @@ struct BatchLaunch {
     const BatchClock* clk;
     const unsigned* start;
     int nprob;
+    int vec;  // points per lane along z of every problem's tiles
     unsigned grid;
 };
 

@@ -287,6 +287,7 @@ struct shx_system {
     int variant = 0;
     int march_ty = 4;
     int march_xchunk = 16;
+    int march_vec = 0;  // points per lane along z; 0: 2 when n2 is even, else 1
     // x-marching form: class byte per point of D, the special-point records
     // and the per-direction special-row tables
     unsigned char* cls = nullptr;
@@ -846,7 +847,9 @@ int march_args(shx_system* s, const void* usD, RPtrs usR, const double* u0, doub
     a.rows = s->crows;
     for (int c = 0; c < 3; ++c) a.nrows[c] = s->nrows[c];
     a.xchunk = s->march_xchunk;
-    a.nbz = (s->n[2] + 63) / 64;
+    // two points per lane along z when every pair is aligned (even n2)
+    a.vec = s->march_vec != 0 ? s->march_vec : (s->n[2] % 2 == 0 ? 2 : 1);
+    a.nbz = (s->n[2] + 64 * a.vec - 1) / (64 * a.vec);
     a.nby = (s->n[1] + s->march_ty - 1) / s->march_ty;
     const long nbx = (s->n[0] + a.xchunk - 1) / a.xchunk;
     const long nb = (long)a.nbz * a.nby * nbx;
@@ -1324,7 +1327,7 @@ int shx_time_fused_stage_kernel(shx_system* s, shx_field* u0, double t, double d
 int shx_system_set_tuning(shx_system* s, int variant, int march_ty, int march_xchunk)
 {
     if (!s || variant < 0 || variant > 4) return SHX_EINVAL;
-    if (variant >= 1 && !(march_ty == 4 || march_ty == 8 || march_ty == 16)) return SHX_EINVAL;
+    if (variant >= 1 && !(march_ty == 4 || march_ty == 8)) return SHX_EINVAL;
     if (variant >= 1 && march_xchunk < 1) return SHX_EINVAL;
     if (variant >= 1 && !march_ok(s)) return fail(s->ctx, SHX_EUNSUPPORTED, "x-marching form: unsupported system");
     if (variant == 0 && s->precision == SHX_F32_COMP)
@@ -1402,7 +1405,7 @@ int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt,
     if (s->precision == SHX_F64) cands.push_back({0, 4, 32});
     if (march_ok(s))
         for (int v = 1; v <= 4; ++v)
-            for (int ty : {4, 8, 16})
+            for (int ty : {4, 8})
                 for (int xc : {16, 32, 64, 1 << 16}) cands.push_back({v, ty, std::min(xc, s->n[0])});
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
@@ -1449,7 +1452,7 @@ int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt,
 struct shx_batch {
     shx_ctx* ctx = nullptr;
     int n = 0;
-    int kind = 0, width = 0, precision = SHX_F64, ty = 4;
+    int kind = 0, width = 0, precision = SHX_F64, ty = 4, vec = 1;
     std::vector<shx_system*> sys;
     std::vector<shx_field> u;
     BatchEntry* ent[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -1468,7 +1471,7 @@ int batch_step(shx_batch* b)
 {
     hipStream_t st = b->ctx->stream;
     for (int stage = 0; stage < 4; ++stage) {
-        BatchLaunch bl{b->ent[stage], b->clk, b->start, b->n, b->grid};
+        BatchLaunch bl{b->ent[stage], b->clk, b->start, b->n, b->vec, b->grid};
         const int lrc = b->precision == SHX_F32_COMP
                             ? launch_batch_stage_mixed(st, b->kind, stage, b->width, b->ty, bl)
                             : launch_batch_stage(st, b->kind, stage, b->width, b->ty, bl);
@@ -1495,7 +1498,7 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
                      const double* dt, int n, int march_ty, shx_batch** out)
 {
     if (!ctx || !systems || !u0 || !t0 || !dt || n < 1 || !out) return SHX_EINVAL;
-    if (!(march_ty == 4 || march_ty == 8 || march_ty == 16)) return SHX_EINVAL;
+    if (!(march_ty == 4 || march_ty == 8)) return SHX_EINVAL;
     *out = nullptr;
     auto b = std::unique_ptr<shx_batch>(new (std::nothrow) shx_batch());
     if (!b) return SHX_ENOMEM;
@@ -1505,6 +1508,10 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
     b->kind = systems[0]->family;
     b->width = systems[0]->width;
     b->precision = systems[0]->precision;
+    // two points per lane along z when every problem's z extent is even
+    b->vec = 2;
+    for (int p = 0; p < n; ++p)
+        if (systems[p] && systems[p]->n[2] % 2 != 0) b->vec = 1;
     std::vector<BatchEntry> ent[4];
     std::vector<BatchClock> clk;
     std::vector<unsigned> start;
@@ -1514,13 +1521,15 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
         if (!s || s->ctx != ctx || s->family != b->kind || s->width != b->width || s->precision != b->precision)
             return fail(ctx, SHX_EINVAL, "batch: problems differ in context, system, stencil width or precision");
         if (!march_ok(s)) return fail(ctx, SHX_EUNSUPPORTED, "batch: the x-marching form does not apply");
-        const int saved_ty = s->march_ty;
-        s->march_ty = march_ty;  // tile height shared by the batch
+        const int saved_ty = s->march_ty, saved_vec = s->march_vec;
+        s->march_ty = march_ty;  // tile shape shared by the batch
+        s->march_vec = b->vec;
         if (s->variant == 0) s->variant = 3;
         int rc = ensure_fused(s);
         if (rc == SHX_OK) rc = ensure_classes(s);
         if (rc != SHX_OK) {
             s->march_ty = saved_ty;
+            s->march_vec = saved_vec;
             return rc;
         }
         b->sys.push_back(s);
@@ -1539,6 +1548,7 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
             rc = march_args(s, usV, usR, up->D, s->accD, outV, outR, e.m, e.sp, nitems);
             if (rc != SHX_OK) {
                 s->march_ty = saved_ty;
+                s->march_vec = saved_vec;
                 return rc;
             }
             const long threads = 64L * march_ty;
@@ -1548,6 +1558,7 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
             ent[stage].push_back(e);
         }
         s->march_ty = saved_ty;
+        s->march_vec = saved_vec;
         next += 8ull * (sg + mg);
         clk.push_back(BatchClock{t0[p], dt[p], 0});
         b->points += s->N;

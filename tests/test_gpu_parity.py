@@ -22,6 +22,8 @@ HEAT_CFGS = {
     "heat2d_21": H.heat2d_cfg(21),
     "heat3d_E4_2": H.heat3d_cfg(),
     "heat3d_E2_2": H.heat3d_cfg(scheme=O.E2_2),
+    # odd z extent: one point per lane in the x-marching tiles
+    "heat3d_odd": H.heat3d_cfg(n=(21, 23, 27)),
 }
 ALL = {**WAVE_CFGS, **HEAT_CFGS}
 
@@ -155,8 +157,8 @@ def test_rk4_steps(ctx, name, fused):
 
 
 @pytest.mark.parametrize("name", ["wave48_E4_1", "wave30_radial_E4u", "wave20_E4u_overlap",
-                                  "heat3d_E4_2", "heat3d_E2_2", "heat2d_21"])
-@pytest.mark.parametrize("variant,ty,xchunk", [(1, 4, 16), (2, 8, 7), (3, 16, 64), (4, 8, 1000)])
+                                  "heat3d_E4_2", "heat3d_E2_2", "heat2d_21", "heat3d_odd"])
+@pytest.mark.parametrize("variant,ty,xchunk", [(1, 4, 16), (2, 8, 7), (3, 8, 64), (4, 4, 1000)])
 def test_march_kernel_bitexact(ctx, name, variant, ty, xchunk):
     """The x-marching form (interior and solid points in registers and LDS,
     special points from a list) gives the same doubles as the point-wise
@@ -341,7 +343,7 @@ def test_mixed_form_same_doubles_for_every_tiling(ctx, name):
     dt = osys.dt(0.5, 0.8)
     ref = _run(prob, u0.flat(), dt, 3, S.F32_COMP, variant=3, ty=4, xc=16)
     assert np.all(np.isfinite(ref))
-    for v, ty, xc in [(1, 8, 7), (2, 16, 64), (4, 4, 1000)]:
+    for v, ty, xc in [(1, 8, 7), (2, 8, 64), (4, 4, 1000)]:
         assert np.array_equal(_run(prob, u0.flat(), dt, 3, S.F32_COMP, v, ty, xc), ref)
 
 
@@ -474,12 +476,15 @@ def test_driver_zero_steps_without_bounds(ctx):
 BATCH_CFGS = [H.heat3d_cfg(n=(20, 24, 28)),
               H.heat3d_cfg(n=(26, 22, 30), center=(0.45, 0.5, 0.52), radius=0.25),
               H.heat3d_cfg(n=(18, 34, 24), center=(0.55, 0.52, 0.47), radius=0.18)]
+# one odd z extent: the whole batch takes one point per lane, the problems
+# stepped alone two (where even)
+BATCH_CFGS_ODD = BATCH_CFGS[:2] + [H.heat3d_cfg(n=(19, 22, 25), center=(0.52, 0.5, 0.49), radius=0.2)]
 
 
-def _batch_setup(ctx, precision):
+def _batch_setup(ctx, precision, cfgs=BATCH_CFGS):
     import shoccs_amd as S
 
-    pairs = [_pair(ctx, c) for c in BATCH_CFGS]
+    pairs = [_pair(ctx, c) for c in cfgs]
     init, dts = [], []
     for osys, prob in pairs:
         u0 = osys.field()
@@ -493,14 +498,15 @@ def _batch_setup(ctx, precision):
 
 
 @pytest.mark.parametrize("precision", ["F64", "F32_COMP"])
-def test_batch_matches_single_problems(ctx, precision):
+@pytest.mark.parametrize("odd", [False, True])
+def test_batch_matches_single_problems(ctx, precision, odd):
     """A batch of heat problems of different shapes steps every problem to
     the same doubles as stepping it alone (same kernels, same stage scalars),
     eagerly and by graph replay, and keeps their clocks."""
     import shoccs_amd as S
 
     prec = getattr(S, precision)
-    pairs, init, dts = _batch_setup(ctx, prec)
+    pairs, init, dts = _batch_setup(ctx, prec, BATCH_CFGS_ODD if odd else BATCH_CFGS)
     want = []
     for (osys, prob), u, dt in zip(pairs, init, dts):
         g = prob.field()
@@ -513,7 +519,7 @@ def test_batch_matches_single_problems(ctx, precision):
         f = prob.field()
         f.upload(u)
         fields.append(f)
-    b = S.Batch(ctx, [p for _, p in pairs], fields, [0.0] * 3, dts)
+    b = S.Batch(ctx, [p for _, p in pairs], fields, [0.0] * len(pairs), dts)
     b.steps(2)
     b.capture(1)
     b.launch()
