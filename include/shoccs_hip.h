@@ -277,6 +277,41 @@ int shx_batch_launch_async(shx_batch* b);
 /* current time of every problem */
 int shx_batch_times(shx_batch* b, double* t);
 
+/* ----------------------------------------------------------------- adjoint
+ * No reference counterpart: the reference has no adjoint.  Transposes of the
+ * operators above and the discrete adjoint of the RK4 step, for gradients of a
+ * final-time misfit with respect to the initial field and the interior
+ * stencil coefficients. */
+
+/* Transpose of one direction's derivative.  With du.D = O u.D + B u.R[dir]
+ * the eq-form apply of shx_derivative_apply (no Neumann N term), sets
+ * ubar.D = O^T dbar.D and ubar.R[dir] = B^T dbar.D (accumulate = 0), or adds
+ * them (1).  Columns whose rows are all circulant rows take the reversed
+ * stencil; every other column and every cut point sums its entries with rows
+ * ascending.  Built on the first call (host, once per operator). */
+int shx_derivative_apply_T(shx_derivative* d, const shx_field* dbar, shx_field* ubar, int accumulate);
+
+/* wbar = (d rhs / d u)^T kbar for the system's right-hand side (rhs rows on
+ * D; the rows of the cut points are zero for Dirichlet bodies): scalar wave
+ * sum_c D_c^T (gG_c kbar), heat sum_c D_c^T (k kbar), zero weight on
+ * Dirichlet faces.  D and R parts of wbar are written. */
+int shx_system_rhs_T(shx_system* s, const shx_field* kbar, shx_field* wbar);
+
+/* Discrete adjoint of one RK4 step (rk4.cpp:16-47, FP64) taken from time t
+ * with step dt: u holds the step's input u_n; on entry ubar holds
+ * dJ/du_{n+1}, on exit dJ/du_n.  The stage inputs are recomputed from u_n;
+ * entries that the boundary update overwrites (Dirichlet faces and cut
+ * points) carry no sensitivity through it.  When coef_grad is not NULL, the
+ * step's dJ/dcoef[c][m] (coef: the stored interior coefficients of direction
+ * c's operator) is added to coef_grad[9 c + m] on the host.  Synchronous. */
+int shx_rk4_adjoint(shx_system* s, const shx_field* u, shx_field* ubar, double t, double dt,
+                    double* coef_grad);
+
+/* Interior (circulant) stencil coefficients of direction dir's operator:
+ * replace them (width must match the operator's), or read them (up to 9). */
+int shx_system_set_interior(shx_system* s, int dir, const double* coef, int width);
+int shx_system_get_interior(const shx_system* s, int dir, double coef[9], int* width);
+
 /* ------------------------------------------------------------- measurement */
 /* Run `nsteps` fused steps `reps` times and report the mean device time per
  * step in milliseconds, measured with HIP events on the context's stream. */

@@ -116,6 +116,17 @@ class Field:
               self.problem.ctx.p, "field download")
         return out
 
+    def copy_from(self, other: "Field"):
+        """Device copy of every part (D, Rx, Ry, Rz) of another field of the
+        same problem (shx_copy)."""
+        assert other.problem is self.problem
+        ctxp = self.problem.ctx.p
+        check(lib().shx_copy(ctxp, self.c.D, other.c.D, self.problem.N), ctxp, "copy")
+        for d in range(3):
+            n = self.problem.nR[d]
+            if n:
+                check(lib().shx_copy(ctxp, self.c.R[d], other.c.R[d], n), ctxp, "copy")
+
     def free(self):
         if getattr(self, "c", None) is not None and self.problem.p:
             lib().shoccs_field_free(self.problem.p, C.byref(self.c))
@@ -292,6 +303,53 @@ class Problem:
                   "laplacian")
 
         return apply
+
+    def derivative_T(self, d: int):
+        """Transpose of direction d's derivative (shx_derivative_apply_T):
+        apply(dbar, ubar, op) sets (op = eq) or adds (op = plus_eq)
+        ubar.D = O^T dbar.D and ubar.R[d] = B^T dbar.D."""
+        self._need_device()
+        dev = lib().shoccs_problem_derivative(self.p, d)
+
+        def apply(dbar: Field, ubar: Field, op: int = eq):
+            check(lib().shx_derivative_apply_T(dev, dbar.ref(), ubar.ref(), op), self.ctx.p,
+                  "shx_derivative_apply_T")
+
+        return apply
+
+    def rhs_T(self, kbar: Field, wbar: Field):
+        """wbar = (d rhs / d u)^T kbar (shx_system_rhs_T)."""
+        self._need_device()
+        check(lib().shx_system_rhs_T(self.sys, kbar.ref(), wbar.ref()), self.ctx.p, "rhs_T")
+
+    def rk4_adjoint(self, u: Field, ubar: Field, t: float, dt: float,
+                    coef_grad: np.ndarray | None = None):
+        """Adjoint of one RK4 step from u at time t (shx_rk4_adjoint): ubar
+        holds dJ/du_{n+1} on entry and dJ/du_n on exit; the step's
+        interior-coefficient gradient is added to coef_grad (3 x 9) when given."""
+        self._need_device()
+        gp = None
+        if coef_grad is not None:
+            assert coef_grad.dtype == np.float64 and coef_grad.size == 27 and coef_grad.flags.c_contiguous
+            gp = coef_grad.ctypes.data_as(C.POINTER(C.c_double))
+        check(lib().shx_rk4_adjoint(self.sys, u.ref(), ubar.ref(), t, dt, gp), self.ctx.p,
+              "rk4_adjoint")
+
+    def device_interior(self, d: int) -> np.ndarray:
+        """Interior stencil coefficients that direction d's device operator
+        applies (those of the assembly unless replaced by set_interior)."""
+        self._need_device()
+        c = np.zeros(9)
+        w = C.c_int()
+        check(lib().shx_system_get_interior(self.sys, d, c.ctypes.data_as(C.POINTER(C.c_double)),
+                                            C.byref(w)), self.ctx.p, "get_interior")
+        return c[:w.value].copy()
+
+    def set_interior(self, d: int, coef) -> None:
+        self._need_device()
+        c = np.ascontiguousarray(coef, dtype=np.float64)
+        check(lib().shx_system_set_interior(self.sys, d, c.ctypes.data_as(C.POINTER(C.c_double)),
+                                            c.size), self.ctx.p, "set_interior")
 
     def rhs(self, u: Field, out: Field, t: float, fused: bool = True):
         self._need_device()

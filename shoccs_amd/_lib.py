@@ -164,6 +164,13 @@ _SIGS = [
     ("shx_time_batch", C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, C.POINTER(C.c_int64)]),
     ("shoccs_run", C.c_int, [C.c_void_p, C.POINTER(RunDesc), C.POINTER(ShxField),
                              C.POINTER(RunResult)]),
+    ("shx_derivative_apply_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
+                                         C.c_int]),
+    ("shx_system_rhs_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField)]),
+    ("shx_rk4_adjoint", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField), C.c_double,
+                                  C.c_double, dp]),
+    ("shx_system_set_interior", C.c_int, [C.c_void_p, C.c_int, dp, C.c_int]),
+    ("shx_system_get_interior", C.c_int, [C.c_void_p, C.c_int, dp, C.POINTER(C.c_int)]),
 ]
 
 EXPORTED = [s[0] for s in _SIGS]

@@ -445,6 +445,7 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     d->h_blocks.assign(desc->blocks, desc->blocks + desc->nblocks);
     d->h_pool.assign(desc->pool, desc->pool + desc->npool);
     d->h_pair_ptr = std::move(pair_ptr);
+    for (int m = 0; m < 9; ++m) d->h_coef[m] = desc->interior[m];
     d->dev.pair_ptr = d->pair_ptr;
     d->dev.fast = d->fast;
     d->dev.blocks = d->blocks;
@@ -457,6 +458,7 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
 int shx_derivative_destroy(shx_derivative* d)
 {
     if (!d) return SHX_EINVAL;
+    destroy_transpose(d->T);
     hipFree(d->pair_ptr);
     hipFree(d->fast);
     hipFree(d->blocks);

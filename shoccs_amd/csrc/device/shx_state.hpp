@@ -15,6 +15,8 @@ struct shx_ctx {
     std::string err;
 };
 
+struct shx_deriv_T;  // transposed operator (shx_adjoint.hip), built on first use
+
 struct shx_derivative {
     shx_ctx* ctx = nullptr;
     int dir = 0;
@@ -36,6 +38,8 @@ struct shx_derivative {
     std::vector<shx_line_block> h_blocks;
     std::vector<double> h_pool;
     std::vector<int> h_pair_ptr;
+    double h_coef[9] = {};  // interior stencil
+    shx_deriv_T* T = nullptr;
 };
 
 struct shx_csr {
@@ -58,6 +62,7 @@ int launch_neumann(shx_derivative* d, const double* nu, double* du);
 int launch_csr_full(shx_csr* A, const double* x, double* b);
 int zero_field(shx_ctx* ctx, const shx_field* f, const int n[3], const long nR[3]);
 int sync(shx_ctx* ctx);
+void destroy_transpose(shx_deriv_T* T);
 template <typename T>
 int upload(shx_ctx* ctx, T** dst, const T* src, size_t n);
 

@@ -1,0 +1,70 @@
+// shx_system_state.hpp -- host-side state behind the opaque shx_system handle
+// (shared by the translation units that implement the system entry points).
+#pragma once
+
+#include "shx_internal.hpp"
+#include "shx_state.hpp"
+#include "shx_sys.hpp"
+
+struct shx_adjoint_ws;
+
+struct shx_system {
+    shx_ctx* ctx = nullptr;
+    int kind = 0;
+    int family = 0;  // kernel family: kind, or kHeatGauss for heat with the gaussian MMS
+    int n[3] = {1, 1, 1};
+    long N = 0;
+    long nR[3] = {0, 0, 0};
+    shx_derivative* d[3] = {nullptr, nullptr, nullptr};
+    shx::DevSys dev;
+    shx::DevOps ops;
+    shx::Tiling tiling;
+    // device buffers
+    double* xyz[3] = {nullptr, nullptr, nullptr};
+    double* sintab[3] = {nullptr, nullptr, nullptr};  // sin-product MMS tables
+    double* mms = nullptr;
+    unsigned* fluid_mask = nullptr;
+    int* fluid = nullptr;
+    long nfluid = 0;
+    int* faces = nullptr;
+    long nfaces = 0;
+    double* rpos[3] = {nullptr, nullptr, nullptr};
+    signed char* rdir[3] = {nullptr, nullptr, nullptr};
+    // fused RK4 registers (stage ping-pong and accumulator), allocated lazily
+    double* stageD[2] = {nullptr, nullptr};
+    float* stageF[2] = {nullptr, nullptr};  // mixed form: FP32 stage register
+    int precision = SHX_F64;
+    double* stageR[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    double* accD = nullptr;
+    // reference-schedule buffers, allocated lazily
+    bool unfused_ready = false;
+    double* gG[3][4] = {};
+    double* du[3][4] = {};
+    double* srcD = nullptr;
+    shx_csr* Bfull[3] = {nullptr, nullptr, nullptr};
+    // statistics partials
+    double* part = nullptr;
+    long* part_idx = nullptr;
+    // fused-stage kernel choice: 0 point-wise tiles; 1-4 x-marching tiles
+    // with the special work in a second launch (1), or in the same launch
+    // before the tiles (2), after them (3) or spread among them (4)
+    int variant = 0;
+    int march_ty = 4;
+    int march_xchunk = 16;
+    int march_vec = 0;  // points per lane along z; 0: 2 when n2 is even, else 1
+    // x-marching form: class byte per point of D, the special-point records
+    // and the per-direction special-row tables
+    unsigned char* cls = nullptr;
+    shx::SpecialRec* special = nullptr;
+    shx::SpecialRow* srows[3] = {nullptr, nullptr, nullptr};
+    shx::ClosureRow* crows = nullptr;  // 3 x kMaxClosureRows
+    int nrows[3] = {0, 0, 0};
+    long nspecial = 0;
+    bool classes_ready = false;
+    int width = 5;  // interior stencil width shared by the three derivatives
+    shx_adjoint_ws* adj = nullptr;  // adjoint workspace, allocated on first use
+};
+
+namespace shx {
+void destroy_adjoint_ws(shx_adjoint_ws* w);
+}  // namespace shx

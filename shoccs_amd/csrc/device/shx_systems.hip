@@ -17,6 +17,7 @@
 #include "shx_state.hpp"
 #include "shx_point.hpp"
 #include "shx_sys.hpp"
+#include "shx_system_state.hpp"
 
 using namespace shx;
 
@@ -244,61 +245,6 @@ inline unsigned grid_for(long n) { return (unsigned)((n + 255) / 256); }
 }  // namespace
 
 // ---------------------------------------------------------------- system state
-struct shx_system {
-    shx_ctx* ctx = nullptr;
-    int kind = 0;
-    int family = 0;  // kernel family: kind, or kHeatGauss for heat with the gaussian MMS
-    int n[3] = {1, 1, 1};
-    long N = 0;
-    long nR[3] = {0, 0, 0};
-    shx_derivative* d[3] = {nullptr, nullptr, nullptr};
-    DevSys dev;
-    DevOps ops;
-    Tiling tiling;
-    // device buffers
-    double* xyz[3] = {nullptr, nullptr, nullptr};
-    double* sintab[3] = {nullptr, nullptr, nullptr};  // sin-product MMS tables
-    double* mms = nullptr;
-    unsigned* fluid_mask = nullptr;
-    int* fluid = nullptr;
-    long nfluid = 0;
-    int* faces = nullptr;
-    long nfaces = 0;
-    double* rpos[3] = {nullptr, nullptr, nullptr};
-    signed char* rdir[3] = {nullptr, nullptr, nullptr};
-    // fused RK4 registers (stage ping-pong and accumulator), allocated lazily
-    double* stageD[2] = {nullptr, nullptr};
-    float* stageF[2] = {nullptr, nullptr};  // mixed form: FP32 stage register
-    int precision = SHX_F64;
-    double* stageR[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
-    double* accD = nullptr;
-    // reference-schedule buffers, allocated lazily
-    bool unfused_ready = false;
-    double* gG[3][4] = {};
-    double* du[3][4] = {};
-    double* srcD = nullptr;
-    shx_csr* Bfull[3] = {nullptr, nullptr, nullptr};
-    // statistics partials
-    double* part = nullptr;
-    long* part_idx = nullptr;
-    // fused-stage kernel choice: 0 point-wise tiles; 1-4 x-marching tiles
-    // with the special work in a second launch (1), or in the same launch
-    // before the tiles (2), after them (3) or spread among them (4)
-    int variant = 0;
-    int march_ty = 4;
-    int march_xchunk = 16;
-    int march_vec = 0;  // points per lane along z; 0: 2 when n2 is even, else 1
-    // x-marching form: class byte per point of D, the special-point records
-    // and the per-direction special-row tables
-    unsigned char* cls = nullptr;
-    SpecialRec* special = nullptr;
-    SpecialRow* srows[3] = {nullptr, nullptr, nullptr};
-    ClosureRow* crows = nullptr;  // 3 x kMaxClosureRows
-    int nrows[3] = {0, 0, 0};
-    long nspecial = 0;
-    bool classes_ready = false;
-    int width = 5;  // interior stencil width shared by the three derivatives
-};
 
 namespace {
 
@@ -1116,6 +1062,7 @@ int shx_system_destroy(shx_system* s)
         }
         if (s->Bfull[c]) shx_csr_destroy(s->Bfull[c]);
     }
+    destroy_adjoint_ws(s->adj);
     hipFree(s->stageD[0]);
     hipFree(s->stageD[1]);
     hipFree(s->stageF[0]);
