@@ -264,6 +264,114 @@ def heat_batch64(args):
     print(json.dumps(result))
 
 
+def smooth_field(prob, seed, modes=8):
+    """A seeded smooth field on the mesh: a sum of `modes` sines of integer
+    wave vectors in [1, 3]^3 with random phases and N(0, 1/modes) amplitudes,
+    sampled at D's points and the cut points."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    kv = rng.integers(1, 4, size=(modes, 3))
+    ph = rng.uniform(0, 2 * np.pi, modes)
+    am = rng.standard_normal(modes) / modes
+    n = prob.n
+    axes = [np.linspace(0.0, 1.0, n[d]) for d in range(3)]
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+
+    def f(x, y, z):
+        v = np.zeros_like(x, dtype=np.float64)
+        for k, p_, a in zip(kv, ph, am):
+            v += a * np.sin(2 * np.pi * (k[0] * x + k[1] * y + k[2] * z) + p_)
+        return v
+
+    parts = [f(X, Y, Z).ravel()]
+    for d in range(3):
+        pos = prob.r_arrays(d)["pos"] if prob.nR[d] else np.zeros((0, 3))
+        parts.append(f(pos[:, 0], pos[:, 1], pos[:, 2]) if len(pos) else np.zeros(0))
+    return np.concatenate(parts)
+
+
+def adjoint128(args):
+    """The adjoint workload: J = 1/2 ||u(T) - u_obs||^2 of the 128^3 scalar
+    wave with the sphere after 400 RK4 steps (checkpoints every 20), u_obs from
+    a forward run of a seeded smooth field; gradient in the initial field and
+    the interior coefficients by the discrete adjoint, checked by central
+    differences along 16 seeded random directions."""
+    import numpy as np
+
+    import shoccs_amd as S
+    from shoccs_amd.adjoint import Solve
+
+    n = 128 if args.n == N_SIDE else args.n
+    nsteps, every = 400, 20
+    ctx = S.Context(0)
+    cfg = workload_cfg(n)
+    # E4u closures with alpha = (0, 0) at CFL 0.05 (default): the smooth
+    # field stays O(1e3) over the 400 steps; with the scalar-wave configs'
+    # alpha, or CFL 0.5, the solution grows without bound on this body
+    prob = S.Problem(ctx, cfg["kind"], cfg["n"], cfg["lo"], cfg["hi"], spheres=cfg["spheres"],
+                     shape_bc=cfg["shape_bc"], grid_bc=cfg["grid_bc"], scheme=cfg["scheme"],
+                     alpha=(0.0, 0.0), params=cfg["params"])
+    dt = prob.dt(args.adj_cfl, 1.0)
+    sol = Solve(prob, 0.0, dt, nsteps, every)
+    u_true = smooth_field(prob, 5)
+    u0 = u_true + 0.1 * smooth_field(prob, 6)
+    c0 = sol.coef()
+    u_obs = sol.run(u_true)
+    ctx.sync()
+    t0 = time.perf_counter()
+    uT = sol.run(u0)
+    ctx.sync()
+    t_fwd = time.perf_counter() - t0
+    J0 = 0.5 * float((uT - u_obs) @ (uT - u_obs))
+    t0 = time.perf_counter()
+    g_u, g_c = sol.gradient(uT - u_obs)
+    ctx.sync()
+    t_adj = time.perf_counter() - t0
+    w = c0.shape[1]
+    rng = np.random.default_rng(7)
+    # 12 directions in u0 (J is quadratic in u0: the central difference is
+    # exact up to rounding) and 4 in the coefficients (4th-order central
+    # differences: J is a high-degree polynomial in them)
+    err_u, err_c = [], []
+    for k in range(12):
+        vu = rng.standard_normal(u0.size)
+        eps = 1e-2
+        fd = (sol.misfit(u0 + eps * vu, u_obs) - sol.misfit(u0 - eps * vu, u_obs)) / (2 * eps)
+        ad = float(g_u @ vu)
+        err_u.append(abs(fd - ad) / max(abs(fd), abs(ad)))
+    for k in range(4):
+        vc = np.abs(c0).max() * rng.standard_normal(c0.shape)
+        eps = 1e-6
+
+        def J_at(e):
+            sol.set_coef(c0 + e * vc)
+            return sol.misfit(u0, u_obs)
+
+        fd = (8 * (J_at(eps) - J_at(-eps)) - (J_at(2 * eps) - J_at(-2 * eps))) / (12 * eps)
+        sol.set_coef(c0)
+        ad = float(np.sum(g_c[:, :w] * vc))
+        err_c.append(abs(fd - ad) / max(abs(fd), abs(ad)))
+    errs = err_u + err_c
+    result = {
+        "metric": "adjoint gradient of a 400-step 128^3 scalar-wave solve: wall seconds and central-difference agreement",
+        "value": t_adj, "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3-D scalar wave u_t + a.grad u = 0, {n}^3, Dirichlet sphere r=0.3, E4u alpha (0, 0), "
+                               f"CFL {args.adj_cfl}, 400 RK4 steps, checkpoints every 20; J = 1/2 ||u(T) - u_obs||^2, "
+                               "u_obs from a seeded smooth initial field; gradient in u0 and the 3 x 5 interior "
+                               "coefficients"},
+        "forward_s": t_fwd, "gradient_s": t_adj, "gradient_over_forward": t_adj / t_fwd,
+        "J": J0, "fd_directions": 16, "fd_max_rel_err": max(errs), "fd_median_rel_err": float(np.median(errs)),
+        "fd_max_rel_err_u0": max(err_u), "fd_max_rel_err_coef": max(err_c), "cfl": args.adj_cfl,
+        "max_abs_u_T": float(np.max(np.abs(uT))),
+    }
+    sol.close()
+    prob.close()
+    ctx.close()
+    print(json.dumps(result))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -275,10 +383,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default fused-stage kernel shape (short profiling runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
-    ap.add_argument("--workload", default="wave256", choices=["wave256", "heat-batch64"],
+    ap.add_argument("--workload", default="wave256", choices=["wave256", "heat-batch64", "adjoint128"],
                     help="wave256: the headline 256^3 scalar wave; heat-batch64: the batched heat "
-                         "configuration (its own JSON line)")
+                         "configuration; adjoint128: the adjoint gradient of a 128^3 solve (each "
+                         "its own JSON line)")
     ap.add_argument("--n", type=int, default=N_SIDE)
+    ap.add_argument("--adj-cfl", type=float, default=0.05, help="CFL of the adjoint workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box here has one "
@@ -287,6 +397,9 @@ def main():
 
     if args.workload == "heat-batch64":
         heat_batch64(args)
+        return
+    if args.workload == "adjoint128":
+        adjoint128(args)
         return
 
     import numpy as np
