@@ -106,6 +106,32 @@ typedef struct shoccs_run_result {
  * final solution.  Synchronous. */
 int shoccs_run(shoccs_problem* p, const shoccs_run_desc* d, shx_field* u, shoccs_run_result* r);
 
+/* ------------------------------------------------------ Lua configuration
+ * The reference's Lua `simulation` table (system::from_lua,
+ * src/systems/system.cpp:103-132, and the from_lua readers it calls), read by
+ * a Lua-subset evaluator (no Lua interpreter is linked): statements
+ * `[local] name = exp`; numbers, strings, booleans, tables, arithmetic,
+ * comparisons, `..`, and the math library; comments.  Function values are
+ * parsed but cannot be called, so manufactured_solution.type = "lua" is
+ * refused.  Keys read: mesh{index_extents, domain_bounds{min, max} | {...}},
+ * domain_boundaries{xmin..zmax}, shapes{{type = "sphere", center, radius,
+ * boundary_condition}}, scheme{order, type, alpha} (E2/E4 order 2, E4/E4u
+ * order 1), system{type = "heat" (diffusivity) | "scalar wave" (center,
+ * radius, max_error)}, manufactured_solution{type = "gaussian", {center,
+ * variance, amplitude, frequency}...}, integrator{type = "rk4" | "euler"},
+ * step_controller{max_step, max_time, min_dt, cfl{hyperbolic, parabolic}};
+ * io and logging are ignored.  Extensions of this library:
+ * system.velocity = {ax, ay, az} with system.initial_condition = {center,
+ * width, amplitude} (constant-velocity wave with a gaussian pulse),
+ * manufactured_solution.type = "sin_product", integrator.form = "fused".
+ * On failure err (errlen bytes) holds the message. */
+typedef struct shoccs_config shoccs_config;
+int shoccs_config_parse(const char* text, shoccs_config** out, char* err, int64_t errlen);
+/* the problem and run descriptors of a parsed configuration (the problem
+ * descriptor's arrays belong to the configuration) */
+int shoccs_config_desc(const shoccs_config* c, shoccs_problem_desc* desc, shoccs_run_desc* run);
+int shoccs_config_destroy(shoccs_config* c);
+
 /* device fields of the problem's shape: D then Rx, Ry, Rz */
 int shoccs_field_alloc(shoccs_problem* p, shx_field* f);
 int shoccs_field_free(shoccs_problem* p, shx_field* f);

@@ -139,6 +139,53 @@ class Field:
             pass
 
 
+class Config:
+    """A Lua `simulation` configuration (shoccs_config_parse): the problem
+    it describes and the driver settings of its integrator and
+    step_controller."""
+
+    def __init__(self, text: str):
+        L = lib()
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = L.shoccs_config_parse(text.encode(), C.byref(h), err, len(err))
+        if rc != 0:
+            raise ShoccsError(f"Lua configuration: {err.value.decode()}")
+        try:
+            d, r = ProblemDesc(), RunDesc()
+            check(L.shoccs_config_desc(h, C.byref(d), C.byref(r)), None, "config")
+            self.kind = d.kind
+            self.n = tuple(d.n[a] for a in range(3))
+            self.lo = tuple(d.min[a] for a in range(3))
+            self.hi = tuple(d.max[a] for a in range(3))
+            self.spheres = [[d.spheres[4 * i + c] for c in range(4)] for i in range(d.nshapes)]
+            self.shape_bc = [d.shape_bc[i] for i in range(d.nshapes)]
+            self.grid_bc = tuple(d.grid_bc[f] for f in range(6))
+            self.scheme = d.scheme
+            self.alpha = (d.alpha[0], d.alpha[1])
+            self.params = [d.params[i] for i in range(d.nparams)]
+            self.integrator = r.integrator
+            self.max_step = None if r.max_step == 2 ** 31 - 1 else r.max_step
+            self.max_time = None if r.max_time >= np.finfo(np.float64).max else r.max_time
+            self.h_cfl, self.p_cfl, self.min_dt = r.h_cfl, r.p_cfl, r.min_dt
+        finally:
+            L.shoccs_config_destroy(h)
+
+    @classmethod
+    def from_file(cls, path: str) -> "Config":
+        with open(path) as f:
+            return cls(f.read())
+
+    def problem(self, ctx: "Context | None") -> "Problem":
+        return Problem(ctx, self.kind, self.n, self.lo, self.hi, spheres=self.spheres,
+                       shape_bc=self.shape_bc, grid_bc=self.grid_bc, scheme=self.scheme,
+                       alpha=self.alpha, params=self.params)
+
+    def run_kwargs(self) -> dict:
+        return dict(integrator=self.integrator, max_step=self.max_step, max_time=self.max_time,
+                    h_cfl=self.h_cfl, p_cfl=self.p_cfl, min_dt=self.min_dt)
+
+
 class Problem:
     """Mesh, boundary conditions, stencil and system on one device.
 
@@ -401,7 +448,13 @@ class Problem:
         d.h_cfl, d.p_cfl, d.min_dt = h_cfl, p_cfl, min_dt
         cap = 0
         if keep_stats:
-            cap = (max_step if max_step is not None else 100000) + 1
+            if max_step is None and max_time is not None:
+                # steps the controller takes to reach max_time (plus slack)
+                dt = self.dt(h_cfl, p_cfl)
+                cap = int(min(max_time / dt, 10 ** 7)) + 2 if dt > 0 else 100000
+            else:
+                cap = (max_step if max_step is not None else 100000)
+            cap += 1
         buf = np.zeros((max(cap, 1), 11))
         d.stats = buf.ctypes.data_as(C.POINTER(C.c_double)) if cap else None
         d.stats_cap = cap

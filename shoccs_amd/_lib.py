@@ -164,6 +164,9 @@ _SIGS = [
     ("shx_time_batch", C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, C.POINTER(C.c_int64)]),
     ("shoccs_run", C.c_int, [C.c_void_p, C.POINTER(RunDesc), C.POINTER(ShxField),
                              C.POINTER(RunResult)]),
+    ("shoccs_config_parse", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_int64]),
+    ("shoccs_config_desc", C.c_int, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(RunDesc)]),
+    ("shoccs_config_destroy", C.c_int, [C.c_void_p]),
     ("shx_derivative_apply_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
                                          C.c_int]),
     ("shx_system_rhs_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField)]),
