@@ -293,3 +293,35 @@ int orc_dirichlet_line(const orc_mesh* m, const int start[3], int dir, const int
     }
     return result;
 }
+
+/* mesh::interp_line (mesh.cpp:170-207): start from the grid ends of the line
+ * through pt along dir and move them to the cut points of R(dir) on that line
+ * that face pt (normal component >= 0 on the left, < 0 on the right). */
+long orc_interp_line(const orc_mesh* m, int dir, const int pt[3], orc_bound* left, orc_bound* right)
+{
+    const int f = dir == 2 ? 1 : 2, s = dir == 0 ? 1 : 0; /* index::dirs (indexing.hpp:21-37) */
+    memset(left, 0, sizeof *left);
+    memset(right, 0, sizeof *right);
+    for (int c = 0; c < 3; c++) left->mesh[c] = right->mesh[c] = pt[c];
+    left->mesh[dir] = 0;
+    right->mesh[dir] = m->n[dir] - 1;
+    for (long i = 0; i < m->nR[dir]; i++) {
+        const orc_rinfo* r = &m->R[dir][i];
+        if (r->solid[s] > pt[s]) break;
+        if (r->solid[s] != pt[s] || r->solid[f] != pt[f]) continue;
+        orc_bound* b = NULL;
+        if (r->normal[dir] >= 0.0 && r->solid[dir] <= pt[dir]) {
+            if (r->solid[dir] >= left->mesh[dir]) b = left;
+        } else if (r->normal[dir] < 0.0 && r->solid[dir] >= pt[dir]) {
+            if (r->solid[dir] <= right->mesh[dir]) b = right;
+        }
+        if (b) {
+            for (int c = 0; c < 3; c++) b->mesh[c] = r->solid[c];
+            b->has_obj = 1;
+            b->obj_coord = i;
+            b->obj_id = r->shape_id;
+            b->psi = r->psi;
+        }
+    }
+    return dir == 0 ? (long)m->n[1] * m->n[2] : dir == 1 ? (long)m->n[2] : 1;
+}

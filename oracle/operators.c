@@ -17,6 +17,7 @@
  */
 #include "oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -104,6 +105,155 @@ static void push_block(orc_deriv* d, long* cap, const orc_iblock* b)
     d->blocks[d->nblocks++] = *b;
 }
 
+
+/* ---------------------------------------------------------------- cut rows */
+
+/* stencil::interp (stencil.hpp:213-270): the interpolant along dir at
+ * closest + y between the line bounds; writes the bounds of the points it
+ * spans to ol/or and returns its length */
+static int stencil_interp(const orc_stencil* st, int dir, const int closest[3], double y,
+                          const orc_bound* left, const orc_bound* right, double* c, int cap,
+                          orc_bound* ol, orc_bound* orr)
+{
+    const orc_iinfo q = orc_query_interp(st);
+    const int p = q.p, t = q.t;
+    const int ic = closest[dir], lc = left->mesh[dir], rc = right->mesh[dir];
+    int st1 = (p & 1) ? (p - 1) / 2 : p / 2;
+    int st2 = (p - 1) / 2;
+    if (y > 0) {
+        const int tmp = st1;
+        st1 = st2;
+        st2 = tmp;
+    }
+    if ((ic - lc > st1 || (ic - lc == st1 && !left->has_obj)) &&
+        (rc - ic > st2 || (rc - ic == st2 && !right->has_obj))) {
+        memset(ol, 0, sizeof *ol);
+        memset(orr, 0, sizeof *orr);
+        for (int k = 0; k < 3; k++) ol->mesh[k] = orr->mesh[k] = closest[k];
+        ol->mesh[dir] -= st1;
+        orr->mesh[dir] += st2;
+        return orc_interp_interior(st, y, c, cap);
+    }
+    if (ic - lc < rc - ic) {
+        *ol = *left;
+        memset(orr, 0, sizeof *orr);
+        for (int k = 0; k < 3; k++) orr->mesh[k] = left->mesh[k];
+        orr->mesh[dir] += t - 1;
+        const double psi = left->has_obj ? left->psi : 1.0;
+        if (left->has_obj && lc == ic) y += psi - 1;
+        return orc_interp_wall(st, ic - lc, y, psi, c, 0, cap);
+    }
+    memset(ol, 0, sizeof *ol);
+    for (int k = 0; k < 3; k++) ol->mesh[k] = right->mesh[k];
+    ol->mesh[dir] -= t - 1;
+    *orr = *right;
+    const double psi = right->has_obj ? right->psi : 1.0;
+    if (right->has_obj && rc == ic) y += 1 - psi;
+    return orc_interp_wall(st, rc - ic, y, psi, c, 1, cap);
+}
+
+/* interp_deriv_coefficients (derivative.cpp:23-67): the derivative row along
+ * dir used at a cut point of R(r), r != dir, and the offset of its first
+ * point from the cut point's solid coordinate along dir */
+static const double* interp_deriv_coeffs(int dir, int n, double h, const orc_rinfo* obj, int bc,
+                                         const orc_stencil* st, double* c, double* ex, int* count,
+                                         int* offset)
+{
+    const orc_info q = orc_query(st, bc);
+    if (fabs(obj->normal[dir]) > 1e-12) {
+        const int rw = obj->normal[dir] < 0.0;
+        orc_nbs(st, h, bc, 1.0, rw, c, ex);
+        *count = q.t;
+        *offset = rw ? 1 - q.t : 0;
+        return rw ? c + (q.r - 1) * q.t : c;
+    }
+    const int right_dist = n - obj->solid[dir], left_dist = obj->solid[dir];
+    if (right_dist >= q.p && left_dist >= q.p) {
+        orc_interior(st, h, c);
+        *count = 2 * q.p + 1;
+        *offset = -q.p;
+        return c;
+    }
+    const int rw = right_dist < left_dist;
+    orc_nbs(st, h, bc, 1.0, rw, c, ex);
+    *count = q.t;
+    if (rw) {
+        const int qq = 1 + right_dist;
+        *offset = qq - q.t;
+        return c + (q.r - qq) * q.t;
+    }
+    *offset = -left_dist;
+    return c + left_dist * q.t;
+}
+
+/* cut_discretization (derivative.cpp:159-257) with OB_builder (:69-157):
+ * rows of R(r) for the derivative along dir, columns D (O) and R(r) (B) */
+static void cut_discretization(int r, int dir, const orc_mesh* m, const orc_stencil* st,
+                               const int* obj_bc, orc_csr* O, orc_csr* B)
+{
+    const long sz = m->nR[r];
+    tbuilder Ob = {0}, Bb = {0};
+    int all_dirichlet = 1;
+    for (int k = 0; k < m->nshapes; k++) all_dirichlet &= obj_bc[k] == ORC_DIRICHLET;
+    if (sz > 0 && !all_dirichlet) {
+        const orc_info mx = orc_query_max(st);
+        const double h = m->h[dir];
+        double* c = (double*)calloc((size_t)(mx.r * mx.t + 16), sizeof(double));
+        double* ic = (double*)calloc((size_t)(mx.t + 16), sizeof(double));
+        double extra[16];
+        const long stride = dir == 0 ? (long)m->n[1] * m->n[2] : dir == 1 ? (long)m->n[2] : 1;
+        for (long row = 0; row < sz; row++) {
+            const orc_rinfo* obj = &m->R[r][row];
+            const int bc_t = obj_bc[obj->shape_id];
+            if (bc_t == ORC_DIRICHLET) continue;
+            if (dir == r) {
+                /* the object's own closure row, from its solid coordinate */
+                const orc_info q = orc_query(st, bc_t);
+                orc_nbs(st, h, bc_t, obj->psi, obj->ray_outside, c, extra);
+                const long sic = ic3(m, obj->solid);
+                if (obj->ray_outside) {
+                    const double* sub = c + (q.r - 1) * q.t; /* last row, reversed */
+                    tb_add(&Bb, row, row, sub[q.t - 1]);
+                    for (int i = 1; i < q.t; i++) tb_add(&Ob, row, sic - i * stride, sub[q.t - 1 - i]);
+                } else {
+                    tb_add(&Bb, row, row, c[0]);
+                    for (int i = 1; i < q.t; i++) tb_add(&Ob, row, sic + i * stride, c[i]);
+                }
+                continue;
+            }
+            int count, off;
+            const double* cl = interp_deriv_coeffs(dir, m->n[dir] - 1, h, obj, bc_t, st, c, extra, &count, &off);
+            int cp[3] = {obj->solid[0], obj->solid[1], obj->solid[2]};
+            if (obj->psi <= 0.5) cp[r] += 1 - 2 * obj->ray_outside;
+            cp[dir] += off;
+            const double y = (obj->ray_outside ? 1.0 : -1.0) * (obj->psi <= 0.5 ? obj->psi : obj->psi - 1);
+            for (int k = 0; k < count; k++, ++cp[dir]) {
+                const double v = cl[k];
+                if (cp[dir] == obj->solid[dir]) {
+                    tb_add(&Bb, row, row, v); /* add_cut_point */
+                    continue;
+                }
+                orc_bound lb, rb, ol, orr;
+                const long rs = orc_interp_line(m, r, cp, &lb, &rb);
+                stencil_interp(st, r, cp, y, &lb, &rb, ic, mx.t, &ol, &orr);
+                /* add_interp_row */
+                const long lic = ic3(m, ol.mesh), ric = ic3(m, orr.mesh);
+                int e = 0;
+                if (ol.has_obj) tb_add(&Bb, row, ol.obj_coord, v * ic[e]);
+                else tb_add(&Ob, row, lic, v * ic[e]);
+                ++e;
+                for (long x = lic + rs; x < ric; x += rs) tb_add(&Ob, row, x, v * ic[e++]);
+                if (orr.has_obj) tb_add(&Bb, row, orr.obj_coord, v * ic[e]);
+                else tb_add(&Ob, row, ric, v * ic[e]);
+            }
+        }
+        free(c);
+        free(ic);
+    }
+    tb_to_csr(&Ob, sz, O);
+    tb_to_csr(&Bb, sz, B);
+}
+
 orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
                             const int grid_bc[6], const int* obj_bc)
 {
@@ -115,15 +265,14 @@ orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
         d->B.rows = d->N;
         d->Nm.u = (long*)calloc((size_t)d->N + 1, sizeof(long));
         d->Nm.rows = d->N;
+        for (int r = 0; r < 3; r++) {
+            d->Bf[r].u = (long*)calloc((size_t)m->nR[r] + 1, sizeof(long));
+            d->Bf[r].rows = m->nR[r];
+            d->Br[r].u = (long*)calloc((size_t)m->nR[r] + 1, sizeof(long));
+            d->Br[r].rows = m->nR[r];
+        }
         return d;
     }
-    /* cut_discretization (derivative.cpp:159-257) is not restated: only bodies with
-     * Dirichlet conditions are accepted, for which it exits early (:173-175). */
-    for (int s = 0; s < m->nshapes; s++)
-        if (obj_bc[s] != ORC_DIRICHLET && (m->nR[0] || m->nR[1] || m->nR[2])) {
-            free(d);
-            return NULL;
-        }
 
     orc_info mx = orc_query_max(st);
     const double h = m->h[dir];
@@ -251,6 +400,8 @@ orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
     tb_to_csr(&Nb, d->N, &d->Nm);
     free(left);
     free(right);
+    /* derivative.cpp:470-472 */
+    for (int r = 0; r < 3; r++) cut_discretization(r, dir, m, st, obj_bc, &d->Bf[r], &d->Br[r]);
     return d;
 }
 
@@ -268,6 +419,14 @@ void orc_deriv_destroy(orc_deriv* d)
     free(d->Nm.u);
     free(d->Nm.v);
     free(d->Nm.w);
+    for (int r = 0; r < 3; r++) {
+        free(d->Bf[r].u);
+        free(d->Bf[r].v);
+        free(d->Bf[r].w);
+        free(d->Br[r].u);
+        free(d->Br[r].v);
+        free(d->Br[r].w);
+    }
     free(d);
 }
 
@@ -314,10 +473,30 @@ void orc_csr_apply(const orc_csr* A, const double* x, double* b)
         for (long i = A->u[row]; i < A->u[row + 1]; i++) b[row] += A->w[i] * x[A->v[i]];
 }
 
-/* derivative.cpp:475-519 (R-space operators are empty for Dirichlet bodies) */
+/* an R-space CSR apply: as csr::operator(), with column indices clamped to
+ * [0, ncols) (a degenerate interpolation line of a scheme without
+ * interpolants can name a point past the mesh; its weight is zero) */
+static void csr_apply_clamped(const orc_csr* A, const double* x, long ncols, double* b)
+{
+    const long nr = A->rows;
+#pragma omp parallel for schedule(static)
+    for (long row = 0; row < nr; row++)
+        for (long i = A->u[row]; i < A->u[row + 1]; i++) {
+            long c = A->v[i];
+            c = c < 0 ? 0 : (c >= ncols ? ncols - 1 : c);
+            b[row] += A->w[i] * x[c];
+        }
+}
+
+/* derivative.cpp:475-519: R rows (always +=: Bf from D, then Br from R), then
+ * the D rows and their B terms */
 void orc_deriv_apply(const orc_deriv* d, const orc_field* u, const orc_field* nu,
                      orc_field* du, int plus_eq)
 {
+    for (int r = 0; r < 3; r++)
+        if (d->Bf[r].rows) csr_apply_clamped(&d->Bf[r], u->D, d->N, du->R[r]);
+    for (int r = 0; r < 3; r++)
+        if (d->Br[r].rows) csr_apply_clamped(&d->Br[r], u->R[r], d->Br[r].rows, du->R[r]);
     orc_block_apply(d, u->D, du->D, plus_eq);
     orc_csr_apply(&d->B, u->R[d->dir], du->D);
     if (nu) orc_csr_apply(&d->Nm, nu->D, du->D);

@@ -26,9 +26,10 @@
  * Arithmetic is compiled with -ffp-contract=off so that every sum and product
  * rounds exactly as the reference's x86-64 build does.
  *
- * Not restated (documented as gaps): cut_discretization for non-Dirichlet
- * bodies (src/operators/derivative.cpp:159-257); such configurations are
- * rejected with an error code.
+ *   cut rows   src/operators/derivative.cpp:17-257 (interp_deriv_coefficients,
+ *              OB_builder, cut_discretization), src/mesh/mesh.cpp:170-207
+ *              (interp_line), src/stencils/stencil.hpp:213-270 (interp),
+ *              E2_2.cpp:31-86, E4_2.cpp:31-183 (interpolants)
  */
 #ifndef SHOCCS_ORACLE_H
 #define SHOCCS_ORACLE_H
@@ -44,6 +45,7 @@ enum { ORC_DIRICHLET = 0, ORC_FLOATING = 1, ORC_NEUMANN = 2 };
 enum { ORC_E2_2 = 0, ORC_E4_2 = 1, ORC_E4_1 = 2, ORC_E4U_1 = 3 };
 
 typedef struct { int p, r, t, x; } orc_info;
+typedef struct { int p, t; } orc_iinfo; /* stencil.hpp:29 interp_info */
 typedef struct { int kind; double alpha[2]; } orc_stencil;
 
 /* src/mesh/mesh_types.hpp:30-38 */
@@ -112,6 +114,9 @@ typedef struct {
     orc_iblock* blocks;
     long nblocks;
     orc_csr B, Nm;
+    /* R-space rows (cut_discretization, derivative.cpp:159-257): rows R(r),
+     * columns D (Bf) and R(r) (Br); empty unless some body is not Dirichlet */
+    orc_csr Bf[3], Br[3];
 } orc_deriv;
 
 /* a scalar field: D plus the three cut-point buffers, src/fields/scalar.hpp */
@@ -163,6 +168,14 @@ orc_info orc_query_max(const orc_stencil* st);
 void orc_interior(const orc_stencil* st, double h, double* c);
 void orc_nbs(const orc_stencil* st, double h, int bc, double psi, int right, double* c,
              double* extra);
+/* interpolants: coefficients written to c, count returned (cap: the length of
+ * c, returned unchanged by schemes without interpolants) */
+orc_iinfo orc_query_interp(const orc_stencil* st);
+int orc_interp_interior(const orc_stencil* st, double y, double* c, int cap);
+int orc_interp_wall(const orc_stencil* st, int i, double y, double psi, double* c, int right, int cap);
+/* mesh::interp_line (mesh.cpp:170-207): bounds of the line through pt along
+ * dir; returns the stride */
+long orc_interp_line(const orc_mesh* m, int dir, const int pt[3], orc_bound* left, orc_bound* right);
 
 /* returns NULL on unsupported configuration */
 orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
@@ -173,6 +186,10 @@ void orc_csr_apply(const orc_csr* A, const double* x, double* b);
 void orc_deriv_apply(const orc_deriv* d, const orc_field* u, const orc_field* nu,
                      orc_field* du, int plus_eq);
 
+/* Non-Dirichlet bodies are accepted for the operators (derivative,
+ * gradient, laplacian, with their cut-point rows); the system functions
+ * (rhs, boundary update, stats, steps) restate the reference for Dirichlet
+ * bodies only. */
 orc_system* orc_system_create(int kind, const int n[3], const double min[3],
                               const double max[3], int nshapes, const double* spheres,
                               const int* obj_bc, const int grid_bc[6],

@@ -109,6 +109,8 @@ class Deriv(C.Structure):
         ("nblocks", C.c_long),
         ("B", Csr),
         ("Nm", Csr),
+        ("Bf", Csr * 3),
+        ("Br", Csr * 3),
     ]
 
 

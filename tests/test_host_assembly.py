@@ -61,12 +61,3 @@ def test_rejects_closures_outside_the_mesh():
     cfg = H.sphere_wave_cfg(24)
     with pytest.raises(ValueError):
         H.oracle_system(cfg)
-
-
-def test_unsupported_bodies_rejected():
-    """Floating bodies need the R-space rows of cut_discretization, which the
-    device path does not implement: the oracle refuses the configuration."""
-    cfg = dict(H.sphere_wave_cfg(32, scheme=O.E4U_1, alpha=(0.1, 0.2)))
-    cfg["obj_bc"] = [O.FLOATING]
-    with pytest.raises(ValueError):
-        H.oracle_system(cfg)

@@ -2,6 +2,7 @@
 extracted from the reference's unit tests) and what the mathematics fixes
 (polynomial exactness of interior stencils, closures and whole operators,
 measured convergence orders, RK4 on a solution linear in time)."""
+import ctypes
 import json
 import math
 import os
@@ -425,3 +426,123 @@ def test_euler_step_is_the_first_order_taylor_polynomial():
     out, srhs = s.field(), s.field()
     s.euler_step(u0, out, srhs, 0.0, dt)
     assert np.array_equal(out.D, want)
+
+
+# ------------------------------------------------- cut rows of non-Dirichlet bodies
+def _floating_system(n, lo, hi, sphere, grid_bc, scheme, bc=None):
+    st = O.stencil(scheme)
+    return O.OracleSystem(O.SYS_HEAT, n, lo, hi, [sphere], [O.FLOATING if bc is None else bc], grid_bc, st,
+                          [1.0, O.MMS_GAUSS, 0])
+
+
+def _eval(osys, f):
+    m = osys.mesh
+    u = osys.field()
+    X, Y, Z = np.meshgrid(*[np.asarray(m.m.xyz[d][:m.m.n[d]]) for d in range(3)], indexing="ij")
+    u.D[:] = f(X.ravel(), Y.ravel(), Z.ravel())
+    for d in range(3):
+        if m.nR(d):
+            p = m.r_arrays(d)["pos"]
+            u.R[d][:] = f(p[:, 0], p[:, 1], p[:, 2])
+    return u
+
+
+def _f2(x, y, z):  # derivative.t.cpp:22-26: linear in each variable
+    return x * (y + z) + y * (x + z) + z * (x + y) + 3 * x * y * z
+
+
+@pytest.mark.parametrize("grid", ["nn_dd_ff", "ff_nn_dd"])
+def test_cut_rows_e2_second_derivative_of_f2(grid):
+    """derivative.t.cpp:515-568 ("E2 with Objects"): on the 25 x 26 x 27 mesh
+    with a floating sphere, the E2 second derivative of f2 (linear in each
+    variable) vanishes at the fluid points and at every cut point, the latter
+    through the R-space rows (same-direction closures and cross-direction
+    interpolation rows).  Only floating bodies are exact here: the reference
+    gives a Neumann body its closure rows but never adds their boundary-data
+    term (derivative.cpp:335-365 builds no N entries for objects)."""
+    gb = {"nn_dd_ff": [O.NEUMANN, O.NEUMANN, O.DIRICHLET, O.DIRICHLET, O.FLOATING, O.FLOATING],
+          "ff_nn_dd": [O.FLOATING, O.FLOATING, O.NEUMANN, O.NEUMANN, O.DIRICHLET, O.DIRICHLET]}[grid]
+    dd_nu = {"nn_dd_ff": 0, "ff_nn_dd": 1}[grid]
+    osys = _floating_system((25, 26, 27), (0.1, 0.2, 0.3), (1, 2, 2.2), [0.45, 1.011, 1.31, 0.25], gb, O.E2_2)
+    m = osys.mesh
+    assert all(m.nR(d) > 0 for d in range(3))
+    u = _eval(osys, _f2)
+    nu = _eval(osys, [lambda x, y, z: 2.0 * (y + z) + 3.0 * y * z,  # f2_dx, f2_dy
+                      lambda x, y, z: 2.0 * (x + z) + 3.0 * x * z][dd_nu])
+    scale = np.max(np.abs(u.flat())) / min(m.h()) ** 2
+    fluid = np.zeros(m.N, bool)
+    fluid[m.fluid()] = True
+    for d in range(3):
+        du = osys.field()
+        lib = O.lib()
+        lib.orc_deriv_apply(osys.deriv(d), u.ref(), nu.ref() if d == dd_nu else None, du.ref(), 0)
+        assert np.max(np.abs(du.D[fluid])) <= 1e-10 * scale, f"dir {d} fluid"
+        dR = osys.deriv(d).contents
+        for r in range(3):
+            if m.nR(r):
+                assert dR.Bf[r].nnz > 0 and dR.Bf[r].rows == m.nR(r)
+                assert np.max(np.abs(du.R[r])) <= 1e-10 * scale, f"dir {d} R{r}"
+
+
+def test_cut_rows_e4_second_derivative_exact_on_cubics():
+    """E4 second derivatives with a floating sphere are exact on a cubic at
+    the fluid points and at the cut points (closures exact to degree 3,
+    5-point interpolants exact to degree 4)."""
+    osys = _floating_system((28, 30, 26), (0, 0, 0), (1, 1.1, 0.95), [0.52, 0.49, 0.47, 0.27],
+                            [O.FLOATING] * 6, O.E4_2)
+    m = osys.mesh
+    f = lambda x, y, z: x ** 3 - 2 * x * y * y + y ** 3 + 0.5 * z ** 3 + x * y * z + z * z  # noqa: E731
+    dd = [lambda x, y, z: 6 * x + 0 * y,
+          lambda x, y, z: -4 * x + 6 * y,
+          lambda x, y, z: 3 * z + 2 + 0 * x]
+    u = _eval(osys, f)
+    scale = np.max(np.abs(u.flat())) / min(m.h()) ** 2
+    fluid = np.zeros(m.N, bool)
+    fluid[m.fluid()] = True
+    for d in range(3):
+        du = osys.field()
+        O.lib().orc_deriv_apply(osys.deriv(d), u.ref(), None, du.ref(), 0)
+        want = _eval(osys, dd[d])
+        assert np.max(np.abs(du.D[fluid] - want.D[fluid])) <= 1e-9 * scale, f"dir {d} fluid"
+        for r in range(3):
+            if m.nR(r):
+                err = np.max(np.abs(du.R[r] - want.R[r]))
+                assert err <= 1e-9 * scale, f"dir {d} R{r}: {err / scale:.2e}"
+
+
+@pytest.mark.parametrize("kind,deg_wall,deg_int", [(O.E2_2, 1, 1), (O.E4_2, 3, 4)])
+def test_interpolants_reproduce_polynomials(kind, deg_wall, deg_int):
+    """E2_2.cpp:31-86 and E4_2.cpp:31-183 interpolants.  Interior: nodes
+    -p/2..p/2 (E2: the two points either side of y), exact to degree p-1.
+    Wall (left): nodes 1-psi (the boundary), 1, 2, .. t-1 relative to the
+    boundary point, evaluated at y + i (y + 1 - psi for i = 0, the shift
+    stencil.hpp:213-270 applies when the closest point is the solid one),
+    exact to degree 1 (E2) or 3 (E4); the right wall mirrors the nodes and
+    the offset (y - i)."""
+    L = O.lib()
+    L.orc_interp_wall.restype = ctypes.c_int
+    L.orc_interp_interior.restype = ctypes.c_int
+    st = O.stencil(kind)
+    t = 3 if kind == O.E2_2 else 5
+    rng = np.random.default_rng(11)
+    c = (ctypes.c_double * 8)()
+    for _ in range(40):
+        y = rng.uniform(-0.95, 0.95)
+        n = L.orc_interp_interior(ctypes.byref(st), ctypes.c_double(y), c, 8)
+        x = (np.array([0.0, 1.0]) if y > 0 else np.array([-1.0, 0.0])) if kind == O.E2_2 else np.arange(-2.0, 3.0)
+        w = np.array(c[:n])
+        for m in range(deg_int + 1):
+            assert abs(w @ x ** m - y ** m) < 1e-13, (y, m)
+        for right in (0, 1):
+            for i in range(t - 1 if kind == O.E2_2 else 3):
+                psi = rng.uniform(0.05, 1.0)
+                n = L.orc_interp_wall(ctypes.byref(st), i, ctypes.c_double(y), ctypes.c_double(psi), c, right, 8)
+                assert n == t
+                w = np.array(c[:n])
+                x = np.concatenate([[1.0 - psi], np.arange(1.0, t)])
+                off = i if i else 1.0 - psi
+                e = y - off if right else y + off
+                if right:
+                    x = -x[::-1]
+                for m in range(deg_wall + 1):
+                    assert abs(w @ x ** m - e ** m) < 1e-12 * max(1.0, abs(e) ** m), (right, i, psi, y, m)
