@@ -48,6 +48,10 @@ typedef struct shoccs_problem_desc {
     int32_t nparams;
 } shoccs_problem_desc;
 
+/* ctx == NULL builds the host structures only.  The system (heat or scalar
+ * wave) is built for Dirichlet bodies on grids without Neumann faces; other
+ * problems hold their derivative operators (with the cut-point rows of
+ * floating and Neumann bodies) and shoccs_problem_system returns NULL. */
 int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_problem** out);
 int shoccs_problem_destroy(shoccs_problem* p);
 shx_system* shoccs_problem_system(shoccs_problem* p);
@@ -70,6 +74,13 @@ int shoccs_deriv_blocks(const shoccs_problem* p, int dir, shx_line_block* out);
 int64_t shoccs_deriv_npool(const shoccs_problem* p, int dir);
 int shoccs_deriv_pool(const shoccs_problem* p, int dir, double* out);
 int shoccs_deriv_interior(const shoccs_problem* p, int dir, double* out9);
+/* The R-space rows of non-Dirichlet bodies (cut_discretization,
+ * src/operators/derivative.cpp:159-257) in the layout of
+ * shx_derivative_desc: ncut returns the row count and sets *nnz; cut copies
+ * rset[rows], rrow[rows], rptr[rows + 1], rcol[nnz], rw[nnz]. */
+int64_t shoccs_deriv_ncut(const shoccs_problem* p, int dir, int64_t* nnz);
+int shoccs_deriv_cut(const shoccs_problem* p, int dir, int32_t* rset, int64_t* rrow, int64_t* rptr,
+                     int64_t* rcol, double* rw);
 
 /* ---------------------------------------------------------------- driver */
 /* integrators of shoccs_run */

@@ -91,6 +91,17 @@ typedef struct shx_derivative_desc {
     const int64_t* neu_col;
     const double* neu_w;
     int64_t nneu;
+    /* R-space rows of non-Dirichlet bodies (derivative.cpp:159-257): row k is
+     * row rrow[k] of R(rset[k]) with entries rptr[k]..rptr[k+1] (rptr has
+     * nrrow + 1 values); an entry's column rcol >= 0 names a point of D,
+     * rcol = -1 - c point c of R(rset[k]).  Within a row the D entries come
+     * first; each row is summed in its stored order. */
+    int64_t nrrow;
+    const int32_t* rset;
+    const int64_t* rrow;
+    const int64_t* rptr;
+    const int64_t* rcol;
+    const double* rw;
 } shx_derivative_desc;
 
 /* ---------------------------------------------------------------- context */
@@ -122,9 +133,10 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc,
 int shx_derivative_destroy(shx_derivative* d);
 
 /* ccs::derivative::operator()(u, du, op) and (u, nu, du, op)
- * (src/operators/derivative.cpp:502-519): du.D op= O u.D, du.D += B u.R[dir]
- * (+= N nu.D when nu is given).  accumulate = 0 is eq_t, 1 is plus_eq_t.
- * R-space rows of du are untouched (bodies are Dirichlet).  Synchronous. */
+ * (src/operators/derivative.cpp:475-519): du.R[r] += the cut-point rows of
+ * non-Dirichlet bodies over u.D and u.R[r] (always added, whatever the op),
+ * du.D op= O u.D, du.D += B u.R[dir] (+= N nu.D when nu is given).
+ * accumulate = 0 is eq_t, 1 is plus_eq_t.  Synchronous. */
 int shx_derivative_apply(shx_derivative* d, const shx_field* u, const shx_field* nu,
                          shx_field* du, int accumulate);
 
@@ -141,12 +153,13 @@ int shx_csr_destroy(shx_csr* A);
 int shx_csr_apply(shx_csr* A, const double* x, double* b);
 
 /* ccs::gradient::operator() (src/operators/gradient.cpp:34-46): zero the
- * three outputs, then dx, dy, dz with eq_t on the directions with extent > 1.
- * d[i] may be NULL for a direction of extent 1. */
+ * three outputs (R parts included), then dx, dy, dz with eq_t on the
+ * directions with extent > 1.  d[i] may be NULL for a direction of extent 1. */
 int shx_gradient_apply(shx_derivative* const d[3], const shx_field* u, shx_field* const du[3]);
 
-/* ccs::laplacian::operator() (src/operators/laplacian.cpp:37-58): zero du,
- * then dx, dy, dz with plus_eq_t (and N nu when nu is given). */
+/* ccs::laplacian::operator() (src/operators/laplacian.cpp:37-58): zero du
+ * (R parts included), then dx, dy, dz with plus_eq_t (and N nu when nu is
+ * given); the cut-point rows of the three directions add up in du.R. */
 int shx_laplacian_apply(shx_derivative* const d[3], const shx_field* u, const shx_field* nu,
                         shx_field* du);
 
@@ -288,7 +301,8 @@ int shx_batch_times(shx_batch* b, double* t);
  * ubar.D = O^T dbar.D and ubar.R[dir] = B^T dbar.D (accumulate = 0), or adds
  * them (1).  Columns whose rows are all circulant rows take the reversed
  * stencil; every other column and every cut point sums its entries with rows
- * ascending.  Built on the first call (host, once per operator). */
+ * ascending.  Built on the first call (host, once per operator).  Operators
+ * with cut-point rows (non-Dirichlet bodies) return SHX_EUNSUPPORTED. */
 int shx_derivative_apply_T(shx_derivative* d, const shx_field* dbar, shx_field* ubar, int accumulate);
 
 /* wbar = (d rhs / d u)^T kbar for the system's right-hand side (rhs rows on

@@ -225,7 +225,7 @@ class Problem:
         self.N = L.shoccs_problem_size(p, 0)
         self.nR = tuple(L.shoccs_problem_size(p, a + 1) for a in range(3))
         self.flat_size = self.N + sum(self.nR)
-        self.sys = L.shoccs_problem_system(p)
+        self._sys = L.shoccs_problem_system(p)
         self._fields = weakref.WeakSet()
         if ctx is not None:
             ctx._problems.add(self)
@@ -291,6 +291,24 @@ class Problem:
         check(lib().shoccs_deriv_pool(self.p, d, a.ctypes.data_as(C.POINTER(C.c_double))), None, "pool")
         return a[:n]
 
+    def cut_rows(self, d: int) -> dict:
+        """Direction d's R-space rows (cut points of non-Dirichlet bodies):
+        rset, rrow, rptr, rcol (>= 0: a point of D; -1 - c: point c of
+        R(rset)), rw."""
+        L = lib()
+        nnz = C.c_int64()
+        n = L.shoccs_deriv_ncut(self.p, d, C.byref(nnz))
+        out = dict(rset=np.zeros(max(n, 1), np.int32), rrow=np.zeros(max(n, 1), np.int64),
+                   rptr=np.zeros(n + 1, np.int64), rcol=np.zeros(max(nnz.value, 1), np.int64),
+                   rw=np.zeros(max(nnz.value, 1)))
+        args = [out[k].ctypes.data_as(C.POINTER(C.c_int32 if k == "rset" else
+                                                C.c_double if k == "rw" else C.c_int64))
+                for k in ("rset", "rrow", "rptr", "rcol", "rw")]
+        check(L.shoccs_deriv_cut(self.p, d, *args), None, "cut rows")
+        out["rset"], out["rrow"] = out["rset"][:n], out["rrow"][:n]
+        out["rcol"], out["rw"] = out["rcol"][:nnz.value], out["rw"][:nnz.value]
+        return out
+
     def interior(self, d: int) -> np.ndarray:
         a = np.zeros(9)
         w = lib().shoccs_deriv_interior(self.p, d, a.ctypes.data_as(C.POINTER(C.c_double)))
@@ -300,8 +318,18 @@ class Problem:
         return lib().shoccs_problem_dt(self.p, h_cfl, p_cfl)
 
     # ---------------------------------------------------------- compute
+    @property
+    def sys(self):
+        """The device system (shx_system*): built for Dirichlet bodies on grids
+        without Neumann faces; other problems have operators only."""
+        self._need_device()
+        if not self._sys:
+            raise ShoccsError("system: only Dirichlet bodies and non-Neumann grid faces are "
+                              "supported (this problem has its derivative operators only)")
+        return self._sys
+
     def _need_device(self):
-        if self.ctx is None or not self.sys:
+        if self.ctx is None:
             raise ShoccsError("problem was built without a device context")
 
     def field(self) -> Field:
@@ -446,6 +474,7 @@ class Problem:
         d.max_step = 2 ** 31 - 1 if max_step is None else max_step
         d.max_time = float(np.finfo(np.float64).max) if max_time is None else max_time
         d.h_cfl, d.p_cfl, d.min_dt = h_cfl, p_cfl, min_dt
+        self.sys  # raises for a problem without a system
         cap = 0
         if keep_stats:
             if max_step is None and max_time is not None:

@@ -148,6 +148,9 @@ _SIGS = [
     ("shoccs_deriv_npool", C.c_int64, [C.c_void_p, C.c_int]),
     ("shoccs_deriv_pool", C.c_int, [C.c_void_p, C.c_int, dp]),
     ("shoccs_deriv_interior", C.c_int, [C.c_void_p, C.c_int, dp]),
+    ("shoccs_deriv_ncut", C.c_int64, [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]),
+    ("shoccs_deriv_cut", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64), dp]),
     ("shoccs_field_alloc", C.c_int, [C.c_void_p, C.POINTER(ShxField)]),
     ("shoccs_field_free", C.c_int, [C.c_void_p, C.POINTER(ShxField)]),
     ("shoccs_field_upload", C.c_int, [C.c_void_p, C.POINTER(ShxField), dp]),

@@ -564,6 +564,9 @@ int shx_derivative_apply_T(shx_derivative* d, const shx_field* dbar, shx_field* 
 {
     if (!d || !dbar || !ubar || !dbar->D || !ubar->D) return SHX_EINVAL;
     if (d->nR[d->dir] > 0 && !ubar->R[d->dir]) return SHX_EINVAL;
+    if (d->nrrow > 0)
+        return fail(d->ctx, SHX_EUNSUPPORTED,
+                    "derivative transpose: operators with cut-point rows (non-Dirichlet bodies) are not supported");
     if (!d->dev.present) {
         if (!accumulate) {
             const long N = (long)d->n[0] * d->n[1] * d->n[2];

@@ -63,6 +63,30 @@ __global__ void k_rows_csr(long nrows, const long* __restrict__ rows, const int*
     b[row] = acc;
 }
 
+// R-space rows of non-Dirichlet bodies (the Bf and Br products of
+// src/operators/derivative.cpp:479-485, always +=): one lane per row,
+// du.R[set][row] += the row's entries over u.D (column >= 0) and u.R[set]
+// (column -1 - c), summed in stored order.
+__global__ void k_cut_rows(long nrows, const int* __restrict__ rset, const long* __restrict__ rrow,
+                           const long* __restrict__ rptr, const long* __restrict__ rcol,
+                           const double* __restrict__ rw, const double* __restrict__ uD,
+                           const double* uR0, const double* uR1, const double* uR2, double* duR0,
+                           double* duR1, double* duR2)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nrows) return;
+    const int s = rset[k];
+    const double* uR = s == 0 ? uR0 : s == 1 ? uR1 : uR2;
+    double* duR = s == 0 ? duR0 : s == 1 ? duR1 : duR2;
+    const long row = rrow[k];
+    double acc = duR[row];
+    for (long e = rptr[k]; e < rptr[k + 1]; ++e) {
+        const long c = rcol[e];
+        acc += rw[e] * (c >= 0 ? uD[c] : uR[-1 - c]);
+    }
+    duR[row] = acc;
+}
+
 // The reference's csr apply over every row pointer (src/matrices/csr.cpp:37-51).
 __global__ void k_csr_full(long nrows, const long* __restrict__ u, const long* __restrict__ v,
                            const double* __restrict__ w, const double* __restrict__ x,
@@ -154,6 +178,17 @@ int launch_neumann(shx_derivative* d, const double* nu, double* du)
                                              d->neu_w, nu, du);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "neumann kernel");
+}
+
+int launch_cut_rows(shx_derivative* d, const shx_field* u, const shx_field* du)
+{
+    if (d->nrrow == 0) return SHX_OK;
+    shx_ctx* ctx = d->ctx;
+    const unsigned grid = (unsigned)((d->nrrow + 255) / 256);
+    k_cut_rows<<<grid, 256, 0, ctx->stream>>>(d->nrrow, d->rset, d->rrow, d->rptr, d->rcol, d->rw, u->D,
+                                             u->R[0], u->R[1], u->R[2], du->R[0], du->R[1], du->R[2]);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "cut-row kernel");
 }
 
 int launch_csr_full(shx_csr* A, const double* x, double* b)
@@ -438,6 +473,28 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
         if (rc == SHX_OK) rc = upload(ctx, &d->neu_col, col.data(), col.size());
         if (rc == SHX_OK) rc = upload(ctx, &d->neu_w, w.data(), w.size());
     }
+    // R-space rows: (set, row) strictly increasing, columns inside their buffer
+    if (rc == SHX_OK && desc->nrrow > 0) {
+        const long nr = desc->nrrow;
+        bool ok = desc->rset && desc->rrow && desc->rptr && desc->rcol && desc->rw && desc->rptr[0] == 0;
+        for (long k = 0; ok && k < nr; ++k) {
+            const int set = desc->rset[k];
+            ok = set >= 0 && set < 3 && desc->rrow[k] >= 0 && desc->rrow[k] < desc->nR[set] &&
+                 desc->rptr[k + 1] >= desc->rptr[k];
+            if (ok && k > 0)
+                ok = set > desc->rset[k - 1] || (set == desc->rset[k - 1] && desc->rrow[k] > desc->rrow[k - 1]);
+            for (long e = desc->rptr[k]; ok && e < desc->rptr[k + 1]; ++e)
+                ok = desc->rcol[e] < N && desc->rcol[e] >= -desc->nR[set];
+        }
+        if (!ok) rc = fail(ctx, SHX_EINVAL, "derivative: R-space rows out of range or unsorted");
+        const long nnz = ok ? desc->rptr[nr] : 0;
+        d->nrrow = ok ? nr : 0;
+        if (rc == SHX_OK) rc = upload(ctx, &d->rset, desc->rset, (size_t)nr);
+        if (rc == SHX_OK) rc = upload(ctx, &d->rrow, (const long*)desc->rrow, (size_t)nr);
+        if (rc == SHX_OK) rc = upload(ctx, &d->rptr, (const long*)desc->rptr, (size_t)nr + 1);
+        if (rc == SHX_OK && nnz) rc = upload(ctx, &d->rcol, (const long*)desc->rcol, (size_t)nnz);
+        if (rc == SHX_OK && nnz) rc = upload(ctx, &d->rw, desc->rw, (size_t)nnz);
+    }
     if (rc != SHX_OK) {
         shx_derivative_destroy(d);
         return rc;
@@ -468,6 +525,11 @@ int shx_derivative_destroy(shx_derivative* d)
     hipFree(d->neu_ptr);
     hipFree(d->neu_col);
     hipFree(d->neu_w);
+    hipFree(d->rset);
+    hipFree(d->rrow);
+    hipFree(d->rptr);
+    hipFree(d->rcol);
+    hipFree(d->rw);
     delete d;
     return SHX_OK;
 }
@@ -476,7 +538,8 @@ int shx_derivative_apply(shx_derivative* d, const shx_field* u, const shx_field*
                          int accumulate)
 {
     if (!d || !u || !du) return SHX_EINVAL;
-    int rc = launch_derivative(d, u->D, u->R[d->dir], du->D, accumulate != 0, true);
+    int rc = launch_cut_rows(d, u, du);
+    if (rc == SHX_OK) rc = launch_derivative(d, u->D, u->R[d->dir], du->D, accumulate != 0, true);
     if (rc == SHX_OK && nu) rc = launch_neumann(d, nu->D, du->D);
     if (rc == SHX_OK) rc = sync(d->ctx);
     return rc;
@@ -565,7 +628,10 @@ int shx_gradient_apply(shx_derivative* const d[3], const shx_field* u, shx_field
     int rc = SHX_OK;
     for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = zero_field(ctx, du[c], any->n, any->nR);
     for (int c = 0; c < 3 && rc == SHX_OK; ++c)
-        if (d[c] && d[c]->n[c] > 1) rc = launch_derivative(d[c], u->D, u->R[c], du[c]->D, false, true);
+        if (d[c] && d[c]->n[c] > 1) {
+            rc = launch_cut_rows(d[c], u, du[c]);
+            if (rc == SHX_OK) rc = launch_derivative(d[c], u->D, u->R[c], du[c]->D, false, true);
+        }
     if (rc == SHX_OK) rc = sync(ctx);
     return rc;
 }
@@ -579,7 +645,8 @@ int shx_laplacian_apply(shx_derivative* const d[3], const shx_field* u, const sh
     int rc = zero_field(ctx, du, any->n, any->nR);
     for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
         if (!d[c] || d[c]->n[c] < 2) continue;
-        rc = launch_derivative(d[c], u->D, u->R[c], du->D, true, true);
+        rc = launch_cut_rows(d[c], u, du);
+        if (rc == SHX_OK) rc = launch_derivative(d[c], u->D, u->R[c], du->D, true, true);
         if (rc == SHX_OK && nu) rc = launch_neumann(d[c], nu->D, du->D);
     }
     if (rc == SHX_OK) rc = sync(ctx);

@@ -33,6 +33,13 @@ struct shx_derivative {
     int* neu_ptr = nullptr;
     long* neu_col = nullptr;
     double* neu_w = nullptr;
+    // R-space rows of non-Dirichlet bodies (shx_derivative_desc.rrow ...)
+    long nrrow = 0;
+    int* rset = nullptr;
+    long* rrow = nullptr;
+    long* rptr = nullptr;
+    long* rcol = nullptr;
+    double* rw = nullptr;
     // host copies kept to build the reference's full-length B matrix and the
     // special-point rows of the x-marching form
     std::vector<shx_line_block> h_blocks;
@@ -59,6 +66,7 @@ int launch_fill(shx_ctx* ctx, double* dst, double v, long n);
 int launch_derivative(shx_derivative* d, const double* u, const double* uR, double* du, bool plus,
                       bool with_b);
 int launch_neumann(shx_derivative* d, const double* nu, double* du);
+int launch_cut_rows(shx_derivative* d, const shx_field* u, const shx_field* du);
 int launch_csr_full(shx_csr* A, const double* x, double* b);
 int zero_field(shx_ctx* ctx, const shx_field* f, const int n[3], const long nR[3]);
 int sync(shx_ctx* ctx);

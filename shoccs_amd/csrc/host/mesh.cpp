@@ -187,4 +187,28 @@ bool mesh::dirichlet_line(const int3& start, int dir, const bcs::Grid& grid) con
     return result;
 }
 
+line mesh::interp_line(int dir, const int3& pt) const
+{
+    line out{stride(dir), boundary{pt, std::nullopt}, boundary{pt, std::nullopt}};
+    out.start.mesh_coordinate[dir] = 0;
+    out.end.mesh_coordinate[dir] = n_[dir] - 1;
+    const int S = slow_axis(dir), F = fast_axis(dir);
+    // R(dir) is ordered by (slow, fast): stop once past pt's slow coordinate
+    const auto& cuts = R_[dir];
+    for (int64_t i = 0; i < (int64_t)cuts.size(); ++i) {
+        const mesh_object_info& o = cuts[i];
+        if (o.solid_coord[S] > pt[S]) break;
+        if (o.solid_coord[S] != pt[S] || o.solid_coord[F] != pt[F]) continue;
+        boundary* b = nullptr;
+        if (o.normal[dir] >= 0.0 && o.solid_coord[dir] <= pt[dir]) {
+            // >=: a solid point on the mesh end still bounds the line
+            if (o.solid_coord[dir] >= out.start.mesh_coordinate[dir]) b = &out.start;
+        } else if (o.normal[dir] < 0.0 && o.solid_coord[dir] >= pt[dir]) {
+            if (o.solid_coord[dir] <= out.end.mesh_coordinate[dir]) b = &out.end;
+        }
+        if (b) *b = boundary{o.solid_coord, object_boundary{i, o.shape_id, o.psi}};
+    }
+    return out;
+}
+
 }  // namespace ccs

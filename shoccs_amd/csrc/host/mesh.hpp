@@ -89,6 +89,10 @@ public:
     }
     // mesh::dirichlet_line (src/mesh/mesh.cpp:154-168)
     bool dirichlet_line(const int3& start, int dir, const bcs::Grid& grid) const;
+    // mesh::interp_line (src/mesh/mesh.cpp:170-207): the line along dir
+    // through pt, bounded by the mesh ends or by the nearest cut points of
+    // R(dir) on either side of pt
+    line interp_line(int dir, const int3& pt) const;
 
 private:
     int3 n_{1, 1, 1};

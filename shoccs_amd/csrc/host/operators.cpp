@@ -4,10 +4,13 @@
 //   submatrix_size / domain_discretization  src/operators/derivative.cpp:259-449
 //   inner_block placement of L | C | R      src/matrices/inner_block.cpp:17-40
 //   derivative ctor                         src/operators/derivative.cpp:452-473
+//   cut_discretization (R-space rows)       src/operators/derivative.cpp:17-257
 //   gradient / laplacian                    src/operators/gradient.cpp:34-46,
 //                                           src/operators/laplacian.cpp:37-58
 #include "operators.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -39,6 +42,139 @@ private:
     std::vector<double>& pool_;
     std::unordered_map<std::string, int32_t> index_;
 };
+
+struct cut_entry {
+    int64_t row;
+    int64_t col;
+    double w;
+};
+
+// stencil::interp (src/stencils/stencil.hpp:213-270): weights of the
+// interpolant at closest + y along dir within the line ln, and the two points
+// (or cut points) it spans.
+int interp_weights(const stencil& st, int dir, const int3& closest, double y, const line& ln,
+                   double* c, int cap, boundary& lo, boundary& hi)
+{
+    const stencils::interp_info q = st.query_interp();
+    const int ic = closest[dir];
+    const int lc = ln.start.mesh_coordinate[dir], rc = ln.end.mesh_coordinate[dir];
+    int before = (q.p & 1) ? (q.p - 1) / 2 : q.p / 2, after = (q.p - 1) / 2;
+    if (y > 0) std::swap(before, after);
+    const bool room_left = ic - lc > before || (ic - lc == before && !ln.start.object);
+    const bool room_right = rc - ic > after || (rc - ic == after && !ln.end.object);
+    if (room_left && room_right) {
+        lo = boundary{closest, std::nullopt};
+        hi = boundary{closest, std::nullopt};
+        lo.mesh_coordinate[dir] -= before;
+        hi.mesh_coordinate[dir] += after;
+        return st.interp_interior(y, c, cap);
+    }
+    if (ic - lc < rc - ic) {  // the left end is nearer
+        lo = ln.start;
+        hi = boundary{ln.start.mesh_coordinate, std::nullopt};
+        hi.mesh_coordinate[dir] += q.t - 1;
+        const double psi = ln.start.object ? ln.start.object->psi : 1.0;
+        if (ln.start.object && lc == ic) y += psi - 1;
+        return st.interp_wall(ic - lc, y, psi, c, false, cap);
+    }
+    lo = boundary{ln.end.mesh_coordinate, std::nullopt};
+    lo.mesh_coordinate[dir] -= q.t - 1;
+    hi = ln.end;
+    const double psi = ln.end.object ? ln.end.object->psi : 1.0;
+    if (ln.end.object && rc == ic) y += 1 - psi;
+    return st.interp_wall(rc - ic, y, psi, c, true, cap);
+}
+
+// cut_discretization (src/operators/derivative.cpp:159-257): the derivative
+// along dir at the cut points R(r) of non-Dirichlet bodies.  For r == dir the
+// body's own closure row; otherwise the derivative row along dir through the
+// cut point's neighbourhood (interp_deriv_coefficients, :22-67), each of its
+// points interpolated across direction r (OB_builder::add_interp_row,
+// :87-128).  Entries go to D (on_D) or to R(r) (on_R).
+void cut_rows(int r, int dir, const mesh& m, const stencil& st, const bcs::Object& obj,
+              std::vector<cut_entry>& on_D, std::vector<cut_entry>& on_R)
+{
+    const auto& cuts = m.R(r);
+    if (cuts.empty() ||
+        std::all_of(obj.begin(), obj.end(), [](bcs::type b) { return b == bcs::type::Dirichlet; }))
+        return;
+    const info mx = st.query_max();
+    const double h = m.h(dir);
+    const int64_t stride = m.stride(dir);
+    const int last = m.extents()[dir] - 1;
+    std::vector<double> c(mx.r * mx.t + 16, 0.0), wts(mx.t + 16, 0.0), extra(16, 0.0);
+    for (int64_t row = 0; row < (int64_t)cuts.size(); ++row) {
+        const mesh_object_info& o = cuts[row];
+        const bcs::type bc = obj[o.shape_id];
+        if (bc == bcs::type::Dirichlet) continue;
+        const info q = st.query(bc);
+        if (dir == r) {
+            // the closure row that touches the body: the first one, or the
+            // last one read backwards when the ray leaves the body
+            st.nbs(h, bc, o.psi, o.ray_outside, c.data(), extra.data());
+            const double* cr = o.ray_outside ? &c[(q.r - 1) * q.t] : &c[0];
+            const int64_t step = o.ray_outside ? -stride : stride;
+            const int64_t solid = m.ic(o.solid_coord);
+            for (int i = 0; i < q.t; ++i) {
+                const double w = o.ray_outside ? cr[q.t - 1 - i] : cr[i];
+                if (i == 0) on_R.push_back({row, row, w});
+                else on_D.push_back({row, solid + i * step, w});
+            }
+            continue;
+        }
+        const double* cl = c.data();
+        int count = 0, offset = 0;
+        if (std::abs(o.normal[dir]) > 1e-12) {
+            // a wall closure along dir, started at the cut point
+            const bool rw = o.normal[dir] < 0.0;
+            st.nbs(h, bc, 1.0, rw, c.data(), extra.data());
+            cl = rw ? &c[(q.r - 1) * q.t] : &c[0];
+            count = q.t;
+            offset = rw ? 1 - q.t : 0;
+        } else {
+            const int to_right = last - o.solid_coord[dir], to_left = o.solid_coord[dir];
+            if (to_right >= q.p && to_left >= q.p) {
+                st.interior(h, c.data());
+                count = 2 * q.p + 1;
+                offset = -q.p;
+            } else if (to_right < to_left) {
+                st.nbs(h, bc, 1.0, true, c.data(), extra.data());
+                const int k = 1 + to_right;
+                cl = &c[(q.r - k) * q.t];
+                count = q.t;
+                offset = k - q.t;
+            } else {
+                st.nbs(h, bc, 1.0, false, c.data(), extra.data());
+                cl = &c[to_left * q.t];
+                count = q.t;
+                offset = -to_left;
+            }
+        }
+        // the point nearest the cut on the fluid side (psi > 1/2: the solid one)
+        int3 cp = o.solid_coord;
+        if (o.psi <= 0.5) cp[r] += o.ray_outside ? -1 : 1;
+        cp[dir] += offset;
+        const double y = (o.ray_outside ? 1.0 : -1.0) * (o.psi <= 0.5 ? o.psi : o.psi - 1);
+        for (int k = 0; k < count; ++k, ++cp[dir]) {
+            const double v = cl[k];
+            if (cp[dir] == o.solid_coord[dir]) {
+                on_R.push_back({row, row, v});
+                continue;
+            }
+            const line ln = m.interp_line(r, cp);
+            boundary lo, hi;
+            interp_weights(st, r, cp, y, ln, wts.data(), mx.t, lo, hi);
+            const int64_t lic = m.ic(lo.mesh_coordinate), ric = m.ic(hi.mesh_coordinate);
+            int e = 0;
+            if (lo.object) on_R.push_back({row, lo.object->object_coordinate, v * wts[e]});
+            else on_D.push_back({row, lic, v * wts[e]});
+            ++e;
+            for (int64_t x = lic + ln.stride; x < ric; x += ln.stride) on_D.push_back({row, x, v * wts[e++]});
+            if (hi.object) on_R.push_back({row, hi.object->object_coordinate, v * wts[e]});
+            else on_D.push_back({row, ric, v * wts[e]});
+        }
+    }
+}
 
 }  // namespace
 
@@ -154,6 +290,42 @@ derivative::derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st,
         }
     }
 
+    // R-space rows, one row list over R(x), R(y), R(z): within a row the D
+    // entries, then the R entries, each ordered by (column, weight).  Columns
+    // are clamped to their buffer: a scheme without interpolants gives a
+    // degenerate line whose zero-weight ends may lie past the mesh.
+    rptr_.assign(1, 0);
+    if (n[dir] >= 2) {
+        for (int r = 0; r < 3; ++r) {
+            std::vector<cut_entry> on_D, on_R;
+            cut_rows(r, dir, m, st, obj, on_D, on_R);
+            auto order = [](const cut_entry& a, const cut_entry& b) {
+                if (a.row != b.row) return a.row < b.row;
+                if (a.col != b.col) return a.col < b.col;
+                return a.w < b.w;
+            };
+            std::sort(on_D.begin(), on_D.end(), order);
+            std::sort(on_R.begin(), on_R.end(), order);
+            const int64_t nD = m.size(), nRr = (int64_t)m.R(r).size();
+            size_t a = 0, b = 0;
+            while (a < on_D.size() || b < on_R.size()) {
+                const int64_t row = std::min(a < on_D.size() ? on_D[a].row : INT64_MAX,
+                                             b < on_R.size() ? on_R[b].row : INT64_MAX);
+                for (; a < on_D.size() && on_D[a].row == row; ++a) {
+                    rcol_.push_back(std::clamp<int64_t>(on_D[a].col, 0, nD - 1));
+                    rw_.push_back(on_D[a].w);
+                }
+                for (; b < on_R.size() && on_R[b].row == row; ++b) {
+                    rcol_.push_back(-1 - std::clamp<int64_t>(on_R[b].col, 0, nRr - 1));
+                    rw_.push_back(on_R[b].w);
+                }
+                rset_.push_back(r);
+                rrow_.push_back(row);
+                rptr_.push_back((int64_t)rw_.size());
+            }
+        }
+    }
+
     shx_derivative_desc desc;
     std::memset(&desc, 0, sizeof desc);
     desc.dir = dir;
@@ -172,6 +344,12 @@ derivative::derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st,
     desc.neu_col = neu_col_.data();
     desc.neu_w = neu_w_.data();
     desc.nneu = (int64_t)neu_w_.size();
+    desc.nrrow = (int64_t)rrow_.size();
+    desc.rset = rset_.data();
+    desc.rrow = rrow_.data();
+    desc.rptr = rptr_.data();
+    desc.rcol = rcol_.data();
+    desc.rw = rw_.data();
     // without a context only the host assembly is built (structure checks)
     if (ctx) status_ = shx_derivative_create(ctx, &desc, &dev_);
 }

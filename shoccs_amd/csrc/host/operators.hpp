@@ -32,6 +32,14 @@ public:
     const std::vector<shx_line_block>& blocks() const { return blocks_; }
     const std::vector<double>& pool() const { return pool_; }
     const std::vector<double>& interior() const { return interior_; }
+    // R-space rows (the cut-point rows of non-Dirichlet bodies): row k is row
+    // rrow[k] of R(rset[k]) with entries rptr[k]..rptr[k+1]; rcol >= 0 names a
+    // point of D, -1 - c point c of R(rset[k])
+    const std::vector<int32_t>& rset() const { return rset_; }
+    const std::vector<int64_t>& rrow() const { return rrow_; }
+    const std::vector<int64_t>& rptr() const { return rptr_; }
+    const std::vector<int64_t>& rcol() const { return rcol_; }
+    const std::vector<double>& rw() const { return rw_; }
 
     // derivative::operator()(u, du, op) (src/operators/derivative.cpp:502-509)
     int operator()(const shx_field& u, shx_field& du, bool plus_eq = false) const;
@@ -45,6 +53,9 @@ private:
     std::vector<double> interior_;
     std::vector<int64_t> neu_row_, neu_col_;
     std::vector<double> neu_w_;
+    std::vector<int32_t> rset_;
+    std::vector<int64_t> rrow_, rptr_, rcol_;
+    std::vector<double> rw_;
 };
 
 // ccs::gradient / ccs::laplacian (src/operators/gradient.cpp:13-46,

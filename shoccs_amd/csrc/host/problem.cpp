@@ -126,8 +126,13 @@ int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_
             sd.mms = mms.data();
         }
     }
-    // a NULL context builds the host structures only (no device, no system)
-    int rc = ctx ? shx_system_create(ctx, &sd, &p->sys) : SHX_OK;
+    // A NULL context builds the host structures only (no device, no system).
+    // The systems step Dirichlet bodies on non-Neumann grids; other problems
+    // get their operators (cut-point rows included) and no system.
+    bool system_supported = true;
+    for (bcs::type b : p->obj) system_supported = system_supported && b == bcs::type::Dirichlet;
+    for (int f = 0; f < 6; ++f) system_supported = system_supported && desc->grid_bc[f] != SHX_BC_NEUMANN;
+    int rc = ctx && system_supported ? shx_system_create(ctx, &sd, &p->sys) : SHX_OK;
     if (rc != SHX_OK) return rc;
     *out = p.release();
     return SHX_OK;
@@ -237,6 +242,26 @@ int shoccs_deriv_pool(const shoccs_problem* p, int dir, double* out)
     if (!p || dir < 0 || dir > 2 || !out) return SHX_EINVAL;
     const auto& v = p->d[dir]->pool();
     std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return SHX_OK;
+}
+
+int64_t shoccs_deriv_ncut(const shoccs_problem* p, int dir, int64_t* nnz)
+{
+    if (!p || dir < 0 || dir > 2) return -1;
+    if (nnz) *nnz = (int64_t)p->d[dir]->rw().size();
+    return (int64_t)p->d[dir]->rrow().size();
+}
+
+int shoccs_deriv_cut(const shoccs_problem* p, int dir, int32_t* rset, int64_t* rrow, int64_t* rptr,
+                     int64_t* rcol, double* rw)
+{
+    if (!p || dir < 0 || dir > 2 || !rset || !rrow || !rptr || !rcol || !rw) return SHX_EINVAL;
+    const derivative& d = *p->d[dir];
+    std::memcpy(rset, d.rset().data(), d.rset().size() * sizeof(int32_t));
+    std::memcpy(rrow, d.rrow().data(), d.rrow().size() * sizeof(int64_t));
+    std::memcpy(rptr, d.rptr().data(), d.rptr().size() * sizeof(int64_t));
+    std::memcpy(rcol, d.rcol().data(), d.rcol().size() * sizeof(int64_t));
+    std::memcpy(rw, d.rw().data(), d.rw().size() * sizeof(double));
     return SHX_OK;
 }
 

@@ -517,6 +517,118 @@ void e22_neumann(double h, double psi, double* c, double* x, int right)
 }
 
 
+// Interpolants of the cut-point rows (E2_2.cpp:31-86, E4_2.cpp:31-183): weights
+// over the boundary and the t - 1 points beyond it at offset y (in the
+// i + y form of stencil.hpp:213-270), i the closest point's distance from
+// the boundary point.
+int e22_interp_wall(int i, double y, double psi, double* c, bool right)
+{
+    if (right) {
+        const double t6 = 1 + psi, t7 = 1.0 / (t6), t8 = -1 + y, t9 = psi * t8, t5 = -1 + psi;
+        const double t12 = -1 * psi * y, t20 = 1 + y;
+        if (i == 0) {
+            c[0] = (psi + y + psi * y) * t5;
+            c[1] = 1 + t12 + -1 * psi * psi * t20 + y;
+            c[2] = psi * t20;
+        } else if (i == 1) {
+            c[0] = (t9 + y) * t5 * t7;
+            c[1] = psi + t12;
+            c[2] = (1 + t9 + y) * t7;
+        }
+    } else {
+        const double t8 = -1 + y, t11 = psi * y, t13 = -1 + psi, t17 = 1 + psi, t18 = 1.0 / (t17);
+        if (i == 0) {
+            c[0] = psi + -1 * psi * y;
+            c[1] = 1 + t11 + psi * psi * t8 + -1 * y;
+            c[2] = (psi * t8 + y) * -1 * t13;
+        } else if (i == 1) {
+            c[0] = (-1 + psi + t11 + y) * -1 * t18;
+            c[1] = (1 + y) * psi;
+            c[2] = (psi + t11 + y) * -1 * t13 * t18;
+        }
+    }
+    return 3;
+}
+
+int e42_interp_wall(int i, double y, double psi, double* c, bool right)
+{
+    const double y2 = y * y, y3 = y2 * y;
+    const double psi2 = psi * psi, psi3 = psi2 * psi, psi4 = psi2 * psi2;
+    if (right) {
+        if (i == 0) {
+            c[0] = ((-1 + psi) * (6 * y * (2 + 3 * y + y2) + 6 * psi * (2 + 8 * y + 6 * y2 + y3) +
+                                  psi3 * (6 + 11 * y + 6 * y2 + y3) + 2 * psi2 * (9 + 22 * y + 12 * y2 + 2 * y3))) /
+                   36.;
+            c[1] = (6 * y * (3 + 4 * y + y2) + 6 * psi * (3 + 8 * y + 3 * y2) - 3 * psi2 * (-2 + 5 * y + 6 * y2 + y3) -
+                    psi4 * (6 + 11 * y + 6 * y2 + y3) - 2 * psi3 * (9 + 22 * y + 12 * y2 + 2 * y3)) /
+                   12.;
+            c[2] = (-6 * y * (6 + 5 * y + y2) - 6 * psi * (6 + 10 * y + 3 * y2) + 6 * psi2 * (1 + 8 * y + 6 * y2 + y3) +
+                    psi4 * (6 + 11 * y + 6 * y2 + y3) + psi3 * (24 + 55 * y + 30 * y2 + 5 * y3)) /
+                   12.;
+            c[3] = (-6 * psi * (-5 - y + 3 * y2 + y3) - 6 * psi3 * (5 + 11 * y + 6 * y2 + y3) +
+                    6 * (6 + 11 * y + 6 * y2 + y3) - psi4 * (6 + 11 * y + 6 * y2 + y3) -
+                    psi2 * (30 + 103 * y + 66 * y2 + 11 * y3)) /
+                   36.;
+            c[4] = (psi * (6 + 11 * y + 6 * y2 + y3)) / 6.;
+        } else if (i == 1) {
+            c[0] = ((-1 + psi) * (psi * (-1 + y) + y) * (2 + 3 * y + y2)) / (2. * (3 + psi));
+            c[1] = -((1 + y) * (2 * psi * (3 + y) - 2 * y * (3 + y) + 3 * psi2 * (-2 + y + y2))) / (2. * (2 + psi));
+            c[2] = ((2 + y) * (psi * (3 + y) - y * (3 + y) + 3 * psi2 * (-1 + y2))) / (2. * (1 + psi));
+            c[3] = (psi * (2 + y - 2 * y2 - y3)) / 2.;
+            c[4] = ((3 + 3 * psi * (-1 + y) + y) * (2 + 3 * y + y2)) / (6 + 11 * psi + 6 * psi2 + psi3);
+        } else {
+            c[0] = (y * (-24 * (-1 + y2) + 2 * psi2 * (-2 - y + 2 * y2 + y3) + 3 * psi3 * (-2 - y + 2 * y2 + y3) +
+                         psi4 * (-2 - y + 2 * y2 + y3))) /
+                   144.;
+            c[1] = (y * (24 * (-2 + y + y2) + psi4 * (2 + y - 2 * y2 - y3) - 3 * psi2 * (-2 - y + 2 * y2 + y3) -
+                         4 * psi3 * (-2 - y + 2 * y2 + y3))) /
+                   48.;
+            c[2] = ((-24 + 6 * psi2 * y + 5 * psi3 * y + psi4 * y) * (-2 - y + 2 * y2 + y3)) / 48.;
+            c[3] = -(y * (-24 * (2 + 3 * y + y2) + 6 * psi * (-2 - y + 2 * y2 + y3) +
+                          11 * psi2 * (-2 - y + 2 * y2 + y3) + 6 * psi3 * (-2 - y + 2 * y2 + y3) +
+                          psi4 * (-2 - y + 2 * y2 + y3))) /
+                   144.;
+            c[4] = (psi * y * (-2 - y + 2 * y2 + y3)) / 24.;
+        }
+    } else {
+        if (i == 0) {
+            c[0] = -(psi * (-6 + 11 * y - 6 * y2 + y3)) / 6.;
+            c[1] = (-6 * (-6 + 11 * y - 6 * y2 + y3) + psi4 * (-6 + 11 * y - 6 * y2 + y3) +
+                    6 * psi3 * (-5 + 11 * y - 6 * y2 + y3) + 6 * psi * (5 - y - 3 * y2 + y3) +
+                    psi2 * (-30 + 103 * y - 66 * y2 + 11 * y3)) /
+                   36.;
+            c[2] = (6 * y * (6 - 5 * y + y2) - 6 * psi * (6 - 10 * y + 3 * y2) + psi3 * (24 - 55 * y + 30 * y2 - 5 * y3) -
+                    6 * psi2 * (-1 + 8 * y - 6 * y2 + y3) - psi4 * (-6 + 11 * y - 6 * y2 + y3)) /
+                   12.;
+            c[3] = (-6 * y * (3 - 4 * y + y2) + 6 * psi * (3 - 8 * y + 3 * y2) + 3 * psi2 * (2 + 5 * y - 6 * y2 + y3) +
+                    psi4 * (-6 + 11 * y - 6 * y2 + y3) + 2 * psi3 * (-9 + 22 * y - 12 * y2 + 2 * y3)) /
+                   12.;
+            c[4] = -((-1 + psi) * (6 * y * (2 - 3 * y + y2) + 6 * psi * (-2 + 8 * y - 6 * y2 + y3) +
+                                   psi3 * (-6 + 11 * y - 6 * y2 + y3) + 2 * psi2 * (-9 + 22 * y - 12 * y2 + 2 * y3))) /
+                   36.;
+        } else if (i == 1) {
+            c[0] = -(((2 - 3 * y + y2) * (-3 + y + 3 * psi * (1 + y))) / ((1 + psi) * (2 + psi) * (3 + psi)));
+            c[1] = (psi * (2 - y - 2 * y2 + y3)) / 2.;
+            c[2] = -((-2 + y) * (-(psi * (-3 + y)) - (-3 + y) * y + 3 * psi2 * (-1 + y2))) / (2. * (1 + psi));
+            c[3] = ((-1 + y) * (-2 * psi * (-3 + y) - 2 * (-3 + y) * y + 3 * psi2 * (-2 - y + y2))) / (2. * (2 + psi));
+            c[4] = -((-1 + psi) * (psi + y + psi * y) * (2 - 3 * y + y2)) / (2. * (3 + psi));
+        } else {
+            c[0] = (psi * y * (2 - y - 2 * y2 + y3)) / 24.;
+            c[1] = -(y * (24 * (2 - 3 * y + y2) + 6 * psi * (2 - y - 2 * y2 + y3) + 11 * psi2 * (2 - y - 2 * y2 + y3) +
+                          6 * psi3 * (2 - y - 2 * y2 + y3) + psi4 * (2 - y - 2 * y2 + y3))) /
+                   144.;
+            c[2] = ((24 + 6 * psi2 * y + 5 * psi3 * y + psi4 * y) * (2 - y - 2 * y2 + y3)) / 48.;
+            c[3] = (y * (24 * (2 + y - y2) + psi4 * (-2 + y + 2 * y2 - y3) - 3 * psi2 * (2 - y - 2 * y2 + y3) -
+                         4 * psi3 * (2 - y - 2 * y2 + y3))) /
+                   48.;
+            c[4] = (y * (24 * (-1 + y2) + 2 * psi2 * (2 - y - 2 * y2 + y3) + 3 * psi3 * (2 - y - 2 * y2 + y3) +
+                         psi4 * (2 - y - 2 * y2 + y3))) /
+                   144.;
+        }
+    }
+    return 5;
+}
+
 }  // namespace
 
 info stencil::query_max() const
@@ -609,6 +721,46 @@ std::optional<stencil> stencil::make(int kind, const double* alpha, int nalpha)
     // E4_1 denominator is singular for alpha[1] < 197/288 (src/stencils/E4_1.cpp:34-41)
     if (kind == (int)scheme::E4_1 && a[1] < 197.0 / 288.0) return std::nullopt;
     return stencil{(scheme)kind, a};
+}
+
+// stencil.hpp:29 query_interp: E2 2 interior points over 3 wall points, E4 5
+// and 5; the first-derivative schemes have none
+interp_info stencil::query_interp() const
+{
+    if (kind_ == scheme::E2_2) return {2, 3};
+    if (kind_ == scheme::E4_2) return {5, 5};
+    return {};
+}
+
+int stencil::interp_interior(double y, double* c, int cap) const
+{
+    if (kind_ == scheme::E2_2) {
+        if (y > 0) {
+            c[0] = 1 + -1 * y;
+            c[1] = y;
+        } else {
+            c[0] = -1 * y;
+            c[1] = 1 + y;
+        }
+        return 2;
+    }
+    if (kind_ == scheme::E4_2) {
+        const double y2 = y * y, y3 = y * y * y, y4 = y2 * y2;
+        c[0] = (y * (2 - y - 2 * y2 + y3)) / 24.;
+        c[1] = (y * (-4 + 4 * y + y2 - y3)) / 6.;
+        c[2] = (4 - 5 * y2 + y4) / 4.;
+        c[3] = -(y * (-4 - 4 * y + y2 + y3)) / 6.;
+        c[4] = (y * (-2 - y + 2 * y2 + y3)) / 24.;
+        return 5;
+    }
+    return cap;
+}
+
+int stencil::interp_wall(int i, double y, double psi, double* c, bool right, int cap) const
+{
+    if (kind_ == scheme::E2_2) return e22_interp_wall(i, y, psi, c, right);
+    if (kind_ == scheme::E4_2) return e42_interp_wall(i, y, psi, c, right);
+    return cap;
 }
 
 }  // namespace ccs::stencils

@@ -18,6 +18,12 @@ struct info {
     int nextra = 0;
 };
 
+// stencil.hpp:29 interp_info: p interior interpolation points, t wall points
+struct interp_info {
+    int p = 0;
+    int t = 0;
+};
+
 // scheme{order, type}: order 2 "E2"/"E4" and order 1 "E4"/"E4u"
 enum class scheme : int { E2_2 = 0, E4_2 = 1, E4_1 = 2, E4u_1 = 3 };
 
@@ -36,6 +42,11 @@ public:
     // closure of r x t coefficients (row-major) for a wall at distance psi*h;
     // Neumann closures also fill `extra`
     void nbs(double h, bcs::type b, double psi, bool right, double* c, double* extra) const;
+    // interpolation weights written to c, their count returned; schemes
+    // without interpolants leave c as it is and return cap
+    interp_info query_interp() const;
+    int interp_interior(double y, double* c, int cap) const;
+    int interp_wall(int i, double y, double psi, double* c, bool right, int cap) const;
     scheme kind() const { return kind_; }
 
 private:

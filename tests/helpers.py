@@ -118,3 +118,51 @@ def product_problem(ctx, cfg):
     return S.Problem(ctx, cfg["kind"], cfg["n"], cfg["lo"], cfg["hi"], spheres=cfg["spheres"],
                      shape_bc=cfg["obj_bc"], grid_bc=cfg["grid_bc"], scheme=cfg["scheme"],
                      alpha=cfg["alpha"], params=cfg["params"])
+
+
+def cut_cfgs():
+    """Problems with floating and Neumann bodies: the derivative operators
+    carry R-space rows (the system is not built for them)."""
+    F, N, D = O.FLOATING, O.NEUMANN, O.DIRICHLET
+
+    def heat(n, lo, hi, spheres, obj_bc, grid_bc, scheme, alpha=(0.0, 0.0)):
+        return dict(kind=O.SYS_HEAT, n=n, lo=lo, hi=hi, spheres=spheres, obj_bc=obj_bc, grid_bc=grid_bc,
+                    scheme=scheme, alpha=alpha, params=[1.0, O.MMS_SINPROD])
+
+    return {
+        # derivative.t.cpp:515-568 ("E2 with Objects")
+        "objects_E2": heat((25, 26, 27), (0.1, 0.2, 0.3), (1, 2, 2.2), [[0.45, 1.011, 1.31, 0.25]], [F],
+                           [N, N, D, D, F, F], O.E2_2),
+        "floating_E4_2": heat((28, 30, 26), (0, 0, 0), (1, 1.1, 0.95), [[0.52, 0.49, 0.47, 0.27]], [F],
+                              [F] * 6, O.E4_2),
+        "neumann_E4_2": heat((20, 22, 24), (0, 0, 0), (1, 1, 1), [[0.5, 0.45, 0.55, 0.22]], [N], [D] * 6, O.E4_2),
+        "two_spheres_E2": heat((30, 28, 26), (0, 0, 0), (1, 1, 1), [[0.3, 0.35, 0.5, 0.17], [0.7, 0.62, 0.48, 0.16]],
+                               [D, F], [F, F, D, D, N, N], O.E2_2),
+        "floating_E4u_1": heat((32, 32, 32), (0, 0, 0), (1, 1, 1), [[0.5, 0.5, 0.5, 0.25]], [F], [D, F] * 3, O.E4U_1,
+                               (-0.7733323791884821, 0.1623961700641681)),
+        "floating_E4_1": heat((36, 36, 36), (0, 0, 0), (1, 1, 1), [[0.51, 0.48, 0.5, 0.27]], [F], [F] * 6, O.E4_1,
+                              (0.1, 0.7)),
+        "floating2d_E4_2": heat((41, 43, 1), (0, 0, 0), (1, 1, 0), [[0.5, 0.52, 0.0, 0.23]], [F], [F] * 6, O.E4_2),
+    }
+
+
+def oracle_cut_rows(osys, d):
+    """The oracle's Bf / Br CSRs of direction d in the product's row layout
+    (rows with entries only; D entries then R entries; columns clamped)."""
+    dv = osys.deriv(d).contents
+    N = osys.mesh.N
+    rset, rrow, rptr, rcol, rw = [], [], [0], [], []
+    for r in range(3):
+        Bf, Br = dv.Bf[r], dv.Br[r]
+        for row in range(Bf.rows):
+            ents = [(min(max(Bf.v[i], 0), N - 1), Bf.w[i]) for i in range(Bf.u[row], Bf.u[row + 1])]
+            ents += [(-1 - min(max(Br.v[i], 0), Br.rows - 1), Br.w[i]) for i in range(Br.u[row], Br.u[row + 1])]
+            if not ents:
+                continue
+            rset.append(r)
+            rrow.append(row)
+            rcol += [c for c, _ in ents]
+            rw += [w for _, w in ents]
+            rptr.append(len(rw))
+    return dict(rset=np.array(rset, np.int32), rrow=np.array(rrow, np.int64), rptr=np.array(rptr, np.int64),
+                rcol=np.array(rcol, np.int64), rw=np.array(rw))

@@ -61,3 +61,41 @@ def test_rejects_closures_outside_the_mesh():
     cfg = H.sphere_wave_cfg(24)
     with pytest.raises(ValueError):
         H.oracle_system(cfg)
+
+
+CUT = H.cut_cfgs()
+
+
+@pytest.mark.parametrize("name", list(CUT))
+def test_cut_rows_bitexact(name):
+    """Cut-point rows of floating and Neumann bodies (cut_discretization,
+    derivative.cpp:159-257) and the D-space segments next to them: the host
+    assembly against the oracle, bit for bit."""
+    cfg = CUT[name]
+    osys = H.oracle_system(cfg)
+    prob = H.product_problem(None, cfg)
+    total = 0
+    for d in range(3):
+        if prob.n[d] < 2:
+            continue
+        want, got = H.oracle_cut_rows(osys, d), prob.cut_rows(d)
+        for k in ("rset", "rrow", "rptr", "rcol", "rw"):
+            assert np.array_equal(want[k], got[k]), (name, d, k)
+        total += len(got["rw"])
+        so, Bo = H.oracle_segments(osys.deriv(d), prob.n, d)
+        sp, Bp = H.product_segments(prob, d)
+        assert len(so) == len(sp)
+        for a, b in zip(so, sp):
+            for k in a:
+                assert np.array_equal(a[k], b[k]) if isinstance(a[k], np.ndarray) else a[k] == b[k], (d, k)
+        for x, y in zip(Bo, Bp):
+            assert np.array_equal(x, y), d
+    assert total > 0
+
+
+def test_operator_only_problem_has_no_system():
+    import shoccs_amd as S
+
+    prob = H.product_problem(None, CUT["objects_E2"])
+    with pytest.raises(S.ShoccsError, match="device context"):
+        prob.sys
