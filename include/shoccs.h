@@ -62,6 +62,14 @@ int64_t shoccs_problem_size(const shoccs_problem* p, int which);
 double shoccs_problem_dt(const shoccs_problem* p, double h_cfl, double p_cfl);
 
 /* assembled structures, for bit-exact comparison with the reference's */
+/* mesh extents and domain bounds (mesh::from_lua, src/mesh/cartesian.cpp:45-89) */
+int shoccs_problem_geometry(const shoccs_problem* p, int32_t n[3], double lo[3], double hi[3]);
+/* write_scalar_error (src/systems/detail/scalar_system_utils.hpp:224-301) on
+ * host flat fields (D, Rx, Ry, Rz): err = |u - sol| at the fluid points and
+ * at the cut points of non-Dirichlet bodies, 0 elsewhere and on Dirichlet
+ * grid faces */
+int shoccs_error_field(const shoccs_problem* p, const double* u, const double* sol, double* err);
+/* outputs may be NULL */
 int shoccs_mesh_r(const shoccs_problem* p, int dir, double* psi, double* pos, int32_t* solid,
                   int32_t* ray_outside, int32_t* shape_id);
 int64_t shoccs_mesh_nlines(const shoccs_problem* p, int dir);
@@ -99,6 +107,8 @@ typedef struct shoccs_run_desc {
     double h_cfl, p_cfl, min_dt;
     double* stats;     /* optional: up to stats_cap rows of the 11 statistics, */
     int64_t stats_cap; /* row 0 at t = 0, then one row per step */
+    struct shoccs_io* io; /* optional field writer: U and Error at t = 0 and after
+                           * each step, when its interval fires */
 } shoccs_run_desc;
 
 typedef struct shoccs_run_result {
@@ -110,8 +120,8 @@ typedef struct shoccs_run_result {
     int64_t nstats;    /* rows written to stats */
 } shoccs_run_result;
 
-/* simulation_cycle::run (src/simulation/simulation_cycle.cpp:31-143) without
- * I/O: initialize u at t = 0, update its boundary, then step while the
+/* simulation_cycle::run (src/simulation/simulation_cycle.cpp:31-143):
+ * initialize u at t = 0, update its boundary, then step while the
  * controller runs and system::valid(stats) holds (scalar_wave.cpp:108-112:
  * finite and |L-inf| <= max_error; heat.cpp:54-58: <= 1e6).  u holds the
  * final solution.  Synchronous. */
@@ -130,8 +140,9 @@ int shoccs_run(shoccs_problem* p, const shoccs_run_desc* d, shx_field* u, shoccs
  * order 1), system{type = "heat" (diffusivity) | "scalar wave" (center,
  * radius, max_error)}, manufactured_solution{type = "gaussian", {center,
  * variance, amplitude, frequency}...}, integrator{type = "rk4" | "euler"},
- * step_controller{max_step, max_time, min_dt, cfl{hyperbolic, parabolic}};
- * io and logging are ignored.  Extensions of this library:
+ * step_controller{max_step, max_time, min_dt, cfl{hyperbolic, parabolic}},
+ * io{write_every_step, write_every_time, dir, suffix_length, xdmf_filename}
+ * (shoccs_config_io); logging is ignored.  Extensions of this library:
  * system.velocity = {ax, ay, az} with system.initial_condition = {center,
  * width, amplitude} (constant-velocity wave with a gaussian pulse),
  * manufactured_solution.type = "sin_product", integrator.form = "fused".
@@ -142,6 +153,33 @@ int shoccs_config_parse(const char* text, shoccs_config** out, char* err, int64_
  * descriptor's arrays belong to the configuration) */
 int shoccs_config_desc(const shoccs_config* c, shoccs_problem_desc* desc, shoccs_run_desc* run);
 int shoccs_config_destroy(shoccs_config* c);
+
+/* ------------------------------------------------------------ field output
+ * field_io (src/io/field_io.cpp:32-65, field_data.cpp, xdmf.cpp): when the
+ * dump interval fires (every write_every_step steps, every write_every_time
+ * of simulated time within dt, and at step 0; interval.hpp:10-103), dump k
+ * writes dir/<name>.<k, zero-padded to suffix_length> per variable (raw
+ * native FP64: D, then Rx, Ry, Rz), the cut-point positions dir/rx, ry, rz at
+ * the first dump (x, y, z per point; z, y, x on a mesh with one z point) and
+ * the XDMF3 time series dir/<xdmf_filename>.  Neither interval set: nothing
+ * is written. */
+typedef struct shoccs_io shoccs_io;
+typedef struct shoccs_io_desc {
+    const char* dir;           /* io.dir; NULL or "": "io" */
+    const char* xdmf_filename; /* io.xdmf_filename; NULL or "": "view.xmf" */
+    int32_t suffix_length;     /* io.suffix_length; <= 0: 6 */
+    int32_t write_every_step;  /* io.write_every_step; <= 0: unset */
+    double write_every_time;   /* io.write_every_time; <= 0: unset */
+} shoccs_io_desc;
+int shoccs_io_create(const shoccs_problem* p, const shoccs_io_desc* d, shoccs_io** out);
+int shoccs_io_destroy(shoccs_io* io);
+/* host flat fields of the problem's shape; *wrote (may be NULL) = 1 when a
+ * dump was written */
+int shoccs_io_write(shoccs_io* io, int32_t nvars, const char* const* names, const double* const* fields,
+                    int32_t step, double time, double dt, int32_t* wrote);
+/* the configuration's io table (1 returned when it sets an interval, else
+ * 0); the strings belong to the configuration */
+int shoccs_config_io(const shoccs_config* c, shoccs_io_desc* io);
 
 /* device fields of the problem's shape: D then Rx, Ry, Rz */
 int shoccs_field_alloc(shoccs_problem* p, shx_field* f);

@@ -26,11 +26,11 @@ import numpy as np
 
 from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, E2_2, E4_1, E4_2, E4U_1, EULER,
                    F32_COMP, F64, MMS_GAUSS, MMS_SINPROD, RK4, RK4_FUSED, SYS_HEAT,
-                   SYS_WAVE_LINEAR, SYS_WAVE_RADIAL, ProblemDesc, RunDesc, RunResult, ShoccsError,
+                   SYS_WAVE_LINEAR, SYS_WAVE_RADIAL, IODesc, ProblemDesc, RunDesc, RunResult, ShoccsError,
                    ShxField, ShxLineBlock, check, lib)
 
 __all__ = [
-    "Context", "Problem", "Field", "Batch", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
+    "Context", "Problem", "Field", "Batch", "FieldIO", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
     "E2_2", "E4_2", "E4_1", "E4U_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
     "MMS_GAUSS", "MMS_SINPROD", "F64", "F32_COMP", "RK4", "RK4_FUSED", "EULER", "eq", "plus_eq",
 ]
@@ -168,6 +168,13 @@ class Config:
             self.max_step = None if r.max_step == 2 ** 31 - 1 else r.max_step
             self.max_time = None if r.max_time >= np.finfo(np.float64).max else r.max_time
             self.h_cfl, self.p_cfl, self.min_dt = r.h_cfl, r.p_cfl, r.min_dt
+            io = IODesc()
+            has_io = L.shoccs_config_io(h, C.byref(io))
+            self.io = None if has_io != 1 else dict(
+                dir=io.dir.decode(), xdmf_filename=io.xdmf_filename.decode(),
+                suffix_length=io.suffix_length,
+                write_every_step=io.write_every_step if io.write_every_step > 0 else None,
+                write_every_time=io.write_every_time if io.write_every_time > 0 else None)
         finally:
             L.shoccs_config_destroy(h)
 
@@ -184,6 +191,48 @@ class Config:
     def run_kwargs(self) -> dict:
         return dict(integrator=self.integrator, max_step=self.max_step, max_time=self.max_time,
                     h_cfl=self.h_cfl, p_cfl=self.p_cfl, min_dt=self.min_dt)
+
+
+class FieldIO:
+    """The reference's field output (shoccs_io_*): raw FP64 dumps of each
+    variable (D, then Rx, Ry, Rz), the cut-point geometry and an XDMF3 time
+    series, written when the dump interval fires (and at step 0)."""
+
+    def __init__(self, problem: "Problem", dir: str = "io", write_every_step: int | None = None,
+                 write_every_time: float | None = None, suffix_length: int = 6,
+                 xdmf_filename: str = "view.xmf"):
+        d = IODesc()
+        self._keep = (dir.encode(), xdmf_filename.encode())
+        d.dir, d.xdmf_filename = self._keep
+        d.suffix_length = suffix_length
+        d.write_every_step = write_every_step or 0
+        d.write_every_time = write_every_time or 0.0
+        self.problem = problem
+        self.h = C.c_void_p()
+        check(lib().shoccs_io_create(problem.p, C.byref(d), C.byref(self.h)), None, "io create")
+
+    def write(self, fields: dict, step: int, time: float, dt: float) -> bool:
+        """Dump the host flat fields {name: array}; True when written."""
+        names = (C.c_char_p * len(fields))(*[k.encode() for k in fields])
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in fields.values()]
+        for a in arrs:
+            assert a.size == self.problem.flat_size
+        ptrs = (C.POINTER(C.c_double) * len(arrs))(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+        wrote = C.c_int32()
+        check(lib().shoccs_io_write(self.h, len(arrs), names, ptrs, step, time, dt, C.byref(wrote)), None,
+              "io write")
+        return bool(wrote.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shoccs_io_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Problem:
@@ -290,6 +339,17 @@ class Problem:
         a = np.zeros(max(n, 1))
         check(lib().shoccs_deriv_pool(self.p, d, a.ctypes.data_as(C.POINTER(C.c_double))), None, "pool")
         return a[:n]
+
+    def error_field(self, u: np.ndarray, sol: np.ndarray) -> np.ndarray:
+        """|u - sol| at the fluid points and the cut points of non-Dirichlet
+        bodies, 0 elsewhere and on Dirichlet grid faces (shoccs_error_field)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        sol = np.ascontiguousarray(sol, dtype=np.float64)
+        out = np.empty(self.flat_size)
+        dpp = C.POINTER(C.c_double)
+        check(lib().shoccs_error_field(self.p, u.ctypes.data_as(dpp), sol.ctypes.data_as(dpp),
+                                       out.ctypes.data_as(dpp)), None, "error field")
+        return out
 
     def cut_rows(self, d: int) -> dict:
         """Direction d's R-space rows (cut points of non-Dirichlet bodies):
@@ -463,7 +523,7 @@ class Problem:
 
     def run(self, u: Field, integrator: int = RK4, max_step: int | None = None,
             max_time: float | None = None, h_cfl: float = 1.0, p_cfl: float = 1.0,
-            min_dt: float = 1e-6, keep_stats: bool = True) -> dict:
+            min_dt: float = 1e-6, keep_stats: bool = True, io: "FieldIO | None" = None) -> dict:
         """simulation_cycle::run without I/O (shoccs_run): returns the final
         time, step count, whether the controller ran out (`completed`), the
         final L-inf error (NaN when ended prematurely) and the statistics of
@@ -487,6 +547,7 @@ class Problem:
         buf = np.zeros((max(cap, 1), 11))
         d.stats = buf.ctypes.data_as(C.POINTER(C.c_double)) if cap else None
         d.stats_cap = cap
+        d.io = io.h if io is not None else None
         r = RunResult()
         check(lib().shoccs_run(self.p, C.byref(d), u.ref(), C.byref(r)), self.ctx.p, "run")
         return dict(time=r.time, step=r.step, completed=bool(r.completed), linf=r.linf,

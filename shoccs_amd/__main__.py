@@ -1,9 +1,11 @@
 """Run a Lua `simulation` configuration on the device, as the reference's
-shoccs binary does (src/app/shoccs.cpp: input file, --check), without its
-field I/O: the statistics of every step go to a CSV file with the columns of
-the reference's system.csv (src/systems/heat.cpp:46-48).
+shoccs binary does (src/app/shoccs.cpp: input file, --check): the
+statistics of every step go to a CSV file with the columns of the reference's
+system.csv (src/systems/heat.cpp:46-48), and the configuration's io table
+writes the U and Error fields, the cut-point geometry and the XDMF3 series
+(src/io/field_io.cpp).
 
-    python -m shoccs_amd config.lua [--check] [--csv logs/system.csv] [--fused]
+    python -m shoccs_amd config.lua [--check] [--csv logs/system.csv] [--fused] [--no-io]
 """
 from __future__ import annotations
 
@@ -19,6 +21,7 @@ def main(argv=None) -> int:
     ap.add_argument("--check", action="store_true", help="read the configuration and the problem, do not run")
     ap.add_argument("--csv", default=os.path.join("logs", "system.csv"), help="statistics file ('' for none)")
     ap.add_argument("--fused", action="store_true", help="fused RK4 form (integrator.form = 'fused')")
+    ap.add_argument("--no-io", action="store_true", help="ignore the configuration's io table")
     args = ap.parse_args(argv)
 
     import shoccs_amd as S
@@ -36,8 +39,9 @@ def main(argv=None) -> int:
     kw = cfg.run_kwargs()
     if args.fused and kw["integrator"] == S.RK4:
         kw["integrator"] = S.RK4_FUSED
+    io = S.FieldIO(prob, **cfg.io) if cfg.io and not args.no_io else None
     t0 = time.perf_counter()
-    res = prob.run(u, **kw)
+    res = prob.run(u, io=io, **kw)
     wall = (time.perf_counter() - t0) * 1e3
     if args.csv:
         os.makedirs(os.path.dirname(args.csv) or ".", exist_ok=True)
@@ -52,6 +56,8 @@ def main(argv=None) -> int:
                 f.write(f"{stamp},{tk:.17g},{k},{vals},{wall if k == len(res['stats']) - 1 else 0:.3f}\n")
     print(f"steps {res['step']}, time {res['time']:.6g}, completed {res['completed']}, "
           f"linf {res['linf']:.6e}")
+    if io is not None:
+        io.close()
     prob.close()
     ctx.close()
     return 0 if res["completed"] else 1

@@ -61,6 +61,17 @@ class RunDesc(C.Structure):
         ("min_dt", C.c_double),
         ("stats", dp),
         ("stats_cap", C.c_int64),
+        ("io", C.c_void_p),
+    ]
+
+
+class IODesc(C.Structure):
+    _fields_ = [
+        ("dir", C.c_char_p),
+        ("xdmf_filename", C.c_char_p),
+        ("suffix_length", C.c_int32),
+        ("write_every_step", C.c_int32),
+        ("write_every_time", C.c_double),
     ]
 
 
@@ -169,6 +180,13 @@ _SIGS = [
                              C.POINTER(RunResult)]),
     ("shoccs_config_parse", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_int64]),
     ("shoccs_config_desc", C.c_int, [C.c_void_p, C.POINTER(ProblemDesc), C.POINTER(RunDesc)]),
+    ("shoccs_config_io", C.c_int, [C.c_void_p, C.POINTER(IODesc)]),
+    ("shoccs_io_create", C.c_int, [C.c_void_p, C.POINTER(IODesc), C.POINTER(C.c_void_p)]),
+    ("shoccs_io_destroy", C.c_int, [C.c_void_p]),
+    ("shoccs_io_write", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(dp), C.c_int32,
+                                  C.c_double, C.c_double, C.POINTER(C.c_int32)]),
+    ("shoccs_error_field", C.c_int, [C.c_void_p, dp, dp, dp]),
+    ("shoccs_problem_geometry", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), dp, dp]),
     ("shoccs_config_destroy", C.c_int, [C.c_void_p]),
     ("shx_derivative_apply_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
                                          C.c_int]),

@@ -24,7 +24,7 @@ ARCH = "gfx950"
 DEVICE_SOURCES = ["device/shx_core.hip", "device/shx_systems.hip", "device/shx_march.hip",
                   "device/shx_mixed.hip", "device/shx_adjoint.hip"]
 HOST_SOURCES = ["host/mesh.cpp", "host/stencils.cpp", "host/operators.cpp", "host/problem.cpp",
-                "host/lua_config.cpp"]
+                "host/lua_config.cpp", "host/field_io.cpp"]
 
 COMMON = ["-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-Wall", "-Wno-unused-result", "-Wno-unused-value",

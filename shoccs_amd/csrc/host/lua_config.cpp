@@ -661,6 +661,10 @@ struct shoccs_config {
     std::vector<double> params;
     shoccs_run_desc run{};
     int fused = 0;
+    // field_io::from_lua (src/io/field_io.cpp:67-106)
+    shoccs_io_desc io{};
+    std::string io_dir = "io", io_xmf = "view.xmf";
+    bool has_io = false;
 };
 
 namespace {
@@ -847,6 +851,15 @@ int build(const Value& sim, shoccs_config& c, std::string& err)
     c.run.min_dt = num_or(sc, "min_dt", 1e-6);
     c.run.h_cfl = num_or(cfl, "hyperbolic", 1.0);
     c.run.p_cfl = num_or(cfl, "parabolic", 1.0);
+    // io (field_io.cpp:67-106): write_every_step or write_every_time, dir,
+    // suffix_length, xdmf_filename
+    const Value io = sub(sim, "io");
+    c.io_dir = str_or(io, "dir", "io");
+    c.io_xmf = str_or(io, "xdmf_filename", "view.xmf");
+    c.io.suffix_length = (int32_t)num_or(io, "suffix_length", 6.0);
+    c.io.write_every_step = (int32_t)num_or(io, "write_every_step", 0.0);
+    c.io.write_every_time = num_or(io, "write_every_time", 0.0);
+    c.has_io = c.io.write_every_step > 0 || c.io.write_every_time > 0.0;
     return SHX_OK;
 }
 
@@ -890,6 +903,15 @@ int shoccs_config_desc(const shoccs_config* c, shoccs_problem_desc* desc, shoccs
     if (desc) *desc = c->desc;
     if (run) *run = c->run;
     return SHX_OK;
+}
+
+int shoccs_config_io(const shoccs_config* c, shoccs_io_desc* io)
+{
+    if (!c || !io) return SHX_EINVAL;
+    *io = c->io;
+    io->dir = c->io_dir.c_str();
+    io->xdmf_filename = c->io_xmf.c_str();
+    return c->has_io ? 1 : 0;
 }
 
 int shoccs_config_destroy(shoccs_config* c)

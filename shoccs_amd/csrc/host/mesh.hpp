@@ -81,6 +81,8 @@ public:
     const std::vector<mesh_object_info>& R(int dir) const { return R_[dir]; }
     const std::vector<line>& lines(int dir) const { return lines_[dir]; }
     const std::vector<int64_t>& fluid() const { return fluid_; }
+    const real3& lo() const { return min_; }
+    const real3& hi() const { return max_; }
     const std::vector<sphere_shape>& shapes() const { return shapes_; }
     int64_t stride(int dir) const;
     int64_t ic(const int3& ijk) const

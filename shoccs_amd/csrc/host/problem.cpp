@@ -1,6 +1,7 @@
 // problem.cpp -- C entry points of the host library (include/shoccs.h): build
 // a mesh, its boundary conditions, stencil and system, upload them through
 // shoccs_hip.h, and expose the assembled structures.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -169,6 +170,52 @@ double shoccs_problem_dt(const shoccs_problem* p, double h_cfl, double p_cfl)
     const double hmin = std::min(h[0], std::min(h[1], h[2]));
     if (p->kind == SHX_SYS_HEAT) return p_cfl * hmin * hmin / (4 * p->diffusivity);
     return h_cfl * hmin;
+}
+
+int shoccs_problem_geometry(const shoccs_problem* p, int32_t n[3], double lo[3], double hi[3])
+{
+    if (!p || !n || !lo || !hi) return SHX_EINVAL;
+    for (int a = 0; a < 3; ++a) {
+        n[a] = p->m->extents()[a];
+        lo[a] = p->m->lo()[a];
+        hi[a] = p->m->hi()[a];
+    }
+    return SHX_OK;
+}
+
+int shoccs_error_field(const shoccs_problem* p, const double* u, const double* sol, double* err)
+{
+    if (!p || !u || !sol || !err) return SHX_EINVAL;
+    const mesh& m = *p->m;
+    const int64_t N = m.size();
+    int64_t total = N;
+    for (int a = 0; a < 3; ++a) total += (int64_t)m.R(a).size();
+    std::fill(err, err + total, 0.0);
+    for (int64_t i : m.fluid()) err[i] = std::abs(u[i] - sol[i]);
+    int64_t off = N;
+    for (int a = 0; a < 3; ++a) {
+        const auto& R = m.R(a);
+        for (size_t i = 0; i < R.size(); ++i)
+            if (p->obj[R[i].shape_id] != bcs::type::Dirichlet)
+                err[off + (int64_t)i] = std::abs(u[off + (int64_t)i] - sol[off + (int64_t)i]);
+        off += (int64_t)R.size();
+    }
+    // Dirichlet grid faces carry no error
+    const int3& n = m.extents();
+    for (int a = 0; a < 3; ++a) {
+        const bcs::type side[2] = {p->grid[a].left, p->grid[a].right};
+        for (int s = 0; s < 2; ++s) {
+            if (side[s] != bcs::type::Dirichlet) continue;
+            const int plane = s == 0 ? 0 : n[a] - 1;
+            for (int i = 0; i < n[0]; ++i)
+                for (int j = 0; j < n[1]; ++j)
+                    for (int k = 0; k < n[2]; ++k) {
+                        const int q[3] = {i, j, k};
+                        if (q[a] == plane) err[m.ic({i, j, k})] = 0.0;
+                    }
+        }
+    }
+    return SHX_OK;
 }
 
 int shoccs_mesh_r(const shoccs_problem* p, int dir, double* psi, double* pos, int32_t* solid,
@@ -371,6 +418,31 @@ int shoccs_run(shoccs_problem* p, const shoccs_run_desc* d, shx_field* u, shoccs
     if (rc == SHX_OK) rc = shx_system_update_boundary(sys, u, 0.0);
     if (rc == SHX_OK) rc = shx_system_stats(sys, &u1, 0.0, stats);
     if (rc == SHX_OK) record();
+    // system::write (scalar_system_utils.hpp:224-301) through the field
+    // writer: U and its Error against the exact solution
+    std::vector<double> hu, hs, he;
+    shx_field ex{};
+    auto write = [&](const shx_field* f, int32_t k, double time, double dt) {
+        if (!d->io) return SHX_OK;
+        int r2 = SHX_OK;
+        if (!ex.D) {
+            r2 = shoccs_field_alloc(p, &ex);
+            int64_t total = shoccs_problem_size(p, 0);
+            for (int a = 1; a <= 3; ++a) total += shoccs_problem_size(p, a);
+            hu.resize((size_t)total);
+            hs.resize((size_t)total);
+            he.resize((size_t)total);
+        }
+        if (r2 == SHX_OK) r2 = shx_system_exact(sys, &ex, time);
+        if (r2 == SHX_OK) r2 = shoccs_field_download(p, f, hu.data());
+        if (r2 == SHX_OK) r2 = shoccs_field_download(p, &ex, hs.data());
+        if (r2 == SHX_OK) r2 = shoccs_error_field(p, hu.data(), hs.data(), he.data());
+        const char* names[2] = {"U", "Error"};
+        const double* fields[2] = {hu.data(), he.data()};
+        if (r2 == SHX_OK) r2 = shoccs_io_write(d->io, 2, names, fields, k, time, dt, nullptr);
+        return r2;
+    };
+    if (rc == SHX_OK) rc = write(u, 0, 0.0, 0.0);
     bool dt_ok = true;
     while (rc == SHX_OK && running() && valid()) {
         const double dt = shoccs_problem_dt(p, d->h_cfl, d->p_cfl);
@@ -391,10 +463,12 @@ int shoccs_run(shoccs_problem* p, const shoccs_run_desc* d, shx_field* u, shoccs
         step += 1;
         rc = shx_system_stats(sys, fused ? u : &u1, t, stats);
         if (rc == SHX_OK) record();
+        if (rc == SHX_OK) rc = write(fused ? u : &u1, step, t, dt);
         if (rc == SHX_OK && !fused) rc = copy_field(ctx, p, u, &u1);  // u0 <- u1
     }
     if (rc == SHX_OK) rc = shx_sync(ctx);
     shoccs_field_free(p, &u1);
+    if (ex.D) shoccs_field_free(p, &ex);
     if (rk.D) shoccs_field_free(p, &rk);
     if (srhs.D) shoccs_field_free(p, &srhs);
     if (rc != SHX_OK) return rc;

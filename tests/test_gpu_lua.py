@@ -71,3 +71,45 @@ def test_lua_configured_run_matches_oracle(ctx, text, steps):
     err = np.max(np.abs(got - want)) / np.max(np.abs(want))
     assert err <= 1e-11, f"relative max-norm error {err:.3e}"
     prob.close()
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_configured_run_writes_fields(ctx, tmp_path, fused):
+    """The io table of a configuration (field_io, src/io/field_io.cpp:32-65):
+    dumps at step 0 and every second step, U as the run left it, Error =
+    |U - exact| by write_scalar_error, geometry and XDMF alongside."""
+    import xml.etree.ElementTree as ET
+
+    import shoccs_amd as S
+
+    text = RADIAL_2D % (', form = "fused"' if fused else "")
+    text = text.replace("step_controller = { max_step = 25,", "step_controller = { max_step = 5,")
+    text = text.replace("\n}\n", f"\n    io = {{ write_every_step = 2, dir = '{tmp_path / 'io'}' }},\n}}\n")
+    cfg = S.Config(text)
+    assert cfg.io["write_every_step"] == 2
+    prob = cfg.problem(ctx)
+    u = prob.field()
+    io = S.FieldIO(prob, **cfg.io)
+    res = prob.run(u, io=io, **cfg.run_kwargs())
+    io.close()
+    assert res["step"] == 5
+    d = tmp_path / "io"
+    assert sorted(p.name for p in d.iterdir() if p.name.startswith("U.")) == ["U.000000", "U.000001", "U.000002"]
+    # dump 0: the initialized field with its boundary values
+    u0 = prob.field()
+    prob.initialize(u0, 0.0)
+    prob.update_boundary(u0, 0.0)
+    np.testing.assert_array_equal(np.fromfile(d / "U.000000"), u0.download())
+    # dump 2 is step 4: rerun 4 steps and compare U and its Error
+    u4 = prob.field()
+    res4 = prob.run(u4, **dict(cfg.run_kwargs(), max_step=4))
+    U = np.fromfile(d / "U.000002")
+    np.testing.assert_array_equal(U, u4.download())
+    ex = prob.field()
+    prob.exact(ex, res4["time"])
+    np.testing.assert_array_equal(np.fromfile(d / "Error.000002"), prob.error_field(U, ex.download()))
+    root = ET.fromstring((d / "view.xmf").read_text().split("\n", 2)[2])
+    grids = root.find("Domain").find("Grid").findall("Grid")
+    assert [g.get("Name") for g in grids] == ["0", "1", "2"]
+    assert float(grids[2].find("Time").get("Value")) == res4["time"]
+    prob.close()
