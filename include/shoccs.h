@@ -48,10 +48,10 @@ typedef struct shoccs_problem_desc {
     int32_t nparams;
 } shoccs_problem_desc;
 
-/* ctx == NULL builds the host structures only.  The system (heat or scalar
- * wave) is built for Dirichlet bodies on grids without Neumann faces; other
- * problems hold their derivative operators (with the cut-point rows of
- * floating and Neumann bodies) and shoccs_problem_system returns NULL. */
+/* ctx == NULL builds the host structures only.  Problems with floating or
+ * Neumann bodies, or Neumann grid faces, step with the reference schedule
+ * (SHOCCS_RK4, SHOCCS_EULER); their fused form, batches and adjoint return
+ * SHX_EUNSUPPORTED. */
 int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_problem** out);
 int shoccs_problem_destroy(shoccs_problem* p);
 shx_system* shoccs_problem_system(shoccs_problem* p);

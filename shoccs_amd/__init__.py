@@ -380,12 +380,8 @@ class Problem:
     # ---------------------------------------------------------- compute
     @property
     def sys(self):
-        """The device system (shx_system*): built for Dirichlet bodies on grids
-        without Neumann faces; other problems have operators only."""
+        """The device system (shx_system*)."""
         self._need_device()
-        if not self._sys:
-            raise ShoccsError("system: only Dirichlet bodies and non-Neumann grid faces are "
-                              "supported (this problem has its derivative operators only)")
         return self._sys
 
     def _need_device(self):

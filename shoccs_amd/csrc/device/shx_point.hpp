@@ -79,6 +79,32 @@ static __device__ __noinline__ double mms_lap(int kind, int dims, int n,
     return sol;
 }
 
+// Gradient component d of the manufactured solution (the Neumann face data of
+// heat.cpp:318-346): the sin product's closed form, and gauss3d::gradient
+// (src/mms/gauss3d.cpp:45-75), each term in the reference's order.
+static __device__ __noinline__ double mms_grad(int kind, int dims, int n, const double* __restrict__ p,
+                                               double t, double x, double y, double z, int d)
+{
+    if (kind == SHX_MMS_SINPROD) {
+        const double sx = sin(kTwoPi * x), sy = sin(kTwoPi * y);
+        const double cx = cos(kTwoPi * x), cy = cos(kTwoPi * y);
+        const double sz = dims == 3 ? sin(kTwoPi * z) : 1.0;
+        const double cz = dims == 3 ? cos(kTwoPi * z) : 0.0;
+        const double e = exp(-t);
+        if (d == 0) return kTwoPi * cx * sy * sz * e;
+        if (d == 1) return kTwoPi * sx * cy * sz * e;
+        return dims == 3 ? kTwoPi * sx * sy * cz * e : 0.0;
+    }
+    const double loc[3] = {x, y, z};
+    double sol = 0;
+    for (int i = 0; i < n; ++i) {
+        const double* g = p + 8 * i;
+        const double base = -exp(g3_expo(g, x, y, z)) * g[6] * cos(t * g[7]);
+        sol += base * (loc[d] - g[d]) * pow(g[3 + d], -2);
+    }
+    return sol;
+}
+
 // heat source S_t - k lap S (heat.cpp:111-117)
 __device__ __forceinline__ double heat_source(const DevSys& s, double t, double x, double y, double z)
 {

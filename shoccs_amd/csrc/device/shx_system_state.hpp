@@ -28,6 +28,14 @@ struct shx_system {
     long nfluid = 0;
     int* faces = nullptr;
     long nfaces = 0;
+    // Floating/Neumann bodies or Neumann grid faces: the reference schedule
+    // only (cut-point rows, R sources, Neumann data); fused forms refuse
+    bool general = false;
+    bool any_free_r = false;      // some cut point of a non-Dirichlet body
+    double* neuD = nullptr;       // heat: Neumann face data (N values)
+    int* neu_faces = nullptr;     // points of the Neumann faces, face by face
+    long neu_off[7] = {0, 0, 0, 0, 0, 0, 0};  // face f = 2 dir + side: [neu_off[f], neu_off[f+1])
+    double* exR[3] = {nullptr, nullptr, nullptr};  // exact values at the cut points (statistics)
     double* rpos[3] = {nullptr, nullptr, nullptr};
     signed char* rdir[3] = {nullptr, nullptr, nullptr};
     // fused RK4 registers (stage ping-pong and accumulator), allocated lazily

@@ -124,6 +124,20 @@ __global__ void k_gG_D(DevSys s, long N, double* __restrict__ gx, double* __rest
     gz[idx] = f ? 0.0 : wave_neg_g(s, 2, x, y, z);
 }
 
+// scalar_wave.cpp:70-95 on the cut points: -g at those of non-Dirichlet
+// bodies, zero at Dirichlet ones
+__global__ void k_gG_R(DevSys s, long n, const double* __restrict__ pos, const signed char* __restrict__ dirich,
+                       double* __restrict__ gx, double* __restrict__ gy, double* __restrict__ gz)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    const bool f = dirich[i] != 0;
+    gx[i] = f ? 0.0 : wave_neg_g(s, 0, x, y, z);
+    gy[i] = f ? 0.0 : wave_neg_g(s, 1, x, y, z);
+    gz[i] = f ? 0.0 : wave_neg_g(s, 2, x, y, z);
+}
+
 // scalar_wave.cpp:294-317, one buffer
 __global__ void k_dot3(long n, const double* __restrict__ gx, const double* __restrict__ dx,
                        const double* __restrict__ gy, const double* __restrict__ dy,
@@ -164,6 +178,35 @@ __global__ void k_gather_fill(long n, const int* __restrict__ sel, double* __res
 {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) dst[sel[q]] = v;
+}
+
+// heat.cpp:140-146: the source at the cut points of non-Dirichlet bodies
+__global__ void k_src_R_add(DevSys s, long n, const double* __restrict__ pos,
+                            const signed char* __restrict__ dirich, double* __restrict__ R, double t)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || dirich[i]) return;
+    R[i] += heat_source(s, t, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+}
+
+// heat.cpp:154-158: the cut points of Dirichlet bodies take zero
+__global__ void k_zero_dirichlet(long n, const signed char* __restrict__ dirich, double* __restrict__ R)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && dirich[i]) R[i] = 0.0;
+}
+
+// heat.cpp:318-346: the gradient component dir of the solution on one
+// Neumann face
+__global__ void k_neumann_data(DevSys s, long n, const int* __restrict__ sel, double* __restrict__ neu,
+                               int dir, double t)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long idx = sel[q];
+    int i, j, k;
+    point_coords(idx, s.n1, s.n2, i, j, k);
+    neu[idx] = mms_grad(s.mms_kind, s.mms_dims, s.mms_n, s.mms, t, s.x[i], s.y[j], s.z[k], dir);
 }
 
 // slot_ops.hpp:23-59
@@ -362,9 +405,13 @@ int ensure_unfused(shx_system* s)
         if (rc != SHX_OK) return rc;
         k_gG_D<<<grid_for(s->N), 256, 0, s->ctx->stream>>>(s->dev, s->N, s->gG[0][0], s->gG[1][0],
                                                           s->gG[2][0]);
-        // gG on cut points: every body is Dirichlet, so the reference zeroes them
-        for (int c = 0; c < 3; ++c)
-            for (int r = 0; r < 3; ++r) rc = rc == SHX_OK ? launch_fill(s->ctx, s->gG[c][r + 1], 0.0, s->nR[r]) : rc;
+        // gG on cut points: -g at those of non-Dirichlet bodies, zero at
+        // Dirichlet ones
+        for (int r = 0; r < 3; ++r)
+            if (s->nR[r] > 0)
+                k_gG_R<<<grid_for(s->nR[r]), 256, 0, s->ctx->stream>>>(s->dev, s->nR[r], s->rpos[r], s->rdir[r],
+                                                                      s->gG[0][r + 1], s->gG[1][r + 1],
+                                                                      s->gG[2][r + 1]);
         rc = rc == SHX_OK ? check(s->ctx, "gG kernel") : rc;
     } else {
         rc = alloc_d(s->ctx, &s->srcD, s->N);
@@ -376,8 +423,13 @@ int ensure_unfused(shx_system* s)
     return rc;
 }
 
+int rhs_unfused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double t);
+
 int rhs_fused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double t)
 {
+    // the point-wise form covers Dirichlet bodies on grids without Neumann
+    // faces; other systems take the reference's sequence
+    if (s->general) return rhs_unfused_launch(s, u, rhs, t);
     const Tiling& tl = s->tiling;
     hipStream_t st = s->ctx->stream;
     switch (s->family * 10 + s->width) {
@@ -413,7 +465,12 @@ int rhs_unfused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double
         }
         for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
             if (!s->d[c] || s->n[c] < 2) continue;
-            rc = launch_derivative(s->d[c], u->D, nullptr, s->du[c][0], false, false);
+            // derivative.cpp:479-485: the cut-point rows of non-Dirichlet bodies
+            shx_field duc{};
+            duc.D = s->du[c][0];
+            for (int r = 0; r < 3; ++r) duc.R[r] = s->du[c][r + 1];
+            rc = launch_cut_rows(s->d[c], u, &duc);
+            if (rc == SHX_OK) rc = launch_derivative(s->d[c], u->D, nullptr, s->du[c][0], false, false);
             if (rc == SHX_OK) rc = launch_csr_full(s->Bfull[c], u->R[c], s->du[c][0]);
         }
         if (rc != SHX_OK) return rc;
@@ -431,8 +488,12 @@ int rhs_unfused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double
     for (int r = 0; r < 3 && rc == SHX_OK; ++r) rc = launch_fill(ctx, rhs->R[r], 0.0, s->nR[r]);
     for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
         if (!s->d[c] || s->n[c] < 2) continue;
-        rc = launch_derivative(s->d[c], u->D, nullptr, rhs->D, true, false);
+        // laplacian.cpp:48-57 with derivative.cpp:475-519: cut-point rows,
+        // O and B, then N nu with the Neumann face data
+        rc = launch_cut_rows(s->d[c], u, rhs);
+        if (rc == SHX_OK) rc = launch_derivative(s->d[c], u->D, nullptr, rhs->D, true, false);
         if (rc == SHX_OK) rc = launch_csr_full(s->Bfull[c], u->R[c], rhs->D);
+        if (rc == SHX_OK && s->neuD) rc = launch_neumann(s->d[c], s->neuD, rhs->D);
     }
     if (rc != SHX_OK) return rc;
     k_scale<<<grid_for(s->N), 256, 0, st>>>(s->N, rhs->D, s->dev.k);
@@ -440,17 +501,30 @@ int rhs_unfused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double
         if (s->nR[r] > 0) k_scale<<<grid_for(s->nR[r]), 256, 0, st>>>(s->nR[r], rhs->R[r], s->dev.k);
     k_src_D<<<grid_for(s->N), 256, 0, st>>>(s->dev, s->N, s->srcD, t);
     if (s->nfluid) k_gather_add<<<grid_for(s->nfluid), 256, 0, st>>>(s->nfluid, s->fluid, s->srcD, rhs->D);
+    for (int r = 0; r < 3; ++r)
+        if (s->nR[r] > 0 && s->any_free_r)
+            k_src_R_add<<<grid_for(s->nR[r]), 256, 0, st>>>(s->dev, s->nR[r], s->rpos[r], s->rdir[r], rhs->R[r], t);
     if (s->nfaces) k_gather_fill<<<grid_for(s->nfaces), 256, 0, st>>>(s->nfaces, s->faces, rhs->D, 0.0);
-    rc = check(ctx, "heat source kernels");
-    // every body is Dirichlet: the cut-point rows are filled with zero
-    for (int r = 0; r < 3 && rc == SHX_OK; ++r) rc = launch_fill(ctx, rhs->R[r], 0.0, s->nR[r]);
-    return rc;
+    for (int r = 0; r < 3; ++r)
+        if (s->nR[r] > 0) k_zero_dirichlet<<<grid_for(s->nR[r]), 256, 0, st>>>(s->nR[r], s->rdir[r], rhs->R[r]);
+    return check(ctx, "heat source kernels");
 }
 
 int update_boundary_launch(shx_system* s, shx_field* u, double t)
 {
     int rc = face_exact(s, u->D, t);
     if (rc == SHX_OK) rc = r_exact_all(s, u->R, t);
+    // heat.cpp:318-346: Neumann face data, face by face in direction order
+    // (an edge point shared by two faces keeps the later direction's value)
+    if (rc == SHX_OK && s->neuD) {
+        for (int f = 0; f < 6; ++f) {
+            const long n = s->neu_off[f + 1] - s->neu_off[f];
+            if (n > 0)
+                k_neumann_data<<<grid_for(n), 256, 0, s->ctx->stream>>>(s->dev, n, s->neu_faces + s->neu_off[f],
+                                                                       s->neuD, f / 2, t);
+        }
+        rc = check(s->ctx, "neumann data kernel");
+    }
     return rc;
 }
 
@@ -872,6 +946,10 @@ int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const dou
 // point-wise form's cut-point update runs outside the brackets).
 int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t* ev)
 {
+    if (s->general)
+        return fail(s->ctx, SHX_EUNSUPPORTED,
+                    "fused form: floating/Neumann bodies and Neumann grid faces take the reference schedule "
+                    "(shx_rk4_step)");
     const bool march = s->variant >= 1;
     const bool mixed = s->precision == SHX_F32_COMP;
     if (mixed && !march)
@@ -918,19 +996,18 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
     *out = nullptr;
     if (desc->kind < SHX_SYS_WAVE_RADIAL || desc->kind > SHX_SYS_HEAT)
         return fail(ctx, SHX_EINVAL, "system: unknown kind");
+    bool free_r = false, neumann = false;
     for (int c = 0; c < 3; ++c)
-        for (long i = 0; i < desc->nR[c]; ++i)
-            if (!desc->r_dirichlet[c][i])
-                return fail(ctx, SHX_EUNSUPPORTED, "system: only Dirichlet bodies are supported");
-    for (int f = 0; f < 6; ++f)
-        if (desc->grid_bc[f] == SHX_BC_NEUMANN)
-            return fail(ctx, SHX_EUNSUPPORTED, "system: Neumann grid faces are not supported");
+        for (long i = 0; i < desc->nR[c]; ++i) free_r = free_r || !desc->r_dirichlet[c][i];
+    for (int f = 0; f < 6; ++f) neumann = neumann || desc->grid_bc[f] == SHX_BC_NEUMANN;
 
     auto* s = new (std::nothrow) shx_system();
     if (!s) return SHX_ENOMEM;
     s->ctx = ctx;
     s->kind = desc->kind;
     s->family = desc->kind == SHX_SYS_HEAT && desc->mms_kind == SHX_MMS_GAUSS ? kHeatGauss : desc->kind;
+    s->general = free_r || neumann;
+    s->any_free_r = free_r;
     for (int c = 0; c < 3; ++c) {
         s->n[c] = desc->n[c];
         s->nR[c] = desc->nR[c];
@@ -1000,6 +1077,28 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
         }
     s->nfaces = (long)faces.size();
     if (rc == SHX_OK) rc = upload(ctx, &s->faces, faces.data(), faces.size());
+    // heat with Neumann faces: the points of each Neumann face and the data
+    // field N nu reads (zero until update_boundary fills it)
+    if (rc == SHX_OK && neumann && desc->kind == SHX_SYS_HEAT) {
+        std::vector<int> nf;
+        for (int f = 0; f < 6; ++f) {
+            s->neu_off[f] = (long)nf.size();
+            if (desc->grid_bc[f] != SHX_BC_NEUMANN) continue;
+            const int dir = f / 2, p = f % 2 ? s->n[dir] - 1 : 0;
+            for (int i = 0; i < nx; ++i)
+                for (int j = 0; j < ny; ++j)
+                    for (int k = 0; k < nz; ++k) {
+                        const int q[3] = {i, j, k};
+                        if (q[dir] == p) nf.push_back((int)(((long)i * ny + j) * nz + k));
+                    }
+        }
+        s->neu_off[6] = (long)nf.size();
+        rc = upload(ctx, &s->neu_faces, nf.data(), nf.size());
+        if (rc == SHX_OK) rc = alloc_d(ctx, &s->neuD, s->N);
+        if (rc == SHX_OK) rc = launch_fill(ctx, s->neuD, 0.0, s->N);
+    }
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c)
+        if (free_r && s->nR[c] > 0) rc = alloc_d(ctx, &s->exR[c], s->nR[c]);
     for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
         rc = upload(ctx, &s->rpos[c], desc->r_pos[c], (size_t)(3 * s->nR[c]));
         std::vector<signed char> fl8(s->nR[c]);
@@ -1055,6 +1154,7 @@ int shx_system_destroy(shx_system* s)
         hipFree(s->sintab[c]);
         hipFree(s->rpos[c]);
         hipFree(s->rdir[c]);
+        hipFree(s->exR[c]);
         for (int b = 0; b < 2; ++b) hipFree(s->stageR[b][c]);
         for (int r = 0; r < 4; ++r) {
             hipFree(s->gG[c][r]);
@@ -1072,6 +1172,8 @@ int shx_system_destroy(shx_system* s)
     hipFree(s->special);
     for (int c = 0; c < 3; ++c) hipFree(s->srows[c]);
     hipFree(s->crows);
+    hipFree(s->neuD);
+    hipFree(s->neu_faces);
     hipFree(s->srcD);
     hipFree(s->mms);
     hipFree(s->fluid_mask);
@@ -1169,6 +1271,53 @@ int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[1
         stats[2] = mx;
         stats[3] = err;
         stats[4] = (double)ei;
+    }
+    // the cut points of non-Dirichlet bodies (scalar_system_utils.hpp:120-169):
+    // their values join min and max, their errors give Rx/Ry/Rz Linf and
+    // index (first strict maximum), and the overall Linf; few points, so on
+    // the host
+    if (s->any_free_r) {
+        if (s->nfluid == 0) stats[1] = stats[2] = stats[3] = 0.0;
+        double mn = s->nfluid ? stats[1] : 0.0, mx = s->nfluid ? stats[2] : 0.0;
+        double err = stats[3];
+        for (int c = 0; c < 3; ++c) {
+            const long n = s->nR[c];
+            if (n == 0) continue;
+            k_exact_R<<<grid_for(n), 256, 0, s->ctx->stream>>>(s->dev, n, s->rpos[c], s->exR[c], t);
+            int rc = check(s->ctx, "exact kernel");
+            std::vector<double> hu(n), hx(n);
+            std::vector<signed char> fl(n);
+            hipError_t e = hipMemcpyAsync(hu.data(), u->R[c], n * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(hx.data(), s->exR[c], n * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(fl.data(), s->rdir[c], n, hipMemcpyDeviceToHost, s->ctx->stream);
+            if (e != hipSuccess) return hip_fail(s->ctx, e, "stats copy");
+            if (rc == SHX_OK) rc = sync(s->ctx);
+            if (rc != SHX_OK) return rc;
+            double best = -DBL_MAX;
+            long bi = 2147483647L;
+            bool any = false;
+            for (long i = 0; i < n; ++i) {
+                if (fl[i]) continue;
+                any = true;
+                if (hu[i] < mn) mn = hu[i];
+                if (hu[i] > mx) mx = hu[i];
+                const double ev = std::abs(hu[i] - hx[i]);
+                if (ev > best) {
+                    best = ev;
+                    bi = i;
+                }
+            }
+            if (any) {
+                stats[5 + 2 * c] = best;
+                stats[6 + 2 * c] = (double)bi;
+                if (best > err) err = best;
+            }
+        }
+        stats[0] = err;
+        stats[1] = mn;
+        stats[2] = mx;
     }
     return SHX_OK;
 }
@@ -1447,6 +1596,9 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
     if (!ctx || !systems || !u0 || !t0 || !dt || n < 1 || !out) return SHX_EINVAL;
     if (!(march_ty == 4 || march_ty == 8)) return SHX_EINVAL;
     *out = nullptr;
+    for (int p = 0; p < n; ++p)
+        if (!systems[p] || systems[p]->general)
+            return fail(ctx, SHX_EUNSUPPORTED, "batch: every problem needs Dirichlet bodies on grids without Neumann faces");
     auto b = std::unique_ptr<shx_batch>(new (std::nothrow) shx_batch());
     if (!b) return SHX_ENOMEM;
     b->ctx = ctx;

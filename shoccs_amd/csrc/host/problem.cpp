@@ -127,13 +127,8 @@ int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_
             sd.mms = mms.data();
         }
     }
-    // A NULL context builds the host structures only (no device, no system).
-    // The systems step Dirichlet bodies on non-Neumann grids; other problems
-    // get their operators (cut-point rows included) and no system.
-    bool system_supported = true;
-    for (bcs::type b : p->obj) system_supported = system_supported && b == bcs::type::Dirichlet;
-    for (int f = 0; f < 6; ++f) system_supported = system_supported && desc->grid_bc[f] != SHX_BC_NEUMANN;
-    int rc = ctx && system_supported ? shx_system_create(ctx, &sd, &p->sys) : SHX_OK;
+    // a NULL context builds the host structures only (no device, no system)
+    int rc = ctx ? shx_system_create(ctx, &sd, &p->sys) : SHX_OK;
     if (rc != SHX_OK) return rc;
     *out = p.release();
     return SHX_OK;
