@@ -92,6 +92,17 @@ static double g3_expo(const double* g, const double* loc)
              -0.5 * pow((loc[2] - c[2]), 2) * pow(v[2], -2)));
 }
 
+/* the gaussian exponent of the mesh's dimension: gauss1d.cpp:14-16,
+ * gauss2d.cpp:14-19, gauss3d (build_ms_gauss, mms.cpp:16-31) */
+static double g_expo(const orc_system* s, const double* g, const double* loc)
+{
+    const double *c = g, *v = g + 3;
+    if (s->mms_dims == 1) return -0.5 * pow((loc[0] - c[0]), 2) * pow(v[0], -2);
+    if (s->mms_dims == 2)
+        return (-0.5 * pow((loc[0] - c[0]), 2) * pow(v[0], -2) + -0.5 * pow((loc[1] - c[1]), 2) * pow(v[1], -2));
+    return g3_expo(g, loc);
+}
+
 static double mms_call(const orc_system* s, double t, const double* loc)
 {
     if (s->mms_kind == ORC_MMS_SINPROD) {
@@ -102,7 +113,7 @@ static double mms_call(const orc_system* s, double t, const double* loc)
     double sol = 0;
     for (int i = 0; i < s->mms_n; i++) {
         const double* g = s->mms + 8 * i;
-        sol += exp(g3_expo(g, loc)) * g[6] * cos(t * g[7]);
+        sol += exp(g_expo(s, g, loc)) * g[6] * cos(t * g[7]);
     }
     return sol;
 }
@@ -113,7 +124,7 @@ static double mms_ddt(const orc_system* s, double t, const double* loc)
     double sol = 0;
     for (int i = 0; i < s->mms_n; i++) {
         const double* g = s->mms + 8 * i;
-        sol += -exp(g3_expo(g, loc)) * g[6] * g[7] * sin(t * g[7]);
+        sol += -exp(g_expo(s, g, loc)) * g[6] * g[7] * sin(t * g[7]);
     }
     return sol;
 }
@@ -134,8 +145,8 @@ static void mms_grad(const orc_system* s, double t, const double* loc, double* o
     }
     for (int i = 0; i < s->mms_n; i++) {
         const double* g = s->mms + 8 * i;
-        double base = -exp(g3_expo(g, loc)) * g[6] * cos(t * g[7]);
-        for (int d = 0; d < 3; d++) out[d] += base * (loc[d] - g[d]) * pow(g[3 + d], -2);
+        double base = -exp(g_expo(s, g, loc)) * g[6] * cos(t * g[7]);
+        for (int d = 0; d < s->mms_dims; d++) out[d] += base * (loc[d] - g[d]) * pow(g[3 + d], -2);
     }
 }
 
@@ -145,6 +156,32 @@ static double mms_lap(const orc_system* s, double t, const double* loc)
         return -(s->mms_dims * TWO_PI * TWO_PI) * mms_call(s, t, loc);
     double sol = 0;
     const double x = loc[0], y = loc[1], z = loc[2];
+    if (s->mms_dims == 1) { /* gauss1d.cpp:47-60 */
+        for (int i = 0; i < s->mms_n; i++) {
+            const double* c = s->mms + 8 * i;
+            const double* v = c + 3;
+            sol += c[6] * cos(t * c[7]) *
+                   (exp(-0.5 * pow((x - c[0]), 2) * pow(v[0], -2)) * pow((x - c[0]), 2) * pow(v[0], -4) -
+                    exp(-0.5 * pow((x - c[0]), 2) * pow(v[0], -2)) * pow(v[0], -2));
+        }
+        return sol;
+    }
+    if (s->mms_dims == 2) { /* gauss2d.cpp:65-85 */
+        for (int i = 0; i < s->mms_n; i++) {
+            const double* c = s->mms + 8 * i;
+            const double* v = c + 3;
+            sol += exp((-0.5 * pow((x - c[0]), 2) * pow(v[0], -2) + -0.5 * pow((y - c[1]), 2) * pow(v[1], -2))) *
+                   c[6] * cos(t * c[7]) * pow(v[0], -4) * pow(v[1], -4) *
+                   (((pow(y, 2) * pow(v[0], 4) +
+                      (-2 * y * c[1] * pow(v[0], 4) +
+                       (pow(c[1], 2) * pow(v[0], 4) +
+                        (pow(x, 2) * pow(v[1], 4) +
+                         (-2 * x * c[0] * pow(v[1], 4) + pow(c[0], 2) * pow(v[1], 4)))))) -
+                     pow(v[0], 2) * pow(v[1], 4)) -
+                    pow(v[0], 4) * pow(v[1], 2));
+        }
+        return sol;
+    }
     for (int i = 0; i < s->mms_n; i++) {
         const double* c = s->mms + 8 * i;
         const double* v = c + 3;

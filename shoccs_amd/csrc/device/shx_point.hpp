@@ -19,6 +19,16 @@ __device__ __forceinline__ double g3_expo(const double* g, double x, double y, d
             (-0.5 * pow((y - c[1]), 2) * pow(v[1], -2) + -0.5 * pow((z - c[2]), 2) * pow(v[2], -2)));
 }
 
+// The gaussian exponent of the mesh's dimension (build_ms_gauss,
+// src/mms/mms.cpp:16-31: gauss1d.cpp:14-16, gauss2d.cpp:14-19, gauss3d).
+__device__ __forceinline__ double g_expo(int dims, const double* g, double x, double y, double z)
+{
+    const double *c = g, *v = g + 3;
+    if (dims == 1) return -0.5 * pow((x - c[0]), 2) * pow(v[0], -2);
+    if (dims == 2) return (-0.5 * pow((x - c[0]), 2) * pow(v[0], -2) + -0.5 * pow((y - c[1]), 2) * pow(v[1], -2));
+    return g3_expo(g, x, y, z);
+}
+
 // Manufactured solutions; scalar arguments only (called out of line).
 static __device__ __noinline__ double mms_call(int kind, int dims, int n,
                                                const double* __restrict__ p, double t, double x,
@@ -32,7 +42,7 @@ static __device__ __noinline__ double mms_call(int kind, int dims, int n,
     double sol = 0;
     for (int i = 0; i < n; ++i) {
         const double* g = p + 8 * i;
-        sol += exp(g3_expo(g, x, y, z)) * g[6] * cos(t * g[7]);
+        sol += exp(g_expo(dims, g, x, y, z)) * g[6] * cos(t * g[7]);
     }
     return sol;
 }
@@ -45,7 +55,7 @@ static __device__ __noinline__ double mms_ddt(int kind, int dims, int n,
     double sol = 0;
     for (int i = 0; i < n; ++i) {
         const double* g = p + 8 * i;
-        sol += -exp(g3_expo(g, x, y, z)) * g[6] * g[7] * sin(t * g[7]);
+        sol += -exp(g_expo(dims, g, x, y, z)) * g[6] * g[7] * sin(t * g[7]);
     }
     return sol;
 }
@@ -56,6 +66,32 @@ static __device__ __noinline__ double mms_lap(int kind, int dims, int n,
 {
     if (kind == SHX_MMS_SINPROD) return -(dims * kTwoPi * kTwoPi) * mms_call(kind, dims, n, p, t, x, y, z);
     double sol = 0;
+    if (dims == 1) {  // gauss1d.cpp:47-60
+        for (int i = 0; i < n; ++i) {
+            const double* c = p + 8 * i;
+            const double* v = c + 3;
+            sol += c[6] * cos(t * c[7]) *
+                   (exp(-0.5 * pow((x - c[0]), 2) * pow(v[0], -2)) * pow((x - c[0]), 2) * pow(v[0], -4) -
+                    exp(-0.5 * pow((x - c[0]), 2) * pow(v[0], -2)) * pow(v[0], -2));
+        }
+        return sol;
+    }
+    if (dims == 2) {  // gauss2d.cpp:65-85
+        for (int i = 0; i < n; ++i) {
+            const double* c = p + 8 * i;
+            const double* v = c + 3;
+            sol += exp((-0.5 * pow((x - c[0]), 2) * pow(v[0], -2) + -0.5 * pow((y - c[1]), 2) * pow(v[1], -2))) *
+                   c[6] * cos(t * c[7]) * pow(v[0], -4) * pow(v[1], -4) *
+                   (((pow(y, 2) * pow(v[0], 4) +
+                      (-2 * y * c[1] * pow(v[0], 4) +
+                       (pow(c[1], 2) * pow(v[0], 4) +
+                        (pow(x, 2) * pow(v[1], 4) +
+                         (-2 * x * c[0] * pow(v[1], 4) + pow(c[0], 2) * pow(v[1], 4)))))) -
+                     pow(v[0], 2) * pow(v[1], 4)) -
+                    pow(v[0], 4) * pow(v[1], 2));
+        }
+        return sol;
+    }
     for (int i = 0; i < n; ++i) {
         const double* c = p + 8 * i;
         const double* v = c + 3;
@@ -80,8 +116,9 @@ static __device__ __noinline__ double mms_lap(int kind, int dims, int n,
 }
 
 // Gradient component d of the manufactured solution (the Neumann face data of
-// heat.cpp:318-346): the sin product's closed form, and gauss3d::gradient
-// (src/mms/gauss3d.cpp:45-75), each term in the reference's order.
+// heat.cpp:318-346): the sin product's closed form, and the gaussians'
+// gradient (src/mms/gauss3d.cpp:45-75, gauss2d.cpp:37-57, gauss1d.cpp:30-40),
+// each term in the reference's order.
 static __device__ __noinline__ double mms_grad(int kind, int dims, int n, const double* __restrict__ p,
                                                double t, double x, double y, double z, int d)
 {
@@ -95,11 +132,12 @@ static __device__ __noinline__ double mms_grad(int kind, int dims, int n, const 
         if (d == 1) return kTwoPi * sx * cy * sz * e;
         return dims == 3 ? kTwoPi * sx * sy * cz * e : 0.0;
     }
+    if (d >= dims) return 0.0;  // gauss1d/gauss2d leave the other components zero
     const double loc[3] = {x, y, z};
     double sol = 0;
     for (int i = 0; i < n; ++i) {
         const double* g = p + 8 * i;
-        const double base = -exp(g3_expo(g, x, y, z)) * g[6] * cos(t * g[7]);
+        const double base = -exp(g_expo(dims, g, x, y, z)) * g[6] * cos(t * g[7]);
         sol += base * (loc[d] - g[d]) * pow(g[3 + d], -2);
     }
     return sol;

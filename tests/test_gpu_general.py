@@ -27,6 +27,9 @@ CFGS = {
     "heat_gauss_floating": _heat((20, 20, 20), [[0.5, 0.5, 0.5, 0.2]], [F], [N, D, D, N, D, D],
                                  params=(0.7, O.MMS_GAUSS, 1, *GAUSS)),
     "heat2d_floating": _heat((31, 33, 1), [[0.5, 0.52, 0.0, 0.23]], [F], [D, N, N, D, F, F]),
+    "heat2d_gauss_floating": _heat((33, 31, 1), [[0.5, 0.5, 0.0, 0.2]], [F], [N, D, D, D, F, F],
+                                   params=(0.8, O.MMS_GAUSS, 2, 0.45, 0.55, 0.0, 0.2, 0.25, 0.0, 1.3, 2.5,
+                                           0.6, 0.3, 0.0, 0.3, 0.2, 0.0, -0.7, 1.0)),
     "wave_floating_E4u": dict(H.sphere_wave_cfg(24, scheme=O.E4U_1, alpha=(-0.7733323791884821, 0.1623961700641681)),
                               obj_bc=[F]),
 }
@@ -128,7 +131,8 @@ def test_rk4_and_euler_steps_match_oracle(ctx, name):
     prob.close()
 
 
-@pytest.mark.parametrize("name", ["heat_floating_neumann", "heat2d_floating", "wave_floating_E4u"])
+@pytest.mark.parametrize("name", ["heat_floating_neumann", "heat2d_floating", "heat2d_gauss_floating",
+                                  "wave_floating_E4u"])
 def test_driver_matches_oracle_cycle(ctx, name):
     """shoccs_run with the reference schedule for 8 steps against the
     oracle's cycle (initialize, update_boundary, rk4 steps): 1e-11."""

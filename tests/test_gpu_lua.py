@@ -35,6 +35,25 @@ simulation = {
 """
 
 
+HEAT_FLOATING_2D = """
+-- 2-D heat with a floating circle, two gaussians, a Neumann face (the kind of
+-- problem the reference's heat.lua sets up)
+local dx = 1 / 40
+simulation = {
+    mesh = { index_extents = {41, 41}, domain_bounds = {1, 1} },
+    domain_boundaries = { xmin = "neumann", xmax = "dirichlet", ymin = "dirichlet", ymax = "dirichlet" },
+    shapes = { { type = "sphere", center = {0.52, 0.5}, radius = 0.2, boundary_condition = "floating" } },
+    scheme = { order = 2, type = "E4" },
+    system = { type = "heat", diffusivity = 1 / 30 },
+    manufactured_solution = { type = "gaussian",
+        { center = {0.4, 0.6}, variance = {0.25, 0.2}, amplitude = 1.5, frequency = 2 },
+        { center = {0.7, 0.3}, variance = {0.3, 0.35}, amplitude = -10 * dx, frequency = 0.5 } },
+    integrator = { type = "rk4" },
+    step_controller = { max_step = 12, cfl = { parabolic = 0.5 } },
+}
+"""
+
+
 def _oracle_cycle(cfg, nsteps, h_cfl, p_cfl):
     st = O.stencil(cfg.scheme, cfg.alpha)
     obj_bc = cfg.shape_bc
@@ -56,7 +75,7 @@ def _oracle_cycle(cfg, nsteps, h_cfl, p_cfl):
 
 
 @pytest.mark.parametrize("text,steps", [(RADIAL_2D % "", 25), (RADIAL_2D % ', form = "fused"', 25),
-                                        (HEAT_2D, 20)])
+                                        (HEAT_2D, 20), (HEAT_FLOATING_2D, 12)])
 def test_lua_configured_run_matches_oracle(ctx, text, steps):
     import shoccs_amd as S
 

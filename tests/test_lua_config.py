@@ -131,8 +131,9 @@ def test_cli_check(tmp_path):
 @pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "lua-configs")), reason="reference tree absent")
 def test_reference_configs():
     """The reference's own configuration files: the 2-D radial-wave configs
-    read completely; the others are refused with the reason (a template
-    placeholder, the E2-poly scheme, a floating body)."""
+    and heat.lua (floating circle, gaussian solution) read completely; the
+    others are refused with the reason (a template placeholder, the E2-poly
+    scheme)."""
     import shoccs_amd as S
 
     c = S.Config.from_file(os.path.join(REF, "lua-configs", "brady_livescu_4_3_n61.lua"))
@@ -152,3 +153,6 @@ def test_reference_configs():
     h = S.Config.from_file(os.path.join(REF, "heat.lua"))
     assert h.kind == S.SYS_HEAT and h.shape_bc == [S.BC_FLOATING]
     assert h.params[:3] == [1 / 30, S.MMS_GAUSS, 1]  # the commented-out gaussians are skipped
+    p = h.problem(None)  # a floating circle: cut-point rows in every direction
+    assert sum(p.nR) > 0 and all(len(p.cut_rows(d)["rw"]) > 0 for d in range(2))
+    p.close()

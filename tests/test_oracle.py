@@ -576,3 +576,59 @@ def test_general_heat_rhs_converges():
         errs.append((e_fluid, max(free)))
     assert errs[1][0] < 0.5 * errs[0][0], errs
     assert errs[1][1] < 0.6 * errs[0][1], errs
+
+
+@pytest.mark.parametrize("dims", [1, 2])
+def test_gaussian_solutions_of_lower_dimension(dims):
+    """gauss1d.cpp / gauss2d.cpp, chosen by the mesh's dimension
+    (mms.cpp:16-31; the configured z (and y) variance is 0 there, where the
+    3-D form would divide by it).  With diffusivity 1 the rhs of the exact
+    solution approaches its time derivative under refinement (laplacian
+    against the solution, gradient through the Neumann data); with
+    diffusivity 0 the rhs is the time derivative, checked against a central
+    difference in time."""
+    F, N, D = O.FLOATING, O.NEUMANN, O.DIRICHLET
+    g = [0.45, 0.55, 0.0, 0.2, 0.25, 0.0, 1.3, 2.5]
+    if dims == 1:
+        g[1] = g[4] = 0.0
+
+    def cfg(n, k):
+        if dims == 1:
+            return dict(kind=O.SYS_HEAT, n=(n, 1, 1), lo=(0, 0, 0), hi=(1, 0, 0), spheres=[], obj_bc=[],
+                        grid_bc=[N, D, F, F, F, F], scheme=O.E4_2, alpha=(0.0, 0.0), params=[k, O.MMS_GAUSS, 1] + g)
+        # one Neumann face: where two meet, the reference's shared data field
+        # holds the later direction's gradient (heat.cpp:318-346), which the
+        # other direction's N row then reads, so that edge does not converge
+        return dict(kind=O.SYS_HEAT, n=(n, n, 1), lo=(0, 0, 0), hi=(1, 1, 0), spheres=[[0.5, 0.48, 0.0, 0.21]],
+                    obj_bc=[F], grid_bc=[N, D, D, F, F, F], scheme=O.E4_2, alpha=(0.0, 0.0),
+                    params=[k, O.MMS_GAUSS, 1] + g)
+
+    t = 0.35
+    errs = []
+    for n in (41, 81):
+        osys = H.oracle_system(cfg(n, 1.0))
+        u = osys.exact(t)
+        assert np.all(np.isfinite(u.flat()))
+        osys.update_boundary(u, t)
+        r = osys.rhs(u, t).flat()
+        h = 1e-5
+        ut = (osys.exact(t + h).flat() - osys.exact(t - h).flat()) / (2 * h)
+        # fluid points off the Dirichlet faces (the rhs is zero there by design)
+        n3 = (n, 1, 1) if dims == 1 else (n, n, 1)
+        ii = np.unravel_index(osys.mesh.fluid(), n3)
+        on_face = (ii[0] == n - 1) if dims == 1 else ((ii[0] == n - 1) | (ii[1] == 0))
+        f = osys.mesh.fluid()[~on_face]
+        free = np.arange(osys.mesh.N, r.size)
+        errs.append(max(np.max(np.abs(r[f] - ut[f])), np.max(np.abs(r[free] - ut[free])) if free.size else 0.0))
+    assert errs[1] < 0.6 * errs[0], errs
+    # diffusivity 0: the source alone, S_t
+    osys = H.oracle_system(cfg(41, 0.0))
+    u = osys.exact(t)
+    r = osys.rhs(u, t).flat()
+    h = 1e-5
+    ut = (osys.exact(t + h).flat() - osys.exact(t - h).flat()) / (2 * h)
+    n3 = (41, 1, 1) if dims == 1 else (41, 41, 1)
+    ii = np.unravel_index(osys.mesh.fluid(), n3)
+    on_face = (ii[0] == 40) if dims == 1 else ((ii[0] == 40) | (ii[1] == 0))
+    f = osys.mesh.fluid()[~on_face]
+    assert np.max(np.abs(r[f] - ut[f])) <= 1e-7 * np.max(np.abs(ut[f]))
