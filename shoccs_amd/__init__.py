@@ -340,6 +340,14 @@ class Problem:
         check(lib().shoccs_deriv_pool(self.p, d, a.ctypes.data_as(C.POINTER(C.c_double))), None, "pool")
         return a[:n]
 
+    def h(self) -> tuple:
+        """Grid spacing per axis, (max - min) / (n - 1) (0 on an axis of one
+        point)."""
+        n = (C.c_int32 * 3)()
+        lo, hi = (C.c_double * 3)(), (C.c_double * 3)()
+        check(lib().shoccs_problem_geometry(self.p, n, lo, hi), None, "geometry")
+        return tuple((hi[a] - lo[a]) / (n[a] - 1) if n[a] > 1 else 0.0 for a in range(3))
+
     def error_field(self, u: np.ndarray, sol: np.ndarray) -> np.ndarray:
         """|u - sol| at the fluid points and the cut points of non-Dirichlet
         bodies, 0 elsewhere and on Dirichlet grid faces (shoccs_error_field)."""

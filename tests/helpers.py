@@ -166,3 +166,28 @@ def oracle_cut_rows(osys, d):
             rptr.append(len(rw))
     return dict(rset=np.array(rset, np.int32), rrow=np.array(rrow, np.int64), rptr=np.array(rptr, np.int64),
                 rcol=np.array(rcol, np.int64), rw=np.array(rw))
+
+
+def oracle_operator_matrix(cfg, d, rows_cols):
+    """Direction d's operator assembled by the oracle (one apply per unit
+    vector) on the flat positions rows_cols (D indices, then N + offset of
+    R(d) indices)."""
+    osys = oracle_system(cfg)
+    m = osys.mesh
+    N = m.N
+    size = N + sum(m.nR(r) for r in range(3))
+    A = np.zeros((rows_cols.size, rows_cols.size))
+    for j, pj in enumerate(rows_cols):
+        flat = np.zeros(size)
+        flat[pj] = 1.0
+        e = osys.field()
+        e.D[:] = flat[:N]
+        o = N
+        for r in range(3):
+            if m.nR(r):
+                e.R[r][:] = flat[o:o + m.nR(r)]
+            o += m.nR(r)
+        du = osys.field()
+        osys.deriv_apply(d, e, du)
+        A[:, j] = du.flat()[rows_cols]
+    return A

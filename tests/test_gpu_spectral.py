@@ -1,0 +1,87 @@
+"""Spectral diagnostics (shoccs_amd.spectral): the dense operator of one
+direction over its unknowns, as the reference's eigenvalue_visitor forms it
+(src/operators/eigenvalue_visitor.cpp:10-52), and the hyperbolic eigenvalue
+statistic (src/systems/hyperbolic_eigenvalues.cpp:78-89)."""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle_ffi as O
+
+pytestmark = pytest.mark.gpu
+
+F, D = O.FLOATING, O.DIRICHLET
+E4U_ALPHA = (-0.7733323791884821, 0.1623961700641681)
+
+
+def _line(n, scheme, grid, alpha=(0.0, 0.0), kind=O.SYS_HEAT, params=(1.0, O.MMS_SINPROD)):
+    return dict(kind=kind, n=(n, 1, 1), lo=(0, 0, 0), hi=(1, 0, 0), spheres=[], obj_bc=[],
+                grid_bc=list(grid) + [F] * 4, scheme=scheme, alpha=alpha, params=list(params))
+
+
+def _positions(prob, d):
+    import shoccs_amd.spectral as SP
+
+    Dk, Rk = SP.unknowns(prob, d)
+    return np.concatenate([Dk, prob.N + sum(prob.nR[:d]) + Rk])
+
+
+def test_e2_dirichlet_eigenvalues_closed_form(ctx):
+    """E2 second derivative with Dirichlet ends on n points: the tridiagonal
+    (1, -2, 1)/h^2 on the n - 2 interior points, eigenvalues
+    -(4/h^2) sin^2(k pi / (2 (n - 1))), k = 1 .. n - 2."""
+    import shoccs_amd.spectral as SP
+
+    n = 21
+    prob = H.product_problem(ctx, _line(n, O.E2_2, [D, D]))
+    lam = SP.eigenvalues(prob, 0)
+    assert lam.size == n - 2 and np.max(np.abs(lam.imag)) == 0.0
+    h = prob.h()[0]
+    want = -(4 / h ** 2) * np.sin(np.arange(1, n - 1) * np.pi / (2 * (n - 1))) ** 2
+    np.testing.assert_allclose(np.sort(lam.real), np.sort(want), rtol=1e-12)
+    prob.close()
+
+
+@pytest.mark.parametrize("case", ["line_E4u", "circle_E4_2_x", "circle_E4_2_y"])
+def test_operator_matrix_matches_oracle(ctx, case):
+    """The device-assembled matrix over the unknowns (mesh rows, then the
+    free cut points of R(dir)) equals the oracle's, bit for bit."""
+    import shoccs_amd.spectral as SP
+
+    if case == "line_E4u":
+        cfg, d = _line(31, O.E4U_1, [D, F], alpha=E4U_ALPHA, kind=O.SYS_WAVE_RADIAL, params=(-0.25, 0, 0, 0)), 0
+    else:
+        cfg = dict(kind=O.SYS_HEAT, n=(17, 15, 1), lo=(0, 0, 0), hi=(1, 1, 0), spheres=[[0.5, 0.52, 0.0, 0.24]],
+                   obj_bc=[F], grid_bc=[D, F, F, D, F, F], scheme=O.E4_2, alpha=(0.0, 0.0),
+                   params=[1.0, O.MMS_SINPROD])
+        d = 0 if case.endswith("x") else 1
+    prob = H.product_problem(ctx, cfg)
+    pos = _positions(prob, d)
+    A = SP.operator_matrix(prob, d)
+    assert A.shape == (pos.size, pos.size)
+    if case != "line_E4u":
+        assert pos.size > prob.N // 2 and pos[-1] >= prob.N  # free cut points are unknowns
+    assert np.array_equal(A, H.oracle_operator_matrix(cfg, d, pos))
+    prob.close()
+
+
+def test_hyperbolic_eigenvalue_statistic(ctx):
+    """-h min Re(eig D_x) for E4u with the reference's scalar-wave closure
+    parameters, Dirichlet inflow, floating outflow: not positive (every
+    eigenvalue of D has a positive real part, so u_t + u_x = 0 is damped),
+    small, shrinking under refinement, and equal to LAPACK on the oracle's
+    matrix."""
+    import shoccs_amd.spectral as SP
+
+    stats = []
+    for n in (21, 41):
+        cfg = _line(n, O.E4U_1, [D, F], alpha=E4U_ALPHA, kind=O.SYS_WAVE_RADIAL, params=(-0.25, 0, 0, 0))
+        prob = H.product_problem(ctx, cfg)
+        st = SP.hyperbolic_eigenvalue_stat(prob)
+        A = H.oracle_operator_matrix(cfg, 0, _positions(prob, 0))
+        want = -prob.h()[0] * np.min(np.linalg.eigvals(A).real)
+        assert st == pytest.approx(want, rel=1e-10)
+        stats.append(st)
+        prob.close()
+    assert stats[0] <= 0.0 and stats[1] <= 0.0
+    assert abs(stats[0]) < 0.05 and abs(stats[1]) < 0.25 * abs(stats[0])
