@@ -154,6 +154,9 @@ typedef struct {
     orc_field neu;
 } orc_system;
 
+/* compressible Euler (euler.c; no reference implementation, see there) */
+typedef struct orc_euler orc_euler;
+
 /* ------------------------------------------------------------------ API */
 void orc_set_threads(int n);
 
@@ -216,4 +219,13 @@ void orc_laplacian_apply(const orc_system* s, const orc_field* u, const orc_fiel
 #ifdef __cplusplus
 }
 #endif
+/* Euler: fields are arrays of five orc_field (rho, rho u, rho v, rho w, E) */
+orc_euler* orc_euler_create(const orc_mesh* m, const orc_deriv* const d[3], double gamma,
+                            const int grid_bc[6]);
+void orc_euler_destroy(orc_euler* e);
+void orc_euler_wall(const orc_euler* e, orc_field* U);
+void orc_euler_rhs(orc_euler* e, const orc_field* U, orc_field* rhs);
+void orc_euler_rk4_step(orc_euler* e, orc_field* u0, orc_field* out, orc_field* rk, orc_field* srhs,
+                        double dt);
+
 #endif

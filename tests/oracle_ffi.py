@@ -166,6 +166,12 @@ def lib():
     L.orc_gradient_apply.argtypes = [C.c_void_p] + [C.POINTER(Field)] * 4
     L.orc_laplacian_apply.argtypes = [C.c_void_p] + [C.POINTER(Field)] * 3
     L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_euler_create.restype = C.c_void_p
+    L.orc_euler_create.argtypes = [C.POINTER(Mesh), C.POINTER(C.POINTER(Deriv)), C.c_double, ip]
+    L.orc_euler_destroy.argtypes = [C.c_void_p]
+    L.orc_euler_wall.argtypes = [C.c_void_p, C.POINTER(Field)]
+    L.orc_euler_rhs.argtypes = [C.c_void_p, C.POINTER(Field), C.POINTER(Field)]
+    L.orc_euler_rk4_step.argtypes = [C.c_void_p] + [C.POINTER(Field)] * 4 + [C.c_double]
     _lib = L
     return L
 
@@ -413,3 +419,45 @@ class OracleSystem:
 
     def deriv_apply(self, d: int, u: OracleField, du: OracleField, plus_eq=False):
         lib().orc_deriv_apply(self.deriv(d), u.ref(), None, du.ref(), int(plus_eq))
+
+
+class OracleEuler:
+    """The compressible Euler system on the CPU (oracle/euler.c): five
+    conserved fields (rho, rho u, rho v, rho w, E) over the mesh and
+    derivative operators of a configuration (bodies Dirichlet in the
+    operators' sense: their cut points carry the slip-wall state)."""
+
+    def __init__(self, cfg, gamma=1.4):
+        L = lib()
+        st = stencil(cfg["scheme"], cfg["alpha"])
+        self.sys = OracleSystem(SYS_HEAT, cfg["n"], cfg["lo"], cfg["hi"], cfg["spheres"],
+                                [DIRICHLET] * len(cfg["spheres"]), cfg["grid_bc"], st, [1.0, MMS_SINPROD])
+        self.mesh = self.sys.mesh
+        ds = (C.POINTER(Deriv) * 3)(*[self.sys.deriv(d) for d in range(3)])
+        gb = np.array(cfg["grid_bc"], dtype=np.intc)
+        self._keep = (ds, gb)
+        self.p = L.orc_euler_create(L.orc_system_mesh(self.sys.p), ds, gamma, _arr(gb, C.c_int))
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            lib().orc_euler_destroy(self.p)
+            self.p = None
+
+    def fields(self):
+        return [self.sys.field() for _ in range(5)]
+
+    @staticmethod
+    def _refs(U):
+        return (Field * 5)(*[u.c for u in U])
+
+    def wall(self, U):
+        lib().orc_euler_wall(self.p, self._refs(U))
+
+    def rhs(self, U):
+        out = self.fields()
+        lib().orc_euler_rhs(self.p, self._refs(U), self._refs(out))
+        return out
+
+    def rk4_step(self, u0, out, rk, srhs, dt):
+        lib().orc_euler_rk4_step(self.p, self._refs(u0), self._refs(out), self._refs(rk), self._refs(srhs), dt)
+
