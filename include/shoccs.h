@@ -181,6 +181,12 @@ int shoccs_io_write(shoccs_io* io, int32_t nvars, const char* const* names, cons
  * 0); the strings belong to the configuration */
 int shoccs_config_io(const shoccs_config* c, shoccs_io_desc* io);
 
+/* The compressible Euler system over a problem's mesh and first-derivative
+ * operators (its bodies Dirichlet): the wall data (each cut point's fluid
+ * neighbour along its ray and unit normal) from the mesh, then
+ * shx_euler_create with the problem's grid boundary conditions. */
+int shoccs_euler_create(shoccs_problem* p, double gamma, shx_euler** out);
+
 /* device fields of the problem's shape: D then Rx, Ry, Rz */
 int shoccs_field_alloc(shoccs_problem* p, shx_field* f);
 int shoccs_field_free(shoccs_problem* p, shx_field* f);

@@ -163,6 +163,32 @@ int shx_gradient_apply(shx_derivative* const d[3], const shx_field* u, shx_field
 int shx_laplacian_apply(shx_derivative* const d[3], const shx_field* u, const shx_field* nu,
                         shx_field* du);
 
+/* ------------------------------------------------------ compressible Euler
+ * No reference counterpart (src/systems/inviscid_vortex.cpp:118-154 is a
+ * stub).  Five conserved fields (rho, rho u, rho v, rho w, E; arrays of five
+ * shx_field below), ideal gas p = (gamma - 1)(E - |m|^2 / 2 rho);
+ * rhs_k = -(D_x F_x,k + D_y F_y,k + D_z F_z,k) with the given first-derivative
+ * operators (bodies Dirichlet in their sense: the cut points carry the wall
+ * state); the slip wall sets each cut point to its fluid neighbour's state
+ * with the normal momentum removed; Dirichlet grid faces hold their state. */
+typedef struct shx_euler shx_euler;
+typedef struct shx_euler_desc {
+    double gamma;
+    int32_t grid_bc[6];
+    const int64_t* wall_fluid[3];  /* host, nR[d]: flat index of each cut point's fluid neighbour */
+    const double* wall_normal[3];  /* host, 3 nR[d]: unit normal at each cut point */
+} shx_euler_desc;
+int shx_euler_create(shx_ctx* ctx, shx_derivative* const d[3], const shx_euler_desc* desc, shx_euler** out);
+int shx_euler_destroy(shx_euler* e);
+int shx_euler_wall(shx_euler* e, shx_field* U);
+int shx_euler_rhs(shx_euler* e, const shx_field* U, shx_field* rhs);
+/* rk4::operator() (src/temporal/rk4.cpp:16-47) over the five fields, the wall
+ * update after each stage input and at the end.  Synchronous. */
+int shx_euler_rk4_step(shx_euler* e, shx_field* u0, shx_field* out, shx_field* rk, shx_field* srhs, double dt);
+/* nsteps steps (u0 <- out after each), mean device time per step */
+int shx_time_euler_steps(shx_euler* e, shx_field* u0, shx_field* out, shx_field* rk, shx_field* srhs, double dt,
+                         int nsteps, double* ms_per_step);
+
 /* ---------------------------------------------------------------- systems */
 #define SHX_SYS_WAVE_RADIAL 0 /* src/systems/scalar_wave.cpp:28-40 */
 #define SHX_SYS_WAVE_LINEAR 1 /* constant velocity, translated gaussian */

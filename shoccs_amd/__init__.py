@@ -30,7 +30,7 @@ from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, E2_2, E4_1, E4_2, E4U_
                    ShxField, ShxLineBlock, check, lib)
 
 __all__ = [
-    "Context", "Problem", "Field", "Batch", "FieldIO", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
+    "Context", "Problem", "Field", "Batch", "FieldIO", "Euler", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
     "E2_2", "E4_2", "E4_1", "E4U_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
     "MMS_GAUSS", "MMS_SINPROD", "F64", "F32_COMP", "RK4", "RK4_FUSED", "EULER", "eq", "plus_eq",
 ]
@@ -622,6 +622,63 @@ class Problem:
         check(lib().shx_time_unfused_steps(self.sys, u0.ref(), out.ref(), rk.ref(), srhs.ref(), t, dt,
                                            nsteps, C.byref(ms)), self.ctx.p, "time_unfused")
         return ms.value
+
+
+class Euler:
+    """Compressible Euler (shx_euler_*) over a problem's mesh and
+    first-derivative operators: five conserved fields (rho, rho u, rho v,
+    rho w, E), slip-wall bodies, Dirichlet grid faces held.  The problem's
+    bodies must be Dirichlet (its operators then carry no cut-point rows)."""
+
+    def __init__(self, problem: "Problem", gamma: float = 1.4):
+        problem._need_device()
+        self.problem = problem
+        self.gamma = gamma
+        self.h = C.c_void_p()
+        check(lib().shoccs_euler_create(problem.p, gamma, C.byref(self.h)), problem.ctx.p, "euler create")
+
+    def fields(self) -> list:
+        return [self.problem.field() for _ in range(5)]
+
+    @staticmethod
+    def _refs(U):
+        return (ShxField * 5)(*[u.c for u in U])
+
+    def upload(self, U, arrays):
+        for u, a in zip(U, arrays):
+            u.upload(a)
+
+    def download(self, U) -> list:
+        return [u.download() for u in U]
+
+    def wall(self, U):
+        check(lib().shx_euler_wall(self.h, self._refs(U)), self.problem.ctx.p, "euler wall")
+
+    def rhs(self, U, out):
+        check(lib().shx_euler_rhs(self.h, self._refs(U), self._refs(out)), self.problem.ctx.p, "euler rhs")
+
+    def rk4_step(self, u0, out, rk, srhs, dt: float):
+        check(lib().shx_euler_rk4_step(self.h, self._refs(u0), self._refs(out), self._refs(rk), self._refs(srhs),
+                                       dt), self.problem.ctx.p, "euler rk4")
+
+    def time_steps(self, u0, out, rk, srhs, dt: float, nsteps: int) -> float:
+        """Mean device milliseconds per step over nsteps steps (u0 <- out
+        after each)."""
+        ms = C.c_double()
+        check(lib().shx_time_euler_steps(self.h, self._refs(u0), self._refs(out), self._refs(rk), self._refs(srhs),
+                                         dt, nsteps, C.byref(ms)), self.problem.ctx.p, "euler time")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().shx_euler_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -186,6 +186,14 @@ _SIGS = [
     ("shoccs_io_write", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_char_p), C.POINTER(dp), C.c_int32,
                                   C.c_double, C.c_double, C.POINTER(C.c_int32)]),
     ("shoccs_error_field", C.c_int, [C.c_void_p, dp, dp, dp]),
+    ("shoccs_euler_create", C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_void_p)]),
+    ("shx_euler_create", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("shx_euler_destroy", C.c_int, [C.c_void_p]),
+    ("shx_euler_wall", C.c_int, [C.c_void_p, C.POINTER(ShxField)]),
+    ("shx_euler_rhs", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField)]),
+    ("shx_euler_rk4_step", C.c_int, [C.c_void_p] + [C.POINTER(ShxField)] * 4 + [C.c_double]),
+    ("shx_time_euler_steps", C.c_int, [C.c_void_p] + [C.POINTER(ShxField)] * 4 +
+     [C.c_double, C.c_int, C.POINTER(C.c_double)]),
     ("shoccs_problem_geometry", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), dp, dp]),
     ("shoccs_config_destroy", C.c_int, [C.c_void_p]),
     ("shx_derivative_apply_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),

@@ -22,7 +22,7 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 
 DEVICE_SOURCES = ["device/shx_core.hip", "device/shx_systems.hip", "device/shx_march.hip",
-                  "device/shx_mixed.hip", "device/shx_adjoint.hip"]
+                  "device/shx_mixed.hip", "device/shx_adjoint.hip", "device/shx_euler.hip"]
 HOST_SOURCES = ["host/mesh.cpp", "host/stencils.cpp", "host/operators.cpp", "host/problem.cpp",
                 "host/lua_config.cpp", "host/field_io.cpp"]
 

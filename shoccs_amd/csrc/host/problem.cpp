@@ -167,6 +167,36 @@ double shoccs_problem_dt(const shoccs_problem* p, double h_cfl, double p_cfl)
     return h_cfl * hmin;
 }
 
+int shoccs_euler_create(shoccs_problem* p, double gamma, shx_euler** out)
+{
+    if (!p || !out || !p->ctx) return SHX_EINVAL;
+    const mesh& m = *p->m;
+    const int3& n = m.extents();
+    const int64_t st[3] = {(int64_t)n[1] * n[2], n[2], 1};
+    std::vector<int64_t> fl[3];
+    std::vector<double> nr[3];
+    shx_euler_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.gamma = gamma;
+    for (int f = 0; f < 6; ++f) d.grid_bc[f] = (int32_t)(f % 2 ? p->grid[f / 2].right : p->grid[f / 2].left);
+    for (int a = 0; a < 3; ++a) {
+        // the fluid side of a cut point: solid + 1 along the ray, or - 1 when
+        // the ray leaves the body (cut_discretization's closure columns,
+        // src/operators/derivative.cpp:189-206)
+        for (const auto& r : m.R(a)) {
+            fl[a].push_back(m.ic(r.solid_coord) + (r.ray_outside ? -st[a] : st[a]));
+            nr[a].insert(nr[a].end(), r.normal.begin(), r.normal.end());
+        }
+        fl[a].push_back(0);
+        nr[a].push_back(0.0);
+        d.wall_fluid[a] = fl[a].data();
+        d.wall_normal[a] = nr[a].data();
+    }
+    shx_derivative* dv[3];
+    for (int a = 0; a < 3; ++a) dv[a] = n[a] > 1 ? p->d[a]->device() : nullptr;
+    return shx_euler_create(p->ctx, dv, &d, out);
+}
+
 int shoccs_problem_geometry(const shoccs_problem* p, int32_t n[3], double lo[3], double hi[3])
 {
     if (!p || !n || !lo || !hi) return SHX_EINVAL;

@@ -1,0 +1,122 @@
+"""The compressible Euler system on the device (shx_euler_*) against the CPU
+oracle (oracle/euler.c): wall update, rhs and RK4 steps, bit for bit (every
+step is plain FP64 arithmetic and the derivative applies are bit-identical)."""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle_ffi as O
+from test_euler_oracle import E4U_ALPHA, euler_cfg, set_state, vortex
+
+pytestmark = pytest.mark.gpu
+
+F, D = O.FLOATING, O.DIRICHLET
+
+CASES = {
+    "sphere3d": euler_cfg((24, 22, 20), (0, 0, 0), (1, 1, 1), spheres=[(0.5, 0.48, 0.52, 0.26)],
+                          grid_bc=[D, F, F, F, F, F]),
+    "sphere3d_noncubic": euler_cfg((30, 18, 26), (0, 0, 0), (1.2, 0.8, 1.0), spheres=[(0.6, 0.4, 0.5, 0.22)],
+                                   grid_bc=[D, F, D, F, F, F]),
+    "vortex2d": euler_cfg((41, 41), (-5, -5, 0), (5, 5, 0)),
+}
+
+
+def _state(e, name, rng):
+    """A smooth flow (or the vortex) with a small random part, positive
+    density and pressure everywhere (solid points included)."""
+    if name == "vortex2d":
+        f = vortex(0.2)
+    else:
+        def f(x, y, z):
+            rho = 1.0 + 0.1 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.cos(np.pi * z)
+            u = 0.3 + 0.05 * np.cos(2 * np.pi * z)
+            v = 0.05 * np.sin(2 * np.pi * x)
+            w = -0.04 * np.cos(2 * np.pi * y)
+            p = 1 / 1.4 + 0.05 * np.cos(2 * np.pi * (x + y))
+            return [rho, rho * u, rho * v, rho * w, p / 0.4 + 0.5 * rho * (u * u + v * v + w * w)]
+    U = e.fields()
+    set_state(e, U, f)
+    for k in range(5):
+        U[k].D[:] *= 1.0 + 1e-3 * rng.standard_normal(U[k].D.size)
+    return U
+
+
+def _pair(ctx, name):
+    import shoccs_amd as S
+
+    cfg = CASES[name]
+    e = O.OracleEuler(cfg)
+    pcfg = dict(kind=O.SYS_HEAT, n=cfg["n"], lo=cfg["lo"], hi=cfg["hi"], spheres=cfg["spheres"],
+                obj_bc=[D] * len(cfg["spheres"]), grid_bc=cfg["grid_bc"], scheme=cfg["scheme"], alpha=cfg["alpha"],
+                params=[1.0, O.MMS_SINPROD])
+    prob = H.product_problem(ctx, pcfg)
+    return e, prob, S.Euler(prob)
+
+
+def _upload(ge, U):
+    G = ge.fields()
+    ge.upload(G, [u.flat() for u in U])
+    return G
+
+
+def _flat(U):
+    return [u.flat() for u in U]
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_wall_and_rhs_bitexact(ctx, name):
+    e, prob, ge = _pair(ctx, name)
+    U = _state(e, name, np.random.default_rng(3))
+    G = _upload(ge, U)
+    e.wall(U)
+    ge.wall(G)
+    for a, b in zip(ge.download(G), _flat(U)):
+        assert np.array_equal(a, b)
+    want = e.rhs(U)
+    out = ge.fields()
+    ge.rhs(G, out)
+    for k, (a, b) in enumerate(zip(ge.download(out), _flat(want))):
+        assert np.array_equal(a, b), f"{name} component {k}: {np.max(np.abs(a - b))}"
+    assert max(np.max(np.abs(w.D)) for w in want) > 0
+    ge.close()
+    prob.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_rk4_steps_bitexact(ctx, name):
+    """Four RK4 steps (rk4.cpp:16-47 over the five fields), u0 <- out between
+    steps, at dt = 0.4 h / (|u| + c)_max."""
+    e, prob, ge = _pair(ctx, name)
+    U = _state(e, name, np.random.default_rng(4))
+    e.wall(U)
+    rho, m, E = U[0].D, np.stack([U[k].D for k in (1, 2, 3)]), U[4].D
+    p = 0.4 * (E - 0.5 * np.sum(m * m, axis=0) / rho)
+    smax = np.max(np.sqrt(np.sum(m * m, axis=0)) / rho + np.sqrt(1.4 * p / rho))
+    dt = 0.4 * min(h for h in e.mesh.h() if h > 0) / smax
+    G = _upload(ge, U)
+    out, rk, srhs = e.fields(), e.fields(), e.fields()
+    Go, Gr, Gs = ge.fields(), ge.fields(), ge.fields()
+    for _ in range(4):
+        e.rk4_step(U, out, rk, srhs, dt)
+        ge.rk4_step(G, Go, Gr, Gs, dt)
+        for k in range(5):
+            U[k].D[:] = out[k].D
+            for d in range(3):
+                U[k].R[d][:] = out[k].R[d]
+            G[k].copy_from(Go[k])
+    for k, (a, b) in enumerate(zip(ge.download(G), _flat(U))):
+        assert np.array_equal(a, b), f"{name} component {k}: {np.max(np.abs(a - b))}"
+    ms = ge.time_steps(G, Go, Gr, Gs, dt, 2)
+    assert ms > 0
+    ge.close()
+    prob.close()
+
+
+def test_euler_refuses_cut_point_rows(ctx):
+    import shoccs_amd as S
+
+    cfg = H.cut_cfgs()["floating_E4u_1"]
+    prob = H.product_problem(ctx, cfg)
+    with pytest.raises(S.ShoccsError):
+        S.Euler(prob)
+    prob.close()
