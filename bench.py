@@ -372,6 +372,136 @@ def adjoint128(args):
     print(json.dumps(result))
 
 
+def euler192(args):
+    """Configuration 4 in FP64: 3-D compressible Euler on 192^3 with a
+    slip-wall sphere (centre 0.5, radius 0.25), Mach 0.3 along x, pressure
+    1/gamma, density 1 +- 0.1 from a seeded low-mode perturbation, 50 RK4 steps
+    at CFL 0.4, E4u first-derivative operators; the inflow face x = 0 holds
+    its state, the other faces take the operators' closures.  The timing does
+    not depend on the step size; a companion run at CFL 0.1 with the E4u
+    closure parameters (0, 0) reports whether the flow stays bounded (the
+    cut-cell E4u closures, psi-independent as the reference defines them,
+    give growing modes at the body; without the body the run is stable)."""
+    import numpy as np
+
+    import shoccs_amd as S
+
+    n = 192 if args.n == N_SIDE else args.n
+    nsteps = 50
+    gam = 1.4
+    ctx = S.Context(0)
+    D_, F_ = S.BC_DIRICHLET, S.BC_FLOATING
+    grid_bc = [D_, F_, F_, F_, F_, F_]
+    sphere = [0.5, 0.5, 0.5, 0.25]
+
+    def state(x, y, z, rng):
+        s = np.zeros_like(x)
+        for _ in range(3):
+            k = rng.integers(1, 4, size=3)
+            s += np.sin(2 * np.pi * (k[0] * x + k[1] * y + k[2] * z) + rng.uniform(0, 2 * np.pi))
+        rho = 1.0 + 0.1 * s / 3.0
+        u, p = 0.3, 1.0 / gam
+        return [rho, rho * u, 0 * rho, 0 * rho, p / (gam - 1) + 0.5 * rho * u * u]
+
+    def run(alpha, cfl, timed):
+        prob = S.Problem(ctx, S.SYS_HEAT, (n, n, n), (0, 0, 0), (1, 1, 1), spheres=[sphere], shape_bc=[D_],
+                         grid_bc=grid_bc, scheme=S.E4U_1, alpha=alpha, params=[1.0, S.MMS_SINPROD])
+        e = S.Euler(prob, gam)
+        xs = np.linspace(0.0, 1.0, n)
+        X, Y, Z = np.meshgrid(xs, xs, xs, indexing="ij")
+        vals = state(X.ravel(), Y.ravel(), Z.ravel(), np.random.default_rng(4))
+        del X, Y, Z
+        pos = [prob.r_arrays(d)["pos"] for d in range(3)]
+        rvals = [state(p[:, 0], p[:, 1], p[:, 2], np.random.default_rng(4)) for p in pos]
+        U = e.fields()
+        e.upload(U, [np.concatenate([vals[k]] + [rvals[d][k] for d in range(3)]) for k in range(5)])
+        e.wall(U)
+        rho, mx, E = vals[0], vals[1], vals[4]
+        c = np.sqrt(gam * (gam - 1) * (E - 0.5 * mx * mx / rho) / rho)
+        dt = cfl * (1.0 / (n - 1)) / float(np.max(np.abs(mx / rho) + c))
+        del vals, rvals, rho, mx, E, c
+        out, rk, srhs = e.fields(), e.fields(), e.fields()
+        if timed:
+            e.time_steps(U, out, rk, srhs, dt, 1)  # warm-up step
+        ms = e.time_steps(U, out, rk, srhs, dt, nsteps)
+        got = e.download(U)
+        N = prob.N
+        rho = got[0][:N]
+        p = (gam - 1) * (got[4][:N] - 0.5 * (got[1][:N] ** 2 + got[2][:N] ** 2 + got[3][:N] ** 2) / rho)
+        fl = prob.fluid()
+        bounded = bool(np.all(np.isfinite(rho[fl])) and rho[fl].min() > 0 and np.all(np.isfinite(p[fl]))
+                       and p[fl].min() > 0)
+        info = {"cfl": cfl, "alpha": list(alpha), "dt": dt, "bounded": bounded,
+                "max_density_deviation": float(np.max(np.abs(rho[fl] - 1.0)))}
+        e.close()
+        prob.close()
+        return ms, N, info
+
+    ms, N, info = run(E4U_ALPHA, 0.4, True)
+    _, _, companion = run((0.0, 0.0), 0.1, False)
+    # bytes the schedule moves per point and step: per rhs, zero (40), per
+    # direction fluxes (80) and five applies (80 eq, 120 plus_eq), negate (80);
+    # slot ops: stage inputs 3 x 120, accumulations 4 x 120, final 120, copies
+    # and zeroing 240
+    bytes_pt = 4 * (40 + (80 + 80) + 2 * (80 + 120) + 80) + 360 + 480 + 120 + 240
+    achieved = bytes_pt * N / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = euler_cpu_baseline(state)
+    result = {
+        "metric": "Mpts/s per RK4 step (3-D compressible Euler, 192^3 slip-wall sphere, FP64)",
+        "value": N / (ms * 1e-3) / 1e6, "unit": "Mpts/s", "higher_is_better": True, "n_gpus": 1,
+        "steps": nsteps, "ms_per_step": ms, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3-D compressible Euler, {n}^3, slip-wall sphere c=0.5 r=0.25, Mach 0.3 along x, "
+                               "density 1 +- 0.1 seeded low-mode perturbation, pressure 1/gamma, gamma 1.4, "
+                               f"{nsteps} RK4 steps at CFL 0.4 (dt {info['dt']:.4g}), E4u operators (the scalar-wave "
+                               "closure parameters), inflow face held",
+                   "form": "reference schedule over five fields: flux kernel + five derivative applies per "
+                           "direction, slot ops"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None, "bytes_per_pt_step": bytes_pt},
+        "timed_run": info, "stability_companion": companion,
+        "cpu_baseline": cpu, "vs_baseline": None,
+    }
+    ctx.close()
+    print(json.dumps(result))
+
+
+def euler_cpu_baseline(state):
+    """The CPU oracle's Euler step (OpenMP in the flux and derivative loops)
+    on a 64^3 sample of the same problem, scaled to points per second."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+
+    n = 64
+    cfg = dict(n=(n, n, n), lo=(0, 0, 0), hi=(1, 1, 1), spheres=[[0.5, 0.5, 0.5, 0.25]],
+               grid_bc=[O.DIRICHLET] + [O.FLOATING] * 5, scheme=O.E4U_1, alpha=E4U_ALPHA)
+    e = O.OracleEuler(cfg)
+    U = e.fields()
+    m = e.mesh
+    X, Y, Z = np.meshgrid(*[np.asarray(m.m.xyz[d][:n]) for d in range(3)], indexing="ij")
+    vals = state(X.ravel(), Y.ravel(), Z.ravel(), np.random.default_rng(4))
+    for k in range(5):
+        U[k].D[:] = vals[k]
+    for d in range(3):
+        p = m.r_arrays(d)["pos"]
+        rv = state(p[:, 0], p[:, 1], p[:, 2], np.random.default_rng(4))
+        for k in range(5):
+            U[k].R[d][:] = rv[k]
+    e.wall(U)
+    out, rk, srhs = e.fields(), e.fields(), e.fields()
+    steps = 3
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.rk4_step(U, out, rk, srhs, 1e-4)
+    t = (time.perf_counter() - t0) / steps
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count()))
+    return {"value": n ** 3 / t / 1e6, "unit": "Mpts/s per RK4 step", "cores": cores, "kind": "port",
+            "sample": f"{steps} RK4 steps of the 64^3 Euler problem ({t:.2f} s/step, the oracle's schedule)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,10 +513,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default fused-stage kernel shape (short profiling runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision form")
-    ap.add_argument("--workload", default="wave256", choices=["wave256", "heat-batch64", "adjoint128"],
+    ap.add_argument("--workload", default="wave256",
+                    choices=["wave256", "heat-batch64", "adjoint128", "euler192"],
                     help="wave256: the headline 256^3 scalar wave; heat-batch64: the batched heat "
-                         "configuration; adjoint128: the adjoint gradient of a 128^3 solve (each "
-                         "its own JSON line)")
+                         "configuration; adjoint128: the adjoint gradient of a 128^3 solve; euler192: "
+                         "the 192^3 compressible Euler configuration (each its own JSON line)")
     ap.add_argument("--n", type=int, default=N_SIDE)
     ap.add_argument("--adj-cfl", type=float, default=0.05, help="CFL of the adjoint workload")
     args = ap.parse_args()
@@ -400,6 +531,9 @@ def main():
         return
     if args.workload == "adjoint128":
         adjoint128(args)
+        return
+    if args.workload == "euler192":
+        euler192(args)
         return
 
     import numpy as np
