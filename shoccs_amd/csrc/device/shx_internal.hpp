@@ -31,6 +31,7 @@ namespace shx {
 constexpr int kTileZ = 64;  // lanes along the contiguous axis (one wavefront)
 constexpr int kTileY = 4;   // wavefronts per workgroup, stacked along y
 constexpr int kBlock = kTileZ * kTileY;
+constexpr int kDerivPlanes = 4;  // x planes per lane of the point-wise derivative apply
 
 // Device view of one direction's derivative operator (kernel argument).  Only
 // scalars and pointers: a kernel argument that is indexed dynamically or whose
@@ -233,6 +234,26 @@ __device__ __forceinline__ bool tile_point(const Tiling& t, int& i, int& j, int&
     const int by = (int)(rest / (unsigned)t.nbz);
     k = bz * kTileZ + (int)(threadIdx.x % kTileZ);
     j = by * kTileY + (int)(threadIdx.x / kTileZ);
+    if (k >= t.n2 || j >= t.n1) return false;
+    idx = ((long)i * t.n1 + j) * t.n2 + k;
+    return true;
+}
+
+// The same tile with each wave on all kTileY rows of a patch of
+// 64 / kTileY points along z, for rows along z: the closure rows at the ends
+// of a z line then fall in the end waves of the end tiles only, instead of
+// diverging every wave of those tiles.
+__device__ __forceinline__ bool tile_point_zpatch(const Tiling& t, int& i, int& j, int& k, long& idx)
+{
+    constexpr int ZW = 64 / kTileY;
+    static_assert(kTileZ == 64 && 64 % kTileY == 0, "a wave holds whole rows of its z patch");
+    const unsigned L = xcd_remap(blockIdx.x, t.nblocks);
+    i = (int)(L % (unsigned)t.n0);
+    const unsigned rest = L / (unsigned)t.n0;
+    const int bz = (int)(rest % (unsigned)t.nbz);
+    const int by = (int)(rest / (unsigned)t.nbz);
+    k = bz * kTileZ + (int)(threadIdx.x / 64) * ZW + (int)(threadIdx.x % ZW);
+    j = by * kTileY + (int)((threadIdx.x % 64) / ZW);
     if (k >= t.n2 || j >= t.n1) return false;
     idx = ((long)i * t.n1 + j) * t.n2 + k;
     return true;

@@ -26,6 +26,10 @@ HEAT_CFGS = {
     "heat3d_odd": H.heat3d_cfg(n=(21, 23, 27)),
 }
 ALL = {**WAVE_CFGS, **HEAT_CFGS}
+# large enough for the derivative's several-planes-per-lane launch along x and
+# y (at least 2048 workgroups of four planes); extents that are not multiples
+# of the tile or of the four planes
+MULTI_PLANE = {"heat3d_multi_plane": H.heat3d_cfg(n=(130, 66, 258), center=(0.5, 0.45, 0.55), radius=0.3)}
 
 
 def _rand_field(osys, seed):
@@ -43,10 +47,10 @@ def _pair(ctx, cfg):
     return osys, prob
 
 
-@pytest.mark.parametrize("name", list(ALL))
+@pytest.mark.parametrize("name", list(ALL) + list(MULTI_PLANE))
 @pytest.mark.parametrize("plus", [False, True])
 def test_derivative_apply_bitexact(ctx, name, plus):
-    cfg = ALL[name]
+    cfg = {**ALL, **MULTI_PLANE}[name]
     osys, prob = _pair(ctx, cfg)
     u = _rand_field(osys, 1)
     for d in range(3):
