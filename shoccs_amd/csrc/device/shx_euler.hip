@@ -9,12 +9,16 @@
 // reference's operators, and every step is plain FP64 arithmetic (no
 // transcendentals), so a CPU restatement of it agrees to the last bit.
 //
-// Per rhs: for each direction, one flux kernel (five fields, mesh and cut
-// points), then five derivative applies (O and B in one launch, eq for the
-// first direction, plus_eq after); then one kernel negates, zeroes the
-// Dirichlet faces and the cut points.  HBM-bound: about 5 x (8 reads + 8
-// writes) B/pt per direction for the fluxes and 5 x 24 B/pt per direction for
-// the applies.
+// Per rhs: one fill zeroes the five rhs fields; for each direction, one flux
+// kernel over the five fields' mesh and cut points writes the negated fluxes,
+// then five derivative applies (O and B in one launch, eq for the first
+// direction, plus_eq after) accumulate -D F directly; a last kernel zeroes the
+// Dirichlet faces.  Negation is exact and commutes with rounding to nearest,
+// so -(sum of products) and the sum of the products of the negated fluxes
+// are the same number (an exact zero may take the other sign).  Every slot
+// operation of the RK4 schedule is one launch over all five fields.
+// HBM-bound: about 5 x (8 reads + 8 writes) B/pt per direction for the
+// fluxes and 5 x 24 B/pt per direction for the applies.
 #include <hip/hip_runtime.h>
 
 #include <new>
@@ -24,13 +28,6 @@
 #include "shx_state.hpp"
 
 using namespace shx;
-
-struct FivePtr {
-    double* p[5];
-};
-struct FiveCPtr {
-    const double* p[5];
-};
 
 struct shx_euler {
     shx_ctx* ctx = nullptr;
@@ -58,168 +55,181 @@ int check(shx_ctx* ctx, const char* what)
     return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, what);
 }
 
-// The five fluxes along dir at n points (mesh or cut points), the pressure
-// term added to the momentum component of dir after its convective part.
-__global__ void k_flux(long n, int dir, double g, FiveCPtr U, FivePtr F)
+// The five fields' arrays as one index range [0, off[4]): part 0 holds the
+// mesh points [0, off[1]), parts 1-3 the cut points of R(x), R(y), R(z).
+// Pointers are picked by comparisons (no dynamic index into a kernel
+// argument, which would copy it to scratch memory).
+struct Five4 {
+    double* d[5][4];
+    const double* a[5][4];
+    const double* b[5][4];
+    long off[5];
+};
+
+__device__ __forceinline__ int part_of(const Five4& s, long i)
+{
+    return (int)(i >= s.off[1]) + (int)(i >= s.off[2]) + (int)(i >= s.off[3]);
+}
+template <typename T>
+__device__ __forceinline__ T pick4(int p, T x0, T x1, T x2, T x3)
+{
+    return p == 0 ? x0 : p == 1 ? x1 : p == 2 ? x2 : x3;
+}
+__device__ __forceinline__ long base_of(const Five4& s, int p)
+{
+    return pick4(p, 0L, s.off[1], s.off[2], s.off[3]);
+}
+
+// The five fluxes along dir, negated, at every mesh and cut point: the
+// pressure term added to the momentum component of dir after its convective
+// part, as the rhs takes them; field k of U is s.a[k], of F s.d[k].
+__global__ void k_flux_neg(Five4 s, int dir, double g)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double r = U.p[0][i], mx = U.p[1][i], my = U.p[2][i], mz = U.p[3][i], E = U.p[4][i];
-    const double p = (g - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) / r);
+    if (i >= s.off[4]) return;
+    const int p = part_of(s, i);
+    const long j = i - base_of(s, p);
+    double U[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) U[k] = pick4(p, s.a[k][0], s.a[k][1], s.a[k][2], s.a[k][3])[j];
+    const double r = U[0], mx = U[1], my = U[2], mz = U[3], E = U[4];
+    const double pr = (g - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) / r);
     const double md = dir == 0 ? mx : dir == 1 ? my : mz;
     const double ud = md / r;
     double f1 = mx * ud, f2 = my * ud, f3 = mz * ud;
-    if (dir == 0) f1 += p;
-    else if (dir == 1) f2 += p;
-    else f3 += p;
-    F.p[0][i] = md;
-    F.p[1][i] = f1;
-    F.p[2][i] = f2;
-    F.p[3][i] = f3;
-    F.p[4][i] = (E + p) * ud;
+    if (dir == 0) f1 += pr;
+    else if (dir == 1) f2 += pr;
+    else f3 += pr;
+    const double F[5] = {md, f1, f2, f3, (E + pr) * ud};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pick4(p, s.d[k][0], s.d[k][1], s.d[k][2], s.d[k][3])[j] = -F[k];
 }
 
 // Slip wall: each cut point takes its fluid neighbour's state with the
-// normal momentum removed (density and pressure kept).
-__global__ void k_wall(long n, double g1, const long* __restrict__ fluid, const double* __restrict__ nrm,
-                       FiveCPtr UD, FivePtr UR)
+// normal momentum removed (density and pressure kept).  Cut points of the
+// three sets as one range (parts 1-3 of s); field k on D is s.a[k][0].
+struct WallArgs {
+    const long* fluid[3];
+    const double* nrm[3];
+};
+__global__ void k_wall(Five4 s, WallArgs w, double g1)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const long f = fluid[i];
-    const double n0 = nrm[3 * i], n1 = nrm[3 * i + 1], n2 = nrm[3 * i + 2];
-    const double r = UD.p[0][f], E = UD.p[4][f];
-    double m0 = UD.p[1][f], m1 = UD.p[2][f], m2 = UD.p[3][f];
-    const double p = g1 * (E - 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r);
+    const long i = s.off[1] + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.off[4]) return;
+    const int p = part_of(s, i);
+    const long q = i - base_of(s, p);
+    const long f = pick4(p, (const long*)nullptr, w.fluid[0], w.fluid[1], w.fluid[2])[q];
+    const double* nr = pick4(p, (const double*)nullptr, w.nrm[0], w.nrm[1], w.nrm[2]) + 3 * q;
+    const double n0 = nr[0], n1 = nr[1], n2 = nr[2];
+    const double r = s.a[0][0][f], E = s.a[4][0][f];
+    double m0 = s.a[1][0][f], m1 = s.a[2][0][f], m2 = s.a[3][0][f];
+    const double pr = g1 * (E - 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r);
     const double mn = m0 * n0 + m1 * n1 + m2 * n2;
     m0 = m0 - mn * n0;
     m1 = m1 - mn * n1;
     m2 = m2 - mn * n2;
-    UR.p[0][i] = r;
-    UR.p[1][i] = m0;
-    UR.p[2][i] = m1;
-    UR.p[3][i] = m2;
-    UR.p[4][i] = p / g1 + 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r;
-}
-
-__global__ void k_negate5(long n, FivePtr x)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const double V[5] = {r, m0, m1, m2, pr / g1 + 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r};
 #pragma unroll
-    for (int k = 0; k < 5; ++k) x.p[k][i] = -x.p[k][i];
+    for (int k = 0; k < 5; ++k) pick4(p, s.d[k][0], s.d[k][1], s.d[k][2], s.d[k][3])[q] = V[k];
 }
 
-__global__ void k_zero_at5(long n, const int* __restrict__ sel, FivePtr x)
+__global__ void k_zero_at5(long n, const int* __restrict__ sel, Five4 s)
 {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const long i = sel[q];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) x.p[k][i] = 0.0;
+    for (int k = 0; k < 5; ++k) s.d[k][0][i] = 0.0;
 }
 
-// slot_ops.hpp:23-59
-__global__ void k_lc(long n, double* __restrict__ d, const double* __restrict__ s0, double c,
-                     const double* __restrict__ r)
+// Slot operations of rk4.cpp:16-47 (slot_ops.hpp:23-59) on field
+// blockIdx.y of five: d = 0, d = a, d = a + c b, d += c b, d = 0 + c b.
+enum { kSlotZero, kSlotCopy, kSlotLc, kSlotAcc, kSlotAccFromZero };
+template <int OP>
+__global__ void k_slot5(Five4 s, double c)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = s0[i] + c * r[i];
-}
-
-__global__ void k_acc(long n, double* __restrict__ d, double c, const double* __restrict__ r)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] += c * r[i];
-}
-
-FiveCPtr cD(const shx_field* U)
-{
-    FiveCPtr p;
-    for (int k = 0; k < 5; ++k) p.p[k] = U[k].D;
-    return p;
-}
-FivePtr mD(shx_field* U)
-{
-    FivePtr p;
-    for (int k = 0; k < 5; ++k) p.p[k] = U[k].D;
-    return p;
-}
-
-int zero5(shx_euler* e, shx_field* X)
-{
-    int rc = SHX_OK;
-    for (int k = 0; k < 5 && rc == SHX_OK; ++k) {
-        rc = launch_fill(e->ctx, X[k].D, 0.0, e->N);
-        for (int c = 0; c < 3 && rc == SHX_OK; ++c)
-            if (e->nR[c] > 0) rc = launch_fill(e->ctx, X[k].R[c], 0.0, e->nR[c]);
+    if (i >= s.off[4]) return;
+    const int k = (int)blockIdx.y;
+    const int p = part_of(s, i);
+    const long j = i - base_of(s, p);
+    double* d = pick4(p, s.d[k][0], s.d[k][1], s.d[k][2], s.d[k][3]);
+    if (OP == kSlotZero) {
+        d[j] = 0.0;
+    } else if (OP == kSlotCopy) {
+        d[j] = pick4(p, s.a[k][0], s.a[k][1], s.a[k][2], s.a[k][3])[j];
+    } else if (OP == kSlotLc) {
+        const double x = pick4(p, s.a[k][0], s.a[k][1], s.a[k][2], s.a[k][3])[j];
+        d[j] = x + c * pick4(p, s.b[k][0], s.b[k][1], s.b[k][2], s.b[k][3])[j];
+    } else if (OP == kSlotAcc) {
+        d[j] += c * pick4(p, s.b[k][0], s.b[k][1], s.b[k][2], s.b[k][3])[j];
+    } else {
+        d[j] = 0.0 + c * pick4(p, s.b[k][0], s.b[k][1], s.b[k][2], s.b[k][3])[j];
     }
-    return rc;
+}
+
+Five4 five4(const shx_euler* e, shx_field* d, const shx_field* a, const shx_field* b)
+{
+    Five4 s{};
+    for (int k = 0; k < 5; ++k) {
+        s.d[k][0] = d ? d[k].D : nullptr;
+        s.a[k][0] = a ? a[k].D : nullptr;
+        s.b[k][0] = b ? b[k].D : nullptr;
+        for (int c = 0; c < 3; ++c) {
+            s.d[k][1 + c] = d ? d[k].R[c] : nullptr;
+            s.a[k][1 + c] = a ? a[k].R[c] : nullptr;
+            s.b[k][1 + c] = b ? b[k].R[c] : nullptr;
+        }
+    }
+    s.off[0] = 0;
+    s.off[1] = e->N;
+    for (int c = 0; c < 3; ++c) s.off[2 + c] = s.off[1 + c] + e->nR[c];
+    return s;
+}
+
+template <int OP>
+void slot5(shx_euler* e, shx_field* d, const shx_field* a, double c, const shx_field* b)
+{
+    const Five4 s = five4(e, d, a, b);
+    k_slot5<OP><<<dim3(blocks_for(s.off[4]), 5), kT, 0, e->ctx->stream>>>(s, c);
 }
 
 int wall_launch(shx_euler* e, shx_field* U)
 {
-    hipStream_t st = e->ctx->stream;
+    const Five4 s = five4(e, U, U, nullptr);
+    const long n = s.off[4] - s.off[1];
+    if (n == 0) return SHX_OK;
+    WallArgs w;
     for (int c = 0; c < 3; ++c) {
-        if (e->nR[c] == 0) continue;
-        FivePtr R;
-        for (int k = 0; k < 5; ++k) R.p[k] = U[k].R[c];
-        k_wall<<<blocks_for(e->nR[c]), kT, 0, st>>>(e->nR[c], e->gamma - 1.0, e->wall_fluid[c], e->wall_normal[c],
-                                                     cD(U), R);
+        w.fluid[c] = e->wall_fluid[c];
+        w.nrm[c] = e->wall_normal[c];
     }
+    k_wall<<<blocks_for(n), kT, 0, e->ctx->stream>>>(s, w, e->gamma - 1.0);
     return check(e->ctx, "wall kernel");
 }
 
 int rhs_launch(shx_euler* e, const shx_field* U, shx_field* rhs)
 {
     hipStream_t st = e->ctx->stream;
-    int rc = zero5(e, rhs);
+    slot5<kSlotZero>(e, rhs, nullptr, 0.0, nullptr);
+    int rc = check(e->ctx, "rhs fill");
     bool first = true;
+    Five4 fs = five4(e, nullptr, U, nullptr);
+    for (int k = 0; k < 5; ++k) {
+        fs.d[k][0] = e->F[k];
+        for (int c = 0; c < 3; ++c) fs.d[k][1 + c] = e->FR[k][c];
+    }
     for (int dir = 0; dir < 3 && rc == SHX_OK; ++dir) {
         if (e->n[dir] < 2 || !e->d[dir]) continue;
-        FivePtr F;
-        for (int k = 0; k < 5; ++k) F.p[k] = e->F[k];
-        k_flux<<<blocks_for(e->N), kT, 0, st>>>(e->N, dir, e->gamma, cD(U), F);
-        for (int c = 0; c < 3; ++c) {
-            if (e->nR[c] == 0) continue;
-            FiveCPtr UR;
-            FivePtr FR;
-            for (int k = 0; k < 5; ++k) {
-                UR.p[k] = U[k].R[c];
-                FR.p[k] = e->FR[k][c];
-            }
-            k_flux<<<blocks_for(e->nR[c]), kT, 0, st>>>(e->nR[c], dir, e->gamma, UR, FR);
-        }
+        k_flux_neg<<<blocks_for(fs.off[4]), kT, 0, st>>>(fs, dir, e->gamma);
         rc = check(e->ctx, "flux kernel");
         for (int k = 0; k < 5 && rc == SHX_OK; ++k)
             rc = launch_derivative(e->d[dir], e->F[k], e->FR[k][dir], rhs[k].D, !first, true);
         first = false;
     }
     if (rc != SHX_OK) return rc;
-    k_negate5<<<blocks_for(e->N), kT, 0, st>>>(e->N, mD(rhs));
-    if (e->nfaces) k_zero_at5<<<blocks_for(e->nfaces), kT, 0, st>>>(e->nfaces, e->faces, mD(rhs));
+    if (e->nfaces) k_zero_at5<<<blocks_for(e->nfaces), kT, 0, st>>>(e->nfaces, e->faces, five4(e, rhs, nullptr, nullptr));
     return check(e->ctx, "rhs finish kernels");
-}
-
-// d = s0 + c r (every component, mesh and cut points)
-void lc5(shx_euler* e, shx_field* d, const shx_field* s0, double c, const shx_field* r)
-{
-    hipStream_t st = e->ctx->stream;
-    for (int k = 0; k < 5; ++k) {
-        k_lc<<<blocks_for(e->N), kT, 0, st>>>(e->N, d[k].D, s0[k].D, c, r[k].D);
-        for (int q = 0; q < 3; ++q)
-            if (e->nR[q] > 0) k_lc<<<blocks_for(e->nR[q]), kT, 0, st>>>(e->nR[q], d[k].R[q], s0[k].R[q], c, r[k].R[q]);
-    }
-}
-
-void acc5(shx_euler* e, shx_field* d, double c, const shx_field* r)
-{
-    hipStream_t st = e->ctx->stream;
-    for (int k = 0; k < 5; ++k) {
-        k_acc<<<blocks_for(e->N), kT, 0, st>>>(e->N, d[k].D, c, r[k].D);
-        for (int q = 0; q < 3; ++q)
-            if (e->nR[q] > 0) k_acc<<<blocks_for(e->nR[q]), kT, 0, st>>>(e->nR[q], d[k].R[q], c, r[k].R[q]);
-    }
 }
 
 template <typename T>
@@ -336,29 +346,24 @@ int shx_euler_rk4_step(shx_euler* e, shx_field* u0, shx_field* out, shx_field* r
     if (!e || !u0 || !out || !rk || !srhs) return SHX_EINVAL;
     static const double rki[4] = {0.0, 0.5, 0.5, 1.0};
     static const double rkf[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-    int rc = zero5(e, rk);
-    if (rc == SHX_OK) rc = zero5(e, srhs);
-    for (int k = 0; k < 5 && rc == SHX_OK; ++k) {
-        hipMemcpyAsync(out[k].D, u0[k].D, sizeof(double) * (size_t)e->N, hipMemcpyDeviceToDevice, e->ctx->stream);
-        for (int q = 0; q < 3; ++q)
-            if (e->nR[q] > 0)
-                hipMemcpyAsync(out[k].R[q], u0[k].R[q], sizeof(double) * (size_t)e->nR[q], hipMemcpyDeviceToDevice,
-                               e->ctx->stream);
-        rc = check(e->ctx, "euler copy");
-    }
+    // rk starts from zero (its first update is 0 + c srhs); srhs is
+    // overwritten whole by every rhs
+    slot5<kSlotCopy>(e, out, u0, 0.0, nullptr);
+    int rc = check(e->ctx, "euler copy");
     for (int i = 0; i < 4 && rc == SHX_OK; ++i) {
         if (i > 0) {
-            lc5(e, out, u0, dt * rki[i], srhs);
+            slot5<kSlotLc>(e, out, u0, dt * rki[i], srhs);
             rc = wall_launch(e, out);
         }
         if (rc == SHX_OK) rc = rhs_launch(e, out, srhs);
         if (rc == SHX_OK) {
-            acc5(e, rk, dt * rkf[i], srhs);
+            if (i == 0) slot5<kSlotAccFromZero>(e, rk, nullptr, dt * rkf[i], srhs);
+            else slot5<kSlotAcc>(e, rk, nullptr, dt * rkf[i], srhs);
             rc = check(e->ctx, "euler slot kernels");
         }
     }
     if (rc == SHX_OK) {
-        lc5(e, out, u0, 1.0, rk);
+        slot5<kSlotLc>(e, out, u0, 1.0, rk);
         rc = wall_launch(e, out);
     }
     return rc == SHX_OK ? sync(e->ctx) : rc;
@@ -379,12 +384,9 @@ int shx_time_euler_steps(shx_euler* e, shx_field* u0, shx_field* out, shx_field*
     for (int s = 0; s < nsteps && rc == SHX_OK; ++s) {
         rc = shx_euler_rk4_step(e, u0, out, rk, srhs, dt);
         // u0 <- out for the next step (simulation_cycle.cpp:61)
-        for (int k = 0; k < 5 && rc == SHX_OK; ++k) {
-            hipMemcpyAsync(u0[k].D, out[k].D, sizeof(double) * (size_t)e->N, hipMemcpyDeviceToDevice, e->ctx->stream);
-            for (int q = 0; q < 3; ++q)
-                if (e->nR[q] > 0)
-                    hipMemcpyAsync(u0[k].R[q], out[k].R[q], sizeof(double) * (size_t)e->nR[q],
-                                   hipMemcpyDeviceToDevice, e->ctx->stream);
+        if (rc == SHX_OK) {
+            slot5<kSlotCopy>(e, u0, out, 0.0, nullptr);
+            rc = check(e->ctx, "euler copy");
         }
     }
     hipEventRecord(b, e->ctx->stream);

@@ -1,6 +1,10 @@
 """The compressible Euler system on the device (shx_euler_*) against the CPU
-oracle (oracle/euler.c): wall update, rhs and RK4 steps, bit for bit (every
-step is plain FP64 arithmetic and the derivative applies are bit-identical)."""
+oracle (oracle/euler.c): wall update, rhs and RK4 steps, value for value.
+Every step is plain FP64 arithmetic and the derivative applies are
+bit-identical.  The device applies the derivatives to negated fluxes where the
+oracle negates the summed rhs; negation commutes with rounding to nearest, so
+only an exact zero may carry the other sign (np.array_equal counts -0.0 equal
+to 0.0)."""
 import numpy as np
 import pytest
 
