@@ -36,11 +36,16 @@ __global__ void k_copy_tail(double* __restrict__ dst, const double* __restrict__
     if (threadIdx.x == 0 && blockIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
 }
 
-// A lane evaluates the points of its (y, z) position in XP consecutive x
-// planes.  Every point's taps (and, for plus_eq, its old value) are loaded
-// before any row is evaluated, so that a lane keeps several points' loads in
-// flight; rows along x share one window of planes.  Each row is summed as
-// row_at sums it.  t counts groups of XP planes along x.
+// Interior rows of the fast intervals.  A lane evaluates the points of its
+// (y, z) position in XP consecutive x planes.  Every point's taps (and, for
+// plus_eq, its old value) are loaded before any row is evaluated, so that a
+// lane keeps several points' loads in flight; rows along x share one window
+// of planes.  Each row is summed as row_at sums it.  With XP > 1 (large
+// grids) every other row comes from the slow-row list (k_deriv_list, a launch
+// of its own), so no wave waits on a segment scan; with XP = 1 (grids too
+// small to fill the chip, where a second launch costs more than the
+// divergence) the lane scans its line's segments itself.  Points with no row
+// are left as they are.  t counts groups of XP planes along x.
 template <int DIR, int W, bool PLUS, bool WITH_B, int XP>
 __global__ __launch_bounds__(kBlock) void k_derivative(Tiling t, int n0, DevDeriv d,
                                                        const double* __restrict__ u,
@@ -99,21 +104,33 @@ __global__ __launch_bounds__(kBlock) void k_derivative(Tiling t, int n0, DevDeri
         const int q = DIR == 0 ? i0 + p : DIR == 1 ? j : k;
         const int lo0 = (int)(fd[p] & 0xffffu), hi0 = (int)((fd[p] >> 16) & 0xffffu);
         const int lo1 = (int)((fd[p] >> 32) & 0xffffu), hi1 = (int)(fd[p] >> 48);
-        RowVal r;
-        if ((q >= lo0 && q < hi0) || (q >= lo1 && q < hi1)) {
-            double dot = 0.0;
-#pragma unroll
-            for (int m = 0; m < W; ++m) dot += d.coef[m] * v[p][m];
-            r.dot = dot;
-            r.b0 = r.b1 = 0.0;
-            r.nb = 0;
-            r.row = true;
-        } else {
-            r = row_slow<WITH_B>(d.pair_ptr, d.blocks, d.pool, d.coef, W, d.stride, u, uR, idx[p], pair[p], q);
+        if (!((q >= lo0 && q < hi0) || (q >= lo1 && q < hi1))) {
+            if constexpr (XP == 1) {
+                const RowVal r = row_slow<WITH_B>(d.pair_ptr, d.blocks, d.pool, d.coef, W, d.stride, u, uR,
+                                                  idx[p], pair[p], q);
+                if (r.row) du[idx[p]] = PLUS ? row_plus(old[p], r) : row_eq(r);
+            }
+            continue;
         }
-        if (!r.row) continue;
-        du[idx[p]] = PLUS ? row_plus(old[p], r) : row_eq(r);
+        double dot = 0.0;
+#pragma unroll
+        for (int m = 0; m < W; ++m) dot += d.coef[m] * v[p][m];
+        du[idx[p]] = PLUS ? old[p] + dot : dot;
     }
+}
+
+// The listed rows (closures, B terms, interior rows of a line's third and
+// later segments), one lane each.
+template <int W, bool PLUS, bool WITH_B>
+__global__ __launch_bounds__(256) void k_deriv_list(long n, const SlowRow* __restrict__ rows, DevDeriv d,
+                                                    const double* __restrict__ u,
+                                                    const double* __restrict__ uR, double* __restrict__ du)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SlowRow e = rows[i];
+    const RowVal r = row_in_block<WITH_B>(d.blocks[e.b], d.pool, d.coef, W, d.stride, u, uR, e.idx, e.q);
+    du[e.idx] = PLUS ? row_plus(du[e.idx], r) : row_eq(r);
 }
 
 // N matrix rows, compressed to the rows that hold entries.
@@ -214,12 +231,13 @@ int launch_derivative(shx_derivative* d, const double* u, const double* uR, doub
 {
     shx_ctx* ctx = d->ctx;
     if (!d->dev.present) return SHX_OK;
-    // four planes per lane along x and y (measured 4.2 and 3.2 TB/s at 256^3
-    // against 3.0 with one) unless the grid would then hold fewer than eight
-    // workgroups per CU; rows along z gain nothing from it
+    // four planes per lane and the slow rows from their list (measured at
+    // 256^3: x 4.1, y 3.4, z 2.6 TB/s, against 3.0, 3.0, 1.9 with one plane
+    // and the segment scan in every lane), unless the grid would then hold
+    // fewer than eight workgroups per CU
     const int g4[3] = {(d->n[0] + kDerivPlanes - 1) / kDerivPlanes, d->n[1], d->n[2]};
     const Tiling t4 = make_tiling(g4);
-    const bool multi = d->dir < 2 && t4.nblocks >= 8u * 256u;
+    const bool multi = t4.nblocks >= 8u * 256u;
     const Tiling t = multi ? t4 : make_tiling(d->n);
     const int key = d->dir * 100 + d->dev.width * 10 + (plus ? 2 : 0) + (with_b ? 1 : 0);
     switch (key) {
@@ -229,6 +247,8 @@ int launch_derivative(shx_derivative* d, const double* u, const double* uR, doub
             k_derivative<D, W, P, B, kDerivPlanes><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->n[0], d->dev, u, uR, du); \
         else \
             k_derivative<D, W, P, B, 1><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->n[0], d->dev, u, uR, du); \
+        if (multi && d->nslow > 0) \
+            k_deriv_list<W, P, B><<<(unsigned)((d->nslow + 255) / 256), 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du); \
         break;
 #define SHX_DERIV_DIR(D, W) \
     SHX_DERIV_CASE(D, W, false, false) SHX_DERIV_CASE(D, W, false, true) \
@@ -514,7 +534,29 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     }
     for (long p = 0; p < npairs; ++p) pair_ptr[p + 1] += pair_ptr[p];
 
+    // every row outside its line's fast intervals, sorted by point
+    std::vector<SlowRow> slow;
+    for (long b = 0; b < desc->nblocks; ++b) {
+        const shx_line_block& blk = desc->blocks[b];
+        const unsigned long long fd = fast[blk.pair];
+        const int lo0 = (int)(fd & 0xffffu), hi0 = (int)((fd >> 16) & 0xffffu);
+        const int lo1 = (int)((fd >> 32) & 0xffffu), hi1 = (int)(fd >> 48);
+        const int s = blk.pair / nfast, f = blk.pair % nfast;
+        long base;
+        if (dir == 0) base = (long)s * desc->n[2] + f;
+        else if (dir == 1) base = (long)s * desc->n[1] * desc->n[2] + f;
+        else base = ((long)s * desc->n[1] + f) * desc->n[2];
+        const int total = blk.nl + blk.ni + blk.nr;
+        for (int q = blk.lo; q < blk.lo + total; ++q) {
+            if ((q >= lo0 && q < hi0) || (q >= lo1 && q < hi1)) continue;
+            slow.push_back(SlowRow{base + (long)q * d->dev.stride, (int)b, q});
+        }
+    }
+    std::sort(slow.begin(), slow.end(), [](const SlowRow& a, const SlowRow& b) { return a.idx < b.idx; });
+    d->nslow = (long)slow.size();
+
     int rc = upload(ctx, &d->pair_ptr, pair_ptr.data(), pair_ptr.size());
+    if (rc == SHX_OK && d->nslow) rc = upload(ctx, &d->slow, slow.data(), slow.size());
     if (rc == SHX_OK) rc = upload(ctx, &d->fast, fast.data(), fast.size());
     if (rc == SHX_OK && desc->nblocks)
         rc = upload(ctx, &d->blocks, desc->blocks, (size_t)desc->nblocks);
@@ -593,6 +635,7 @@ int shx_derivative_destroy(shx_derivative* d)
     destroy_transpose(d->T);
     hipFree(d->pair_ptr);
     hipFree(d->fast);
+    hipFree(d->slow);
     hipFree(d->blocks);
     hipFree(d->pool);
     hipFree(d->coef);

@@ -85,6 +85,52 @@ __device__ __forceinline__ void pair_of(int dir, int i, int j, int k, int n1, in
     }
 }
 
+// Row q of segment blk (0 <= q - blk.lo < its row count) at point idx: the
+// closure or interior dot product from 0.0 in column order, then its B
+// products.
+template <bool WITH_B>
+__device__ __forceinline__ RowVal row_in_block(const shx_line_block& blk, const double* __restrict__ pool,
+                                               const double* __restrict__ coef, int width, long stride,
+                                               const double* __restrict__ u,
+                                               const double* __restrict__ uR, long idx, int q)
+{
+    RowVal r;
+    r.b0 = r.b1 = 0.0;
+    r.nb = 0;
+    r.row = true;
+    const int lr = q - blk.lo;
+    const int total = blk.nl + blk.ni + blk.nr;
+    const long base = idx - (long)q * stride;
+    double dot = 0.0;
+    if (lr < blk.nl) {
+        const double* a = pool + blk.lcoef + (long)lr * blk.lcols;
+        for (int jj = 0; jj < blk.lcols; ++jj)
+            dot += a[jj] * u[base + (long)(blk.lcol0 + jj) * stride];
+    } else if (lr < blk.nl + blk.ni) {
+        const int hw = width / 2;
+        for (int jj = 0; jj < width; ++jj) dot += coef[jj] * u[idx + (long)(jj - hw) * stride];
+    } else {
+        const int rr = lr - blk.nl - blk.ni;
+        const double* a = pool + blk.rcoef + (long)rr * blk.rcols;
+        for (int jj = 0; jj < blk.rcols; ++jj)
+            dot += a[jj] * u[base + (long)(blk.rcol0 + jj) * stride];
+    }
+    r.dot = dot;
+    if (WITH_B) {
+        if (blk.lobj >= 0 && lr < blk.nl) {
+            r.b0 = pool[blk.lbw + lr] * uR[blk.lobj];
+            r.nb = 1;
+        }
+        if (blk.robj >= 0 && lr >= total - blk.nr) {
+            const double v = pool[blk.rbw + (lr - (total - blk.nr))] * uR[blk.robj];
+            if (r.nb == 0) r.b0 = v;
+            else r.b1 = v;
+            r.nb++;
+        }
+    }
+    return r;
+}
+
 // Slow path: scan the segments of the line for the one holding row q.  All
 // arguments are scalars or pointers so that nothing is spilled to scratch.
 template <bool WITH_B>
@@ -95,52 +141,31 @@ __device__ __noinline__ RowVal row_slow(const int* __restrict__ pair_ptr,
                                         const double* __restrict__ u,
                                         const double* __restrict__ uR, long idx, int pair, int q)
 {
+    const int b_begin = pair_ptr[pair];
+    const int b_end = pair_ptr[pair + 1];
+    for (int b = b_begin; b < b_end; ++b) {
+        const shx_line_block blk = blocks[b];
+        const int lr = q - blk.lo;
+        if (lr < 0) break;
+        if (lr >= blk.nl + blk.ni + blk.nr) continue;
+        return row_in_block<WITH_B>(blk, pool, coef, width, stride, u, uR, idx, q);
+    }
     RowVal r;
     r.dot = 0.0;
     r.b0 = r.b1 = 0.0;
     r.nb = 0;
     r.row = false;
-    const int b_begin = pair_ptr[pair];
-    const int b_end = pair_ptr[pair + 1];
-    const long base = idx - (long)q * stride;
-    for (int b = b_begin; b < b_end; ++b) {
-        const shx_line_block blk = blocks[b];
-        const int lr = q - blk.lo;
-        if (lr < 0) break;
-        const int total = blk.nl + blk.ni + blk.nr;
-        if (lr >= total) continue;
-        double dot = 0.0;
-        if (lr < blk.nl) {
-            const double* a = pool + blk.lcoef + (long)lr * blk.lcols;
-            for (int jj = 0; jj < blk.lcols; ++jj)
-                dot += a[jj] * u[base + (long)(blk.lcol0 + jj) * stride];
-        } else if (lr < blk.nl + blk.ni) {
-            const int hw = width / 2;
-            for (int jj = 0; jj < width; ++jj) dot += coef[jj] * u[idx + (long)(jj - hw) * stride];
-        } else {
-            const int rr = lr - blk.nl - blk.ni;
-            const double* a = pool + blk.rcoef + (long)rr * blk.rcols;
-            for (int jj = 0; jj < blk.rcols; ++jj)
-                dot += a[jj] * u[base + (long)(blk.rcol0 + jj) * stride];
-        }
-        r.dot = dot;
-        r.row = true;
-        if (WITH_B) {
-            if (blk.lobj >= 0 && lr < blk.nl) {
-                r.b0 = pool[blk.lbw + lr] * uR[blk.lobj];
-                r.nb = 1;
-            }
-            if (blk.robj >= 0 && lr >= total - blk.nr) {
-                const double v = pool[blk.rbw + (lr - (total - blk.nr))] * uR[blk.robj];
-                if (r.nb == 0) r.b0 = v;
-                else r.b1 = v;
-                r.nb++;
-            }
-        }
-        break;
-    }
     return r;
 }
+
+// A row of a derivative outside the fast intervals of its line: point idx,
+// position q along the direction, segment b.  The point-wise apply evaluates
+// these from a list (sorted by idx) in a launch of their own.
+struct SlowRow {
+    long idx;
+    int b;
+    int q;
+};
 
 // Row of direction DIR (stencil width W) at point idx = (i, j, k).  The W taps
 // are loaded before the interval test, at positions clamped to the line, so

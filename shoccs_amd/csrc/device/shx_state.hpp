@@ -25,6 +25,8 @@ struct shx_derivative {
     shx::DevDeriv dev;
     int* pair_ptr = nullptr;
     unsigned long long* fast = nullptr;
+    long nslow = 0;                  // rows outside the fast intervals
+    shx::SlowRow* slow = nullptr;    // sorted by point
     shx_line_block* blocks = nullptr;
     double* pool = nullptr;
     double* coef = nullptr;
