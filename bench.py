@@ -443,11 +443,13 @@ def euler192(args):
     # direction fluxes (80) and five applies (80 eq, 120 plus_eq), negate (80);
     # slot ops: stage inputs 3 x 120, accumulations 4 x 120, final 120, copies
     # and zeroing 240
-    # per rhs: zero five fields (40), per direction the negated fluxes (5 x 16)
-    # and five applies (eq 5 x 16, plus_eq 5 x 24); per step: three stage
-    # inputs (5 x 24 each), rk from zero (5 x 16) then three updates (5 x 24),
-    # the final combination (5 x 24), out <- u0 and u0 <- out (5 x 16 each)
-    bytes_pt = 4 * (40 + (80 + 80) + 2 * (80 + 120)) + 3 * 120 + 80 + 3 * 120 + 120 + 2 * 80
+    # per rhs: zero five fields (40); per direction one flux-derivative pass
+    # that reads the five conserved fields (40) and writes the five rhs
+    # fields (40), reading them too after the first direction (+40); per
+    # step: three stage inputs (5 x 24 each), rk from zero (5 x 16) then
+    # three updates (5 x 24), the final combination (5 x 24), out <- u0 and
+    # u0 <- out (5 x 16 each)
+    bytes_pt = 4 * (40 + 80 + 2 * 120) + 3 * 120 + 80 + 3 * 120 + 120 + 2 * 80
     achieved = bytes_pt * N / (ms * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu_baseline:
@@ -460,8 +462,9 @@ def euler192(args):
                                "density 1 +- 0.1 seeded low-mode perturbation, pressure 1/gamma, gamma 1.4, "
                                f"{nsteps} RK4 steps at CFL 0.4 (dt {info['dt']:.4g}), E4u operators (the scalar-wave "
                                "closure parameters), inflow face held",
-                   "form": "reference schedule over five fields: negated-flux kernel + five derivative applies "
-                           "per direction, each slot operation one launch over the five fields"},
+                   "form": "RK4 slot schedule over five fields; per direction one pass applies the derivative "
+                           "to the fluxes computed at its taps (five fields at once); each slot operation one "
+                           "launch over the five fields"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": None, "bytes_per_pt_step": bytes_pt},
         "timed_run": info, "stability_companion": companion,

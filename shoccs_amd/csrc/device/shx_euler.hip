@@ -9,16 +9,19 @@
 // reference's operators, and every step is plain FP64 arithmetic (no
 // transcendentals), so a CPU restatement of it agrees to the last bit.
 //
-// Per rhs: one fill zeroes the five rhs fields; for each direction, one flux
-// kernel over the five fields' mesh and cut points writes the negated fluxes,
-// then five derivative applies (O and B in one launch, eq for the first
-// direction, plus_eq after) accumulate -D F directly; a last kernel zeroes the
-// Dirichlet faces.  Negation is exact and commutes with rounding to nearest,
-// so -(sum of products) and the sum of the products of the negated fluxes
-// are the same number (an exact zero may take the other sign).  Every slot
+// Per rhs: one fill zeroes the five rhs fields; for each direction, one kernel
+// writes the negated fluxes at the cut points (the B terms' values), then one
+// pass applies the direction's first derivative to the negated fluxes of the
+// five fields at once, computing each tap's flux from the conserved values
+// there (interior rows of the fast intervals), and one more evaluates the
+// derivative's listed rows (closures with their B terms).  The first
+// direction assigns, the others add; a last kernel zeroes the Dirichlet
+// faces.  Negation is exact and commutes with rounding to nearest, so summing
+// the products of the negated fluxes gives -(the sum of the products): an
+// exact zero may take the other sign, no other value changes.  Every slot
 // operation of the RK4 schedule is one launch over all five fields.
-// HBM-bound: about 5 x (8 reads + 8 writes) B/pt per direction for the
-// fluxes and 5 x 24 B/pt per direction for the applies.
+// HBM-bound: per direction the five conserved fields are read once (40 B/pt)
+// and the five rhs fields written (and read after the first direction).
 #include <hip/hip_runtime.h>
 
 #include <new>
@@ -40,8 +43,7 @@ struct shx_euler {
     double* wall_normal[3] = {nullptr, nullptr, nullptr};
     int* faces = nullptr;  // Dirichlet grid-face points
     long nfaces = 0;
-    double* F[5] = {};     // one direction's fluxes on D
-    double* FR[5][3] = {};  // and on the cut points
+    double* FR[5][3] = {};  // one direction's negated fluxes at the cut points
 };
 
 namespace {
@@ -80,18 +82,11 @@ __device__ __forceinline__ long base_of(const Five4& s, int p)
     return pick4(p, 0L, s.off[1], s.off[2], s.off[3]);
 }
 
-// The five fluxes along dir, negated, at every mesh and cut point: the
-// pressure term added to the momentum component of dir after its convective
-// part, as the rhs takes them; field k of U is s.a[k], of F s.d[k].
-__global__ void k_flux_neg(Five4 s, int dir, double g)
+// The five fluxes along dir at one state, negated, as the rhs takes them:
+// the pressure term added to the momentum component of dir after its
+// convective part.
+__device__ __forceinline__ void neg_flux(double g, int dir, const double (&U)[5], double (&F)[5])
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.off[4]) return;
-    const int p = part_of(s, i);
-    const long j = i - base_of(s, p);
-    double U[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) U[k] = pick4(p, s.a[k][0], s.a[k][1], s.a[k][2], s.a[k][3])[j];
     const double r = U[0], mx = U[1], my = U[2], mz = U[3], E = U[4];
     const double pr = (g - 1.0) * (E - 0.5 * (mx * mx + my * my + mz * mz) / r);
     const double md = dir == 0 ? mx : dir == 1 ? my : mz;
@@ -100,9 +95,147 @@ __global__ void k_flux_neg(Five4 s, int dir, double g)
     if (dir == 0) f1 += pr;
     else if (dir == 1) f2 += pr;
     else f3 += pr;
-    const double F[5] = {md, f1, f2, f3, (E + pr) * ud};
+    F[0] = -md;
+    F[1] = -f1;
+    F[2] = -f2;
+    F[3] = -f3;
+    F[4] = -((E + pr) * ud);
+}
+
+// The negated fluxes along dir at every point of the range of s (field k of
+// U is s.a[k], of F s.d[k]).
+__global__ void k_flux_neg(Five4 s, int dir, double g)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.off[4]) return;
+    const int p = part_of(s, i);
+    const long j = i - base_of(s, p);
+    double U[5], F[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) pick4(p, s.d[k][0], s.d[k][1], s.d[k][2], s.d[k][3])[j] = -F[k];
+    for (int k = 0; k < 5; ++k) U[k] = pick4(p, s.a[k][0], s.a[k][1], s.a[k][2], s.a[k][3])[j];
+    neg_flux(g, dir, U, F);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pick4(p, s.d[k][0], s.d[k][1], s.d[k][2], s.d[k][3])[j] = F[k];
+}
+
+struct FiveC {
+    const double* p[5];
+};
+struct FiveD {
+    double* p[5];
+};
+
+// rhs_k (op)= D_dir applied to the negated flux of field k, the five fields
+// at once, for the interior rows of the fast intervals: each tap's flux is
+// computed from the five conserved values there (the same arithmetic as
+// k_flux_neg), and the dot products are summed from 0.0 in tap order, as the
+// derivative apply sums them over a stored flux.  Rows outside the fast
+// intervals come from the derivative's slow-row list (k_euler_dflux_list).
+template <int DIR, int W, bool PLUS>
+__global__ __launch_bounds__(kBlock) void k_euler_dflux(Tiling t, DevDeriv d, double g, FiveC U, FiveD rhs)
+{
+    constexpr int hw = W / 2;
+    int i, j, k;
+    long idx;
+    if (!(DIR == 2 ? tile_point_zpatch(t, i, j, k, idx) : tile_point(t, i, j, k, idx))) return;
+    const int pair = DIR == 0 ? j * t.n2 + k : DIR == 1 ? i * t.n2 + k : i * t.n1 + j;
+    const int q = DIR == 0 ? i : DIR == 1 ? j : k;
+    const unsigned long long fd = d.fast[pair];
+    double v[W][5];
+#pragma unroll
+    for (int m = 0; m < W; ++m) {
+        const int qq = min(max(q + m - hw, 0), d.nq - 1);
+        const long at = idx + (long)(qq - q) * d.stride;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) v[m][f] = U.p[f][at];
+    }
+    double old[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) old[f] = PLUS ? rhs.p[f][idx] : 0.0;
+    const int lo0 = (int)(fd & 0xffffu), hi0 = (int)((fd >> 16) & 0xffffu);
+    const int lo1 = (int)((fd >> 32) & 0xffffu), hi1 = (int)(fd >> 48);
+    if (!((q >= lo0 && q < hi0) || (q >= lo1 && q < hi1))) return;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < W; ++m) {
+        double F[5];
+        neg_flux(g, DIR, v[m], F);
+        const double c = d.coef[m];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) acc[f] += c * F[f];
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) rhs.p[f][idx] = PLUS ? old[f] + acc[f] : acc[f];
+}
+
+// The listed rows of the derivative (closures with their B terms, interior
+// rows of a line's third and later segments) for the five fields, one lane
+// per row: the columns' fluxes computed from U, the B terms from the negated
+// fluxes at the cut points of R(dir) (FR), each field summed as
+// row_in_block sums it.
+template <int W, bool PLUS>
+__global__ __launch_bounds__(256) void k_euler_dflux_list(long n, const SlowRow* __restrict__ rows, DevDeriv d,
+                                                          int dir, double g, FiveC U, FiveC FR, FiveD rhs)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SlowRow e = rows[i];
+    const shx_line_block blk = d.blocks[e.b];
+    const long st = d.stride;
+    const int lr = e.q - blk.lo;
+    const int total = blk.nl + blk.ni + blk.nr;
+    const long base = e.idx - (long)e.q * st;
+    const double* a;
+    long c0;
+    int ncols;
+    if (lr < blk.nl) {
+        a = d.pool + blk.lcoef + (long)lr * blk.lcols;
+        c0 = base + (long)blk.lcol0 * st;
+        ncols = blk.lcols;
+    } else if (lr < blk.nl + blk.ni) {
+        a = d.coef;
+        c0 = e.idx - (long)(W / 2) * st;
+        ncols = W;
+    } else {
+        const int rr = lr - blk.nl - blk.ni;
+        a = d.pool + blk.rcoef + (long)rr * blk.rcols;
+        c0 = base + (long)blk.rcol0 * st;
+        ncols = blk.rcols;
+    }
+    double dot[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < ncols; ++c) {
+        const long at = c0 + (long)c * st;
+        double Uc[5], F[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) Uc[f] = U.p[f][at];
+        neg_flux(g, dir, Uc, F);
+        const double w = a[c];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) dot[f] += w * F[f];
+    }
+    const bool bl = blk.lobj >= 0 && lr < blk.nl;
+    const bool br = blk.robj >= 0 && lr >= total - blk.nr;
+    const double wl = bl ? d.pool[blk.lbw + lr] : 0.0;
+    const double wr = br ? d.pool[blk.rbw + (lr - (total - blk.nr))] : 0.0;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        RowVal r;
+        r.dot = dot[f];
+        r.b0 = r.b1 = 0.0;
+        r.nb = 0;
+        r.row = true;
+        if (bl) {
+            r.b0 = wl * FR.p[f][blk.lobj];
+            r.nb = 1;
+        }
+        if (br) {
+            const double v = wr * FR.p[f][blk.robj];
+            if (r.nb == 0) r.b0 = v;
+            else r.b1 = v;
+            r.nb++;
+        }
+        rhs.p[f][e.idx] = PLUS ? row_plus(rhs.p[f][e.idx], r) : row_eq(r);
+    }
 }
 
 // Slip wall: each cut point takes its fluid neighbour's state with the
@@ -208,23 +341,57 @@ int wall_launch(shx_euler* e, shx_field* U)
     return check(e->ctx, "wall kernel");
 }
 
+template <int DIR, int W>
+void dflux_launch(shx_euler* e, const shx_derivative* d, bool plus, const FiveC& U, const FiveC& FR, const FiveD& R)
+{
+    hipStream_t st = e->ctx->stream;
+    const Tiling t = make_tiling(d->n);
+    const double g = e->gamma;
+    if (plus) k_euler_dflux<DIR, W, true><<<t.nblocks, kBlock, 0, st>>>(t, d->dev, g, U, R);
+    else k_euler_dflux<DIR, W, false><<<t.nblocks, kBlock, 0, st>>>(t, d->dev, g, U, R);
+    if (d->nslow == 0) return;
+    const unsigned grid = (unsigned)((d->nslow + 255) / 256);
+    if (plus) k_euler_dflux_list<W, true><<<grid, 256, 0, st>>>(d->nslow, d->slow, d->dev, DIR, g, U, FR, R);
+    else k_euler_dflux_list<W, false><<<grid, 256, 0, st>>>(d->nslow, d->slow, d->dev, DIR, g, U, FR, R);
+}
+
 int rhs_launch(shx_euler* e, const shx_field* U, shx_field* rhs)
 {
     hipStream_t st = e->ctx->stream;
     slot5<kSlotZero>(e, rhs, nullptr, 0.0, nullptr);
     int rc = check(e->ctx, "rhs fill");
-    bool first = true;
+    // negated fluxes at the cut points (the B terms' values): part 0 empty
     Five4 fs = five4(e, nullptr, U, nullptr);
     for (int k = 0; k < 5; ++k) {
-        fs.d[k][0] = e->F[k];
+        fs.a[k][0] = nullptr;
         for (int c = 0; c < 3; ++c) fs.d[k][1 + c] = e->FR[k][c];
     }
+    fs.off[1] = 0;
+    for (int c = 0; c < 3; ++c) fs.off[2 + c] = fs.off[1 + c] + e->nR[c];
+    FiveC Ud;
+    FiveD Rd;
+    for (int k = 0; k < 5; ++k) {
+        Ud.p[k] = U[k].D;
+        Rd.p[k] = rhs[k].D;
+    }
+    bool first = true;
     for (int dir = 0; dir < 3 && rc == SHX_OK; ++dir) {
-        if (e->n[dir] < 2 || !e->d[dir]) continue;
-        k_flux_neg<<<blocks_for(fs.off[4]), kT, 0, st>>>(fs, dir, e->gamma);
-        rc = check(e->ctx, "flux kernel");
-        for (int k = 0; k < 5 && rc == SHX_OK; ++k)
-            rc = launch_derivative(e->d[dir], e->F[k], e->FR[k][dir], rhs[k].D, !first, true);
+        const shx_derivative* d = e->d[dir];
+        if (e->n[dir] < 2 || !d) continue;
+        if (fs.off[4] > 0) k_flux_neg<<<blocks_for(fs.off[4]), kT, 0, st>>>(fs, dir, e->gamma);
+        FiveC FR;
+        for (int k = 0; k < 5; ++k) FR.p[k] = e->FR[k][dir];
+        const int key = dir * 10 + d->dev.width;
+        switch (key) {
+        case 3: dflux_launch<0, 3>(e, d, !first, Ud, FR, Rd); break;
+        case 5: dflux_launch<0, 5>(e, d, !first, Ud, FR, Rd); break;
+        case 13: dflux_launch<1, 3>(e, d, !first, Ud, FR, Rd); break;
+        case 15: dflux_launch<1, 5>(e, d, !first, Ud, FR, Rd); break;
+        case 23: dflux_launch<2, 3>(e, d, !first, Ud, FR, Rd); break;
+        case 25: dflux_launch<2, 5>(e, d, !first, Ud, FR, Rd); break;
+        default: rc = fail(e->ctx, SHX_EUNSUPPORTED, "euler: interior width must be 3 or 5");
+        }
+        if (rc == SHX_OK) rc = check(e->ctx, "euler flux-derivative kernels");
         first = false;
     }
     if (rc != SHX_OK) return rc;
@@ -297,7 +464,7 @@ int shx_euler_create(shx_ctx* ctx, shx_derivative* const d[3], const shx_euler_d
     e->nfaces = (long)faces.size();
     if (rc == SHX_OK) rc = upload_n(ctx, &e->faces, faces.data(), faces.size());
     for (int k = 0; k < 5 && rc == SHX_OK; ++k) {
-        hipError_t err = hipMalloc((void**)&e->F[k], sizeof(double) * (size_t)e->N);
+        hipError_t err = hipSuccess;
         for (int c = 0; c < 3 && err == hipSuccess; ++c)
             if (e->nR[c] > 0) err = hipMalloc((void**)&e->FR[k][c], sizeof(double) * (size_t)e->nR[c]);
         if (err != hipSuccess) rc = hip_fail(ctx, err, "euler flux buffers");
@@ -320,7 +487,6 @@ int shx_euler_destroy(shx_euler* e)
     }
     hipFree(e->faces);
     for (int k = 0; k < 5; ++k) {
-        hipFree(e->F[k]);
         for (int c = 0; c < 3; ++c) hipFree(e->FR[k][c]);
     }
     delete e;
