@@ -443,13 +443,14 @@ def euler192(args):
     # direction fluxes (80) and five applies (80 eq, 120 plus_eq), negate (80);
     # slot ops: stage inputs 3 x 120, accumulations 4 x 120, final 120, copies
     # and zeroing 240
-    # per rhs: zero five fields (40); per direction one flux-derivative pass
-    # that reads the five conserved fields (40) and writes the five rhs
-    # fields (40), reading them too after the first direction (+40); per
-    # step: three stage inputs (5 x 24 each), rk from zero (5 x 16) then
-    # three updates (5 x 24), the final combination (5 x 24), out <- u0 and
-    # u0 <- out (5 x 16 each)
-    bytes_pt = 4 * (40 + 80 + 2 * 120) + 3 * 120 + 80 + 3 * 120 + 120 + 2 * 80
+    # per rhs and direction one flux-derivative pass that reads the five
+    # conserved fields (40) and writes the five rhs fields (40), reading them
+    # too after the first direction (+40) (the points without a row of the
+    # first direction, zeroed from a list, are not counted); per step: three
+    # stage inputs (5 x 24 each), rk from zero (5 x 16) then three updates
+    # (5 x 24), the final combination (5 x 24), out <- u0 and u0 <- out
+    # (5 x 16 each)
+    bytes_pt = 4 * (80 + 2 * 120) + 3 * 120 + 80 + 3 * 120 + 120 + 2 * 80
     achieved = bytes_pt * N / (ms * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu_baseline:

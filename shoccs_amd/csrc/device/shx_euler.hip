@@ -9,7 +9,8 @@
 // reference's operators, and every step is plain FP64 arithmetic (no
 // transcendentals), so a CPU restatement of it agrees to the last bit.
 //
-// Per rhs: one fill zeroes the five rhs fields; for each direction, one kernel
+// Per rhs: the points without a row of the first direction and the cut
+// points of the five rhs fields are zeroed; for each direction, one kernel
 // writes the negated fluxes at the cut points (the B terms' values), then one
 // pass applies the direction's first derivative to the negated fluxes of the
 // five fields at once, computing each tap's flux from the conserved values
@@ -43,6 +44,8 @@ struct shx_euler {
     double* wall_normal[3] = {nullptr, nullptr, nullptr};
     int* faces = nullptr;  // Dirichlet grid-face points
     long nfaces = 0;
+    int* norow = nullptr;  // mesh points with no row of the first direction
+    long nnorow = 0;
     double* FR[5][3] = {};  // one direction's negated fluxes at the cut points
 };
 
@@ -166,6 +169,50 @@ __global__ __launch_bounds__(kBlock) void k_euler_dflux(Tiling t, DevDeriv d, do
     }
 #pragma unroll
     for (int f = 0; f < 5; ++f) rhs.p[f][idx] = PLUS ? old[f] + acc[f] : acc[f];
+}
+
+// The same for rows along x with XP consecutive x planes per lane: the
+// negated fluxes of the XP + W - 1 planes the lane's rows read are computed
+// once, each from the conserved values of its plane, and shared by the XP
+// rows.  t counts groups of XP planes along x.
+template <int W, bool PLUS, int XP>
+__global__ __launch_bounds__(kBlock) void k_euler_dflux_x(Tiling t, int n0, DevDeriv d, double g, FiveC U,
+                                                          FiveD rhs)
+{
+    constexpr int hw = W / 2;
+    int grp, j, k;
+    long unused;
+    if (!tile_point(t, grp, j, k, unused)) return;
+    const long plane = (long)t.n1 * t.n2, col = (long)j * t.n2 + k;
+    const int i0 = grp * XP;
+    const unsigned long long fd = d.fast[j * t.n2 + k];
+    double F[XP + W - 1][5];
+#pragma unroll
+    for (int m = 0; m < XP + W - 1; ++m) {
+        const int qq = min(max(i0 + m - hw, 0), d.nq - 1);
+        double Um[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) Um[f] = U.p[f][(long)qq * plane + col];
+        neg_flux(g, 0, Um, F[m]);
+    }
+    const int lo0 = (int)(fd & 0xffffu), hi0 = (int)((fd >> 16) & 0xffffu);
+    const int lo1 = (int)((fd >> 32) & 0xffffu), hi1 = (int)(fd >> 48);
+#pragma unroll
+    for (int p = 0; p < XP; ++p) {
+        const int q = i0 + p;
+        if (q >= n0) break;
+        if (!((q >= lo0 && q < hi0) || (q >= lo1 && q < hi1))) continue;
+        const long idx = (long)q * plane + col;
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int m = 0; m < W; ++m) {
+            const double c = d.coef[m];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) acc[f] += c * F[p + m][f];
+        }
+#pragma unroll
+        for (int f = 0; f < 5; ++f) rhs.p[f][idx] = PLUS ? rhs.p[f][idx] + acc[f] : acc[f];
+    }
 }
 
 // The listed rows of the derivative (closures with their B terms, interior
@@ -345,10 +392,19 @@ template <int DIR, int W>
 void dflux_launch(shx_euler* e, const shx_derivative* d, bool plus, const FiveC& U, const FiveC& FR, const FiveD& R)
 {
     hipStream_t st = e->ctx->stream;
-    const Tiling t = make_tiling(d->n);
     const double g = e->gamma;
-    if (plus) k_euler_dflux<DIR, W, true><<<t.nblocks, kBlock, 0, st>>>(t, d->dev, g, U, R);
-    else k_euler_dflux<DIR, W, false><<<t.nblocks, kBlock, 0, st>>>(t, d->dev, g, U, R);
+    // rows along x: four planes per lane when the grid keeps at least eight
+    // workgroups per CU
+    const int g4[3] = {(d->n[0] + kDerivPlanes - 1) / kDerivPlanes, d->n[1], d->n[2]};
+    const Tiling t4 = make_tiling(g4);
+    if (DIR == 0 && t4.nblocks >= 8u * 256u) {
+        if (plus) k_euler_dflux_x<W, true, kDerivPlanes><<<t4.nblocks, kBlock, 0, st>>>(t4, d->n[0], d->dev, g, U, R);
+        else k_euler_dflux_x<W, false, kDerivPlanes><<<t4.nblocks, kBlock, 0, st>>>(t4, d->n[0], d->dev, g, U, R);
+    } else {
+        const Tiling t = make_tiling(d->n);
+        if (plus) k_euler_dflux<DIR, W, true><<<t.nblocks, kBlock, 0, st>>>(t, d->dev, g, U, R);
+        else k_euler_dflux<DIR, W, false><<<t.nblocks, kBlock, 0, st>>>(t, d->dev, g, U, R);
+    }
     if (d->nslow == 0) return;
     const unsigned grid = (unsigned)((d->nslow + 255) / 256);
     if (plus) k_euler_dflux_list<W, true><<<grid, 256, 0, st>>>(d->nslow, d->slow, d->dev, DIR, g, U, FR, R);
@@ -358,7 +414,17 @@ void dflux_launch(shx_euler* e, const shx_derivative* d, bool plus, const FiveC&
 int rhs_launch(shx_euler* e, const shx_field* U, shx_field* rhs)
 {
     hipStream_t st = e->ctx->stream;
-    slot5<kSlotZero>(e, rhs, nullptr, 0.0, nullptr);
+    // the first direction assigns every mesh point that has a row of it;
+    // the others and the cut points start from zero
+    {
+        Five4 z = five4(e, rhs, nullptr, nullptr);
+        for (int k = 0; k < 5; ++k) z.d[k][0] = nullptr;
+        z.off[1] = 0;
+        for (int c = 0; c < 3; ++c) z.off[2 + c] = z.off[1 + c] + e->nR[c];
+        if (z.off[4] > 0) k_slot5<kSlotZero><<<dim3(blocks_for(z.off[4]), 5), kT, 0, st>>>(z, 0.0);
+        if (e->nnorow)
+            k_zero_at5<<<blocks_for(e->nnorow), kT, 0, st>>>(e->nnorow, e->norow, five4(e, rhs, nullptr, nullptr));
+    }
     int rc = check(e->ctx, "rhs fill");
     // negated fluxes at the cut points (the B terms' values): part 0 empty
     Five4 fs = five4(e, nullptr, U, nullptr);
@@ -463,6 +529,34 @@ int shx_euler_create(shx_ctx* ctx, shx_derivative* const d[3], const shx_euler_d
     }
     e->nfaces = (long)faces.size();
     if (rc == SHX_OK) rc = upload_n(ctx, &e->faces, faces.data(), faces.size());
+    // mesh points without a row of the first present direction (solid
+    // points, dropped Dirichlet rows, lines on a Dirichlet face of another
+    // direction): the rhs zeroes them, the first direction assigns the rest
+    {
+        int c0 = -1;
+        for (int c = 0; c < 3 && c0 < 0; ++c)
+            if (e->n[c] >= 2 && d[c]) c0 = c;
+        std::vector<char> row((size_t)e->N, 0);
+        if (c0 >= 0) {
+            const shx_derivative* dc = d[c0];
+            const int nfast = c0 == 2 ? e->n[1] : e->n[2];
+            const long st = c0 == 0 ? (long)e->n[1] * e->n[2] : c0 == 1 ? e->n[2] : 1;
+            for (const shx_line_block& blk : dc->h_blocks) {
+                const int sl = blk.pair / nfast, f = blk.pair % nfast;
+                long base;
+                if (c0 == 0) base = (long)sl * e->n[2] + f;
+                else if (c0 == 1) base = (long)sl * e->n[1] * e->n[2] + f;
+                else base = ((long)sl * e->n[1] + f) * e->n[2];
+                const int total = blk.nl + blk.ni + blk.nr;
+                for (int q = blk.lo; q < blk.lo + total; ++q) row[(size_t)(base + (long)q * st)] = 1;
+            }
+        }
+        std::vector<int> norow;
+        for (long i = 0; i < e->N; ++i)
+            if (!row[(size_t)i]) norow.push_back((int)i);
+        e->nnorow = (long)norow.size();
+        if (rc == SHX_OK) rc = upload_n(ctx, &e->norow, norow.data(), norow.size());
+    }
     for (int k = 0; k < 5 && rc == SHX_OK; ++k) {
         hipError_t err = hipSuccess;
         for (int c = 0; c < 3 && err == hipSuccess; ++c)
@@ -486,6 +580,7 @@ int shx_euler_destroy(shx_euler* e)
         hipFree(e->wall_normal[c]);
     }
     hipFree(e->faces);
+    hipFree(e->norow);
     for (int k = 0; k < 5; ++k) {
         for (int c = 0; c < 3; ++c) hipFree(e->FR[k][c]);
     }

@@ -23,6 +23,10 @@ CASES = {
                                    grid_bc=[D, F, D, F, F, F]),
     "vortex2d": euler_cfg((41, 41), (-5, -5, 0), (5, 5, 0)),
 }
+# large enough for the four-planes-per-lane pass along x (at least 2048
+# workgroups of four planes), extents off the tile and plane multiples
+LARGE = {"sphere3d_multi_plane": euler_cfg((130, 66, 258), (0, 0, 0), (1.0, 0.5, 2.0),
+                                           spheres=[(0.5, 0.26, 1.0, 0.2)], grid_bc=[D, F, F, D, F, F])}
 
 
 def _state(e, name, rng):
@@ -48,7 +52,7 @@ def _state(e, name, rng):
 def _pair(ctx, name):
     import shoccs_amd as S
 
-    cfg = CASES[name]
+    cfg = {**CASES, **LARGE}[name]
     e = O.OracleEuler(cfg)
     pcfg = dict(kind=O.SYS_HEAT, n=cfg["n"], lo=cfg["lo"], hi=cfg["hi"], spheres=cfg["spheres"],
                 obj_bc=[D] * len(cfg["spheres"]), grid_bc=cfg["grid_bc"], scheme=cfg["scheme"], alpha=cfg["alpha"],
@@ -67,7 +71,7 @@ def _flat(U):
     return [u.flat() for u in U]
 
 
-@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("name", list(CASES) + list(LARGE))
 def test_wall_and_rhs_bitexact(ctx, name):
     e, prob, ge = _pair(ctx, name)
     U = _state(e, name, np.random.default_rng(3))
