@@ -343,7 +343,9 @@ int shx_system_rhs_T(shx_system* s, const shx_field* kbar, shx_field* wbar);
  * entries that the boundary update overwrites (Dirichlet faces and cut
  * points) carry no sensitivity through it.  When coef_grad is not NULL, the
  * step's dJ/dcoef[c][m] (coef: the stored interior coefficients of direction
- * c's operator) is added to coef_grad[9 c + m] on the host.  Synchronous. */
+ * c's operator) is added to coef_grad[9 c + m] (host memory, 27 entries; the
+ * sum runs on the device over the step's four stages, in stage order, and is
+ * copied back once).  Synchronous. */
 int shx_rk4_adjoint(shx_system* s, const shx_field* u, shx_field* ubar, double t, double dt,
                     double* coef_grad);
 

@@ -48,7 +48,7 @@ struct shx_adjoint_ws {
     shx_field k, g, kb, wb;
     double* ybar[3] = {nullptr, nullptr, nullptr};
     double* part = nullptr;  // coefficient-gradient partials
-    double* gsum = nullptr;  // 9 reduced values
+    double* gacc = nullptr;  // coefficient gradient accumulated on the device, 3 x 9
     long npart = 0;
 };
 
@@ -200,14 +200,26 @@ __global__ __launch_bounds__(kT) void k_coef_part(long N, long st, const unsigne
     if (threadIdx.x < W) part[(long)blockIdx.x * W + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// out[m] = sum over workgroups, in workgroup order
-__global__ void k_coef_reduce(int nb, int W, const double* __restrict__ part, double* __restrict__ out)
+// acc[m] += (sum over workgroups of part[b][m], from 0.0 in workgroup order).
+// The partials are staged through LDS a chunk at a time by the whole
+// workgroup (coalesced loads in flight together); lane m then adds its
+// column of the chunk in order, so the sum is the sequential one.
+constexpr int kReduceChunk = 256;
+__global__ __launch_bounds__(kT) void k_coef_reduce(int nb, int W, const double* __restrict__ part,
+                                                    double* __restrict__ acc)
 {
+    __shared__ double buf[kReduceChunk * 9];
     const int m = threadIdx.x;
-    if (m >= W) return;
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[(long)b * W + m];
-    out[m] = s;
+    for (int b0 = 0; b0 < nb; b0 += kReduceChunk) {
+        const int nc = min(kReduceChunk, nb - b0);
+        for (int e = threadIdx.x; e < nc * W; e += blockDim.x) buf[e] = part[(long)b0 * W + e];
+        __syncthreads();
+        if (m < W)
+            for (int b = 0; b < nc; ++b) s += buf[b * W + m];
+        __syncthreads();
+    }
+    if (m < W) acc[m] += s;
 }
 
 // ------------------------------------------------------------ transpose build
@@ -422,7 +434,7 @@ int ensure_ws(shx_system* s)
     if (rc == SHX_OK) {
         w->npart = std::min<long>(2048, std::max<long>(1, (s->N + kT - 1) / kT));
         hipError_t e = hipMalloc((void**)&w->part, sizeof(double) * (size_t)w->npart * 9);
-        if (e == hipSuccess) e = hipMalloc((void**)&w->gsum, sizeof(double) * 9);
+        if (e == hipSuccess) e = hipMalloc((void**)&w->gacc, sizeof(double) * 27);
         if (e != hipSuccess) rc = hip_fail(s->ctx, e, "adjoint workspace");
     }
     if (rc != SHX_OK) {
@@ -475,9 +487,10 @@ void project(shx_system* s, shx_field& f)
         if (s->nR[r]) k_zero_flag<<<blocks_for(s->nR[r]), kT, 0, st>>>(s->nR[r], s->rdir[r], f.R[r]);
 }
 
-// wbar = (d rhs / d u)^T kbar, and, when grad is given, the stage's
-// coefficient-gradient terms with stage input w added to grad[9 c + m]
-int rhs_T(shx_system* s, const shx_field& kbar, shx_field& wbar, const shx_field* w, double* grad)
+// wbar = (d rhs / d u)^T kbar, and, when gacc (device, 3 x 9) is given, the
+// stage's coefficient-gradient terms with stage input w added to
+// gacc[9 c + m]
+int rhs_T(shx_system* s, const shx_field& kbar, shx_field& wbar, const shx_field* w, double* gacc)
 {
     shx_adjoint_ws* ws = s->adj;
     hipStream_t st = s->ctx->stream;
@@ -497,7 +510,7 @@ int rhs_T(shx_system* s, const shx_field& kbar, shx_field& wbar, const shx_field
         const double* y = heat ? ws->ybar[0] : ws->ybar[c];
         rc = apply_T(d, y, wbar.D, wbar.R[c], !first);
         first = false;
-        if (rc != SHX_OK || !grad || !w) continue;
+        if (rc != SHX_OK || !gacc || !w) continue;
         const int W = d->dev.width;
         const int nb = (int)ws->npart;
         switch (W) {
@@ -505,12 +518,8 @@ int rhs_T(shx_system* s, const shx_field& kbar, shx_field& wbar, const shx_field
         case 5: k_coef_part<5><<<nb, kT, 0, st>>>(s->N, d->dev.stride, d->T->int_mask, y, w->D, ws->part); break;
         default: return fail(s->ctx, SHX_EUNSUPPORTED, "adjoint: coefficient gradient for widths 3 and 5");
         }
-        k_coef_reduce<<<1, 64, 0, st>>>(nb, W, ws->part, ws->gsum);
-        double h[9];
-        hipError_t e = hipMemcpyAsync(h, ws->gsum, sizeof(double) * W, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) return hip_fail(s->ctx, e, "adjoint: coefficient gradient");
-        for (int m = 0; m < W; ++m) grad[9 * c + m] += h[m];
+        k_coef_reduce<<<1, kT, 0, st>>>(nb, W, ws->part, gacc + 9 * c);
+        rc = check(s->ctx, "adjoint: coefficient gradient");
     }
     if (rc == SHX_OK && first) {  // no direction: the rhs does not depend on u
         k_fill0<<<blocks_for(s->N), kT, 0, st>>>(s->N, wbar.D);
@@ -551,7 +560,7 @@ void destroy_adjoint_ws(shx_adjoint_ws* w)
     free_field(w->wb);
     for (auto& y : w->ybar) hipFree(y);
     hipFree(w->part);
-    hipFree(w->gsum);
+    hipFree(w->gacc);
     delete w;
 }
 
@@ -618,9 +627,15 @@ int shx_rk4_adjoint(shx_system* s, const shx_field* u, shx_field* ubar, double t
     project(s, ws->g);  // g = P ubar'
     copy(ws->g, *ubar);  // ubar = g (u enters the final sum directly)
     field_scale(s, dt * kRkf[3], ws->g, ws->kb);  // kbar_3 = dt rkf[3] g
+    // the coefficient gradient accumulates on the device through the four
+    // stages (each stage's sum added to the running value, in stage order)
+    if (coef_grad) {
+        const hipError_t e = hipMemcpyAsync(ws->gacc, coef_grad, sizeof(double) * 27, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return hip_fail(s->ctx, e, "rk4 adjoint: coefficient gradient");
+    }
     rc = check(s->ctx, "rk4 adjoint");
     for (int stg = 3; stg >= 0 && rc == SHX_OK; --stg) {
-        rc = rhs_T(s, ws->kb, ws->wb, w[stg], coef_grad);
+        rc = rhs_T(s, ws->kb, ws->wb, w[stg], coef_grad ? ws->gacc : nullptr);
         if (rc != SHX_OK) break;
         if (stg > 0) {
             project(s, ws->wb);
@@ -631,6 +646,10 @@ int shx_rk4_adjoint(shx_system* s, const shx_field* u, shx_field* ubar, double t
             field_add(s, ws->wb, *ubar);
         }
         rc = check(s->ctx, "rk4 adjoint");
+    }
+    if (rc == SHX_OK && coef_grad) {
+        const hipError_t e = hipMemcpyAsync(coef_grad, ws->gacc, sizeof(double) * 27, hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return hip_fail(s->ctx, e, "rk4 adjoint: coefficient gradient");
     }
     return rc == SHX_OK ? sync(s->ctx) : rc;
 }
