@@ -446,11 +446,11 @@ def euler192(args):
     # per rhs and direction one flux-derivative pass that reads the five
     # conserved fields (40) and writes the five rhs fields (40), reading them
     # too after the first direction (+40) (the points without a row of the
-    # first direction, zeroed from a list, are not counted); per step: three
-    # stage inputs (5 x 24 each), rk from zero (5 x 16) then three updates
-    # (5 x 24), the final combination (5 x 24), out <- u0 and u0 <- out
-    # (5 x 16 each)
-    bytes_pt = 4 * (80 + 2 * 120) + 3 * 120 + 80 + 3 * 120 + 120 + 2 * 80
+    # first direction, zeroed from a list, are not counted); per stage one
+    # slot pass: rk from zero and the next input (read srhs, u0; write rk,
+    # out: 160), then rk += c srhs with the next input or the result (read
+    # rk, srhs, u0; write rk, out: 200) three times; u0 <- out (80)
+    bytes_pt = 4 * (80 + 2 * 120) + 160 + 3 * 200 + 80
     achieved = bytes_pt * N / (ms * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu_baseline:
@@ -464,8 +464,8 @@ def euler192(args):
                                f"{nsteps} RK4 steps at CFL 0.4 (dt {info['dt']:.4g}), E4u operators (the scalar-wave "
                                "closure parameters), inflow face held",
                    "form": "RK4 slot schedule over five fields; per direction one pass applies the derivative "
-                           "to the fluxes computed at its taps (five fields at once); each slot operation one "
-                           "launch over the five fields"},
+                           "to the fluxes computed at its taps (five fields at once); one slot pass per stage "
+                           "over the five fields"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": None, "bytes_per_pt_step": bytes_pt},
         "timed_run": info, "stability_companion": companion,

@@ -19,8 +19,9 @@
 // direction assigns, the others add; a last kernel zeroes the Dirichlet
 // faces.  Negation is exact and commutes with rounding to nearest, so summing
 // the products of the negated fluxes gives -(the sum of the products): an
-// exact zero may take the other sign, no other value changes.  Every slot
-// operation of the RK4 schedule is one launch over all five fields.
+// exact zero may take the other sign, no other value changes.  After each
+// stage's rhs one pass over all five fields performs the slot operations of
+// the RK4 schedule that follow it (rk update, next stage input or result).
 // HBM-bound: per direction the five conserved fields are read once (40 B/pt)
 // and the five rhs fields written (and read after the first direction).
 #include <hip/hip_runtime.h>
@@ -322,11 +323,10 @@ __global__ void k_zero_at5(long n, const int* __restrict__ sel, Five4 s)
     for (int k = 0; k < 5; ++k) s.d[k][0][i] = 0.0;
 }
 
-// Slot operations of rk4.cpp:16-47 (slot_ops.hpp:23-59) on field
-// blockIdx.y of five: d = 0, d = a, d = a + c b, d += c b, d = 0 + c b.
-enum { kSlotZero, kSlotCopy, kSlotLc, kSlotAcc, kSlotAccFromZero };
+// d = 0 or d = a on field blockIdx.y of five (slot_ops.hpp:23-59).
+enum { kSlotZero, kSlotCopy };
 template <int OP>
-__global__ void k_slot5(Five4 s, double c)
+__global__ void k_slot5(Five4 s, double)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.off[4]) return;
@@ -334,18 +334,33 @@ __global__ void k_slot5(Five4 s, double c)
     const int p = part_of(s, i);
     const long j = i - base_of(s, p);
     double* d = pick4(p, s.d[k][0], s.d[k][1], s.d[k][2], s.d[k][3]);
-    if (OP == kSlotZero) {
-        d[j] = 0.0;
-    } else if (OP == kSlotCopy) {
-        d[j] = pick4(p, s.a[k][0], s.a[k][1], s.a[k][2], s.a[k][3])[j];
-    } else if (OP == kSlotLc) {
-        const double x = pick4(p, s.a[k][0], s.a[k][1], s.a[k][2], s.a[k][3])[j];
-        d[j] = x + c * pick4(p, s.b[k][0], s.b[k][1], s.b[k][2], s.b[k][3])[j];
-    } else if (OP == kSlotAcc) {
-        d[j] += c * pick4(p, s.b[k][0], s.b[k][1], s.b[k][2], s.b[k][3])[j];
-    } else {
-        d[j] = 0.0 + c * pick4(p, s.b[k][0], s.b[k][1], s.b[k][2], s.b[k][3])[j];
-    }
+    d[j] = OP == kSlotZero ? 0.0 : pick4(p, s.a[k][0], s.a[k][1], s.a[k][2], s.a[k][3])[j];
+}
+
+// After stage i's rhs (b = srhs, d = rk, a = u0), in one pass: rk += c1 b
+// (rk = 0 + c1 b at the first stage), then the next stage input o = a + c2 b,
+// or, after the last stage, o = a + 1.0 rk with the updated rk: the slot
+// operations of rk4.cpp:16-47 that follow one another, each with the same
+// arithmetic.
+enum { kStepFirst, kStepMid, kStepLast };
+struct Out4 {
+    double* o[5][4];
+};
+template <int OP>
+__global__ void k_step_slots(Five4 s, Out4 o, double c1, double c2)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.off[4]) return;
+    const int k = (int)blockIdx.y;
+    const int p = part_of(s, i);
+    const long j = i - base_of(s, p);
+    double* rk = pick4(p, s.d[k][0], s.d[k][1], s.d[k][2], s.d[k][3]);
+    const double u0 = pick4(p, s.a[k][0], s.a[k][1], s.a[k][2], s.a[k][3])[j];
+    const double sr = pick4(p, s.b[k][0], s.b[k][1], s.b[k][2], s.b[k][3])[j];
+    double* out = pick4(p, o.o[k][0], o.o[k][1], o.o[k][2], o.o[k][3]);
+    const double r = (OP == kStepFirst ? 0.0 : rk[j]) + c1 * sr;
+    rk[j] = r;
+    out[j] = OP == kStepLast ? u0 + 1.0 * r : u0 + c2 * sr;
 }
 
 Five4 five4(const shx_euler* e, shx_field* d, const shx_field* a, const shx_field* b)
@@ -607,25 +622,29 @@ int shx_euler_rk4_step(shx_euler* e, shx_field* u0, shx_field* out, shx_field* r
     if (!e || !u0 || !out || !rk || !srhs) return SHX_EINVAL;
     static const double rki[4] = {0.0, 0.5, 0.5, 1.0};
     static const double rkf[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-    // rk starts from zero (its first update is 0 + c srhs); srhs is
-    // overwritten whole by every rhs
-    slot5<kSlotCopy>(e, out, u0, 0.0, nullptr);
-    int rc = check(e->ctx, "euler copy");
-    for (int i = 0; i < 4 && rc == SHX_OK; ++i) {
-        if (i > 0) {
-            slot5<kSlotLc>(e, out, u0, dt * rki[i], srhs);
-            rc = wall_launch(e, out);
-        }
-        if (rc == SHX_OK) rc = rhs_launch(e, out, srhs);
-        if (rc == SHX_OK) {
-            if (i == 0) slot5<kSlotAccFromZero>(e, rk, nullptr, dt * rkf[i], srhs);
-            else slot5<kSlotAcc>(e, rk, nullptr, dt * rkf[i], srhs);
-            rc = check(e->ctx, "euler slot kernels");
-        }
+    // Stage 0 reads u0 itself (the schedule's out = u0 copy is its only
+    // use before out is overwritten whole).  After each stage's rhs one pass
+    // updates rk (from zero at the first stage) and forms the next stage
+    // input, or the step's result after the last stage; srhs is overwritten
+    // whole by every rhs.
+    int rc = SHX_OK;
+    Out4 o;
+    for (int k = 0; k < 5; ++k) {
+        o.o[k][0] = out[k].D;
+        for (int c = 0; c < 3; ++c) o.o[k][1 + c] = out[k].R[c];
     }
-    if (rc == SHX_OK) {
-        slot5<kSlotLc>(e, out, u0, 1.0, rk);
-        rc = wall_launch(e, out);
+    const Five4 sl = five4(e, rk, u0, srhs);
+    const dim3 grid(blocks_for(sl.off[4]), 5);
+    hipStream_t st = e->ctx->stream;
+    for (int i = 0; i < 4 && rc == SHX_OK; ++i) {
+        rc = rhs_launch(e, i == 0 ? u0 : out, srhs);
+        if (rc != SHX_OK) break;
+        const double c1 = dt * rkf[i];
+        if (i == 0) k_step_slots<kStepFirst><<<grid, kT, 0, st>>>(sl, o, c1, dt * rki[1]);
+        else if (i < 3) k_step_slots<kStepMid><<<grid, kT, 0, st>>>(sl, o, c1, dt * rki[i + 1]);
+        else k_step_slots<kStepLast><<<grid, kT, 0, st>>>(sl, o, c1, 1.0);
+        rc = check(e->ctx, "euler slot kernels");
+        if (rc == SHX_OK) rc = wall_launch(e, out);
     }
     return rc == SHX_OK ? sync(e->ctx) : rc;
 }
