@@ -439,11 +439,7 @@ def euler192(args):
 
     ms, N, info = run(E4U_ALPHA, 0.4, True)
     _, _, companion = run((0.0, 0.0), 0.1, False)
-    # bytes the schedule moves per point and step: per rhs, zero (40), per
-    # direction fluxes (80) and five applies (80 eq, 120 plus_eq), negate (80);
-    # slot ops: stage inputs 3 x 120, accumulations 4 x 120, final 120, copies
-    # and zeroing 240
-    # per rhs and direction one flux-derivative pass that reads the five
+    # bytes the schedule moves per point and step: per rhs and direction one flux-derivative pass that reads the five
     # conserved fields (40) and writes the five rhs fields (40), reading them
     # too after the first direction (+40) (the points without a row of the
     # first direction, zeroed from a list, are not counted); per stage one

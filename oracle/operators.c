@@ -189,12 +189,14 @@ static const double* interp_deriv_coeffs(int dir, int n, double h, const orc_rin
 /* cut_discretization (derivative.cpp:159-257) with OB_builder (:69-157):
  * rows of R(r) for the derivative along dir, columns D (O) and R(r) (B) */
 static void cut_discretization(int r, int dir, const orc_mesh* m, const orc_stencil* st,
-                               const int* obj_bc, orc_csr* O, orc_csr* B)
+                               const int* pbc, orc_csr* O, orc_csr* B)
 {
     const long sz = m->nR[r];
     tbuilder Ob = {0}, Bb = {0};
+    /* the reference's quick exit tests the shapes' conditions; skipping the
+     * Dirichlet points one by one below gives the same rows either way */
     int all_dirichlet = 1;
-    for (int k = 0; k < m->nshapes; k++) all_dirichlet &= obj_bc[k] == ORC_DIRICHLET;
+    for (long k = 0; k < sz; k++) all_dirichlet &= pbc[k] == ORC_DIRICHLET;
     if (sz > 0 && !all_dirichlet) {
         const orc_info mx = orc_query_max(st);
         const double h = m->h[dir];
@@ -204,7 +206,7 @@ static void cut_discretization(int r, int dir, const orc_mesh* m, const orc_sten
         const long stride = dir == 0 ? (long)m->n[1] * m->n[2] : dir == 1 ? (long)m->n[2] : 1;
         for (long row = 0; row < sz; row++) {
             const orc_rinfo* obj = &m->R[r][row];
-            const int bc_t = obj_bc[obj->shape_id];
+            const int bc_t = pbc[row];
             if (bc_t == ORC_DIRICHLET) continue;
             if (dir == r) {
                 /* the object's own closure row, from its solid coordinate */
@@ -257,6 +259,19 @@ static void cut_discretization(int r, int dir, const orc_mesh* m, const orc_sten
 orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
                             const int grid_bc[6], const int* obj_bc)
 {
+    int* pbc[3];
+    for (int r = 0; r < 3; r++) {
+        pbc[r] = (int*)malloc((size_t)(m->nR[r] + 1) * sizeof(int));
+        for (long k = 0; k < m->nR[r]; k++) pbc[r][k] = obj_bc[m->R[r][k].shape_id];
+    }
+    orc_deriv* d = orc_deriv_create_pbc(m, dir, st, grid_bc, (const int* const*)pbc);
+    for (int r = 0; r < 3; r++) free(pbc[r]);
+    return d;
+}
+
+orc_deriv* orc_deriv_create_pbc(const orc_mesh* m, int dir, const orc_stencil* st,
+                                const int grid_bc[6], const int* const pbc[3])
+{
     orc_deriv* d = (orc_deriv*)calloc(1, sizeof(orc_deriv));
     d->dir = dir;
     d->N = orc_mesh_size(m);
@@ -301,7 +316,7 @@ orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
         memset(&b, 0, sizeof b);
 
         if (ln->start.has_obj) {
-            const int bc_t = obj_bc[ln->start.obj_id];
+            const int bc_t = pbc[dir][ln->start.obj_coord];
             orc_info q = orc_query(st, bc_t);
             orc_nbs(st, h, bc_t, ln->start.psi, 0, left, extra);
             int s = bc_t != ORC_DIRICHLET;
@@ -337,7 +352,7 @@ orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
         }
 
         if (ln->end.has_obj) {
-            const int bc_t = obj_bc[ln->end.obj_id];
+            const int bc_t = pbc[dir][ln->end.obj_coord];
             orc_info q = orc_query(st, bc_t);
             orc_nbs(st, h, bc_t, ln->end.psi, 1, right, extra);
             int s = bc_t != ORC_DIRICHLET;
@@ -401,7 +416,7 @@ orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
     free(left);
     free(right);
     /* derivative.cpp:470-472 */
-    for (int r = 0; r < 3; r++) cut_discretization(r, dir, m, st, obj_bc, &d->Bf[r], &d->Br[r]);
+    for (int r = 0; r < 3; r++) cut_discretization(r, dir, m, st, pbc[r], &d->Bf[r], &d->Br[r]);
     return d;
 }
 

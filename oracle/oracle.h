@@ -38,15 +38,21 @@
 extern "C" {
 #endif
 
-/* boundary condition types, src/operators/boundaries.hpp:13-20 */
-enum { ORC_DIRICHLET = 0, ORC_FLOATING = 1, ORC_NEUMANN = 2 };
+/* boundary condition types, src/operators/boundaries.hpp:13-20.  ORC_INFLOW
+ * (no reference counterpart) is a shape condition of the constant-velocity
+ * wave resolved per cut point: Dirichlet where the velocity leaves the body
+ * into the fluid (a . n > 0, n the body's outward normal), Floating
+ * elsewhere -- the reference's pattern of a Dirichlet upstream body and a
+ * floating downstream body (scalar_wave.lua:10-22) applied point by point. */
+enum { ORC_DIRICHLET = 0, ORC_FLOATING = 1, ORC_NEUMANN = 2, ORC_INFLOW = 3 };
 
 /* stencil families, src/stencils/stencil.cpp:8-106 */
-enum { ORC_E2_2 = 0, ORC_E4_2 = 1, ORC_E4_1 = 2, ORC_E4U_1 = 3 };
+enum { ORC_E2_2 = 0, ORC_E4_2 = 1, ORC_E4_1 = 2, ORC_E4U_1 = 3, ORC_E2P_1 = 4 };
 
 typedef struct { int p, r, t, x; } orc_info;
 typedef struct { int p, t; } orc_iinfo; /* stencil.hpp:29 interp_info */
-typedef struct { int kind; double alpha[2]; } orc_stencil;
+/* alpha: the scheme's free parameters (E4_1, E4u_1: 2; E2-poly: 13) */
+typedef struct { int kind; double alpha[13]; } orc_stencil;
 
 /* src/mesh/mesh_types.hpp:30-38 */
 typedef struct {
@@ -135,6 +141,7 @@ typedef struct {
     orc_mesh* m;
     int grid_bc[6];
     int* obj_bc;
+    int* pbc[3]; /* condition of each cut point of R(x), R(y), R(z) */
     orc_deriv* d[3];
     /* radial scalar wave (scalar_wave.cpp:28-40) */
     double center[3], radius;
@@ -180,9 +187,12 @@ int orc_interp_wall(const orc_stencil* st, int i, double y, double psi, double* 
  * dir; returns the stride */
 long orc_interp_line(const orc_mesh* m, int dir, const int pt[3], orc_bound* left, orc_bound* right);
 
-/* returns NULL on unsupported configuration */
+/* returns NULL on unsupported configuration; obj_bc: one condition per shape */
 orc_deriv* orc_deriv_create(const orc_mesh* m, int dir, const orc_stencil* st,
                             const int grid_bc[6], const int* obj_bc);
+/* the same with one condition per cut point (pbc[r][k] for point k of R(r)) */
+orc_deriv* orc_deriv_create_pbc(const orc_mesh* m, int dir, const orc_stencil* st,
+                                const int grid_bc[6], const int* const pbc[3]);
 void orc_deriv_destroy(orc_deriv*);
 void orc_block_apply(const orc_deriv* d, const double* x, double* b, int plus_eq);
 void orc_csr_apply(const orc_csr* A, const double* x, double* b);

@@ -10,6 +10,10 @@
  *   E4u_1 (P2 R3 T5) query/interior/nbs   src/stencils/E4u_1.cpp:26-126
  *   E4_2  (P2 R3 T5 X3)                   src/stencils/E4_2.cpp:16-42,185-296
  *   E2_2  (P1 R2 T4 X2)                   src/stencils/E2_2.cpp:12-180
+ *   E2-poly (P1 R3 T4; the 13 parameters are floating[6], dirichlet[3],
+ *         interpolant[4])                src/stencils/polyE2_1.cpp:10-257,
+ *                                        split as stencil.cpp:79-89 splits
+ *                                        one alpha list
  *
  * Right-wall closures are the left ones reversed (and negated for the first
  * derivative schemes).
@@ -41,6 +45,7 @@ orc_info orc_query_max(const orc_stencil* st)
     case ORC_E4_2: r.p = 2; r.r = 3; r.t = 5; r.x = 3; break;
     case ORC_E4_1: r.p = 2; r.r = 5; r.t = 7; r.x = 0; break;
     case ORC_E4U_1: r.p = 2; r.r = 3; r.t = 5; r.x = 0; break;
+    case ORC_E2P_1: r.p = 1; r.r = 3; r.t = 4; r.x = 0; break;
     }
     return r;
 }
@@ -54,7 +59,7 @@ orc_info orc_query(const orc_stencil* st, int bc)
     case ORC_FLOATING: m.x = 0; return m;
     case ORC_NEUMANN:
         /* first-derivative schemes define no Neumann closure */
-        if (st->kind == ORC_E4_1 || st->kind == ORC_E4U_1) return z;
+        if (st->kind == ORC_E4_1 || st->kind == ORC_E4U_1 || st->kind == ORC_E2P_1) return z;
         return m;
     }
     return z;
@@ -90,6 +95,68 @@ void orc_interior(const orc_stencil* st, double h, double* c)
         c[3] = -c[1];
         c[4] = -c[0];
         break;
+    case ORC_E2P_1:
+        c[0] = -0.5;
+        c[1] = 0;
+        c[2] = 0.5;
+        for (int i = 0; i < 3; i++) c[i] /= h;
+        break;
+    }
+}
+
+/* ---------------------------------------------------------------- E2-poly
+ * polyE2_1::nbs_floating / nbs_dirichlet (polyE2_1.cpp:166-245): 3 x 4 and
+ * 2 x 4 closures from the floating (fa) and Dirichlet (da) parameters; the
+ * rows are psi-dependent rational functions */
+static void e2p_floating(const double* fa, double h, double psi, double* c, int right)
+{
+    const double a0 = fa[0], a1 = fa[1], a2 = fa[2], a3 = fa[3], a4 = fa[4], a5 = fa[5];
+    const double inv = 1.0 / (1 + psi), two_psi = 2 + psi;
+    const double m1 = -1 + a1, m3 = -1 + a3, m5 = -1 + a5;
+    c[0] = 0.5 * inv * m1;
+    c[1] = (-1 + a0) * 0.5;
+    c[2] = -1 * a0 + -0.5 * two_psi * inv * m1;
+    c[3] = (a0 + a1) * 0.5;
+    c[4] = 0.5 * m3 * inv;
+    c[5] = (-1 + a2) * 0.5;
+    c[6] = -1 * a2 + -0.5 * two_psi * m3 * inv;
+    c[7] = (a3 + a2) * 0.5;
+    c[8] = 0.5 * m5 * inv;
+    c[9] = (-1 + a4) * 0.5;
+    c[10] = -1 * a4 + -0.5 * two_psi * m5 * inv;
+    c[11] = (a5 + a4) * 0.5;
+    for (int i = 0; i < 12; i++) c[i] /= h;
+    if (right) {
+        for (int i = 0; i < 12; i++) c[i] *= -1;
+        reverse(c, 12);
+    }
+}
+
+static void e2p_dirichlet(const double* da, double h, double psi, double* c, int right)
+{
+    const double d0 = da[0], d1 = da[1], d2 = da[2];
+    const double inv = 1.0 / (1 + psi), m0 = -1 + d0, two_psi = 2 * psi;
+    const double three_d2 = 3 * d2, two_psi_d2 = 2 * psi * d2;
+    const double den = 1.0 / (1 + two_psi + three_d2 + two_psi_d2);
+    const double two_psi_d1 = 2 * psi * d1, three_d1d0 = 3 * d1 * d0, two_psi_d1d0 = 2 * psi * d1 * d0;
+    const double neg_d1 = -1 * d1, psi2 = 2 + psi, m2 = -1 + d2;
+    c[0] = 0.5 * inv * m0;
+    c[1] = (-1 + -2 * psi + d1 + two_psi_d1 + three_d1d0 + two_psi_d1d0 + -3 * d0 + -2 * psi * d0) * 0.5 * den;
+    c[2] = (-2 * psi * d1 + -3 * d2 + -2 * psi * d2 + neg_d1 + 3 * d0 + 2 * psi * d0 + -3 * d1 * d0 +
+            -2 * psi * d1 * d0) *
+               den +
+           -0.5 * psi2 * inv * m0;
+    c[3] = (d1 + two_psi_d1 + three_d1d0 + two_psi_d1d0 + three_d2 + two_psi_d2 + -2 * d0 + 3 * d2 * d0 +
+            2 * psi * d2 * d0) *
+           0.5 * den;
+    c[4] = 0.5 * m2 * inv;
+    c[5] = (-1 + d1) * 0.5;
+    c[6] = neg_d1 + -0.5 * psi2 * m2 * inv;
+    c[7] = (d1 + d2) * 0.5;
+    for (int i = 0; i < 8; i++) c[i] /= h;
+    if (right) {
+        for (int i = 0; i < 8; i++) c[i] *= -1;
+        reverse(c, 8);
     }
 }
 
@@ -605,6 +672,10 @@ void orc_nbs(const orc_stencil* st, double h, int bc, double psi, int right, dou
         else if (bc == ORC_DIRICHLET) e22_dirichlet(h, psi, c, right);
         else e22_neumann(h, psi, c, extra, right);
         break;
+    case ORC_E2P_1:
+        if (bc == ORC_FLOATING) e2p_floating(st->alpha, h, psi, c, right);
+        else if (bc == ORC_DIRICHLET) e2p_dirichlet(st->alpha + 6, h, psi, c, right);
+        break;
     }
 }
 
@@ -612,19 +683,21 @@ void orc_nbs(const orc_stencil* st, double h, int bc, double psi, int right, dou
  * query_interp / interp_interior / interp_wall (stencil.hpp:201-211):
  *   E2_2: E2_2.cpp:31-86 (2 points in the interior, 3 at a wall)
  *   E4_2: E4_2.cpp:31-183 (5 points, 5 at a wall)
+ *   E2-poly: polyE2_1.cpp:42-138 (2 points in the interior, 4 at a wall)
  *   E4_1, E4u_1: none (E4_1.cpp:57-62, E4u_1.cpp:40-45): the span passed in
  *   is returned unchanged and its length is the caller's buffer. */
 orc_iinfo orc_query_interp(const orc_stencil* st)
 {
     orc_iinfo r = {0, 0};
     if (st->kind == ORC_E2_2) r.p = 2, r.t = 3;
+    else if (st->kind == ORC_E2P_1) r.p = 2, r.t = 4;
     else if (st->kind == ORC_E4_2) r.p = 5, r.t = 5;
     return r;
 }
 
 int orc_interp_interior(const orc_stencil* st, double y, double* c, int cap)
 {
-    if (st->kind == ORC_E2_2) {
+    if (st->kind == ORC_E2_2 || st->kind == ORC_E2P_1) {
         if (y > 0) {
             c[0] = 1 + -1 * y;
             c[1] = y;
@@ -754,9 +827,53 @@ static int e4_2_wall(int i, double y, double psi, double* c, int right)
     return 5;
 }
 
+/* polyE2_1::interp_wall (polyE2_1.cpp:52-138): rows 0 and 1 from the
+ * floating (fa) and interpolant (ia) parameters */
+static int e2p_wall(const double* fa, const double* ia, int i, double y, double psi, double* c, int right)
+{
+    const double inv = 1.0 / (1 + psi), ny = -1 * y;
+    if (right) {
+        const double f0y = fa[0] * y, f1y = fa[1] * y, f2y = fa[2] * y, f3y = fa[3] * y;
+        if (i == 0) {
+            c[0] = (f0y + f1y + ia[0] + ia[1]) * 0.5;
+            c[1] = (-1 * psi + ny + (f0y + ia[0]) * -2 +
+                    (ny + -2 * ia[1] + -1 * psi * ia[1] + -2 * fa[1] * y + -1 * psi * fa[1] * y) * inv) *
+                   0.5;
+            c[2] = (1 + psi + f0y + ia[0] + y) * 0.5;
+            c[3] = (1 + psi + f1y + ia[1] + y) * 0.5 * inv;
+        } else if (i == 1) {
+            c[0] = (ia[3] + f2y + f3y + ia[2]) * 0.5;
+            c[1] = (ny + (f2y + ia[2]) * -2 +
+                    (psi + -2 * ia[3] + -1 * psi * ia[3] + ny + -2 * fa[3] * y + -1 * psi * fa[3] * y) * inv) *
+                   0.5;
+            c[2] = (1 + f2y + ia[2] + y) * 0.5;
+            c[3] = (1 + ia[3] + f3y + y) * 0.5 * inv;
+        }
+    } else {
+        const double f0y = fa[0] * y, f1y = fa[1] * y, f2y = fa[2] * y, f3y = fa[3] * y;
+        if (i == 0) {
+            c[0] = (1 + psi + ia[1] + ny + f1y) * 0.5 * inv;
+            c[1] = (1 + psi + f0y + ia[0] + ny) * 0.5;
+            c[2] = (-1 * psi + (f0y + ia[0]) * -2 + y +
+                    (-2 * ia[1] + -1 * psi * ia[1] + y + -2 * fa[1] * y + -1 * psi * fa[1] * y) * inv) *
+                   0.5;
+            c[3] = (ia[1] + f0y + ia[0] + f1y) * 0.5;
+        } else if (i == 1) {
+            c[0] = (1 + f3y + ia[3] + ny) * 0.5 * inv;
+            c[1] = (1 + f2y + ia[2] + ny) * 0.5;
+            c[2] = ((f2y + ia[2]) * -2 + y +
+                    (psi + -2 * ia[3] + -1 * psi * ia[3] + y + -2 * fa[3] * y + -1 * psi * fa[3] * y) * inv) *
+                   0.5;
+            c[3] = (f3y + ia[3] + f2y + ia[2]) * 0.5;
+        }
+    }
+    return 4;
+}
+
 int orc_interp_wall(const orc_stencil* st, int i, double y, double psi, double* c, int right, int cap)
 {
     if (st->kind == ORC_E2_2) return e2_2_wall(i, y, psi, c, right);
+    if (st->kind == ORC_E2P_1) return e2p_wall(st->alpha, st->alpha + 9, i, y, psi, c, right);
     if (st->kind == ORC_E4_2) return e4_2_wall(i, y, psi, c, right);
     return cap;
 }

@@ -275,8 +275,26 @@ orc_system* orc_system_create(int kind, const int n[3], const double min[3],
     memcpy(s->grid_bc, grid_bc, sizeof s->grid_bc);
     s->obj_bc = (int*)malloc((size_t)(nshapes + 1) * sizeof(int));
     if (nshapes) memcpy(s->obj_bc, obj_bc, (size_t)nshapes * sizeof(int));
+    const double* vel = kind == ORC_SYS_WAVE_LINEAR && nparams >= 8 ? params : NULL;
     for (int d = 0; d < 3; d++) {
-        s->d[d] = orc_deriv_create(s->m, d, st, grid_bc, s->obj_bc);
+        const orc_mesh* m = s->m;
+        s->pbc[d] = (int*)malloc((size_t)(m->nR[d] + 1) * sizeof(int));
+        for (long k = 0; k < m->nR[d]; k++) {
+            const orc_rinfo* ri = &m->R[d][k];
+            int bc = s->obj_bc[ri->shape_id];
+            if (bc == ORC_INFLOW) {
+                if (!vel) {
+                    orc_system_destroy(s);
+                    return NULL;
+                }
+                const double an = vel[0] * ri->normal[0] + vel[1] * ri->normal[1] + vel[2] * ri->normal[2];
+                bc = an > 0.0 ? ORC_DIRICHLET : ORC_FLOATING;
+            }
+            s->pbc[d][k] = bc;
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        s->d[d] = orc_deriv_create_pbc(s->m, d, st, grid_bc, (const int* const*)s->pbc);
         if (!s->d[d]) {
             orc_system_destroy(s);
             return NULL;
@@ -328,7 +346,7 @@ orc_system* orc_system_create(int kind, const int n[3], const double min[3],
             for_each_dirichlet_face(s, zero_at, s->gG[c].D);
             for (int d = 0; d < 3; d++)
                 for (long i = 0; i < m->nR[d]; i++)
-                    if (s->obj_bc[m->R[d][i].shape_id] == ORC_DIRICHLET) s->gG[c].R[d][i] = 0.0;
+                    if (s->pbc[d][i] == ORC_DIRICHLET) s->gG[c].R[d][i] = 0.0;
         }
     }
     return s;
@@ -347,6 +365,7 @@ void orc_system_destroy(orc_system* s)
     }
     for (int d = 0; d < 3; d++) orc_deriv_destroy(s->d[d]);
     orc_mesh_destroy(s->m);
+    for (int d = 0; d < 3; d++) free(s->pbc[d]);
     free(s->obj_bc);
     free(s->mms);
     free(s);
@@ -414,11 +433,11 @@ void orc_system_rhs(orc_system* s, const orc_field* u, orc_field* rhs, double t)
     for (long q = 0; q < m->nfluid; q++) rhs->D[m->fluid[q]] += s->src.D[m->fluid[q]];
     for (int d = 0; d < 3; d++)
         for (long i = 0; i < m->nR[d]; i++)
-            if (s->obj_bc[m->R[d][i].shape_id] != ORC_DIRICHLET) rhs->R[d][i] += s->src.R[d][i];
+            if (s->pbc[d][i] != ORC_DIRICHLET) rhs->R[d][i] += s->src.R[d][i];
     for_each_dirichlet_face(s, zero_at, rhs->D);
     for (int d = 0; d < 3; d++)
         for (long i = 0; i < m->nR[d]; i++)
-            if (s->obj_bc[m->R[d][i].shape_id] == ORC_DIRICHLET) rhs->R[d][i] = 0.0;
+            if (s->pbc[d][i] == ORC_DIRICHLET) rhs->R[d][i] = 0.0;
 }
 
 /* -------------------------------------------------------- update_boundary */
@@ -432,7 +451,7 @@ void orc_system_update_boundary(orc_system* s, orc_field* u, double t)
     for_each_dirichlet_face(s, copy_at, &cc);
     for (int d = 0; d < 3; d++)
         for (long i = 0; i < m->nR[d]; i++)
-            if (s->obj_bc[m->R[d][i].shape_id] == ORC_DIRICHLET) u->R[d][i] = sol.R[d][i];
+            if (s->pbc[d][i] == ORC_DIRICHLET) u->R[d][i] = sol.R[d][i];
     if (s->kind == ORC_SYS_HEAT) {
         /* Neumann face data (heat.cpp:318-346) */
         for (int dir = 0; dir < 3; dir++) {
@@ -497,7 +516,7 @@ void orc_system_stats(orc_system* s, const orc_field* u, double t, double st[11]
         long bi = 2147483647L;
         int any = 0;
         for (long i = 0; i < m->nR[d]; i++) {
-            if (s->obj_bc[m->R[d][i].shape_id] == ORC_DIRICHLET) continue;
+            if (s->pbc[d][i] == ORC_DIRICHLET) continue;
             any = 1;
             if (u->R[d][i] < umin) umin = u->R[d][i];
             if (u->R[d][i] > umax) umax = u->R[d][i];

@@ -43,6 +43,39 @@ def stencil_cases(ref: str):
     return out
 
 
+def _lua_numbers(text: str):
+    """Numbers of a Lua table body whose entries may be fractions (3/25)."""
+    out = []
+    for tok in text.split(","):
+        tok = tok.strip()
+        if not tok:
+            continue
+        if "/" in tok:
+            a, b = tok.split("/")
+            out.append(float(a) / float(b))
+        else:
+            out.append(float(tok))
+    return out
+
+
+def polye2_cases(ref: str):
+    """The E2-poly Dirichlet closure vector of src/stencils/polyE2_1.t.cpp
+    ("dirichlet coeffs"): parameters, h, psi and the expected 2 x 4 block."""
+    fname = "polyE2_1.t.cpp"
+    src = open(os.path.join(ref, "src", "stencils", fname)).read()
+    start = src.index('TEST_CASE("dirichlet coeffs")')
+    block = src[start:src.index("TEST_CASE", start + 10)]
+    da = _lua_numbers(re.search(r"dirichlet_alpha\s*=\s*\{([^}]*)\}", block).group(1))
+    fa = _lua_numbers(re.search(r"floating_alpha\s*=\s*\{([^}]*)\}", block).group(1))
+    h = float(re.search(r"real h\s*=\s*(" + NUM + ")", block).group(1))
+    psi = float(re.search(r"real psi\s*=\s*(" + NUM + ")", block).group(1))
+    exact = _numbers(re.search(r"T exact\{([^}]*)\}", block).group(1))
+    line = src[:start].count("\n") + 1
+    alpha = (fa + [0.0] * 6)[:6] + (da + [0.0] * 3)[:3] + [0.0] * 4
+    return [dict(scheme="E2P_1", alpha=alpha, h=h, bc="Dirichlet", psi=psi, right=False, expected=exact,
+                 source=f"src/stencils/{fname}:{line}")]
+
+
 def mesh_cases():
     """Line extents, R indices and psi of the reference's mesh test
     (src/mesh/mesh.t.cpp:67-230): extents {21, 22, 23}, bounds
@@ -74,7 +107,7 @@ def mesh_cases():
 
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
-    stencils = stencil_cases(ref)
+    stencils = stencil_cases(ref) + polye2_cases(ref)
     with open(os.path.join(HERE, "stencil_closures.json"), "w") as f:
         json.dump(stencils, f, indent=1)
     with open(os.path.join(HERE, "mesh_lines.json"), "w") as f:

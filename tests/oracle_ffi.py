@@ -16,8 +16,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
-DIRICHLET, FLOATING, NEUMANN = 0, 1, 2
-E2_2, E4_2, E4_1, E4U_1 = 0, 1, 2, 3
+DIRICHLET, FLOATING, NEUMANN, INFLOW = 0, 1, 2, 3
+E2_2, E4_2, E4_1, E4U_1, E2P_1 = 0, 1, 2, 3, 4
 SYS_WAVE_RADIAL, SYS_WAVE_LINEAR, SYS_HEAT = 0, 1, 2
 MMS_GAUSS, MMS_SINPROD = 0, 1
 
@@ -31,7 +31,7 @@ class Info(C.Structure):
 
 
 class Stencil(C.Structure):
-    _fields_ = [("kind", C.c_int), ("alpha", C.c_double * 2)]
+    _fields_ = [("kind", C.c_int), ("alpha", C.c_double * 13)]
 
 
 class RInfo(C.Structure):
@@ -183,7 +183,8 @@ def _arr(a, ctype):
 def stencil(kind: int, alpha=(0.0, 0.0)) -> Stencil:
     s = Stencil()
     s.kind = kind
-    s.alpha[0], s.alpha[1] = alpha
+    for i, a in enumerate(alpha):
+        s.alpha[i] = a
     return s
 
 

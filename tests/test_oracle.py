@@ -14,7 +14,7 @@ import helpers as H
 import oracle_ffi as O
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-SCHEMES = {"E4_1": O.E4_1, "E4u_1": O.E4U_1, "E2_2": O.E2_2, "E4_2": O.E4_2}
+SCHEMES = {"E4_1": O.E4_1, "E4u_1": O.E4U_1, "E2_2": O.E2_2, "E4_2": O.E4_2, "E2P_1": O.E2P_1}
 BCS = {"Floating": O.FLOATING, "Dirichlet": O.DIRICHLET, "Neumann": O.NEUMANN}
 
 
@@ -510,9 +510,11 @@ def test_cut_rows_e4_second_derivative_exact_on_cubics():
                 assert err <= 1e-9 * scale, f"dir {d} R{r}: {err / scale:.2e}"
 
 
-@pytest.mark.parametrize("kind,deg_wall,deg_int", [(O.E2_2, 1, 1), (O.E4_2, 3, 4)])
+@pytest.mark.parametrize("kind,deg_wall,deg_int", [(O.E2_2, 1, 1), (O.E4_2, 3, 4), (O.E2P_1, 1, 1)])
 def test_interpolants_reproduce_polynomials(kind, deg_wall, deg_int):
-    """E2_2.cpp:31-86 and E4_2.cpp:31-183 interpolants.  Interior: nodes
+    """E2_2.cpp:31-86, E4_2.cpp:31-183 and polyE2_1.cpp:42-138 interpolants
+    (E2-poly with random floating and interpolant parameters, as
+    polyE2_1.t.cpp's "interp wall" case draws them).  Interior: nodes
     -p/2..p/2 (E2: the two points either side of y), exact to degree p-1.
     Wall (left): nodes 1-psi (the boundary), 1, 2, .. t-1 relative to the
     boundary point, evaluated at y + i (y + 1 - psi for i = 0, the shift
@@ -522,19 +524,20 @@ def test_interpolants_reproduce_polynomials(kind, deg_wall, deg_int):
     L = O.lib()
     L.orc_interp_wall.restype = ctypes.c_int
     L.orc_interp_interior.restype = ctypes.c_int
-    st = O.stencil(kind)
-    t = 3 if kind == O.E2_2 else 5
     rng = np.random.default_rng(11)
+    st = O.stencil(kind, rng.uniform(size=13) if kind == O.E2P_1 else (0.0, 0.0))
+    t = {O.E2_2: 3, O.E4_2: 5, O.E2P_1: 4}[kind]
+    nwall = {O.E2_2: 2, O.E4_2: 3, O.E2P_1: 2}[kind]
     c = (ctypes.c_double * 8)()
     for _ in range(40):
         y = rng.uniform(-0.95, 0.95)
         n = L.orc_interp_interior(ctypes.byref(st), ctypes.c_double(y), c, 8)
-        x = (np.array([0.0, 1.0]) if y > 0 else np.array([-1.0, 0.0])) if kind == O.E2_2 else np.arange(-2.0, 3.0)
+        x = (np.array([0.0, 1.0]) if y > 0 else np.array([-1.0, 0.0])) if kind != O.E4_2 else np.arange(-2.0, 3.0)
         w = np.array(c[:n])
         for m in range(deg_int + 1):
             assert abs(w @ x ** m - y ** m) < 1e-13, (y, m)
         for right in (0, 1):
-            for i in range(t - 1 if kind == O.E2_2 else 3):
+            for i in range(nwall):
                 psi = rng.uniform(0.05, 1.0)
                 n = L.orc_interp_wall(ctypes.byref(st), i, ctypes.c_double(y), ctypes.c_double(psi), c, right, 8)
                 assert n == t
