@@ -260,6 +260,22 @@ int shx_euler_step(shx_system* s, shx_field* u0, shx_field* out, shx_field* srhs
  * registers are owned by the system.  Asynchronous. */
 int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt, int nsteps);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same steps as shx_rk4_fused_steps_async, bit for bit, with
+ * the fused registers (accumulator, stage pair, stage cut-point values) taken
+ * from a per-device workspace the library holds in its code object (64 MiB
+ * per GPU, kept until the process ends) instead of the system's own buffers;
+ * no hipMalloc, hipFree or other allocating call is made.  Limit: the
+ * registers of the grid must fit the workspace (about 24 bytes per point in
+ * FP64, 16 in the mixed form, plus 16 per cut point; about 2.7 million FP64
+ * points); larger grids return SHX_EINVAL.  The x-marching form's point
+ * classes must have been built beforehand (shx_system_stage_points, or any
+ * fused step).  Every call on one GPU shares that GPU's workspace: calls that
+ * overlap on one GPU (two streams, two contexts) race.  The registers need no
+ * initial contents.  Asynchronous. */
+int shx_rk4_fused_steps_noalloc(shx_system* s, shx_field* u0, double t, double dt, int nsteps);
+
 /* Fused-stage kernel choice (no reference counterpart; the reference has one
  * Kokkos TeamPolicy shape, src/matrices/block.hpp:185-194): variant 0 is the
  * point-wise kernel on 64 x 4 tiles.  Variants 1-4 march tiles of

@@ -132,6 +132,8 @@ _SIGS = [
     ("shx_rk4_step", C.c_int, [C.c_void_p] + [C.POINTER(ShxField)] * 4 + [C.c_double, C.c_double]),
     ("shx_rk4_fused_steps_async", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double,
                                             C.c_int]),
+    ("shx_rk4_fused_steps_noalloc", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double,
+                                            C.c_int]),
     ("shx_time_fused_steps", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double,
                                        C.c_int, C.c_int, dp]),
     ("shx_time_unfused_steps", C.c_int, [C.c_void_p] + [C.POINTER(ShxField)] * 4 +

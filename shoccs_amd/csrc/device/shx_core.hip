@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -329,6 +330,36 @@ template int upload<int8_t>(shx_ctx*, int8_t**, const int8_t*, size_t);
 
 }  // namespace shx
 
+// ------------------------------------------------------------- workspace
+// The workspace of the allocation-free entry points: one copy in the memory of
+// every GPU the library runs on, held until the process ends.  In host code
+// this name is only a stand-in for the symbol, never a device address; each
+// device's address comes from hipGetSymbolAddress at shx_init (with that
+// device current) and is kept per device below and in the context.
+static __device__ __align__(256) unsigned char scratch[shx::kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// the current device's workspace address (resolved once per device)
+hipError_t resolve_scratch(int device, void** addr)
+{
+    *addr = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipSuccess;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = p;
+    }
+    *addr = g_scratch_addr[device];
+    return hipSuccess;
+}
+}  // namespace
+
 // ------------------------------------------------------------------ C ABI
 extern "C" {
 
@@ -343,6 +374,7 @@ int shx_init(int device, shx_ctx** out)
     if (!ctx) return SHX_ENOMEM;
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = resolve_scratch(device, &ctx->scratch);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete ctx;

@@ -13,6 +13,9 @@ struct shx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    // this device's address of the allocation-free workspace (shx_core.hip),
+    // or nullptr when it could not be resolved
+    void* scratch = nullptr;
 };
 
 struct shx_deriv_T;  // transposed operator (shx_adjoint.hip), built on first use
@@ -61,6 +64,11 @@ struct shx_csr {
 };
 
 namespace shx {
+
+// Size of the per-device workspace behind the allocation-free entry points
+// (shx_rk4_fused_steps_noalloc): 64 MiB, i.e. the fused FP64 registers of a
+// grid of up to about 2.7 million points.
+constexpr size_t kScratchBytes = size_t(64) << 20;
 
 int fail(shx_ctx* ctx, int code, const std::string& msg);
 int hip_fail(shx_ctx* ctx, hipError_t e, const char* what);

@@ -905,16 +905,39 @@ int march_args(shx_system* s, const void* usD, RPtrs usR, const double* u0, doub
 
 // Stage register pointers of stage `stage` of a fused step on u0: input,
 // output (FP64 or FP32 stage buffers, u0 at the ends) and cut points.
-void stage_registers(shx_system* s, shx_field* u0, int stage, const void*& usV, void*& outV,
-                     RPtrs& usR, double* outR[3])
+// The fused form's registers: the accumulator, the stage ping-pong (FP64,
+// or FP32 for the mixed form) and the stage's cut-point values (FP64).  They
+// are the system's own buffers, or a carve of the per-device workspace for
+// the allocation-free entry point.
+struct FusedRegs {
+    double* acc = nullptr;
+    double* stageD[2] = {nullptr, nullptr};
+    float* stageF[2] = {nullptr, nullptr};
+    double* stageR[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+};
+
+FusedRegs own_regs(const shx_system* s)
+{
+    FusedRegs r;
+    r.acc = s->accD;
+    for (int b = 0; b < 2; ++b) {
+        r.stageD[b] = s->stageD[b];
+        r.stageF[b] = s->stageF[b];
+        for (int c = 0; c < 3; ++c) r.stageR[b][c] = s->stageR[b][c];
+    }
+    return r;
+}
+
+void stage_registers(shx_system* s, const FusedRegs& g, shx_field* u0, int stage, const void*& usV,
+                     void*& outV, RPtrs& usR, double* outR[3])
 {
     const bool mixed = s->precision == SHX_F32_COMP;
     usV = stage == 0 ? (const void*)u0->D
-                     : mixed ? (const void*)s->stageF[(stage - 1) & 1] : (const void*)s->stageD[(stage - 1) & 1];
-    outV = stage == 3 ? (void*)u0->D : mixed ? (void*)s->stageF[stage & 1] : (void*)s->stageD[stage & 1];
+                     : mixed ? (const void*)g.stageF[(stage - 1) & 1] : (const void*)g.stageD[(stage - 1) & 1];
+    outV = stage == 3 ? (void*)u0->D : mixed ? (void*)g.stageF[stage & 1] : (void*)g.stageD[stage & 1];
     for (int c = 0; c < 3; ++c) {
-        usR.r[c] = stage == 0 ? u0->R[c] : s->stageR[(stage - 1) & 1][c];
-        outR[c] = stage == 3 ? u0->R[c] : s->stageR[stage & 1][c];
+        usR.r[c] = stage == 0 ? u0->R[c] : g.stageR[(stage - 1) & 1][c];
+        outR[c] = stage == 3 ? u0->R[c] : g.stageR[stage & 1][c];
     }
 }
 
@@ -941,27 +964,22 @@ int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const dou
     return check(s->ctx, "rk4 march stage kernel");
 }
 
-// One fused step; when ev is given, event pairs bracket each stage's kernels
-// (the point-wise tiles, or the x-marching launch or launches; the
-// point-wise form's cut-point update runs outside the brackets).
-int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t* ev)
+// One fused step on the registers g; when ev is given, event pairs bracket
+// each stage's kernels (the point-wise tiles, or the x-marching launch or
+// launches; the point-wise form's cut-point update runs outside the
+// brackets).  The registers need no initial contents: stage 0 writes the
+// accumulator and the first stage register before any stage reads them.
+int rk4_fused_step_on(shx_system* s, const FusedRegs& g, shx_field* u0, double t, double dt,
+                      hipEvent_t* ev)
 {
-    if (s->general)
-        return fail(s->ctx, SHX_EUNSUPPORTED,
-                    "fused form: floating/Neumann bodies and Neumann grid faces take the reference schedule "
-                    "(shx_rk4_step)");
     const bool march = s->variant >= 1;
-    const bool mixed = s->precision == SHX_F32_COMP;
-    if (mixed && !march)
-        return fail(s->ctx, SHX_EUNSUPPORTED, "mixed precision needs the x-marching form (variants 1-4)");
-    int rc = ensure_fused(s);
-    if (rc == SHX_OK && march) rc = ensure_classes(s);
+    int rc = SHX_OK;
     for (int stage = 0; stage < 4 && rc == SHX_OK; ++stage) {
         const void* usV;
         void* outV;
         RPtrs usR;
         double* outR[3];
-        stage_registers(s, u0, stage, usV, outV, usR, outR);
+        stage_registers(s, g, u0, stage, usV, outV, usR, outR);
         const double* usD = static_cast<const double*>(usV);
         double* outD = static_cast<double*>(outV);
         const double ts = t + dt * kRki[stage];
@@ -970,19 +988,73 @@ int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t
         const double tn = stage < 3 ? t + dt * kRki[stage + 1] : t + dt;
         if (ev) hipEventRecord(ev[2 * stage], s->ctx->stream);
         if (march)
-            rc = launch_march(s, stage, usV, usR, u0->D, s->accD, outV, outR, ts, c_acc, c_next, tn);
+            rc = launch_march(s, stage, usV, usR, u0->D, g.acc, outV, outR, ts, c_acc, c_next, tn);
         else if (s->family == SHX_SYS_HEAT)
-            rc = launch_stage<SHX_SYS_HEAT>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
+            rc = launch_stage<SHX_SYS_HEAT>(s, stage, usD, usR, u0->D, g.acc, outD, ts, c_acc, c_next, tn);
         else if (s->family == kHeatGauss)
-            rc = launch_stage<kHeatGauss>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
+            rc = launch_stage<kHeatGauss>(s, stage, usD, usR, u0->D, g.acc, outD, ts, c_acc, c_next, tn);
         else if (s->kind == SHX_SYS_WAVE_RADIAL)
-            rc = launch_stage<SHX_SYS_WAVE_RADIAL>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
+            rc = launch_stage<SHX_SYS_WAVE_RADIAL>(s, stage, usD, usR, u0->D, g.acc, outD, ts, c_acc, c_next, tn);
         else
-            rc = launch_stage<SHX_SYS_WAVE_LINEAR>(s, stage, usD, usR, u0->D, s->accD, outD, ts, c_acc, c_next, tn);
+            rc = launch_stage<SHX_SYS_WAVE_LINEAR>(s, stage, usD, usR, u0->D, g.acc, outD, ts, c_acc, c_next, tn);
         if (ev) hipEventRecord(ev[2 * stage + 1], s->ctx->stream);
         if (rc == SHX_OK && !march) rc = r_exact_all(s, outR, tn);
     }
     return rc;
+}
+
+// Whether the fused form can step this system at all.
+int fused_supported(shx_system* s)
+{
+    if (s->general)
+        return fail(s->ctx, SHX_EUNSUPPORTED,
+                    "fused form: floating/Neumann bodies and Neumann grid faces take the reference schedule "
+                    "(shx_rk4_step)");
+    if (s->precision == SHX_F32_COMP && s->variant < 1)
+        return fail(s->ctx, SHX_EUNSUPPORTED, "mixed precision needs the x-marching form (variants 1-4)");
+    return SHX_OK;
+}
+
+// One fused step on the system's own registers (allocated on first use,
+// with the x-marching form's point classes).
+int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t* ev)
+{
+    int rc = fused_supported(s);
+    if (rc == SHX_OK) rc = ensure_fused(s);
+    if (rc == SHX_OK && s->variant >= 1) rc = ensure_classes(s);
+    if (rc == SHX_OK) rc = rk4_fused_step_on(s, own_regs(s), u0, t, dt, ev);
+    return rc;
+}
+
+// The fused registers carved from the per-device workspace, 256-byte aligned.
+int carve_workspace(shx_system* s, FusedRegs& g)
+{
+    auto* base = static_cast<unsigned char*>(s->ctx->scratch);
+    if (!base) return fail(s->ctx, SHX_EHIP, "no-alloc fused steps: the device workspace is not available");
+    const bool mixed = s->precision == SHX_F32_COMP;
+    const size_t N = (size_t)s->N;
+    auto round = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t need = round(N * sizeof(double)) + 2 * round(N * (mixed ? sizeof(float) : sizeof(double)));
+    for (int c = 0; c < 3; ++c) need += 2 * round((size_t)s->nR[c] * sizeof(double));
+    if (need > kScratchBytes)
+        return fail(s->ctx, SHX_EINVAL,
+                    "no-alloc fused steps: the registers of " + std::to_string(s->N) + " points need " +
+                        std::to_string(need) + " bytes, more than the " + std::to_string(kScratchBytes) +
+                        "-byte device workspace");
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        void* p = base + off;
+        off += round(bytes);
+        return p;
+    };
+    g.acc = static_cast<double*>(take(N * sizeof(double)));
+    for (int b = 0; b < 2; ++b) {
+        if (mixed) g.stageF[b] = static_cast<float*>(take(N * sizeof(float)));
+        else g.stageD[b] = static_cast<double*>(take(N * sizeof(double)));
+    }
+    for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 3; ++c) g.stageR[b][c] = static_cast<double*>(take((size_t)s->nR[c] * sizeof(double)));
+    return SHX_OK;
 }
 
 }  // namespace
@@ -1350,6 +1422,20 @@ int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt,
     return rc;
 }
 
+int shx_rk4_fused_steps_noalloc(shx_system* s, shx_field* u0, double t, double dt, int nsteps)
+{
+    if (!s || !u0 || nsteps < 0) return SHX_EINVAL;
+    int rc = fused_supported(s);
+    if (rc == SHX_OK && s->variant >= 1 && !s->classes_ready)
+        return fail(s->ctx, SHX_EINVAL,
+                    "no-alloc fused steps: the point classes of the x-marching form are not built "
+                    "(call shx_system_stage_points, or take one fused step, once first)");
+    FusedRegs g;
+    if (rc == SHX_OK) rc = carve_workspace(s, g);
+    for (int n = 0; n < nsteps && rc == SHX_OK; ++n) rc = rk4_fused_step_on(s, g, u0, t + n * dt, dt, nullptr);
+    return rc;
+}
+
 int shx_time_fused_steps(shx_system* s, shx_field* u0, double t, double dt, int nsteps, int reps,
                          double* ms_per_step)
 {
@@ -1642,7 +1728,7 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
             void* outV;
             RPtrs usR;
             double* outR[3];
-            stage_registers(s, up, stage, usV, outV, usR, outR);
+            stage_registers(s, own_regs(s), up, stage, usV, outV, usR, outR);
             long nitems = 0;
             rc = march_args(s, usV, usR, up->D, s->accD, outV, outR, e.m, e.sp, nitems);
             if (rc != SHX_OK) {
