@@ -27,6 +27,7 @@ extern "C" {
 #define SHOCCS_E4_2 1
 #define SHOCCS_E4_1 2
 #define SHOCCS_E4U_1 3
+#define SHOCCS_E2P_1 4 /* "E2-poly" (src/stencils/polyE2_1.cpp) */
 
 typedef struct shoccs_problem shoccs_problem;
 
@@ -39,13 +40,19 @@ typedef struct shoccs_problem_desc {
     const int32_t* shape_bc; /* per shape boundary_condition */
     int32_t grid_bc[6];    /* xmin xmax ymin ymax zmin zmax */
     int32_t scheme;
-    double alpha[2];
+    /* the scheme's free parameters: E4_1, E4u_1 two; E2-poly 13, its
+     * floating (6), Dirichlet (3) and interpolant (4) parameters in order */
+    double alpha[13];
     /* system parameters:
      *  WAVE_RADIAL: cx cy cz radius [max_error]
      *  WAVE_LINEAR: vx vy vz x0 y0 z0 width amplitude [max_error]
      *  HEAT:        diffusivity mms_kind [n_gauss, n_gauss x {c[3], var[3], amp, freq}] */
     const double* params;
     int32_t nparams;
+    /* nonzero: the mesh and cut points only (no operators, no system), for
+     * meshes whose lines are too short for the scheme's closures, e.g. the
+     * field output of the reference's xdmf test (extents 3 4 5) */
+    int32_t mesh_only;
 } shoccs_problem_desc;
 
 /* ctx == NULL builds the host structures only.  Problems with floating or
@@ -53,6 +60,10 @@ typedef struct shoccs_problem_desc {
  * (SHOCCS_RK4, SHOCCS_EULER); their fused form, batches and adjoint return
  * SHX_EUNSUPPORTED. */
 int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_problem** out);
+/* The message of the calling thread's last failed host assembly
+ * (shoccs_problem_create), or "" (the reference logs it and returns nullopt,
+ * src/operators/derivative.cpp:452-473, system.cpp:103-131). */
+const char* shoccs_last_error(void);
 int shoccs_problem_destroy(shoccs_problem* p);
 shx_system* shoccs_problem_system(shoccs_problem* p);
 shx_derivative* shoccs_problem_derivative(shoccs_problem* p, int dir);

@@ -35,6 +35,11 @@ extern "C" {
 #define SHX_BC_DIRICHLET 0
 #define SHX_BC_FLOATING 1
 #define SHX_BC_NEUMANN 2
+/* A shape condition with no reference counterpart: each cut point of the
+ * shape is Dirichlet where the constant velocity of the linear wave leaves
+ * the body into the fluid (a . n > 0) and Floating elsewhere.  Only
+ * shoccs_problem_create resolves it; operators and systems see the result. */
+#define SHX_BC_INFLOW 3
 
 typedef struct shx_ctx shx_ctx;
 typedef struct shx_derivative shx_derivative;

@@ -391,16 +391,16 @@ orc_deriv* orc_deriv_create_pbc(const orc_mesh* m, int dir, const orc_stencil* s
         b.stride = stride;
         b.right_col_offset = col_offset + stride * (columns - b.right_cols);
         push_block(d, &cap, &b);
-        /* Overlapping closures of a short segment read through flat offsets
-         * that may leave the line; reading outside the mesh buffer would be
-         * undefined behaviour in the reference, so such meshes are rejected. */
+        /* A segment shorter than its two closure blocks would need a
+         * circulant with a negative row count (derivative.cpp:428-436): its
+         * closures overlap, which is undefined in the reference, so such
+         * meshes are rejected; so are closures whose columns leave the mesh
+         * buffer. */
         {
             long l1 = b.col_offset + (b.left_cols - 1) * stride;
             long r1 = b.right_col_offset + (b.right_cols - 1) * stride;
-            long total = b.left_rows + b.interior_rows + b.right_rows;
             if ((b.left_rows > 0 && (b.col_offset < 0 || l1 >= d->N)) ||
-                (b.right_rows > 0 && (b.right_col_offset < 0 || r1 >= d->N)) ||
-                total < b.left_rows || total < b.right_rows) {
+                (b.right_rows > 0 && (b.right_col_offset < 0 || r1 >= d->N)) || b.interior_rows < 0) {
                 free(left);
                 free(right);
                 free(Bb.p);

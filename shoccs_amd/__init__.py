@@ -24,14 +24,14 @@ import weakref
 
 import numpy as np
 
-from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, E2_2, E4_1, E4_2, E4U_1, EULER,
+from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_INFLOW, BC_NEUMANN, E2_2, E2P_1, E4_1, E4_2, E4U_1, EULER,
                    F32_COMP, F64, MMS_GAUSS, MMS_SINPROD, RK4, RK4_FUSED, SYS_HEAT,
                    SYS_WAVE_LINEAR, SYS_WAVE_RADIAL, IODesc, ProblemDesc, RunDesc, RunResult, ShoccsError,
-                   ShxField, ShxLineBlock, check, lib)
+                   SHX_OK, ShxField, ShxLineBlock, check, lib)
 
 __all__ = [
     "Context", "Problem", "Field", "Batch", "FieldIO", "Euler", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
-    "E2_2", "E4_2", "E4_1", "E4U_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
+    "BC_INFLOW", "E2_2", "E4_2", "E4_1", "E4U_1", "E2P_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
     "MMS_GAUSS", "MMS_SINPROD", "F64", "F32_COMP", "RK4", "RK4_FUSED", "EULER", "eq", "plus_eq",
 ]
 
@@ -162,7 +162,7 @@ class Config:
             self.shape_bc = [d.shape_bc[i] for i in range(d.nshapes)]
             self.grid_bc = tuple(d.grid_bc[f] for f in range(6))
             self.scheme = d.scheme
-            self.alpha = (d.alpha[0], d.alpha[1])
+            self.alpha = tuple(d.alpha[i] for i in range(13 if d.scheme == E2P_1 else 2))
             self.params = [d.params[i] for i in range(d.nparams)]
             self.integrator = r.integrator
             self.max_step = None if r.max_step == 2 ** 31 - 1 else r.max_step
@@ -242,7 +242,9 @@ class Problem:
     """
 
     def __init__(self, ctx: Context | None, kind: int, n, lo, hi, spheres=(), shape_bc=(),
-                 grid_bc=(BC_FLOATING,) * 6, scheme: int = E4_1, alpha=(0.0, 0.0), params=()):
+                 grid_bc=(BC_FLOATING,) * 6, scheme: int = E4_1, alpha=(0.0, 0.0), params=(),
+                 mesh_only: bool = False):
+        """mesh_only: the mesh and cut points alone (no operators or system)."""
         L = lib()
         self.ctx = ctx
         d = ProblemDesc()
@@ -262,11 +264,15 @@ class Problem:
         for f in range(6):
             d.grid_bc[f] = int(grid_bc[f])
         d.scheme = scheme
-        d.alpha[0], d.alpha[1] = alpha
+        for i, a in enumerate(alpha):
+            d.alpha[i] = a
         d.params = pr.ctypes.data_as(C.POINTER(C.c_double))
         d.nparams = len(pr) - 1
+        d.mesh_only = int(mesh_only)
         p = C.c_void_p()
         rc = L.shoccs_problem_create(ctx.p if ctx else None, C.byref(d), C.byref(p))
+        if rc != SHX_OK and L.shoccs_last_error():
+            raise ShoccsError(f"shoccs_problem_create failed with code {rc}: {L.shoccs_last_error().decode()}")
         check(rc, ctx.p if ctx else None, "shoccs_problem_create")
         self.p = p
         self.n = tuple(n)

@@ -13,10 +13,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libshoccs.so")
 
 SHX_OK = 0
-BC_DIRICHLET, BC_FLOATING, BC_NEUMANN = 0, 1, 2
+BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, BC_INFLOW = 0, 1, 2, 3
 SYS_WAVE_RADIAL, SYS_WAVE_LINEAR, SYS_HEAT = 0, 1, 2
 MMS_GAUSS, MMS_SINPROD = 0, 1
-E2_2, E4_2, E4_1, E4U_1 = 0, 1, 2, 3
+E2_2, E4_2, E4_1, E4U_1, E2P_1 = 0, 1, 2, 3, 4
 F64, F32_COMP = 0, 1
 
 dp = C.POINTER(C.c_double)
@@ -45,9 +45,10 @@ class ProblemDesc(C.Structure):
         ("shape_bc", i32p),
         ("grid_bc", C.c_int32 * 6),
         ("scheme", C.c_int32),
-        ("alpha", C.c_double * 2),
+        ("alpha", C.c_double * 13),
         ("params", dp),
         ("nparams", C.c_int32),
+        ("mesh_only", C.c_int32),
     ]
 
 
@@ -99,6 +100,7 @@ _SIGS = [
     ("shx_init", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     ("shx_finalize", C.c_int, [C.c_void_p]),
     ("shx_last_error", C.c_char_p, [C.c_void_p]),
+    ("shoccs_last_error", C.c_char_p, []),
     ("shx_sync", C.c_int, [C.c_void_p]),
     ("shx_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     ("shx_free", C.c_int, [C.c_void_p, C.c_void_p]),

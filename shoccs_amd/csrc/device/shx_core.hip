@@ -650,6 +650,10 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     }
     d->h_blocks.assign(desc->blocks, desc->blocks + desc->nblocks);
     d->h_pool.assign(desc->pool, desc->pool + desc->npool);
+    if (d->nrrow > 0) {
+        d->h_rset.assign(desc->rset, desc->rset + d->nrrow);
+        d->h_rrow.assign(desc->rrow, desc->rrow + d->nrrow);
+    }
     d->h_pair_ptr = std::move(pair_ptr);
     for (int m = 0; m < 9; ++m) d->h_coef[m] = desc->interior[m];
     d->dev.pair_ptr = d->pair_ptr;

@@ -420,6 +420,53 @@ __device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRo
     return r;
 }
 
+// A free cut point (of a floating or Neumann body): its rows of the three
+// derivatives (derivative.cpp:479-485: each row from the running value,
+// entries over D then R in stored order; the heat laplacian continues one
+// running sum across the directions, the wave's gradient starts each from
+// zero), the rhs at the cut point (scalar_wave.cpp:208-238: gG . du with gG
+// at the point; heat.cpp:119-162: k lap + source) and the stage update of
+// its value and accumulator (rk4.cpp:27-46), all in FP64.
+template <int KIND, int STAGE, bool MIXED>
+__device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScalars& ss, long f)
+{
+    using TI = typename StageTypes<MIXED, STAGE>::in;
+    const TI* __restrict__ us = static_cast<const TI*>(a.us);
+    const FreeRec fr = a.frees[f];
+    const int set = fr.set;
+    const double* __restrict__ uR = a.usR.r[set];
+    const double* pos = set == 0 ? a.pos0 : set == 1 ? a.pos1 : a.pos2;
+    const double x = pos[3 * fr.q], y = pos[3 * fr.q + 1], z = pos[3 * fr.q + 2];
+    auto row_sum = [&](int c, double acc) {
+        const long k = fr.row[c];
+        for (long e = a.cut.rptr[c][k]; e < a.cut.rptr[c][k + 1]; ++e) {
+            const long col = a.cut.rcol[c][e];
+            acc += a.cut.rw[c][e] * (col >= 0 ? (double)us[col] : uR[-1 - col]);
+        }
+        return acc;
+    };
+    double rhs;
+    if (kIsHeat<KIND>) {
+        double lap = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (fr.row[c] >= 0) lap = row_sum(c, lap);
+        rhs = lap * a.s.k;
+        rhs += heat_source(a.s, ss.t_s, x, y, z);
+    } else {
+        double du[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) du[c] = fr.row[c] >= 0 ? row_sum(c, 0.0) : 0.0;
+        rhs = wave_neg_g(a.s, 0, x, y, z) * du[0] + wave_neg_g(a.s, 1, x, y, z) * du[1] +
+              wave_neg_g(a.s, 2, x, y, z) * du[2];
+    }
+    const double u0v = a.u0R.r[set][fr.q];
+    const double aa = (STAGE == 0 ? 0.0 : a.accF[f]) + ss.c_acc * rhs;
+    if (STAGE < 3) a.accF[f] = aa;
+    double* outR = set == 0 ? a.outR0 : set == 1 ? a.outR1 : a.outR2;
+    outR[fr.q] = STAGE < 3 ? u0v + ss.c_next * rhs : u0v + 1.0 * aa;
+}
+
 // Item q of the special work: a listed point, then the cut points.  Rows are
 // evaluated in FP64 in both forms.
 template <int KIND, int STAGE, int W, bool MIXED>
@@ -487,7 +534,10 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
     if (q < a.nR2) {
         if (a.dir2[q])
             a.outR2[q] = exact_of<KIND>(a.s, a.pos2[3 * q], a.pos2[3 * q + 1], a.pos2[3 * q + 2], ss.t_next);
+        return;
     }
+    q -= a.nR2;
+    if (q < a.nfree) free_item<KIND, STAGE, MIXED>(a, ss, q);
 }
 
 // One fused RK4 stage.  Workgroups go in groups of 8 (one per XCD): `sg`
@@ -631,7 +681,7 @@ template <bool MIXED>
 int launch_stage_form(hipStream_t st, int kind, int stage, int width, int ty, int mode,
                       const MarchArgs& a, const SpecialArgs& sp, const StageScalars& ss)
 {
-    const long nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2;
+    const long nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2 + sp.nfree;
     if (a.nblocks == 0 && nitems == 0) return 0;
     switch (kind * 10 + width) {
     case SHX_SYS_HEAT * 10 + 5: return stage_of<SHX_SYS_HEAT, 5, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);

@@ -50,6 +50,8 @@ struct shx_derivative {
     std::vector<shx_line_block> h_blocks;
     std::vector<double> h_pool;
     std::vector<int> h_pair_ptr;
+    std::vector<int> h_rset;   // R-space rows: set and row of each
+    std::vector<long> h_rrow;
     double h_coef[9] = {};  // interior stencil
     shx_deriv_T* T = nullptr;
 };

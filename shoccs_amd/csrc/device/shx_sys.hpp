@@ -163,8 +163,26 @@ struct SpecialRec {
     unsigned ref[3];
 };
 
+// One free cut point (a cut point of a non-Dirichlet body) of the fused
+// form: point q of R(set), and per direction the index of its row in that
+// derivative's R-space row list (-1: none).
+struct FreeRec {
+    long q;
+    long row[3];
+    int set;
+    int pad;
+};
+
+// The R-space rows of the three derivatives (shx_derivative_desc.rptr ...).
+struct CutRows {
+    const long* rptr[3];
+    const long* rcol[3];
+    const double* rw[3];
+};
+
 // Arguments of the special-point kernel: the listed points of D, then the
-// cut points of the three directions (Dirichlet ones take exact(t_next)).
+// cut points of the three directions (Dirichlet ones take exact(t_next)),
+// then the free cut points (their rows, the rhs and the stage update).
 struct SpecialArgs {
     DevSys s;
     DevOps ops;
@@ -180,6 +198,11 @@ struct SpecialArgs {
     const double *pos0, *pos1, *pos2;
     const signed char *dir0, *dir1, *dir2;
     double *outR0, *outR1, *outR2;
+    long nfree;
+    const FreeRec* frees;
+    CutRows cut;
+    RPtrs u0R;      // the step's u0 at the cut points
+    double* accF;   // accumulator of the free cut points (nfree values)
 };
 
 // Per-stage scalars of one fused RK4 stage (rk4.cpp:27-46): the stage time,

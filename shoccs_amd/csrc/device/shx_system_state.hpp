@@ -32,6 +32,12 @@ struct shx_system {
     // only (cut-point rows, R sources, Neumann data); fused forms refuse
     bool general = false;
     bool any_free_r = false;      // some cut point of a non-Dirichlet body
+    // the fused x-marching form also steps free cut points (no Neumann grid
+    // faces): one record each, with the rows of its three derivatives
+    bool fused_free = false;
+    shx::FreeRec* frees = nullptr;
+    long nfree = 0;
+    double* accF = nullptr;       // their fused accumulator register
     double* neuD = nullptr;       // heat: Neumann face data (N values)
     int* neu_faces = nullptr;     // points of the Neumann faces, face by face
     long neu_off[7] = {0, 0, 0, 0, 0, 0, 0};  // face f = 2 dir + side: [neu_off[f], neu_off[f+1])

@@ -336,6 +336,7 @@ int ensure_fused(shx_system* s)
         rc = alloc_d(s->ctx, &s->accD, s->N);
         for (int b = 0; b < 2 && rc == SHX_OK; ++b)
             for (int c = 0; c < 3 && rc == SHX_OK; ++c) rc = alloc_d(s->ctx, &s->stageR[b][c], s->nR[c]);
+        if (rc == SHX_OK && s->nfree > 0) rc = alloc_d(s->ctx, &s->accF, s->nfree);
     }
     for (int b = 0; b < 2 && rc == SHX_OK; ++b) {
         if (s->precision == SHX_F64 && !s->stageD[b]) rc = alloc_d(s->ctx, &s->stageD[b], s->N);
@@ -854,8 +855,8 @@ int ensure_classes(shx_system* s)
 }
 
 // Tile and special-point arguments of one stage of the x-marching form.
-int march_args(shx_system* s, const void* usD, RPtrs usR, const double* u0, double* acc, void* out,
-               double* const outR[3], MarchArgs& a, SpecialArgs& sp, long& nitems)
+int march_args(shx_system* s, const void* usD, RPtrs usR, const double* u0, RPtrs u0R, double* acc,
+               double* accF, void* out, double* const outR[3], MarchArgs& a, SpecialArgs& sp, long& nitems)
 {
     a.s = s->dev;
     a.ops = s->ops;
@@ -897,7 +898,17 @@ int march_args(shx_system* s, const void* usD, RPtrs usR, const double* u0, doub
     sp.outR0 = outR[0];
     sp.outR1 = outR[1];
     sp.outR2 = outR[2];
-    nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2;
+    sp.nfree = s->fused_free ? s->nfree : 0;
+    sp.frees = s->frees;
+    for (int c = 0; c < 3; ++c) {
+        const shx_derivative* d = s->d[c];
+        sp.cut.rptr[c] = d ? d->rptr : nullptr;
+        sp.cut.rcol[c] = d ? d->rcol : nullptr;
+        sp.cut.rw[c] = d ? d->rw : nullptr;
+    }
+    sp.u0R = u0R;
+    sp.accF = accF;
+    nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2 + sp.nfree;
     if (nb + nitems / 64 + 8 >= (1L << 31)) return fail(s->ctx, SHX_EUNSUPPORTED, "march: too many workgroups");
     a.nblocks = (unsigned)nb;
     return SHX_OK;
@@ -911,6 +922,7 @@ int march_args(shx_system* s, const void* usD, RPtrs usR, const double* u0, doub
 // the allocation-free entry point.
 struct FusedRegs {
     double* acc = nullptr;
+    double* accF = nullptr;  // accumulator of the free cut points
     double* stageD[2] = {nullptr, nullptr};
     float* stageF[2] = {nullptr, nullptr};
     double* stageR[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
@@ -920,6 +932,7 @@ FusedRegs own_regs(const shx_system* s)
 {
     FusedRegs r;
     r.acc = s->accD;
+    r.accF = s->accF;
     for (int b = 0; b < 2; ++b) {
         r.stageD[b] = s->stageD[b];
         r.stageF[b] = s->stageF[b];
@@ -944,14 +957,14 @@ void stage_registers(shx_system* s, const FusedRegs& g, shx_field* u0, int stage
 // One stage of the x-marching form in one launch: march tiles for interior,
 // solid and pattern points; special items for the listed points and the
 // Dirichlet cut points.
-int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const double* u0,
-                 double* acc, void* out, double* const outR[3], double ts, double c_acc,
+int launch_march(shx_system* s, int stage, const void* usD, RPtrs usR, const double* u0, RPtrs u0R,
+                 double* acc, double* accF, void* out, double* const outR[3], double ts, double c_acc,
                  double c_next, double tn)
 {
     MarchArgs a;
     SpecialArgs sp;
     long nitems = 0;
-    int rc = march_args(s, usD, usR, u0, acc, out, outR, a, sp, nitems);
+    int rc = march_args(s, usD, usR, u0, u0R, acc, accF, out, outR, a, sp, nitems);
     if (rc != SHX_OK) return rc;
     const StageScalars ss{ts, c_acc, c_next, tn};
     const int lrc = s->precision == SHX_F32_COMP
@@ -988,7 +1001,8 @@ int rk4_fused_step_on(shx_system* s, const FusedRegs& g, shx_field* u0, double t
         const double tn = stage < 3 ? t + dt * kRki[stage + 1] : t + dt;
         if (ev) hipEventRecord(ev[2 * stage], s->ctx->stream);
         if (march)
-            rc = launch_march(s, stage, usV, usR, u0->D, g.acc, outV, outR, ts, c_acc, c_next, tn);
+            rc = launch_march(s, stage, usV, usR, u0->D, rptrs(u0), g.acc, g.accF, outV, outR, ts, c_acc,
+                              c_next, tn);
         else if (s->family == SHX_SYS_HEAT)
             rc = launch_stage<SHX_SYS_HEAT>(s, stage, usD, usR, u0->D, g.acc, outD, ts, c_acc, c_next, tn);
         else if (s->family == kHeatGauss)
@@ -1006,10 +1020,12 @@ int rk4_fused_step_on(shx_system* s, const FusedRegs& g, shx_field* u0, double t
 // Whether the fused form can step this system at all.
 int fused_supported(shx_system* s)
 {
-    if (s->general)
+    if (s->general && !s->fused_free)
         return fail(s->ctx, SHX_EUNSUPPORTED,
-                    "fused form: floating/Neumann bodies and Neumann grid faces take the reference schedule "
-                    "(shx_rk4_step)");
+                    "fused form: Neumann grid faces take the reference schedule (shx_rk4_step)");
+    if (s->general && s->variant < 1)
+        return fail(s->ctx, SHX_EUNSUPPORTED,
+                    "fused form: floating/Neumann bodies need the x-marching variants (1-4)");
     if (s->precision == SHX_F32_COMP && s->variant < 1)
         return fail(s->ctx, SHX_EUNSUPPORTED, "mixed precision needs the x-marching form (variants 1-4)");
     return SHX_OK;
@@ -1036,6 +1052,7 @@ int carve_workspace(shx_system* s, FusedRegs& g)
     auto round = [](size_t b) { return (b + 255) & ~size_t(255); };
     size_t need = round(N * sizeof(double)) + 2 * round(N * (mixed ? sizeof(float) : sizeof(double)));
     for (int c = 0; c < 3; ++c) need += 2 * round((size_t)s->nR[c] * sizeof(double));
+    need += round((size_t)s->nfree * sizeof(double));
     if (need > kScratchBytes)
         return fail(s->ctx, SHX_EINVAL,
                     "no-alloc fused steps: the registers of " + std::to_string(s->N) + " points need " +
@@ -1054,6 +1071,7 @@ int carve_workspace(shx_system* s, FusedRegs& g)
     }
     for (int b = 0; b < 2; ++b)
         for (int c = 0; c < 3; ++c) g.stageR[b][c] = static_cast<double*>(take((size_t)s->nR[c] * sizeof(double)));
+    g.accF = static_cast<double*>(take((size_t)s->nfree * sizeof(double)));
     return SHX_OK;
 }
 
@@ -1179,6 +1197,35 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
             rc = upload(ctx, reinterpret_cast<int8_t**>(&s->rdir[c]),
                         reinterpret_cast<const int8_t*>(fl8.data()), fl8.size());
     }
+    // the fused form's free cut points (no Neumann grid faces): point q of
+    // R(r) and the index of its row in each derivative's R-space row list
+    // (rows sorted by (set, row))
+    if (rc == SHX_OK && free_r && !neumann) {
+        std::vector<FreeRec> fr;
+        for (int r = 0; r < 3; ++r)
+            for (long q = 0; q < s->nR[r]; ++q) {
+                if (desc->r_dirichlet[r][q]) continue;
+                FreeRec e{};
+                e.q = q;
+                e.set = r;
+                for (int c = 0; c < 3; ++c) {
+                    e.row[c] = -1;
+                    const shx_derivative* d = s->d[c];
+                    if (!d || d->h_rset.empty()) continue;
+                    long lo = 0, hi = (long)d->h_rset.size();
+                    while (lo < hi) {  // first row not before (r, q)
+                        const long mid = (lo + hi) / 2;
+                        if (d->h_rset[mid] < r || (d->h_rset[mid] == r && d->h_rrow[mid] < q)) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    if (lo < (long)d->h_rset.size() && d->h_rset[lo] == r && d->h_rrow[lo] == q) e.row[c] = lo;
+                }
+                fr.push_back(e);
+            }
+        s->nfree = (long)fr.size();
+        rc = upload_vec(ctx, &s->frees, fr);
+        s->fused_free = rc == SHX_OK;
+    }
     if (rc == SHX_OK) {
         const long nparts = 1024;
         hipError_t e = hipMalloc((void**)&s->part, (size_t)nparts * 3 * sizeof(double));
@@ -1240,6 +1287,8 @@ int shx_system_destroy(shx_system* s)
     hipFree(s->stageF[0]);
     hipFree(s->stageF[1]);
     hipFree(s->accD);
+    hipFree(s->accF);
+    hipFree(s->frees);
     hipFree(s->cls);
     hipFree(s->special);
     for (int c = 0; c < 3; ++c) hipFree(s->srows[c]);
@@ -1730,7 +1779,7 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
             double* outR[3];
             stage_registers(s, own_regs(s), up, stage, usV, outV, usR, outR);
             long nitems = 0;
-            rc = march_args(s, usV, usR, up->D, s->accD, outV, outR, e.m, e.sp, nitems);
+            rc = march_args(s, usV, usR, up->D, rptrs(up), s->accD, s->accF, outV, outR, e.m, e.sp, nitems);
             if (rc != SHX_OK) {
                 s->march_ty = saved_ty;
                 s->march_vec = saved_vec;

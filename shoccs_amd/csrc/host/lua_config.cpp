@@ -676,6 +676,8 @@ int32_t bc_of(const std::string& s, int extent, std::string& err, const char* wh
     if (s == "dirichlet") return extent == 1 ? SHX_BC_FLOATING : SHX_BC_DIRICHLET;
     if (s == "floating") return SHX_BC_FLOATING;
     if (s == "neumann") return extent == 1 ? SHX_BC_FLOATING : SHX_BC_NEUMANN;
+    // extension for bodies of the constant-velocity wave (resolved per cut point)
+    if (s == "inflow" && std::string(where) == "shapes.boundary_condition") return SHX_BC_INFLOW;
     err = std::string("incorrect boundary condition '") + s + "' at " + where;
     return -1;
 }
@@ -749,9 +751,24 @@ int build(const Value& sim, shoccs_config& c, std::string& err)
     else if (order == 2 && stype == "E4") d.scheme = SHOCCS_E4_2;
     else if (order == 1 && stype == "E4") d.scheme = SHOCCS_E4_1;
     else if (order == 1 && stype == "E4u") d.scheme = SHOCCS_E4U_1;
-    else {
+    else if (order == 1 && stype == "E2-poly") {
+        // stencil.cpp:79-99: one alpha list (floating 6, Dirichlet 3,
+        // interpolant 4, the list split where it would be), else the three
+        // named lists; missing entries are zero (copy_zero_padded)
+        d.scheme = SHOCCS_E2P_1;
+        for (int i = 0; i < 13; ++i) d.alpha[i] = 0.0;
+        if (alpha.k == Value::Tab && alpha.t->get(1).k != Value::Nil) {
+            for (int i = 0; i < 13; ++i) d.alpha[i] = idx_or(alpha, i + 1, 0.0);
+        } else {
+            const Value fa = sub(sch, "floating_alpha"), da = sub(sch, "dirichlet_alpha"),
+                        ia = sub(sch, "interpolant_alpha");
+            for (int i = 0; i < 6; ++i) d.alpha[i] = idx_or(fa, i + 1, 0.0);
+            for (int i = 0; i < 3; ++i) d.alpha[6 + i] = idx_or(da, i + 1, 0.0);
+            for (int i = 0; i < 4; ++i) d.alpha[9 + i] = idx_or(ia, i + 1, 0.0);
+        }
+    } else {
         err = "scheme.order/type = " + std::to_string(order) + " / " + stype +
-              " not built (E2 and E4 second derivatives, E4 and E4u first derivatives)";
+              " not built (E2 and E4 second derivatives, E4, E4u and E2-poly first derivatives)";
         return SHX_EUNSUPPORTED;
     }
     // system (system.cpp:103-132)

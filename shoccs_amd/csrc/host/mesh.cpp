@@ -211,4 +211,21 @@ line mesh::interp_line(int dir, const int3& pt) const
     return out;
 }
 
+bool point_conditions(const mesh& m, const bcs::Object& obj, const double* vel, bcs::Points& out)
+{
+    for (int d = 0; d < 3; ++d) {
+        out[d].clear();
+        for (const mesh_object_info& r : m.R(d)) {
+            bcs::type bc = obj[r.shape_id];
+            if (bc == bcs::type::Inflow) {
+                if (!vel) return false;
+                const double an = vel[0] * r.normal[0] + vel[1] * r.normal[1] + vel[2] * r.normal[2];
+                bc = an > 0.0 ? bcs::type::Dirichlet : bcs::type::Floating;
+            }
+            out[d].push_back(bc);
+        }
+    }
+    return true;
+}
+
 }  // namespace ccs

@@ -20,14 +20,18 @@ using real3 = std::array<real, 3>;
 using int3 = std::array<int, 3>;
 
 namespace bcs {
-// src/operators/boundaries.hpp:13-20
-enum class type : int { Dirichlet = 0, Floating = 1, Neumann = 2 };
+// src/operators/boundaries.hpp:13-20.  Inflow (no reference counterpart) is a
+// shape condition of the constant-velocity wave that each cut point resolves
+// to Dirichlet or Floating (see point_conditions); operators only ever see
+// the resolved conditions.
+enum class type : int { Dirichlet = 0, Floating = 1, Neumann = 2, Inflow = 3 };
 struct Line {
     type left = type::Floating;
     type right = type::Floating;
 };
 using Grid = std::array<Line, 3>;
-using Object = std::vector<type>;
+using Object = std::vector<type>;  // one condition per shape
+using Points = std::array<std::vector<type>, 3>;  // one per cut point of R(x), R(y), R(z)
 }  // namespace bcs
 
 // index::dir<I>::slow / fast (src/indexing.hpp:21-37)
@@ -65,6 +69,16 @@ struct line {
     boundary start;
     boundary end;
 };
+
+class mesh;
+// The condition of every cut point: its shape's condition, with Inflow
+// resolved against the velocity vel (nullptr: Inflow is refused) to
+// Dirichlet where vel . n > 0 at the cut point (n the body's outward normal:
+// the flow leaves the body into the fluid) and Floating elsewhere -- the
+// reference's pattern of a Dirichlet body upstream and a floating one
+// downstream (scalar_wave.lua:10-22), applied point by point.  Returns false
+// when an Inflow shape has no velocity to resolve it.
+bool point_conditions(const mesh& m, const bcs::Object& obj, const double* vel, bcs::Points& out);
 
 class mesh
 {

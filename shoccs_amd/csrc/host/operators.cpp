@@ -91,12 +91,14 @@ int interp_weights(const stencil& st, int dir, const int3& closest, double y, co
 // cut point's neighbourhood (interp_deriv_coefficients, :22-67), each of its
 // points interpolated across direction r (OB_builder::add_interp_row,
 // :87-128).  Entries go to D (on_D) or to R(r) (on_R).
-void cut_rows(int r, int dir, const mesh& m, const stencil& st, const bcs::Object& obj,
+void cut_rows(int r, int dir, const mesh& m, const stencil& st, const std::vector<bcs::type>& pbc,
               std::vector<cut_entry>& on_D, std::vector<cut_entry>& on_R)
 {
     const auto& cuts = m.R(r);
+    // the reference's quick exit tests the shapes' conditions; skipping the
+    // Dirichlet points one by one gives the same rows
     if (cuts.empty() ||
-        std::all_of(obj.begin(), obj.end(), [](bcs::type b) { return b == bcs::type::Dirichlet; }))
+        std::all_of(pbc.begin(), pbc.end(), [](bcs::type b) { return b == bcs::type::Dirichlet; }))
         return;
     const info mx = st.query_max();
     const double h = m.h(dir);
@@ -105,7 +107,7 @@ void cut_rows(int r, int dir, const mesh& m, const stencil& st, const bcs::Objec
     std::vector<double> c(mx.r * mx.t + 16, 0.0), wts(mx.t + 16, 0.0), extra(16, 0.0);
     for (int64_t row = 0; row < (int64_t)cuts.size(); ++row) {
         const mesh_object_info& o = cuts[row];
-        const bcs::type bc = obj[o.shape_id];
+        const bcs::type bc = pbc[row];
         if (bc == bcs::type::Dirichlet) continue;
         const info q = st.query(bc);
         if (dir == r) {
@@ -178,8 +180,23 @@ void cut_rows(int r, int dir, const mesh& m, const stencil& st, const bcs::Objec
 
 }  // namespace
 
+namespace {
+bcs::Points shape_conditions(const mesh& m, const bcs::Object& obj)
+{
+    bcs::Points p;
+    point_conditions(m, obj, nullptr, p);
+    return p;
+}
+}  // namespace
+
 derivative::derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st,
                        const bcs::Grid& grid, const bcs::Object& obj)
+    : derivative(ctx, dir, m, st, grid, shape_conditions(m, obj))
+{
+}
+
+derivative::derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st,
+                       const bcs::Grid& grid, const bcs::Points& pbc)
     : dir_(dir)
 {
     const int3& n = m.extents();
@@ -206,7 +223,7 @@ derivative::derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st,
             const int64_t base = m.ic(ln.start.mesh_coordinate) - (int64_t)p0 * m.stride(dir);
 
             if (const auto& o = ln.start.object) {
-                const bcs::type bc = obj[o->objectID];
+                const bcs::type bc = pbc[dir][o->object_coordinate];
                 const info q = st.query(bc);
                 st.nbs(h, bc, o->psi, false, left.data(), extra.data());
                 const int s = bc != bcs::type::Dirichlet;
@@ -247,7 +264,7 @@ derivative::derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st,
             }
 
             if (const auto& o = ln.end.object) {
-                const bcs::type bc = obj[o->objectID];
+                const bcs::type bc = pbc[dir][o->object_coordinate];
                 const info q = st.query(bc);
                 st.nbs(h, bc, o->psi, true, right.data(), extra.data());
                 const int s = bc != bcs::type::Dirichlet;
@@ -284,6 +301,17 @@ derivative::derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st,
             }
             b.lo = row_lo;
             b.ni = rows - b.nl - b.nr;
+            if (b.ni < 0 && status_ == SHX_OK) {
+                // the closures of a segment shorter than its two closure
+                // blocks would overlap: the reference builds a circulant with
+                // a negative row count there (derivative.cpp:428-436), which
+                // is undefined, so such meshes are refused
+                status_ = SHX_EUNSUPPORTED;
+                error_ = "derivative along " + std::string(1, "xyz"[dir]) + ": a line segment of " +
+                         std::to_string(rows) + " rows (positions " + std::to_string(p0) + ".." +
+                         std::to_string(p1) + ") is shorter than its closures (" + std::to_string(b.nl) +
+                         " + " + std::to_string(b.nr) + " rows); refine the mesh or move the body";
+            }
             b.lcol0 = col_lo;
             b.rcol0 = col_lo + (columns - b.rcols);
             blocks_.push_back(b);
@@ -298,7 +326,7 @@ derivative::derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st,
     if (n[dir] >= 2) {
         for (int r = 0; r < 3; ++r) {
             std::vector<cut_entry> on_D, on_R;
-            cut_rows(r, dir, m, st, obj, on_D, on_R);
+            cut_rows(r, dir, m, st, pbc[r], on_D, on_R);
             auto order = [](const cut_entry& a, const cut_entry& b) {
                 if (a.row != b.row) return a.row < b.row;
                 if (a.col != b.col) return a.col < b.col;
@@ -351,7 +379,7 @@ derivative::derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st,
     desc.rcol = rcol_.data();
     desc.rw = rw_.data();
     // without a context only the host assembly is built (structure checks)
-    if (ctx) status_ = shx_derivative_create(ctx, &desc, &dev_);
+    if (ctx && status_ == SHX_OK) status_ = shx_derivative_create(ctx, &desc, &dev_);
 }
 
 derivative::~derivative()

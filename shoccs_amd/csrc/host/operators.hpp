@@ -9,6 +9,7 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "mesh.hpp"
@@ -20,6 +21,9 @@ namespace ccs {
 class derivative
 {
 public:
+    // pbc: the condition of every cut point (point_conditions)
+    derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st, const bcs::Grid& grid,
+               const bcs::Points& pbc);
     derivative(shx_ctx* ctx, int dir, const mesh& m, const stencil& st, const bcs::Grid& grid,
                const bcs::Object& obj);
     ~derivative();
@@ -27,6 +31,7 @@ public:
     derivative& operator=(const derivative&) = delete;
 
     int status() const { return status_; }
+    const std::string& error() const { return error_; }
     int dir() const { return dir_; }
     shx_derivative* device() const { return dev_; }
     const std::vector<shx_line_block>& blocks() const { return blocks_; }
@@ -47,6 +52,7 @@ public:
 private:
     int dir_ = 0;
     int status_ = SHX_OK;
+    std::string error_;
     shx_derivative* dev_ = nullptr;
     std::vector<shx_line_block> blocks_;
     std::vector<double> pool_;

@@ -9,6 +9,7 @@
 #include <limits>
 #include <memory>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "operators.hpp"
@@ -22,6 +23,7 @@ struct problem {
     std::unique_ptr<mesh> m;
     bcs::Grid grid;
     bcs::Object obj;
+    bcs::Points pbc;  // the condition of every cut point
     std::unique_ptr<derivative> d[3];
     shx_system* sys = nullptr;
     double diffusivity = 1.0;
@@ -35,18 +37,31 @@ struct shoccs_problem : ccs::problem {
 
 using namespace ccs;
 
+namespace {
+thread_local std::string g_last_error;
+
+int host_fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+}  // namespace
+
 extern "C" {
+
+const char* shoccs_last_error(void) { return g_last_error.c_str(); }
 
 int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_problem** out)
 {
     if (!desc || !out) return SHX_EINVAL;
     *out = nullptr;
+    g_last_error.clear();
     const double* pr = desc->params;
     const int np = desc->nparams;
     if (desc->kind == SHX_SYS_WAVE_RADIAL && np < 4) return SHX_EINVAL;
     if (desc->kind == SHX_SYS_WAVE_LINEAR && np < 8) return SHX_EINVAL;
     if (desc->kind == SHX_SYS_HEAT && np < 2) return SHX_EINVAL;
-    auto st = stencil::make(desc->scheme, desc->alpha, 2);
+    auto st = stencil::make(desc->scheme, desc->alpha, 13);
     if (!st) return SHX_EINVAL;
 
     auto p = std::unique_ptr<shoccs_problem>(new (std::nothrow) shoccs_problem());
@@ -66,9 +81,17 @@ int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_
     p->m = std::make_unique<mesh>(int3{desc->n[0], desc->n[1], desc->n[2]},
                                   real3{desc->min[0], desc->min[1], desc->min[2]},
                                   real3{desc->max[0], desc->max[1], desc->max[2]}, shapes);
+    // Inflow shapes resolve against the constant velocity of the linear wave
+    const double* vel = desc->kind == SHX_SYS_WAVE_LINEAR ? pr : nullptr;
+    if (!point_conditions(*p->m, p->obj, vel, p->pbc))
+        return host_fail(SHX_EINVAL, "an inflow body needs the constant-velocity wave");
+    if (desc->mesh_only) {
+        *out = p.release();
+        return SHX_OK;
+    }
     for (int a = 0; a < 3; ++a) {
-        p->d[a] = std::make_unique<derivative>(ctx, a, *p->m, *st, p->grid, p->obj);
-        if (p->d[a]->status() != SHX_OK) return p->d[a]->status();
+        p->d[a] = std::make_unique<derivative>(ctx, a, *p->m, *st, p->grid, p->pbc);
+        if (p->d[a]->status() != SHX_OK) return host_fail(p->d[a]->status(), p->d[a]->error());
     }
 
     shx_system_desc sd;
@@ -88,9 +111,10 @@ int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_
     std::vector<double> rpos[3];
     std::vector<int32_t> rdir[3];
     for (int a = 0; a < 3; ++a) {
-        for (const auto& r : m.R(a)) {
+        for (size_t q = 0; q < m.R(a).size(); ++q) {
+            const auto& r = m.R(a)[q];
             rpos[a].insert(rpos[a].end(), r.position.begin(), r.position.end());
-            rdir[a].push_back(p->obj[r.shape_id] == bcs::type::Dirichlet);
+            rdir[a].push_back(p->pbc[a][q] == bcs::type::Dirichlet);
         }
         rpos[a].push_back(0.0);
         rdir[a].push_back(0);
@@ -146,7 +170,7 @@ shx_system* shoccs_problem_system(shoccs_problem* p) { return p ? p->sys : nullp
 
 shx_derivative* shoccs_problem_derivative(shoccs_problem* p, int dir)
 {
-    if (!p || dir < 0 || dir > 2) return nullptr;
+    if (!p || dir < 0 || dir > 2 || !p->d[dir]) return nullptr;
     return p->d[dir]->device();
 }
 
@@ -193,7 +217,10 @@ int shoccs_euler_create(shoccs_problem* p, double gamma, shx_euler** out)
         d.wall_normal[a] = nr[a].data();
     }
     shx_derivative* dv[3];
-    for (int a = 0; a < 3; ++a) dv[a] = n[a] > 1 ? p->d[a]->device() : nullptr;
+    for (int a = 0; a < 3; ++a) {
+        if (!p->d[a]) return SHX_EINVAL;  // a mesh-only problem
+        dv[a] = n[a] > 1 ? p->d[a]->device() : nullptr;
+    }
     return shx_euler_create(p->ctx, dv, &d, out);
 }
 
@@ -221,7 +248,7 @@ int shoccs_error_field(const shoccs_problem* p, const double* u, const double* s
     for (int a = 0; a < 3; ++a) {
         const auto& R = m.R(a);
         for (size_t i = 0; i < R.size(); ++i)
-            if (p->obj[R[i].shape_id] != bcs::type::Dirichlet)
+            if (p->pbc[a][i] != bcs::type::Dirichlet)
                 err[off + (int64_t)i] = std::abs(u[off + (int64_t)i] - sol[off + (int64_t)i]);
         off += (int64_t)R.size();
     }
@@ -293,12 +320,12 @@ int shoccs_mesh_fluid(const shoccs_problem* p, int64_t* out)
 
 int64_t shoccs_deriv_nblocks(const shoccs_problem* p, int dir)
 {
-    return p && dir >= 0 && dir < 3 ? (int64_t)p->d[dir]->blocks().size() : -1;
+    return p && dir >= 0 && dir < 3 && p->d[dir] ? (int64_t)p->d[dir]->blocks().size() : -1;
 }
 
 int shoccs_deriv_blocks(const shoccs_problem* p, int dir, shx_line_block* out)
 {
-    if (!p || dir < 0 || dir > 2 || !out) return SHX_EINVAL;
+    if (!p || dir < 0 || dir > 2 || !out || !p->d[dir]) return SHX_EINVAL;
     const auto& b = p->d[dir]->blocks();
     std::memcpy(out, b.data(), b.size() * sizeof(shx_line_block));
     return SHX_OK;
@@ -306,12 +333,12 @@ int shoccs_deriv_blocks(const shoccs_problem* p, int dir, shx_line_block* out)
 
 int64_t shoccs_deriv_npool(const shoccs_problem* p, int dir)
 {
-    return p && dir >= 0 && dir < 3 ? (int64_t)p->d[dir]->pool().size() : -1;
+    return p && dir >= 0 && dir < 3 && p->d[dir] ? (int64_t)p->d[dir]->pool().size() : -1;
 }
 
 int shoccs_deriv_pool(const shoccs_problem* p, int dir, double* out)
 {
-    if (!p || dir < 0 || dir > 2 || !out) return SHX_EINVAL;
+    if (!p || dir < 0 || dir > 2 || !out || !p->d[dir]) return SHX_EINVAL;
     const auto& v = p->d[dir]->pool();
     std::memcpy(out, v.data(), v.size() * sizeof(double));
     return SHX_OK;
@@ -319,7 +346,7 @@ int shoccs_deriv_pool(const shoccs_problem* p, int dir, double* out)
 
 int64_t shoccs_deriv_ncut(const shoccs_problem* p, int dir, int64_t* nnz)
 {
-    if (!p || dir < 0 || dir > 2) return -1;
+    if (!p || dir < 0 || dir > 2 || !p->d[dir]) return -1;
     if (nnz) *nnz = (int64_t)p->d[dir]->rw().size();
     return (int64_t)p->d[dir]->rrow().size();
 }
@@ -327,7 +354,7 @@ int64_t shoccs_deriv_ncut(const shoccs_problem* p, int dir, int64_t* nnz)
 int shoccs_deriv_cut(const shoccs_problem* p, int dir, int32_t* rset, int64_t* rrow, int64_t* rptr,
                      int64_t* rcol, double* rw)
 {
-    if (!p || dir < 0 || dir > 2 || !rset || !rrow || !rptr || !rcol || !rw) return SHX_EINVAL;
+    if (!p || dir < 0 || dir > 2 || !rset || !rrow || !rptr || !rcol || !rw || !p->d[dir]) return SHX_EINVAL;
     const derivative& d = *p->d[dir];
     std::memcpy(rset, d.rset().data(), d.rset().size() * sizeof(int32_t));
     std::memcpy(rrow, d.rrow().data(), d.rrow().size() * sizeof(int64_t));
@@ -339,7 +366,7 @@ int shoccs_deriv_cut(const shoccs_problem* p, int dir, int32_t* rset, int64_t* r
 
 int shoccs_deriv_interior(const shoccs_problem* p, int dir, double* out9)
 {
-    if (!p || dir < 0 || dir > 2 || !out9) return SHX_EINVAL;
+    if (!p || dir < 0 || dir > 2 || !out9 || !p->d[dir]) return SHX_EINVAL;
     const auto& v = p->d[dir]->interior();
     for (size_t j = 0; j < 9; ++j) out9[j] = j < v.size() ? v[j] : 0.0;
     return (int)v.size();

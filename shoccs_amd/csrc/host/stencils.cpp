@@ -8,6 +8,7 @@
 //   E4u_1 (P2 R3 T5)            src/stencils/E4u_1.cpp:26-126
 //   E4_2  (P2 R3 T5 X3)         src/stencils/E4_2.cpp:16-42,185-296
 //   E2_2  (P1 R2 T4 X2)         src/stencils/E2_2.cpp:12-180
+//   E2-poly (P1 R3 T4)          src/stencils/polyE2_1.cpp:10-257
 //
 // Right-wall closures are the left ones reversed (negated for the first
 // derivative).
@@ -629,6 +630,87 @@ int e42_interp_wall(int i, double y, double psi, double* c, bool right)
     return 5;
 }
 
+/* ------------------------------------------------------------- E2-poly
+ * polyE2_1.cpp:166-245 closures (fa: the 6 floating parameters, da: the 3
+ * Dirichlet ones) and :52-138 wall interpolants (fa[0..3], ia: the 4
+ * interpolant parameters). */
+void e2p_closure_floating(const double* fa, double h, double psi, double* c, bool right)
+{
+    const double q = 1.0 / (1 + psi), s = 2 + psi;
+    for (int row = 0; row < 3; ++row) {
+        const double lo = fa[2 * row], hi = fa[2 * row + 1], him = -1 + hi;
+        double* o = c + 4 * row;
+        // (row 0 multiplies its factors in another order than rows 1 and 2)
+        o[0] = row == 0 ? 0.5 * q * him : 0.5 * him * q;
+        o[1] = (-1 + lo) * 0.5;
+        o[2] = -1 * lo + (row == 0 ? -0.5 * s * q * him : -0.5 * s * him * q);
+        o[3] = (row == 0 ? lo + hi : hi + lo) * 0.5;
+    }
+    for (int i = 0; i < 12; ++i) c[i] /= h;
+    if (right) {
+        for (int i = 0; i < 12; ++i) c[i] *= -1;
+        reverse(c, 12);
+    }
+}
+
+void e2p_closure_dirichlet(const double* da, double h, double psi, double* c, bool right)
+{
+    const double d0 = da[0], d1 = da[1], d2 = da[2];
+    const double q = 1.0 / (1 + psi), d0m = -1 + d0, d2m = -1 + d2, s = 2 + psi;
+    const double p2 = 2 * psi, d2x3 = 3 * d2, d2p2 = 2 * psi * d2;
+    const double inv = 1.0 / (1 + p2 + d2x3 + d2p2);
+    const double d1p2 = 2 * psi * d1, d1d0x3 = 3 * d1 * d0, d1d0p2 = 2 * psi * d1 * d0;
+    c[0] = 0.5 * q * d0m;
+    c[1] = (-1 + -2 * psi + d1 + d1p2 + d1d0x3 + d1d0p2 + -3 * d0 + -2 * psi * d0) * 0.5 * inv;
+    c[2] = (-2 * psi * d1 + -3 * d2 + -2 * psi * d2 + -1 * d1 + 3 * d0 + 2 * psi * d0 + -3 * d1 * d0 +
+            -2 * psi * d1 * d0) *
+               inv +
+           -0.5 * s * q * d0m;
+    c[3] = (d1 + d1p2 + d1d0x3 + d1d0p2 + d2x3 + d2p2 + -2 * d0 + 3 * d2 * d0 + 2 * psi * d2 * d0) * 0.5 * inv;
+    c[4] = 0.5 * d2m * q;
+    c[5] = (-1 + d1) * 0.5;
+    c[6] = -1 * d1 + -0.5 * s * d2m * q;
+    c[7] = (d1 + d2) * 0.5;
+    for (int i = 0; i < 8; ++i) c[i] /= h;
+    if (right) {
+        for (int i = 0; i < 8; ++i) c[i] *= -1;
+        reverse(c, 8);
+    }
+}
+
+int e2p_interp_wall(const double* fa, const double* ia, int i, double y, double psi, double* c, bool right)
+{
+    if (i != 0 && i != 1) return 4;
+    const double q = 1.0 / (1 + psi), my = -1 * y;
+    // row i uses the parameter pairs (fa, ia)[2 i], [2 i + 1]
+    const double fl = fa[2 * i], fh = fa[2 * i + 1], il = ia[2 * i], ih = ia[2 * i + 1];
+    const double fly = fl * y, fhy = fh * y;
+    // the row of node 1 - psi carries psi (i = 0) or nothing (i = 1)
+    const double lead = i == 0 ? 1 + psi : 1;
+    if (right) {
+        c[0] = (i == 0 ? fly + fhy + il + ih : ih + fly + fhy + il) * 0.5;
+        c[1] = i == 0 ? (-1 * psi + my + (fly + il) * -2 +
+                         (my + -2 * ih + -1 * psi * ih + -2 * fh * y + -1 * psi * fh * y) * q) *
+                            0.5
+                      : (my + (fly + il) * -2 +
+                         (psi + -2 * ih + -1 * psi * ih + my + -2 * fh * y + -1 * psi * fh * y) * q) *
+                            0.5;
+        c[2] = (lead + fly + il + y) * 0.5;
+        c[3] = (i == 0 ? 1 + psi + fhy + ih + y : 1 + ih + fhy + y) * 0.5 * q;
+    } else {
+        c[0] = (i == 0 ? 1 + psi + ih + my + fhy : 1 + fhy + ih + my) * 0.5 * q;
+        c[1] = (lead + fly + il + my) * 0.5;
+        c[2] = i == 0 ? (-1 * psi + (fly + il) * -2 + y +
+                         (-2 * ih + -1 * psi * ih + y + -2 * fh * y + -1 * psi * fh * y) * q) *
+                            0.5
+                      : ((fly + il) * -2 + y +
+                         (psi + -2 * ih + -1 * psi * ih + y + -2 * fh * y + -1 * psi * fh * y) * q) *
+                            0.5;
+        c[3] = (i == 0 ? ih + fly + il + fhy : fhy + ih + fly + il) * 0.5;
+    }
+    return 4;
+}
+
 }  // namespace
 
 info stencil::query_max() const
@@ -638,6 +720,7 @@ info stencil::query_max() const
     case scheme::E4_2: return {2, 3, 5, 3};
     case scheme::E4_1: return {2, 5, 7, 0};
     case scheme::E4u_1: return {2, 3, 5, 0};
+    case scheme::E2poly_1: return {1, 3, 4, 0};
     }
     return {};
 }
@@ -649,7 +732,7 @@ info stencil::query(bcs::type b) const
     case bcs::type::Dirichlet: m.r -= 1; m.nextra = 0; return m;
     case bcs::type::Floating: m.nextra = 0; return m;
     case bcs::type::Neumann:
-        if (kind_ == scheme::E4_1 || kind_ == scheme::E4u_1) return {};
+        if (kind_ == scheme::E4_1 || kind_ == scheme::E4u_1 || kind_ == scheme::E2poly_1) return {};
         return m;
     }
     return {};
@@ -685,6 +768,11 @@ void stencil::interior(double h, double* c) const
         c[3] = -c[1];
         c[4] = -c[0];
         break;
+    case scheme::E2poly_1:
+        c[0] = -0.5 / h;
+        c[1] = 0 / h;
+        c[2] = 0.5 / h;
+        break;
     }
 }
 
@@ -710,14 +798,18 @@ void stencil::nbs(double h, bcs::type b, double psi, bool right, double* c, doub
         else if (b == bcs::type::Dirichlet) e22_dirichlet(h, psi, c, right);
         else e22_neumann(h, psi, c, extra, right);
         break;
+    case scheme::E2poly_1:
+        if (b == bcs::type::Floating) e2p_closure_floating(a, h, psi, c, right);
+        else if (b == bcs::type::Dirichlet) e2p_closure_dirichlet(a + 6, h, psi, c, right);
+        break;
     }
 }
 
 std::optional<stencil> stencil::make(int kind, const double* alpha, int nalpha)
 {
-    std::array<double, 2> a{0.0, 0.0};
-    for (int i = 0; i < nalpha && i < 2; ++i) a[i] = alpha[i];
-    if (kind < 0 || kind > 3) return std::nullopt;
+    std::array<double, kMaxAlpha> a{};
+    for (int i = 0; i < nalpha && i < kMaxAlpha; ++i) a[i] = alpha[i];
+    if (kind < 0 || kind > 4) return std::nullopt;
     // E4_1 denominator is singular for alpha[1] < 197/288 (src/stencils/E4_1.cpp:34-41)
     if (kind == (int)scheme::E4_1 && a[1] < 197.0 / 288.0) return std::nullopt;
     return stencil{(scheme)kind, a};
@@ -729,12 +821,13 @@ interp_info stencil::query_interp() const
 {
     if (kind_ == scheme::E2_2) return {2, 3};
     if (kind_ == scheme::E4_2) return {5, 5};
+    if (kind_ == scheme::E2poly_1) return {2, 4};
     return {};
 }
 
 int stencil::interp_interior(double y, double* c, int cap) const
 {
-    if (kind_ == scheme::E2_2) {
+    if (kind_ == scheme::E2_2 || kind_ == scheme::E2poly_1) {
         if (y > 0) {
             c[0] = 1 + -1 * y;
             c[1] = y;
@@ -760,6 +853,7 @@ int stencil::interp_wall(int i, double y, double psi, double* c, bool right, int
 {
     if (kind_ == scheme::E2_2) return e22_interp_wall(i, y, psi, c, right);
     if (kind_ == scheme::E4_2) return e42_interp_wall(i, y, psi, c, right);
+    if (kind_ == scheme::E2poly_1) return e2p_interp_wall(alpha_.data(), alpha_.data() + 9, i, y, psi, c, right);
     return cap;
 }
 

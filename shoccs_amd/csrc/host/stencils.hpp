@@ -24,14 +24,18 @@ struct interp_info {
     int t = 0;
 };
 
-// scheme{order, type}: order 2 "E2"/"E4" and order 1 "E4"/"E4u"
-enum class scheme : int { E2_2 = 0, E4_2 = 1, E4_1 = 2, E4u_1 = 3 };
+// scheme{order, type}: order 2 "E2"/"E4" and order 1 "E4"/"E4u"/"E2-poly"
+enum class scheme : int { E2_2 = 0, E4_2 = 1, E4_1 = 2, E4u_1 = 3, E2poly_1 = 4 };
+
+// the most free parameters a scheme takes (E2-poly: floating 6, Dirichlet 3,
+// interpolant 4, in that order)
+constexpr int kMaxAlpha = 13;
 
 class stencil
 {
 public:
     stencil() = default;
-    stencil(scheme k, std::array<double, 2> alpha) : kind_(k), alpha_(alpha) {}
+    stencil(scheme k, std::array<double, kMaxAlpha> alpha) : kind_(k), alpha_(alpha) {}
 
     // returns nullopt for an unknown scheme or an E4_1 alpha[1] below 197/288
     static std::optional<stencil> make(int kind, const double* alpha, int nalpha);
@@ -51,7 +55,7 @@ public:
 
 private:
     scheme kind_ = scheme::E4_2;
-    std::array<double, 2> alpha_{0.0, 0.0};
+    std::array<double, kMaxAlpha> alpha_{};
 };
 
 }  // namespace ccs::stencils

@@ -112,12 +112,28 @@ def oracle_system(cfg):
                           cfg["grid_bc"], st, cfg["params"])
 
 
-def product_problem(ctx, cfg):
+def product_problem(ctx, cfg, mesh_only=False):
     import shoccs_amd as S
 
     return S.Problem(ctx, cfg["kind"], cfg["n"], cfg["lo"], cfg["hi"], spheres=cfg["spheres"],
                      shape_bc=cfg["obj_bc"], grid_bc=cfg["grid_bc"], scheme=cfg["scheme"],
-                     alpha=cfg["alpha"], params=cfg["params"])
+                     alpha=cfg["alpha"], params=cfg["params"], mesh_only=mesh_only)
+
+
+# E2-poly parameters of the reference's scalar_wave.lua:29-40 (floating,
+# Dirichlet, interpolant)
+E2P_LUA = [0.7039278390946743, 0.5390086175376538, -0.647109821986589, 0.2051508287133347,
+           0.6062051039572746, 0.8148425279273044, -0.10739761225713096, 0.8736492896991024,
+           0.40413606410467495, 0.01, -0.01, 0.01, 0.04]
+
+
+def wave_e2p_cfg(n=31, obj_bc=O.INFLOW, center=(0.5, 0.5, 0.5), radius=0.3):
+    """The constant-velocity wave of sphere_wave_cfg with the reference's
+    E2-poly scheme; the sphere's condition defaults to the per-point inflow
+    condition (Dirichlet where a . n > 0, floating elsewhere)."""
+    cfg = sphere_wave_cfg(n, scheme=O.E2P_1, alpha=E2P_LUA, radius=radius, center=center)
+    cfg["obj_bc"] = [obj_bc]
+    return cfg
 
 
 def cut_cfgs():
@@ -140,9 +156,11 @@ def cut_cfgs():
                                [D, F], [F, F, D, D, N, N], O.E2_2),
         "floating_E4u_1": heat((32, 32, 32), (0, 0, 0), (1, 1, 1), [[0.5, 0.5, 0.5, 0.25]], [F], [D, F] * 3, O.E4U_1,
                                (-0.7733323791884821, 0.1623961700641681)),
-        "floating_E4_1": heat((36, 36, 36), (0, 0, 0), (1, 1, 1), [[0.51, 0.48, 0.5, 0.27]], [F], [F] * 6, O.E4_1,
+        "floating_E4_1": heat((48, 48, 48), (0, 0, 0), (1, 1, 1), [[0.51, 0.48, 0.5, 0.27]], [F], [F] * 6, O.E4_1,
                               (0.1, 0.7)),
         "floating2d_E4_2": heat((41, 43, 1), (0, 0, 0), (1, 1, 0), [[0.5, 0.52, 0.0, 0.23]], [F], [F] * 6, O.E4_2),
+        "inflow_E2P": wave_e2p_cfg(31),
+        "floating_E2P": wave_e2p_cfg(33, obj_bc=F, center=(0.52, 0.47, 0.49), radius=0.27),
     }
 
 

@@ -14,9 +14,11 @@ E4U_ALPHA = (-0.7733323791884821, 0.1623961700641681)
 
 
 def euler_cfg(n, lo, hi, spheres=(), grid_bc=None, scheme=O.E4U_1, alpha=E4U_ALPHA):
-    """E4u by default: the E4_1 closures at Dirichlet bodies do not
-    annihilate constants (their rows do not sum to zero, as the reference's
-    coefficients give them), so a uniform state would not stay uniform."""
+    """E4u by default.  E4_1's rows sum to zero, but at grid walls it clamps
+    psi = 1 to 1 - 1e-4 (E4_1.cpp:100-101), where its closure coefficients
+    reach about 4.5e5 / h: no explicit RK4 step of usable size is stable.
+    Short segments between a body and a wall are refused with either scheme
+    when their two closure blocks would overlap."""
     n = tuple(n) + (1,) * (3 - len(n))
     return dict(n=n, lo=lo, hi=hi, spheres=[list(s) for s in spheres],
                 grid_bc=list(grid_bc) if grid_bc else [O.FLOATING] * 6, scheme=scheme, alpha=alpha)

@@ -13,7 +13,9 @@ import oracle_ffi as O
 
 
 def _prob(cfg):
-    return H.product_problem(None, cfg)
+    # the mesh and cut points are all the output needs (extents as small as
+    # 3 leave no room for the closures of any scheme)
+    return H.product_problem(None, cfg, mesh_only=True)
 
 
 def _fires(io, prob, seq):

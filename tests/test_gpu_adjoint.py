@@ -10,12 +10,15 @@ import oracle_ffi as O
 
 pytestmark = pytest.mark.gpu
 
+# small grids (the explicit-transpose test assembles dense matrices), with
+# spheres small enough that no segment is shorter than its two closures
 CFGS = {
-    "wave16_E4u": H.sphere_wave_cfg(16, scheme=O.E4U_1, alpha=(-0.7733323791884821, 0.1623961700641681)),
-    "wave20_E4u_overlap": H.sphere_wave_cfg(20, scheme=O.E4U_1, alpha=(0.1, 0.2)),
-    "wave18_radial": H.sphere_wave_cfg(18, linear=False, scheme=O.E4U_1),
-    "heat3d": H.heat3d_cfg(n=(14, 16, 18)),
-    "heat3d_odd": H.heat3d_cfg(n=(13, 15, 17)),
+    "wave16_E4u": H.sphere_wave_cfg(16, scheme=O.E4U_1, alpha=(-0.7733323791884821, 0.1623961700641681),
+                                    radius=0.2),
+    "wave22_E4u_short": H.sphere_wave_cfg(22, scheme=O.E4U_1, alpha=(0.1, 0.2)),
+    "wave18_radial": H.sphere_wave_cfg(18, linear=False, scheme=O.E4U_1, radius=0.2),
+    "heat3d": H.heat3d_cfg(n=(14, 16, 18), radius=0.18),
+    "heat3d_odd": H.heat3d_cfg(n=(13, 15, 17), radius=0.15),
     "heat2d": H.heat2d_cfg(21),
 }
 

@@ -19,7 +19,7 @@ F, D = O.FLOATING, O.DIRICHLET
 CASES = {
     "sphere3d": euler_cfg((24, 22, 20), (0, 0, 0), (1, 1, 1), spheres=[(0.5, 0.48, 0.52, 0.26)],
                           grid_bc=[D, F, F, F, F, F]),
-    "sphere3d_noncubic": euler_cfg((30, 18, 26), (0, 0, 0), (1.2, 0.8, 1.0), spheres=[(0.6, 0.4, 0.5, 0.22)],
+    "sphere3d_noncubic": euler_cfg((30, 18, 26), (0, 0, 0), (1.2, 0.8, 1.0), spheres=[(0.6, 0.4, 0.5, 0.2)],
                                    grid_bc=[D, F, D, F, F, F]),
     "vortex2d": euler_cfg((41, 41), (-5, -5, 0), (5, 5, 0)),
 }

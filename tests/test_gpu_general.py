@@ -32,8 +32,15 @@ CFGS = {
                                            0.6, 0.3, 0.0, 0.3, 0.2, 0.0, -0.7, 1.0)),
     "wave_floating_E4u": dict(H.sphere_wave_cfg(24, scheme=O.E4U_1, alpha=(-0.7733323791884821, 0.1623961700641681)),
                               obj_bc=[F]),
+    # the reference's E2-poly scheme (scalar_wave.lua) with the per-point
+    # inflow condition (Dirichlet where a . n > 0) and with a floating sphere
+    "heat_floating_dirichlet_faces": _heat((22, 20, 24), [[0.52, 0.47, 0.5, 0.2]], [F], [D] * 6),
+    "heat2d_gauss_floating_D": _heat((33, 31, 1), [[0.5, 0.5, 0.0, 0.2]], [F], [D, D, D, D, F, F],
+                                     params=(0.8, O.MMS_GAUSS, 1, 0.45, 0.55, 0.0, 0.2, 0.25, 0.0, 1.3, 2.5)),
+    "wave_inflow_E2P": H.wave_e2p_cfg(31),
+    "wave_floating_E2P": H.wave_e2p_cfg(33, obj_bc=F, center=(0.52, 0.47, 0.49), radius=0.27),
 }
-WAVES = ["wave_floating_E4u"]
+WAVES = ["wave_floating_E4u", "wave_inflow_E2P", "wave_floating_E2P"]
 
 
 def _pair(ctx, name):
@@ -126,8 +133,62 @@ def test_rk4_and_euler_steps_match_oracle(ctx, name):
         want, got = u.flat(), gu.download()
         err = np.max(np.abs(got - want)) / np.max(np.abs(want))
         assert err <= 1e-11, f"{name} {'euler' if euler else 'rk4'}: {err:.2e}"
-    with pytest.raises(S.ShoccsError):
-        prob.rk4_fused(gu, 0.0, dt, 1)
+    if N in CFGS[name]["grid_bc"]:
+        with pytest.raises(S.ShoccsError, match="Neumann grid faces"):
+            prob.rk4_fused(gu, 0.0, dt, 1)
+    prob.close()
+
+
+FUSED = [k for k, c in CFGS.items() if N not in c["grid_bc"]]
+
+
+@pytest.mark.parametrize("name", FUSED)
+def test_fused_steps_match_oracle(ctx, name):
+    """The fused x-marching form over floating and Neumann bodies: the free
+    cut points are special-work items (their rows in the three directions,
+    the rhs at the point, the stage update of value and accumulator); four
+    steps against the oracle's RK4 schedule to 1e-11 of the max-norm, every
+    kernel variant of the x-marching form giving the same doubles; the mixed
+    form stays within 1e-5 of FP64."""
+    import shoccs_amd as S
+
+    osys, prob = _pair(ctx, name)
+    assert sum(prob.nR) > 0
+    dt = osys.dt(0.4, 0.4)
+    u = osys.field()
+    osys.initialize(u, 0.0)
+    osys.update_boundary(u, 0.0)
+    start = u.flat().copy()
+    out, rk, srhs = osys.field(), osys.field(), osys.field()
+    t = 0.0
+    nsteps = 4
+    for _ in range(nsteps):
+        osys.rk4_step(u, out, rk, srhs, t, dt)
+        u.D[:] = out.D
+        for d in range(3):
+            u.R[d][:] = out.R[d]
+        t += dt
+    want = u.flat()
+    scale = np.max(np.abs(want))
+    ref = None
+    for variant in (1, 2, 3, 4):
+        prob.set_tuning(variant, 4, 16)
+        g = prob.field()
+        g.upload(start)
+        prob.rk4_fused(g, 0.0, dt, nsteps)
+        got = g.download()
+        err = np.max(np.abs(got - want)) / scale
+        assert err <= 1e-11, f"{name} variant {variant}: {err:.2e}"
+        assert np.count_nonzero(got[prob.N:] != start[prob.N:]) > 0  # the free cut points moved
+        if ref is None:
+            ref = got
+        assert np.array_equal(got, ref), f"{name} variant {variant}"
+    prob.set_tuning(3, 4, 16)
+    prob.set_precision(S.F32_COMP)
+    g = prob.field()
+    g.upload(start)
+    prob.rk4_fused(g, 0.0, dt, nsteps)
+    assert np.max(np.abs(g.download() - ref)) / scale <= 1e-5
     prob.close()
 
 
@@ -177,6 +238,14 @@ def test_fused_batch_and_adjoint_refuse(ctx):
         prob.rk4_fused(f, 0.0, 1e-4, 1)
     with pytest.raises(S.ShoccsError):
         S.Batch(ctx, [prob], [f], [0.0], [1e-4])
+    osys2, prob2 = _pair(ctx, "heat_floating_dirichlet_faces")
+    f2 = prob2.field()
+    prob2.set_tuning(0)
+    with pytest.raises(S.ShoccsError, match="x-marching"):
+        prob2.rk4_fused(f2, 0.0, 1e-4, 1)
+    with pytest.raises(S.ShoccsError):
+        S.Batch(ctx, [prob2], [f2], [0.0], [1e-4])
+    prob2.close()
     with pytest.raises(S.ShoccsError):
         prob.rhs_T(f, g)
     prob.close()

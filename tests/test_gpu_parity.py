@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 WAVE_CFGS = {
     "wave48_E4_1": H.sphere_wave_cfg(48),
-    "wave20_E4u_overlap": H.sphere_wave_cfg(20, scheme=O.E4U_1, alpha=(0.1, 0.2)),
+    "wave22_E4u_short": H.sphere_wave_cfg(22, scheme=O.E4U_1, alpha=(0.1, 0.2)),
     "wave30_radial_E4u": H.sphere_wave_cfg(30, linear=False, scheme=O.E4U_1),
 }
 HEAT_CFGS = {
@@ -160,7 +160,7 @@ def test_rk4_steps(ctx, name, fused):
     assert err <= 1e-11, f"{name}: relative max-norm error {err:.3e}"
 
 
-@pytest.mark.parametrize("name", ["wave48_E4_1", "wave30_radial_E4u", "wave20_E4u_overlap",
+@pytest.mark.parametrize("name", ["wave48_E4_1", "wave30_radial_E4u", "wave22_E4u_short",
                                   "heat3d_E4_2", "heat3d_E2_2", "heat2d_21", "heat3d_odd"])
 @pytest.mark.parametrize("variant,ty,xchunk", [(1, 4, 16), (2, 8, 7), (3, 8, 64), (4, 4, 1000)])
 def test_march_kernel_bitexact(ctx, name, variant, ty, xchunk):

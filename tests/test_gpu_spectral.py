@@ -51,7 +51,7 @@ def test_operator_matrix_matches_oracle(ctx, case):
     if case == "line_E4u":
         cfg, d = _line(31, O.E4U_1, [D, F], alpha=E4U_ALPHA, kind=O.SYS_WAVE_RADIAL, params=(-0.25, 0, 0, 0)), 0
     else:
-        cfg = dict(kind=O.SYS_HEAT, n=(17, 15, 1), lo=(0, 0, 0), hi=(1, 1, 0), spheres=[[0.5, 0.52, 0.0, 0.24]],
+        cfg = dict(kind=O.SYS_HEAT, n=(19, 17, 1), lo=(0, 0, 0), hi=(1, 1, 0), spheres=[[0.5, 0.52, 0.0, 0.22]],
                    obj_bc=[F], grid_bc=[D, F, F, D, F, F], scheme=O.E4_2, alpha=(0.0, 0.0),
                    params=[1.0, O.MMS_SINPROD])
         d = 0 if case.endswith("x") else 1

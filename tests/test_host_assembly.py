@@ -9,11 +9,12 @@ import oracle_ffi as O
 
 CFGS = {
     "wave48_E4_1": H.sphere_wave_cfg(48),
-    "wave20_E4u_overlapping_closures": H.sphere_wave_cfg(20, scheme=O.E4U_1, alpha=(0.1, 0.2)),
+    "wave22_E4u_short_segments": H.sphere_wave_cfg(22, scheme=O.E4U_1, alpha=(0.1, 0.2)),
     "wave30_radial": H.sphere_wave_cfg(30, linear=False, scheme=O.E4U_1),
     "heat2d_101_config1": H.heat2d_cfg(101),
     "heat3d_noncubic": H.heat3d_cfg(),
     "heat3d_E2": H.heat3d_cfg(scheme=O.E2_2),
+    "wave31_E2P_inflow": H.wave_e2p_cfg(31),
 }
 
 
@@ -54,13 +55,21 @@ def test_timestep_matches():
         assert osys.dt(0.5, 0.8) == prob.dt(0.5, 0.8)
 
 
-def test_rejects_closures_outside_the_mesh():
-    """A 24^3 E4_1 sphere has 4-row segments whose overlapping right closure
-    would read before the start of the mesh buffer (undefined in the
-    reference); the oracle and the device upload both refuse it."""
-    cfg = H.sphere_wave_cfg(24)
+@pytest.mark.parametrize("n,scheme,alpha", [(24, O.E4_1, (0.1, 0.7)), (20, O.E4U_1, (0.1, 0.2)),
+                                             (41, O.E4_1, (0.1, 0.7))])
+def test_rejects_overlapping_closures(n, scheme, alpha):
+    """A segment between the sphere and a wall shorter than its two closure
+    blocks would need a circulant with a negative row count
+    (derivative.cpp:428-436), undefined in the reference: the oracle and the
+    host assembly both refuse the mesh, the latter with a message that names
+    the segment."""
+    import shoccs_amd as S
+
+    cfg = H.sphere_wave_cfg(n, scheme=scheme, alpha=alpha)
     with pytest.raises(ValueError):
         H.oracle_system(cfg)
+    with pytest.raises(S.ShoccsError, match="shorter than its closures"):
+        H.product_problem(None, cfg)
 
 
 CUT = H.cut_cfgs()

@@ -9,6 +9,8 @@ import sys
 import numpy as np
 import pytest
 
+import helpers as H
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = "/root/reference"
 
@@ -132,8 +134,8 @@ def test_cli_check(tmp_path):
 def test_reference_configs():
     """The reference's own configuration files: the 2-D radial-wave configs
     and heat.lua (floating circle, gaussian solution) read completely; the
-    others are refused with the reason (a template placeholder, the E2-poly
-    scheme)."""
+    others are refused with the reason (a template placeholder); scalar_wave.lua
+    (two circles, Dirichlet and floating, E2-poly) reads completely."""
     import shoccs_amd as S
 
     c = S.Config.from_file(os.path.join(REF, "lua-configs", "brady_livescu_4_3_n61.lua"))
@@ -148,8 +150,14 @@ def test_reference_configs():
     with pytest.raises(S.ShoccsError):  # template placeholders are comments: `{, }` has no values
         c = S.Config.from_file(os.path.join(REF, "lua-configs", "brady_livescu_4_3.lua"))
         c.problem(None)
-    with pytest.raises(S.ShoccsError, match="not built"):
-        S.Config.from_file(os.path.join(REF, "scalar_wave.lua"))
+    w = S.Config.from_file(os.path.join(REF, "scalar_wave.lua"))
+    assert w.kind == S.SYS_WAVE_RADIAL and w.n == (71, 71, 1) and w.scheme == S.E2P_1
+    assert list(w.alpha) == H.E2P_LUA  # floating, Dirichlet, interpolant lists in order
+    assert w.shape_bc == [S.BC_DIRICHLET, S.BC_FLOATING]
+    assert w.params[:4] == [-1.0, -1.0, 0.0, 0.0]
+    p = w.problem(None)  # the floating circle carries cut-point rows
+    assert all(len(p.cut_rows(d)["rw"]) > 0 for d in range(2))
+    p.close()
     h = S.Config.from_file(os.path.join(REF, "heat.lua"))
     assert h.kind == S.SYS_HEAT and h.shape_bc == [S.BC_FLOATING]
     assert h.params[:3] == [1 / 30, S.MMS_GAUSS, 1]  # the commented-out gaussians are skipped
