@@ -427,6 +427,44 @@ __device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRo
 // zero), the rhs at the cut point (scalar_wave.cpp:208-238: gG . du with gG
 // at the point; heat.cpp:119-162: k lap + source) and the stage update of
 // its value and accumulator (rk4.cpp:27-46), all in FP64.
+// One R-space row, from the running value acc, entries in stored order:
+// column >= 0 names a point of D (stage value), -1 - c cut point c of the
+// row's set.  The loads of eight entries are issued before their products
+// are summed, so a row's gathers overlap; the sum keeps its order.
+template <typename TI>
+__device__ __forceinline__ double cut_row_sum(const long* __restrict__ rptr, const long* __restrict__ rcol,
+                                              const double* __restrict__ rw, long k, const TI* __restrict__ us,
+                                              const double* __restrict__ uR, double acc)
+{
+    long e = rptr[k];
+    const long e1 = rptr[k + 1];
+    for (; e + 8 <= e1; e += 8) {
+        double w[8], v[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const long col = rcol[e + m];
+            w[m] = rw[e + m];
+            v[m] = col >= 0 ? (double)us[col] : uR[-1 - col];
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m) acc += w[m] * v[m];
+    }
+    for (; e < e1; ++e) {
+        const long col = rcol[e];
+        acc += rw[e] * (col >= 0 ? (double)us[col] : uR[-1 - col]);
+    }
+    return acc;
+}
+
+// A free cut point (of a floating or Neumann body): its rows of the three
+// derivatives (derivative.cpp:479-485: each row from the running value,
+// entries over D then R in stored order; the heat laplacian continues one
+// running sum across the directions, the wave's gradient starts each from
+// zero), the rhs at the cut point (scalar_wave.cpp:208-238: gG . du with gG
+// at the point; heat.cpp:119-162: k lap + source) and the stage update of
+// its value and accumulator (rk4.cpp:27-46), all in FP64.  Pointers are
+// picked by comparisons, never by a run-time array index, so that argument
+// structures copied to registers stay out of scratch memory.
 template <int KIND, int STAGE, bool MIXED>
 __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScalars& ss, long f)
 {
@@ -434,33 +472,27 @@ __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScala
     const TI* __restrict__ us = static_cast<const TI*>(a.us);
     const FreeRec fr = a.frees[f];
     const int set = fr.set;
-    const double* __restrict__ uR = a.usR.r[set];
+    const double* __restrict__ uR = set == 0 ? a.usR.r[0] : set == 1 ? a.usR.r[1] : a.usR.r[2];
     const double* pos = set == 0 ? a.pos0 : set == 1 ? a.pos1 : a.pos2;
     const double x = pos[3 * fr.q], y = pos[3 * fr.q + 1], z = pos[3 * fr.q + 2];
-    auto row_sum = [&](int c, double acc) {
-        const long k = fr.row[c];
-        for (long e = a.cut.rptr[c][k]; e < a.cut.rptr[c][k + 1]; ++e) {
-            const long col = a.cut.rcol[c][e];
-            acc += a.cut.rw[c][e] * (col >= 0 ? (double)us[col] : uR[-1 - col]);
-        }
-        return acc;
-    };
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
     double rhs;
     if (kIsHeat<KIND>) {
         double lap = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            if (fr.row[c] >= 0) lap = row_sum(c, lap);
+        if (fr.row[0] >= 0) lap = cut_row_sum(a.cut.rptr[0], a.cut.rcol[0], a.cut.rw[0], fr.row[0], us, uR, lap);
+        if (fr.row[1] >= 0) lap = cut_row_sum(a.cut.rptr[1], a.cut.rcol[1], a.cut.rw[1], fr.row[1], us, uR, lap);
+        if (fr.row[2] >= 0) lap = cut_row_sum(a.cut.rptr[2], a.cut.rcol[2], a.cut.rw[2], fr.row[2], us, uR, lap);
         rhs = lap * a.s.k;
         rhs += heat_source(a.s, ss.t_s, x, y, z);
     } else {
-        double du[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) du[c] = fr.row[c] >= 0 ? row_sum(c, 0.0) : 0.0;
-        rhs = wave_neg_g(a.s, 0, x, y, z) * du[0] + wave_neg_g(a.s, 1, x, y, z) * du[1] +
-              wave_neg_g(a.s, 2, x, y, z) * du[2];
+        if (fr.row[0] >= 0) r0 = cut_row_sum(a.cut.rptr[0], a.cut.rcol[0], a.cut.rw[0], fr.row[0], us, uR, 0.0);
+        if (fr.row[1] >= 0) r1 = cut_row_sum(a.cut.rptr[1], a.cut.rcol[1], a.cut.rw[1], fr.row[1], us, uR, 0.0);
+        if (fr.row[2] >= 0) r2 = cut_row_sum(a.cut.rptr[2], a.cut.rcol[2], a.cut.rw[2], fr.row[2], us, uR, 0.0);
+        rhs = wave_neg_g(a.s, 0, x, y, z) * r0 + wave_neg_g(a.s, 1, x, y, z) * r1 +
+              wave_neg_g(a.s, 2, x, y, z) * r2;
     }
-    const double u0v = a.u0R.r[set][fr.q];
+    const double* u0R = set == 0 ? a.u0R.r[0] : set == 1 ? a.u0R.r[1] : a.u0R.r[2];
+    const double u0v = u0R[fr.q];
     const double aa = (STAGE == 0 ? 0.0 : a.accF[f]) + ss.c_acc * rhs;
     if (STAGE < 3) a.accF[f] = aa;
     double* outR = set == 0 ? a.outR0 : set == 1 ? a.outR1 : a.outR2;
@@ -469,7 +501,7 @@ __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScala
 
 // Item q of the special work: a listed point, then the cut points.  Rows are
 // evaluated in FP64 in both forms.
-template <int KIND, int STAGE, int W, bool MIXED>
+template <int KIND, int STAGE, int W, bool MIXED, bool FREE = true>
 __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageScalars& ss, long q)
 {
     using TI = typename StageTypes<MIXED, STAGE>::in;
@@ -537,7 +569,9 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
         return;
     }
     q -= a.nR2;
-    if (q < a.nfree) free_item<KIND, STAGE, MIXED>(a, ss, q);
+    if constexpr (FREE) {
+        if (q < a.nfree) free_item<KIND, STAGE, MIXED>(a, ss, q);
+    }
 }
 
 // One fused RK4 stage.  Workgroups go in groups of 8 (one per XCD): `sg`
@@ -548,7 +582,7 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
 // floor(g sg / T), T = sg + mg).  A launch with sg = 0 or mg = 0 runs one
 // part alone.  Tile workgroups keep their XCD (b mod 8) and map to tiles
 // contiguously per XCD.
-template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
+template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED, bool FREE = true>
 __device__ __forceinline__ void stage_block(const MarchArgs& a, const SpecialArgs& sp,
                                             const StageScalars& ss, unsigned b, unsigned sg,
                                             unsigned mg, int mode)
@@ -574,7 +608,7 @@ __device__ __forceinline__ void stage_block(const MarchArgs& a, const SpecialArg
     }
     if (special) {
         const long blk = (long)si * 8 + lane8;
-        special_item<KIND, STAGE, W, MIXED>(sp, ss, blk * (64 * TY) + threadIdx.x);
+        special_item<KIND, STAGE, W, MIXED, FREE>(sp, ss, blk * (64 * TY) + threadIdx.x);
         return;
     }
     const unsigned m = mi * 8u + lane8;
@@ -630,7 +664,9 @@ __global__ __launch_bounds__(64 * TY) void k_batch_stage(const BatchEntry* __res
     ss.c_acc = dt * kStageWeight[STAGE];
     ss.c_next = STAGE < 3 ? dt * kStageTime[STAGE + 1] : 1.0;
     ss.t_next = STAGE < 3 ? t + dt * kStageTime[STAGE + 1] : t + dt;
-    stage_block<KIND, STAGE, W, TY, VEC, MIXED>(m, sp, ss, blockIdx.x - start[lo], sg, mg, mode);
+    // batches refuse systems with free cut points: that part of the special
+    // work is compiled out, so its arguments are never loaded
+    stage_block<KIND, STAGE, W, TY, VEC, MIXED, false>(m, sp, ss, blockIdx.x - start[lo], sg, mg, mode);
 }
 
 template <int KIND, int STAGE, int W, bool MIXED>

@@ -1633,11 +1633,17 @@ int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt,
         int v, ty, xc;
     };
     std::vector<cand> cands;
-    if (s->precision == SHX_F64) cands.push_back({0, 4, 32});
+    // the point-wise kernel steps neither the mixed form nor free cut points
+    if (s->precision == SHX_F64 && !s->general) cands.push_back({0, 4, 32});
     if (march_ok(s))
         for (int v = 1; v <= 4; ++v)
             for (int ty : {4, 8})
                 for (int xc : {16, 32, 64, 1 << 16}) cands.push_back({v, ty, std::min(xc, s->n[0])});
+    if (cands.empty()) {
+        for (int c = 0; c < 3; ++c) hipFree(tmp.R[c]);
+        hipFree(tmp.D);
+        return fail(s->ctx, SHX_EUNSUPPORTED, "autotune: no fused-stage kernel applies to this system");
+    }
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
