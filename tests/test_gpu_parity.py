@@ -578,6 +578,51 @@ def test_batch_matches_oracle(ctx, odd):
     b.close()
 
 
+def test_batch64_full_size(ctx):
+    """The 64-problem heat configuration at full size (per-axis sizes 32..128
+    from seed 0, a Dirichlet sphere each, sin-product solution): 50 steps as
+    one captured graph; every problem stays finite and within its
+    discretisation error of the exact solution: below 5e-4 and below h_min^2
+    for every problem at t = 50 dt (the second-order bound of the E4
+    closures)."""
+    import shoccs_amd as S
+
+    rng = np.random.default_rng(0)
+    probs, fields, dts, hs = [], [], [], []
+    for _ in range(64):
+        n = tuple(int(v) for v in rng.integers(32, 129, size=3))
+        while True:
+            c = rng.uniform(0.3, 0.7, size=3)
+            r = float(rng.uniform(0.15, 0.35))
+            try:
+                p = S.Problem(ctx, S.SYS_HEAT, n, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0),
+                              spheres=[[c[0], c[1], c[2], r]], shape_bc=[S.BC_DIRICHLET],
+                              grid_bc=[S.BC_DIRICHLET] * 6, scheme=S.E4_2, params=[1.0, S.MMS_SINPROD])
+                break
+            except S.ShoccsError:
+                pass
+        f = p.field()
+        p.initialize(f, 0.0)
+        p.update_boundary(f, 0.0)
+        probs.append(p)
+        fields.append(f)
+        dts.append(p.dt(1.0, 0.5))
+        hs.append(1.0 / (min(n) - 1))
+    b = S.Batch(ctx, probs, fields, [0.0] * 64, dts)
+    b.capture(50)
+    b.launch()
+    times = b.times()
+    errs = np.array([p.stats(f, t)[0] for p, f, t in zip(probs, fields, times)])
+    np.testing.assert_allclose(times, [50 * dt for dt in dts], rtol=1e-12)
+    assert np.all(np.isfinite(errs))
+    # measured: max error 3.3e-4, max error / h_min^2 0.42
+    assert errs.max() < 5e-4
+    assert np.all(errs <= 1.0 * np.array(hs) ** 2)
+    b.close()
+    for p in probs:
+        p.close()
+
+
 def test_batch_rejects_mixed_families(ctx):
     import shoccs_amd as S
 
