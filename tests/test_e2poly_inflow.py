@@ -119,3 +119,36 @@ def test_inflow_needs_a_velocity():
     cfg["obj_bc"] = [O.INFLOW]
     with pytest.raises(ValueError):
         H.oracle_system(cfg)
+
+
+def _linf_history(cfg, steps, every):
+    s = H.oracle_system(cfg)
+    u = s.field()
+    s.initialize(u, 0.0)
+    s.update_boundary(u, 0.0)
+    out, rk, srhs = s.field(), s.field(), s.field()
+    dt, t, hist = s.dt(0.5, 1.0), 0.0, []
+    for k in range(steps):
+        s.rk4_step(u, out, rk, srhs, t, dt)
+        t += dt
+        u, out = out, u
+        if (k + 1) % every == 0:
+            hist.append(s.stats(u, t)[0])
+    return hist
+
+
+def test_reference_schemes_grow_on_the_inflow_sphere():
+    """Config 2's wave with the inflow sphere at 48^3, CFL 0.5: the
+    reference's 4th-order E4u (its scalar-wave parameters) and its E2-poly
+    (scalar_wave.lua's parameters) both grow without bound within 100 steps,
+    while the same wave without the sphere stays at its discretisation
+    error."""
+    e4u = H.sphere_wave_cfg(48, scheme=O.E4U_1, alpha=(-0.7733323791884821, 0.1623961700641681))
+    e4u["obj_bc"] = [O.INFLOW]
+    h = _linf_history(e4u, 50, 25)
+    assert h[-1] > 1e3 * h[0] and h[-1] > 1e3, h
+    h = _linf_history(H.wave_e2p_cfg(48), 100, 25)
+    assert h[-1] > 10.0 and h[-1] > 10 * min(h), h
+    free = dict(e4u, spheres=[], obj_bc=[])
+    h = _linf_history(free, 100, 25)
+    assert max(h) < 0.2, h
