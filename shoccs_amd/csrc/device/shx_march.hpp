@@ -322,14 +322,22 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
                     const unsigned pd = patdec[c - kPattern0];
                     const int ix = (int)(pd & 0xffu), iy = (int)((pd >> 8) & 0xffu), iz = (int)(pd >> 16);
                     if (ix != 0) {
-                        // over the queue: only the row's own columns, in
-                        // column order
+                        // over the queue when the row's columns lie in it,
+                        // else from the stage input (the wall planes of a
+                        // chunk, uniform over the workgroup); the row's own
+                        // columns in column order either way
                         const ClosureRow& cr = crow[0][ix - 1];
                         double s = 0.0;
+                        if (cr.off >= -HQ && cr.off + cr.n - 1 <= HQ) {
 #pragma unroll
-                        for (int m = 0; m < QW; ++m) {
-                            const int ci = m - HQ - cr.off;
-                            if (ci >= 0 && ci < cr.n) s += cr.w[ci] * (double)qv(m);
+                            for (int m = 0; m < QW; ++m) {
+                                const int ci = m - HQ - cr.off;
+                                if (ci >= 0 && ci < cr.n) s += cr.w[ci] * (double)qv(m);
+                            }
+                        } else {
+                            const TI* __restrict__ v = us + idx + e + (long)cr.off * plane;
+#pragma unroll 1
+                            for (int m = 0; m < cr.n; ++m) s += cr.w[m] * (double)v[(long)m * plane];
                         }
                         sx = s;
                     }
@@ -499,14 +507,24 @@ __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScala
     outR[fr.q] = STAGE < 3 ? u0v + ss.c_next * rhs : u0v + 1.0 * aa;
 }
 
-// Item q of the special work: a listed point, then the cut points.  Rows are
-// evaluated in FP64 in both forms.
+// Item q of the special work: a free cut point, a listed point, then the
+// cut points of the three directions.  Rows are evaluated in FP64 in both
+// forms.
 template <int KIND, int STAGE, int W, bool MIXED, bool FREE = true>
 __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageScalars& ss, long q)
 {
     using TI = typename StageTypes<MIXED, STAGE>::in;
     using TO = typename StageTypes<MIXED, STAGE>::out;
     const TI* us = static_cast<const TI*>(a.us);
+    // the free cut points first: their rows are the longest chains of
+    // dependent loads, so they start with the earliest special workgroups
+    if constexpr (FREE) {
+        if (q < a.nfree) {
+            free_item<KIND, STAGE, MIXED>(a, ss, q);
+            return;
+        }
+        q -= a.nfree;
+    }
     if (q < a.nspecial) {
         const SpecialRec rec = a.recs[q];
         const long idx = rec.idx;
@@ -567,10 +585,6 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
         if (a.dir2[q])
             a.outR2[q] = exact_of<KIND>(a.s, a.pos2[3 * q], a.pos2[3 * q + 1], a.pos2[3 * q + 2], ss.t_next);
         return;
-    }
-    q -= a.nR2;
-    if constexpr (FREE) {
-        if (q < a.nfree) free_item<KIND, STAGE, MIXED>(a, ss, q);
     }
 }
 

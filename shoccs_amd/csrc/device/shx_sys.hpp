@@ -115,10 +115,10 @@ constexpr unsigned char kRawFace = 0x01;
 constexpr unsigned char kRawFluid = 0x02;
 constexpr int kMaxClosureRows = 16;   // per direction, staged in LDS
 constexpr int kMaxClosureCols = 8;
-// x closure rows are taken in the tiles when their columns lie within
-// hw + kXReach planes of the point (the register queue); a reach of 2 (all
-// 4th-order wall closures) measured no faster at 256^3 than 0 and costs
-// registers, so x-wall closures go to the special work
+// The register queue of the tiles holds hw + kXReach planes either side of
+// the point; x closure rows whose columns lie outside it (the wall planes)
+// are read from the stage input at their own planes, in the tiles.  A reach
+// of 2 measured no faster at 256^3 than 0 and costs registers.
 constexpr int kXReach = 0;
 
 // A closure row without B terms: the dot of w[c] with the values at offsets

@@ -641,7 +641,6 @@ int ensure_classes(shx_system* s)
     // closures repeat from line to line).
     const int n1 = s->n[1], n2 = s->n[2];
     const int hw = s->width / 2;
-    const int hq = hw + kXReach;
     // 0 = no row, 1 = interior row, 2 = other row (e filled); the segment
     // holding row q is found as the device slow path finds it
     auto find_row = [&](int c, int pair, int q, SpecialRow& e) -> int {
@@ -702,7 +701,9 @@ int ensure_classes(shx_system* s)
     auto reachable = [&](int c, const SpecialRow& e, int q) {
         if (e.nb != 0 || e.ncols < 1 || e.ncols > kMaxClosureCols) return false;
         const int first = q + e.off, last = q + e.off + e.ncols - 1;
-        if (c == 0) return e.off >= -hq && e.off + e.ncols - 1 <= hq && first >= 0 && last < s->n[0];
+        // x rows: from the register queue when they fit, else the tile reads
+        // the stage input at the row's planes
+        if (c == 0) return first >= 0 && last < s->n[0];
         const int tile = c == 1 ? 4 : 64, n = c == 1 ? n1 : n2;
         const int lo = std::max(0, (q / tile) * tile - hw);
         const int hi = std::min(n, (q / tile) * tile + tile + hw);
