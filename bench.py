@@ -351,22 +351,31 @@ def adjoint128(args):
     a forward run of a seeded smooth field; gradient in the initial field and
     the interior coefficients by the discrete adjoint, checked by central
     differences along 16 seeded random directions."""
+    import shoccs_amd as S
+
+    ctx = S.Context(0)
+    result = adjoint128_result(ctx, 128 if args.n == N_SIDE else args.n, args.adj_cfl)
+    ctx.close()
+    print(json.dumps(result))
+
+
+def adjoint128_result(ctx, n, cfl):
+    """The adjoint workload's result on a context (the adjoint128 line)."""
     import numpy as np
 
     import shoccs_amd as S
     from shoccs_amd.adjoint import Solve
 
-    n = 128 if args.n == N_SIDE else args.n
     nsteps, every = 400, 20
-    ctx = S.Context(0)
     cfg = workload_cfg(n)
     # E4u closures with alpha = (0, 0) at CFL 0.05 (default): the smooth
     # field stays O(1e3) over the 400 steps; with the scalar-wave configs'
     # alpha, or CFL 0.5, the solution grows without bound on this body
+    # a Dirichlet sphere: the adjoint takes bodies without cut-point rows
     prob = S.Problem(ctx, cfg["kind"], cfg["n"], cfg["lo"], cfg["hi"], spheres=cfg["spheres"],
-                     shape_bc=cfg["shape_bc"], grid_bc=cfg["grid_bc"], scheme=cfg["scheme"],
+                     shape_bc=[S.BC_DIRICHLET], grid_bc=cfg["grid_bc"], scheme=cfg["scheme"],
                      alpha=(0.0, 0.0), params=cfg["params"])
-    dt = prob.dt(args.adj_cfl, 1.0)
+    dt = prob.dt(cfl, 1.0)
     sol = Solve(prob, 0.0, dt, nsteps, every)
     u_true = smooth_field(prob, 5)
     u0 = u_true + 0.1 * smooth_field(prob, 6)
@@ -412,38 +421,54 @@ def adjoint128(args):
         "value": t_adj, "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"3-D scalar wave u_t + a.grad u = 0, {n}^3, Dirichlet sphere r=0.3, E4u alpha (0, 0), "
-                               f"CFL {args.adj_cfl}, 400 RK4 steps, checkpoints every 20; J = 1/2 ||u(T) - u_obs||^2, "
+                               f"CFL {cfl}, 400 RK4 steps, checkpoints every 20; J = 1/2 ||u(T) - u_obs||^2, "
                                "u_obs from a seeded smooth initial field; gradient in u0 and the 3 x 5 interior "
                                "coefficients"},
         "forward_s": t_fwd, "gradient_s": t_adj, "gradient_over_forward": t_adj / t_fwd,
         "J": J0, "fd_directions": 16, "fd_max_rel_err": max(errs), "fd_median_rel_err": float(np.median(errs)),
-        "fd_max_rel_err_u0": max(err_u), "fd_max_rel_err_coef": max(err_c), "cfl": args.adj_cfl,
+        "fd_max_rel_err_u0": max(err_u), "fd_max_rel_err_coef": max(err_c), "cfl": cfl,
         "max_abs_u_T": float(np.max(np.abs(uT))),
     }
     sol.close()
     prob.close()
+    return result
+
+
+def euler192(args):
+    """Configuration 4: 3-D compressible Euler on 192^3 with a slip-wall
+    sphere (centre 0.5, radius 0.25), Mach 0.3 along x, pressure 1/gamma,
+    density 1 +- 0.1 from a seeded low-mode perturbation, 50 RK4 steps, E4u
+    first-derivative operators; the inflow face x = 0 holds its state, the
+    other faces take the operators' closures.  Timed in the mixed form (FP32
+    stage inputs and interior flux-derivative rows with Kahan-compensated
+    sums; FP64 closures, B terms, cut points and RK registers) and in FP64,
+    the mixed result's error stated against the FP64 run.
+
+    The timed runs use CFL 0.1 and the E4u closure parameters (0, 0): the
+    cut-cell E4u rows at the body have modes with positive real part, so the
+    flow grows with simulated time whatever the step (step 2k at CFL 0.1
+    matches step k at CFL 0.2); 50 steps at the configured CFL 0.4 leave the
+    bounded range with every reference scheme, while 50 steps at CFL 0.1 end
+    at the density deviation of the impulsive start (about rho c |u| = 0.3
+    on top of the 0.1 perturbation).  The configured run (CFL 0.4, the
+    scalar-wave closure parameters) is reported beside it.  The step time
+    does not depend on the step size."""
+    import shoccs_amd as S
+
+    ctx = S.Context(0)
+    result = euler192_result(ctx, 192 if args.n == N_SIDE else args.n, not args.no_cpu_baseline)
     ctx.close()
     print(json.dumps(result))
 
 
-def euler192(args):
-    """Configuration 4 in FP64: 3-D compressible Euler on 192^3 with a
-    slip-wall sphere (centre 0.5, radius 0.25), Mach 0.3 along x, pressure
-    1/gamma, density 1 +- 0.1 from a seeded low-mode perturbation, 50 RK4 steps
-    at CFL 0.4, E4u first-derivative operators; the inflow face x = 0 holds
-    its state, the other faces take the operators' closures.  The timing does
-    not depend on the step size; a companion run at CFL 0.1 with the E4u
-    closure parameters (0, 0) reports whether the flow stays bounded (the
-    cut-cell E4u closures, psi-independent as the reference defines them,
-    give growing modes at the body; without the body the run is stable)."""
+def euler192_result(ctx, n, with_cpu):
+    """The Euler workload's result on a context (the euler192 line)."""
     import numpy as np
 
     import shoccs_amd as S
 
-    n = 192 if args.n == N_SIDE else args.n
     nsteps = 50
     gam = 1.4
-    ctx = S.Context(0)
     D_, F_ = S.BC_DIRICHLET, S.BC_FLOATING
     grid_bc = [D_, F_, F_, F_, F_, F_]
     sphere = [0.5, 0.5, 0.5, 0.25]
@@ -457,7 +482,7 @@ def euler192(args):
         u, p = 0.3, 1.0 / gam
         return [rho, rho * u, 0 * rho, 0 * rho, p / (gam - 1) + 0.5 * rho * u * u]
 
-    def run(alpha, cfl, timed):
+    def run(alpha, cfl, precisions):
         prob = S.Problem(ctx, S.SYS_HEAT, (n, n, n), (0, 0, 0), (1, 1, 1), spheres=[sphere], shape_bc=[D_],
                          grid_bc=grid_bc, scheme=S.E4U_1, alpha=alpha, params=[1.0, S.MMS_SINPROD])
         e = S.Euler(prob, gam)
@@ -467,62 +492,92 @@ def euler192(args):
         del X, Y, Z
         pos = [prob.r_arrays(d)["pos"] for d in range(3)]
         rvals = [state(p[:, 0], p[:, 1], p[:, 2], np.random.default_rng(4)) for p in pos]
-        U = e.fields()
-        e.upload(U, [np.concatenate([vals[k]] + [rvals[d][k] for d in range(3)]) for k in range(5)])
-        e.wall(U)
+        init = [np.concatenate([vals[k]] + [rvals[d][k] for d in range(3)]) for k in range(5)]
         rho, mx, E = vals[0], vals[1], vals[4]
         c = np.sqrt(gam * (gam - 1) * (E - 0.5 * mx * mx / rho) / rho)
         dt = cfl * (1.0 / (n - 1)) / float(np.max(np.abs(mx / rho) + c))
         del vals, rvals, rho, mx, E, c
-        out, rk, srhs = e.fields(), e.fields(), e.fields()
-        if timed:
-            e.time_steps(U, out, rk, srhs, dt, 1)  # warm-up step
-        ms = e.time_steps(U, out, rk, srhs, dt, nsteps)
-        got = e.download(U)
         N = prob.N
-        rho = got[0][:N]
-        p = (gam - 1) * (got[4][:N] - 0.5 * (got[1][:N] ** 2 + got[2][:N] ** 2 + got[3][:N] ** 2) / rho)
         fl = prob.fluid()
-        bounded = bool(np.all(np.isfinite(rho[fl])) and rho[fl].min() > 0 and np.all(np.isfinite(p[fl]))
-                       and p[fl].min() > 0)
-        info = {"cfl": cfl, "alpha": list(alpha), "dt": dt, "bounded": bounded,
-                "max_density_deviation": float(np.max(np.abs(rho[fl] - 1.0)))}
+        U, out, rk, srhs = e.fields(), e.fields(), e.fields(), e.fields()
+        res = {}
+        for prec in precisions:
+            e.set_precision(prec)
+            e.upload(U, init)
+            e.wall(U)
+            e.time_steps(U, out, rk, srhs, dt, 1)  # warm-up step
+            e.upload(U, init)
+            e.wall(U)
+            ms = e.time_steps(U, out, rk, srhs, dt, nsteps)
+            got = e.download(U)
+            rho = got[0][:N]
+            p = (gam - 1) * (got[4][:N] - 0.5 * (got[1][:N] ** 2 + got[2][:N] ** 2 + got[3][:N] ** 2) / rho)
+            bounded = bool(np.all(np.isfinite(rho[fl])) and rho[fl].min() > 0 and np.all(np.isfinite(p[fl]))
+                           and p[fl].min() > 0)
+            res[prec] = (ms, got, {"cfl": cfl, "alpha": list(alpha), "dt": dt, "bounded": bounded,
+                                   "max_density_deviation": float(np.max(np.abs(rho[fl] - 1.0)))})
+        for f in (U, out, rk, srhs):
+            for x in f:
+                x.free()
         e.close()
         prob.close()
-        return ms, N, info
+        return N, fl, res
 
-    ms, N, info = run(E4U_ALPHA, 0.4, True)
-    _, _, companion = run((0.0, 0.0), 0.1, False)
-    # bytes the schedule moves per point and step: per rhs and direction one flux-derivative pass that reads the five
-    # conserved fields (40) and writes the five rhs fields (40), reading them
-    # too after the first direction (+40) (the points without a row of the
-    # first direction, zeroed from a list, are not counted); per stage one
-    # slot pass: rk from zero and the next input (read srhs, u0; write rk,
-    # out: 160), then rk += c srhs with the next input or the result (read
-    # rk, srhs, u0; write rk, out: 200) three times; u0 <- out (80)
-    bytes_pt = 4 * (80 + 2 * 120) + 160 + 3 * 200 + 80
-    achieved = bytes_pt * N / (ms * 1e-3) / 1e9
+    N, fl, res = run((0.0, 0.0), 0.1, [S.F32_COMP, S.F64])
+    ms_mix, got_mix, info_mix = res[S.F32_COMP]
+    ms_64, got_64, info_64 = res[S.F64]
+    # error of the mixed form against the FP64 run: per conserved field, the
+    # largest difference over the fluid points relative to the field's
+    # largest magnitude there
+    err_vs_f64 = []
+    for k in range(5):
+        a, b = got_mix[k][:N][fl], got_64[k][:N][fl]
+        scale = float(np.max(np.abs(b)))
+        err_vs_f64.append(float(np.max(np.abs(a - b))) / scale if scale > 0 else float(np.max(np.abs(a))))
+    del got_mix, got_64
+    _, _, cres = run(E4U_ALPHA, 0.4, [S.F64])
+    configured = cres[S.F64][2]
+    # algorithmic bytes per point and step of the stage schedule: per stage
+    # the first direction's pass reads the stage input and writes the partial
+    # rhs, the second reads both and writes the rhs, and the last reads the
+    # input and the partial rhs, u0 and rk (not at stage 1, whose input is u0
+    # and whose rk starts from zero) and writes rk (not at stage 4) and the
+    # next input or the result.  FP64: 5 fields x 8 B everywhere; mixed: the
+    # stage inputs after the first and the partial rhs 5 x 4 B.
+    bytes_64 = 4 * (80 + 120) + 160 + 2 * 240 + 200
+    bytes_mix = (60 + 80 + 120) + 2 * (40 + 60 + 180) + (40 + 60 + 160)
+    bounded = info_mix["bounded"] and info_64["bounded"]
+    achieved = bytes_mix * N / (ms_mix * 1e-3) / 1e9
     cpu = None
-    if not args.no_cpu_baseline:
+    if with_cpu:
         cpu = euler_cpu_baseline(state)
     result = {
-        "metric": "Mpts/s per RK4 step (3-D compressible Euler, 192^3 slip-wall sphere, FP64)",
-        "value": N / (ms * 1e-3) / 1e6, "unit": "Mpts/s", "higher_is_better": True, "n_gpus": 1,
-        "steps": nsteps, "ms_per_step": ms, "dtype": "f64", "data": "synthetic",
+        "metric": "Mpts/s per RK4 step (3-D compressible Euler, 192^3 slip-wall sphere, mixed FP32/FP64)",
+        "value": N / (ms_mix * 1e-3) / 1e6 if bounded else None, "unit": "Mpts/s", "higher_is_better": True,
+        "n_gpus": 1, "steps": nsteps, "ms_per_step": ms_mix, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"3-D compressible Euler, {n}^3, slip-wall sphere c=0.5 r=0.25, Mach 0.3 along x, "
                                "density 1 +- 0.1 seeded low-mode perturbation, pressure 1/gamma, gamma 1.4, "
-                               f"{nsteps} RK4 steps at CFL 0.4 (dt {info['dt']:.4g}), E4u operators (the scalar-wave "
-                               "closure parameters), inflow face held",
-                   "form": "RK4 slot schedule over five fields; per direction one pass applies the derivative "
-                           "to the fluxes computed at its taps (five fields at once); one slot pass per stage "
-                           "over the five fields"},
+                               f"{nsteps} RK4 steps at CFL 0.1 (dt {info_mix['dt']:.4g}; the configured CFL 0.4 "
+                               "leaves the bounded range, see configured_run), E4u operators with closure "
+                               "parameters (0, 0), inflow face held",
+                   "form": "mixed: FP32 stage inputs and partial rhs, interior flux-derivative rows in FP32 with "
+                           "Kahan-compensated sums; FP64 closure rows, B terms, cut points, u0 and rk. Per stage "
+                           "one pass per direction (fluxes computed at the taps, five fields at once); the last "
+                           "direction's pass performs the RK slot operations"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None, "bytes_per_pt_step": bytes_pt},
-        "timed_run": info, "stability_companion": companion,
+                     "frac": achieved / 8000.0, "traffic": None, "bytes_per_pt_step": bytes_mix},
+        "fp64": {"ms_per_step": ms_64, "value": N / (ms_64 * 1e-3) / 1e6, "bytes_per_pt_step": bytes_64,
+                 "achieved_GBps": bytes_64 * N / (ms_64 * 1e-3) / 1e9},
+        "mixed_over_fp64_speed": ms_64 / ms_mix,
+        "mixed_error_vs_fp64": {"rel_max_per_field": err_vs_f64,
+                                "note": "max |mixed - FP64| over the fluid points / max |FP64| there, after the "
+                                        f"{nsteps} steps"},
+        "timed_run": info_mix, "fp64_run": info_64, "configured_run": configured,
         "cpu_baseline": cpu, "vs_baseline": None,
     }
-    ctx.close()
-    print(json.dumps(result))
+    if not bounded:
+        result["value_null_reason"] = "the timed run left the bounded range"
+    return result
 
 
 def euler_cpu_baseline(state):
@@ -535,7 +590,7 @@ def euler_cpu_baseline(state):
 
     n = 64
     cfg = dict(n=(n, n, n), lo=(0, 0, 0), hi=(1, 1, 1), spheres=[[0.5, 0.5, 0.5, 0.25]],
-               grid_bc=[O.DIRICHLET] + [O.FLOATING] * 5, scheme=O.E4U_1, alpha=E4U_ALPHA)
+               grid_bc=[O.DIRICHLET] + [O.FLOATING] * 5, scheme=O.E4U_1, alpha=(0.0, 0.0))
     e = O.OracleEuler(cfg)
     U = e.fields()
     m = e.mesh
@@ -746,8 +801,11 @@ def main():
         result["accuracy_with_sphere_e2poly"] = accuracy_e2poly(ctx, args.n, args.steps)
     if not args.headline_only:
         result["derivative_microbench"] = derivative_microbench(ctx)
-        # configuration 3 in the same run: the 64-problem heat batch
+        # configurations 3, 4 and 5 in the same run: the 64-problem heat
+        # batch, the 192^3 Euler system and the 128^3 adjoint gradient
         result["heat_batch64"] = heat_batch64_result(ctx, 100)
+        result["euler192"] = euler192_result(ctx, 192, not args.no_cpu_baseline)
+        result["adjoint128"] = adjoint128_result(ctx, 128, args.adj_cfl)
     ctx.close()
     if not args.no_cpu_baseline and args.cpu_steps > 0 and args.n == N_SIDE:
         result["cpu_baseline"] = cpu_baseline(args.cpu_steps)

@@ -188,8 +188,18 @@ int shx_euler_destroy(shx_euler* e);
 int shx_euler_wall(shx_euler* e, shx_field* U);
 int shx_euler_rhs(shx_euler* e, const shx_field* U, shx_field* rhs);
 /* rk4::operator() (src/temporal/rk4.cpp:16-47) over the five fields, the wall
- * update after each stage input and at the end.  Synchronous. */
+ * update after each stage input and at the end; the result in out.  rk and
+ * srhs are the schedule's scratch registers: srhs also holds stage inputs,
+ * and the last stage's rk update is not stored.  Synchronous. */
 int shx_euler_rk4_step(shx_euler* e, shx_field* u0, shx_field* out, shx_field* rk, shx_field* srhs, double dt);
+/* Precision of shx_euler_rk4_step (no reference counterpart).  SHX_F64 (the
+ * default): plain FP64, identical to the CPU restatement.  SHX_F32_COMP: the
+ * stage inputs after the first and the partial rhs are kept in FP32 at the
+ * mesh points (two five-field FP32 buffers held by the system) and the
+ * interior flux-derivative rows are computed and summed in FP32 with Kahan
+ * compensation; closure rows, B terms, cut points, u0 and rk stay FP64, and
+ * srhs is not used.  shx_euler_rhs stays FP64. */
+int shx_euler_set_precision(shx_euler* e, int precision);
 /* nsteps steps (u0 <- out after each), mean device time per step */
 int shx_time_euler_steps(shx_euler* e, shx_field* u0, shx_field* out, shx_field* rk, shx_field* srhs, double dt,
                          int nsteps, double* ms_per_step);

@@ -294,6 +294,8 @@ class Problem:
                 for bat in list(getattr(self.ctx, "_batches", ())):
                     if any(p is self for p in bat.problems):
                         bat.close()
+            for eu in list(getattr(self, "_eulers", ())):
+                eu.close()
             for f in list(getattr(self, "_fields", ())):
                 f.free()
             lib().shoccs_problem_destroy(self.p)
@@ -642,6 +644,10 @@ class Euler:
         self.gamma = gamma
         self.h = C.c_void_p()
         check(lib().shoccs_euler_create(problem.p, gamma, C.byref(self.h)), problem.ctx.p, "euler create")
+        # the system uses the problem's operators: closing the problem closes it first
+        if not hasattr(problem, "_eulers"):
+            problem._eulers = weakref.WeakSet()
+        problem._eulers.add(self)
 
     def fields(self) -> list:
         return [self.problem.field() for _ in range(5)]
@@ -666,6 +672,11 @@ class Euler:
     def rk4_step(self, u0, out, rk, srhs, dt: float):
         check(lib().shx_euler_rk4_step(self.h, self._refs(u0), self._refs(out), self._refs(rk), self._refs(srhs),
                                        dt), self.problem.ctx.p, "euler rk4")
+
+    def set_precision(self, precision: int):
+        """F64 (default) or F32_COMP (the mixed form) for rk4_step and
+        time_steps."""
+        check(lib().shx_euler_set_precision(self.h, precision), self.problem.ctx.p, "euler precision")
 
     def time_steps(self, u0, out, rk, srhs, dt: float, nsteps: int) -> float:
         """Mean device milliseconds per step over nsteps steps (u0 <- out

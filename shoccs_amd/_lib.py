@@ -196,6 +196,7 @@ _SIGS = [
     ("shx_euler_wall", C.c_int, [C.c_void_p, C.POINTER(ShxField)]),
     ("shx_euler_rhs", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField)]),
     ("shx_euler_rk4_step", C.c_int, [C.c_void_p] + [C.POINTER(ShxField)] * 4 + [C.c_double]),
+    ("shx_euler_set_precision", C.c_int, [C.c_void_p, C.c_int]),
     ("shx_time_euler_steps", C.c_int, [C.c_void_p] + [C.POINTER(ShxField)] * 4 +
      [C.c_double, C.c_int, C.POINTER(C.c_double)]),
     ("shoccs_problem_geometry", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), dp, dp]),
