@@ -46,13 +46,17 @@ typedef struct shoccs_problem_desc {
     /* system parameters:
      *  WAVE_RADIAL: cx cy cz radius [max_error]
      *  WAVE_LINEAR: vx vy vz x0 y0 z0 width amplitude [max_error]
-     *  HEAT:        diffusivity mms_kind [n_gauss, n_gauss x {c[3], var[3], amp, freq}] */
+     *  HEAT:        diffusivity mms_kind [n_gauss, n_gauss x {c[3], var[3], amp, freq}]
+     *               (mms_kind SHX_MMS_HOST: the functions in mms_fns) */
     const double* params;
     int32_t nparams;
     /* nonzero: the mesh and cut points only (no operators, no system), for
      * meshes whose lines are too short for the scheme's closures, e.g. the
      * field output of the reference's xdmf test (extents 3 4 5) */
     int32_t mesh_only;
+    /* heat with mms_kind SHX_MMS_HOST: the manufactured solution's functions
+     * (they must outlive the problem) */
+    const shx_mms_fns* mms_fns;
 } shoccs_problem_desc;
 
 /* ctx == NULL builds the host structures only.  Problems with floating or
@@ -142,15 +146,17 @@ int shoccs_run(shoccs_problem* p, const shoccs_run_desc* d, shx_field* u, shoccs
  * The reference's Lua `simulation` table (system::from_lua,
  * src/systems/system.cpp:103-132, and the from_lua readers it calls), read by
  * a Lua-subset evaluator (no Lua interpreter is linked): statements
- * `[local] name = exp`; numbers, strings, booleans, tables, arithmetic,
- * comparisons, `..`, and the math library; comments.  Function values are
- * parsed but cannot be called, so manufactured_solution.type = "lua" is
- * refused.  Keys read: mesh{index_extents, domain_bounds{min, max} | {...}},
+ * `[local] name = exp`, `[local] function name (...) ... end`, call
+ * statements; numbers, strings, booleans, tables, functions, arithmetic,
+ * comparisons, `..`, and the math library; comments.  Function bodies run
+ * locals, assignments, if/elseif/else and return.  Keys read: mesh{index_extents, domain_bounds{min, max} | {...}},
  * domain_boundaries{xmin..zmax}, shapes{{type = "sphere", center, radius,
  * boundary_condition}}, scheme{order, type, alpha} (E2/E4 order 2, E4/E4u
  * order 1), system{type = "heat" (diffusivity) | "scalar wave" (center,
  * radius, max_error)}, manufactured_solution{type = "gaussian", {center,
- * variance, amplitude, frequency}...}, integrator{type = "rk4" | "euler"},
+ * variance, amplitude, frequency}...} | {type = "lua", call, ddt, grad,
+ * div, lap} (functions of (time, loc), called on the host through
+ * shx_mms_fns, src/mms/lua_mms.hpp:12-79), integrator{type = "rk4" | "euler"},
  * step_controller{max_step, max_time, min_dt, cfl{hyperbolic, parabolic}},
  * io{write_every_step, write_every_time, dir, suffix_length, xdmf_filename}
  * (shoccs_config_io); logging is ignored.  Extensions of this library:
@@ -163,6 +169,9 @@ int shoccs_config_parse(const char* text, shoccs_config** out, char* err, int64_
 /* the problem and run descriptors of a parsed configuration (the problem
  * descriptor's arrays belong to the configuration) */
 int shoccs_config_desc(const shoccs_config* c, shoccs_problem_desc* desc, shoccs_run_desc* run);
+/* the first error raised by a function of a "lua" manufactured solution
+ * ("" when none; such an error makes the function's value NaN) */
+const char* shoccs_config_error(const shoccs_config* c);
 int shoccs_config_destroy(shoccs_config* c);
 
 /* ------------------------------------------------------------ field output

@@ -210,6 +210,23 @@ int shx_time_euler_steps(shx_euler* e, shx_field* u0, shx_field* out, shx_field*
 #define SHX_SYS_HEAT 2        /* src/systems/heat.cpp */
 #define SHX_MMS_GAUSS 0       /* src/mms/gauss3d.cpp */
 #define SHX_MMS_SINPROD 1     /* sin(2 pi x) sin(2 pi y) [sin(2 pi z)] exp(-t) */
+#define SHX_MMS_HOST 2        /* functions of (time, location) called on the host (src/mms/lua_mms.hpp:12-79) */
+
+/* A manufactured solution given by functions of time and location (x, y, z),
+ * as the reference's lua_mms calls the Lua functions call, ddt, grad and lap
+ * (src/mms/lua_mms.hpp:12-79, serial there too).  The heat system evaluates
+ * them on the host, at the mesh and cut points that each use needs (initial
+ * values, Dirichlet data, the source ddt - k lap, Neumann data, statistics),
+ * and copies the values to the device; the functions must outlive the
+ * system.  Such systems step with the reference schedule (shx_rk4_step,
+ * shx_euler_step); fused forms, batches and the adjoint refuse them. */
+typedef struct shx_mms_fns {
+    void* ctx;
+    double (*call)(void* ctx, double t, const double* loc);
+    double (*ddt)(void* ctx, double t, const double* loc);
+    void (*grad)(void* ctx, double t, const double* loc, double* g);
+    double (*lap)(void* ctx, double t, const double* loc);
+} shx_mms_fns;
 
 typedef struct shx_system_desc {
     int32_t kind;
@@ -231,6 +248,7 @@ typedef struct shx_system_desc {
     double diffusivity;         /* heat */
     int32_t mms_kind, mms_dims, mms_n;
     const double* mms;          /* mms_n x {center[3], variance[3], amplitude, frequency} */
+    const shx_mms_fns* mms_fns; /* SHX_MMS_HOST: the solution's functions */
 } shx_system_desc;
 
 int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** out);

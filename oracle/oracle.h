@@ -134,7 +134,16 @@ typedef struct {
 /* systems */
 enum { ORC_SYS_WAVE_RADIAL = 0, ORC_SYS_WAVE_LINEAR = 1, ORC_SYS_HEAT = 2 };
 /* manufactured solutions for heat */
-enum { ORC_MMS_GAUSS = 0, ORC_MMS_SINPROD = 1 };
+enum { ORC_MMS_GAUSS = 0, ORC_MMS_SINPROD = 1, ORC_MMS_HOST = 2 };
+/* a manufactured solution given by functions of (time, location), as the
+ * reference's lua_mms (src/mms/lua_mms.hpp:12-79) calls its Lua functions */
+typedef struct {
+    void* ctx;
+    double (*call)(void* ctx, double t, const double* loc);
+    double (*ddt)(void* ctx, double t, const double* loc);
+    void (*grad)(void* ctx, double t, const double* loc, double* g);
+    double (*lap)(void* ctx, double t, const double* loc);
+} orc_mms_fns;
 
 typedef struct {
     int kind;
@@ -154,6 +163,7 @@ typedef struct {
     int mms_dims;
     int mms_n;
     double* mms; /* per gaussian: center[3], variance[3], amplitude, frequency */
+    orc_mms_fns fns; /* ORC_MMS_HOST */
     /* work buffers */
     orc_field gG[3];
     orc_field du[3];
@@ -208,6 +218,7 @@ orc_system* orc_system_create(int kind, const int n[3], const double min[3],
                               const int* obj_bc, const int grid_bc[6],
                               const orc_stencil* st, const double* params, int nparams);
 void orc_system_destroy(orc_system*);
+void orc_system_set_mms(orc_system*, const orc_mms_fns*);
 orc_mesh* orc_system_mesh(orc_system*);
 orc_deriv* orc_system_deriv(orc_system*, int dir);
 void orc_system_rhs(orc_system*, const orc_field* u, orc_field* rhs, double t);

@@ -105,6 +105,7 @@ static double g_expo(const orc_system* s, const double* g, const double* loc)
 
 static double mms_call(const orc_system* s, double t, const double* loc)
 {
+    if (s->mms_kind == ORC_MMS_HOST) return s->fns.call(s->fns.ctx, t, loc);
     if (s->mms_kind == ORC_MMS_SINPROD) {
         double v = sin(TWO_PI * loc[0]) * sin(TWO_PI * loc[1]);
         if (s->mms_dims == 3) v = v * sin(TWO_PI * loc[2]);
@@ -120,6 +121,7 @@ static double mms_call(const orc_system* s, double t, const double* loc)
 
 static double mms_ddt(const orc_system* s, double t, const double* loc)
 {
+    if (s->mms_kind == ORC_MMS_HOST) return s->fns.ddt(s->fns.ctx, t, loc);
     if (s->mms_kind == ORC_MMS_SINPROD) return -mms_call(s, t, loc);
     double sol = 0;
     for (int i = 0; i < s->mms_n; i++) {
@@ -132,6 +134,10 @@ static double mms_ddt(const orc_system* s, double t, const double* loc)
 static void mms_grad(const orc_system* s, double t, const double* loc, double* out)
 {
     out[0] = out[1] = out[2] = 0;
+    if (s->mms_kind == ORC_MMS_HOST) {
+        s->fns.grad(s->fns.ctx, t, loc, out);
+        return;
+    }
     if (s->mms_kind == ORC_MMS_SINPROD) {
         double sx = sin(TWO_PI * loc[0]), sy = sin(TWO_PI * loc[1]);
         double cx = cos(TWO_PI * loc[0]), cy = cos(TWO_PI * loc[1]);
@@ -152,6 +158,7 @@ static void mms_grad(const orc_system* s, double t, const double* loc, double* o
 
 static double mms_lap(const orc_system* s, double t, const double* loc)
 {
+    if (s->mms_kind == ORC_MMS_HOST) return s->fns.lap(s->fns.ctx, t, loc);
     if (s->mms_kind == ORC_MMS_SINPROD)
         return -(s->mms_dims * TWO_PI * TWO_PI) * mms_call(s, t, loc);
     double sol = 0;
@@ -350,6 +357,13 @@ orc_system* orc_system_create(int kind, const int n[3], const double min[3],
         }
     }
     return s;
+}
+
+/* the functions of an ORC_MMS_HOST manufactured solution (lua_mms.hpp:12-79:
+ * call, ddt, grad, lap of (time, location)) */
+void orc_system_set_mms(orc_system* s, const orc_mms_fns* f)
+{
+    s->fns = *f;
 }
 
 void orc_system_destroy(orc_system* s)

@@ -25,15 +25,32 @@ import weakref
 import numpy as np
 
 from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_INFLOW, BC_NEUMANN, E2_2, E2P_1, E4_1, E4_2, E4U_1, EULER,
-                   F32_COMP, F64, MMS_GAUSS, MMS_SINPROD, RK4, RK4_FUSED, SYS_HEAT,
-                   SYS_WAVE_LINEAR, SYS_WAVE_RADIAL, IODesc, ProblemDesc, RunDesc, RunResult, ShoccsError,
+                   F32_COMP, F64, MMS_GAUSS, MMS_HOST, MMS_SINPROD, RK4, RK4_FUSED, SYS_HEAT,
+                   SYS_WAVE_LINEAR, SYS_WAVE_RADIAL, IODesc, MMS_GRAD, MMS_SCALAR, MmsFns, ProblemDesc, RunDesc,
+                   RunResult, ShoccsError,
                    SHX_OK, ShxField, ShxLineBlock, check, lib)
 
 __all__ = [
     "Context", "Problem", "Field", "Batch", "FieldIO", "Euler", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
     "BC_INFLOW", "E2_2", "E4_2", "E4_1", "E4U_1", "E2P_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
-    "MMS_GAUSS", "MMS_SINPROD", "F64", "F32_COMP", "RK4", "RK4_FUSED", "EULER", "eq", "plus_eq",
+    "MMS_GAUSS", "MMS_SINPROD", "MMS_HOST", "F64", "F32_COMP", "RK4", "RK4_FUSED", "EULER", "eq", "plus_eq",
+    "mms_functions",
 ]
+
+
+def mms_functions(call, ddt, grad, lap) -> MmsFns:
+    """A manufactured solution for heat problems with params [k, MMS_HOST]
+    from Python functions of (t, (x, y, z)) (grad returns three values), as
+    the reference's lua_mms calls its Lua functions.  Pass the result as
+    Problem(..., mms=...); the problem keeps it alive."""
+    def scalar(f):
+        return MMS_SCALAR(lambda _c, t, loc: float(f(t, (loc[0], loc[1], loc[2]))))
+
+    def vec(_c, t, loc, g):
+        gx, gy, gz = grad(t, (loc[0], loc[1], loc[2]))
+        g[0], g[1], g[2] = float(gx), float(gy), float(gz)
+
+    return MmsFns(None, scalar(call), scalar(ddt), MMS_GRAD(vec), scalar(lap))
 
 eq = 0
 plus_eq = 1
@@ -151,9 +168,14 @@ class Config:
         rc = L.shoccs_config_parse(text.encode(), C.byref(h), err, len(err))
         if rc != 0:
             raise ShoccsError(f"Lua configuration: {err.value.decode()}")
+        self._h = None
+        self.mms = None
         try:
             d, r = ProblemDesc(), RunDesc()
             check(L.shoccs_config_desc(h, C.byref(d), C.byref(r)), None, "config")
+            # a "lua" manufactured solution runs in the configuration's
+            # evaluator: the configuration then lives as long as this object
+            self.mms = d.mms_fns if d.mms_fns else None
             self.kind = d.kind
             self.n = tuple(d.n[a] for a in range(3))
             self.lo = tuple(d.min[a] for a in range(3))
@@ -176,7 +198,20 @@ class Config:
                 write_every_step=io.write_every_step if io.write_every_step > 0 else None,
                 write_every_time=io.write_every_time if io.write_every_time > 0 else None)
         finally:
-            L.shoccs_config_destroy(h)
+            if self.mms is None:
+                L.shoccs_config_destroy(h)
+            else:
+                self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().shoccs_config_destroy(self._h)
+            self._h = None
+
+    def mms_error(self) -> str:
+        """The first error raised by a function of the "lua" manufactured
+        solution ("" when none)."""
+        return lib().shoccs_config_error(self._h).decode() if self._h else ""
 
     @classmethod
     def from_file(cls, path: str) -> "Config":
@@ -184,9 +219,11 @@ class Config:
             return cls(f.read())
 
     def problem(self, ctx: "Context | None") -> "Problem":
-        return Problem(ctx, self.kind, self.n, self.lo, self.hi, spheres=self.spheres,
-                       shape_bc=self.shape_bc, grid_bc=self.grid_bc, scheme=self.scheme,
-                       alpha=self.alpha, params=self.params)
+        p = Problem(ctx, self.kind, self.n, self.lo, self.hi, spheres=self.spheres,
+                    shape_bc=self.shape_bc, grid_bc=self.grid_bc, scheme=self.scheme,
+                    alpha=self.alpha, params=self.params, mms=self.mms)
+        p._config = self  # the solution's functions live in this configuration
+        return p
 
     def run_kwargs(self) -> dict:
         return dict(integrator=self.integrator, max_step=self.max_step, max_time=self.max_time,
@@ -243,8 +280,10 @@ class Problem:
 
     def __init__(self, ctx: Context | None, kind: int, n, lo, hi, spheres=(), shape_bc=(),
                  grid_bc=(BC_FLOATING,) * 6, scheme: int = E4_1, alpha=(0.0, 0.0), params=(),
-                 mesh_only: bool = False):
-        """mesh_only: the mesh and cut points alone (no operators or system)."""
+                 mesh_only: bool = False, mms=None):
+        """mesh_only: the mesh and cut points alone (no operators or system).
+        mms: for heat with params [k, MMS_HOST], the solution's functions
+        (mms_functions(...), or a configuration's pointer)."""
         L = lib()
         self.ctx = ctx
         d = ProblemDesc()
@@ -269,6 +308,9 @@ class Problem:
         d.params = pr.ctypes.data_as(C.POINTER(C.c_double))
         d.nparams = len(pr) - 1
         d.mesh_only = int(mesh_only)
+        self._mms = mms
+        if mms is not None:
+            d.mms_fns = C.pointer(mms) if isinstance(mms, MmsFns) else mms
         p = C.c_void_p()
         rc = L.shoccs_problem_create(ctx.p if ctx else None, C.byref(d), C.byref(p))
         if rc != SHX_OK and L.shoccs_last_error():

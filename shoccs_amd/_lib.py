@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libshoccs.so")
 SHX_OK = 0
 BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, BC_INFLOW = 0, 1, 2, 3
 SYS_WAVE_RADIAL, SYS_WAVE_LINEAR, SYS_HEAT = 0, 1, 2
-MMS_GAUSS, MMS_SINPROD = 0, 1
+MMS_GAUSS, MMS_SINPROD, MMS_HOST = 0, 1, 2
 E2_2, E4_2, E4_1, E4U_1, E2P_1 = 0, 1, 2, 3, 4
 F64, F32_COMP = 0, 1
 
@@ -34,6 +34,16 @@ class ShxLineBlock(C.Structure):
         "lobj", "lbw", "robj", "rbw", "flags")]
 
 
+# shx_mms_fns: a manufactured solution's functions of (time, location)
+MMS_SCALAR = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double, dp)
+MMS_GRAD = C.CFUNCTYPE(None, C.c_void_p, C.c_double, dp, dp)
+
+
+class MmsFns(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("call", MMS_SCALAR), ("ddt", MMS_SCALAR), ("grad", MMS_GRAD),
+                ("lap", MMS_SCALAR)]
+
+
 class ProblemDesc(C.Structure):
     _fields_ = [
         ("kind", C.c_int32),
@@ -49,6 +59,7 @@ class ProblemDesc(C.Structure):
         ("params", dp),
         ("nparams", C.c_int32),
         ("mesh_only", C.c_int32),
+        ("mms_fns", C.POINTER(MmsFns)),
     ]
 
 
@@ -201,6 +212,7 @@ _SIGS = [
      [C.c_double, C.c_int, C.POINTER(C.c_double)]),
     ("shoccs_problem_geometry", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), dp, dp]),
     ("shoccs_config_destroy", C.c_int, [C.c_void_p]),
+    ("shoccs_config_error", C.c_char_p, [C.c_void_p]),
     ("shx_derivative_apply_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
                                          C.c_int]),
     ("shx_system_rhs_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField)]),

@@ -591,8 +591,9 @@ int shx_derivative_apply_T(shx_derivative* d, const shx_field* dbar, shx_field* 
 int shx_system_rhs_T(shx_system* s, const shx_field* kbar, shx_field* wbar)
 {
     if (!s || !kbar || !wbar) return SHX_EINVAL;
-    if (s->general)
-        return fail(s->ctx, SHX_EUNSUPPORTED, "adjoint: floating/Neumann bodies and Neumann grid faces are not supported");
+    if (s->general || s->hm)
+        return fail(s->ctx, SHX_EUNSUPPORTED,
+                    "adjoint: floating/Neumann bodies, Neumann grid faces and host-evaluated solutions are not supported");
     int rc = ensure_ws(s);
     if (rc == SHX_OK) rc = rhs_T(s, *kbar, *wbar, nullptr, nullptr);
     return rc == SHX_OK ? sync(s->ctx) : rc;
@@ -601,8 +602,9 @@ int shx_system_rhs_T(shx_system* s, const shx_field* kbar, shx_field* wbar)
 int shx_rk4_adjoint(shx_system* s, const shx_field* u, shx_field* ubar, double t, double dt, double* coef_grad)
 {
     if (!s || !u || !ubar) return SHX_EINVAL;
-    if (s->general)
-        return fail(s->ctx, SHX_EUNSUPPORTED, "adjoint: floating/Neumann bodies and Neumann grid faces are not supported");
+    if (s->general || s->hm)
+        return fail(s->ctx, SHX_EUNSUPPORTED,
+                    "adjoint: floating/Neumann bodies, Neumann grid faces and host-evaluated solutions are not supported");
     int rc = ensure_ws(s);
     if (rc != SHX_OK) return rc;
     shx_adjoint_ws* ws = s->adj;

@@ -2,11 +2,28 @@
 // (shared by the translation units that implement the system entry points).
 #pragma once
 
+#include <vector>
+
 #include "shx_internal.hpp"
 #include "shx_state.hpp"
 #include "shx_sys.hpp"
 
 struct shx_adjoint_ws;
+
+// Heat with a manufactured solution whose functions are called on the host
+// (SHX_MMS_HOST, the reference's lua_mms): host copies of the evaluation
+// sites, and staging for the values copied to the device.
+struct HostMms {
+    shx_mms_fns f{};
+    std::vector<double> xyz[3];          // mesh coordinates per axis
+    std::vector<double> pos[3];          // cut-point positions, 3 per point
+    std::vector<signed char> dirich[3];  // cut point of a Dirichlet body
+    std::vector<int> fluid, faces, neu;  // fluid gather, Dirichlet faces, Neumann faces
+    long neu_off[7] = {0, 0, 0, 0, 0, 0, 0};
+    std::vector<double> h;               // host staging
+    double* d = nullptr;                 // device staging (max(N, nR) values)
+    double* exD = nullptr;               // exact values on the fluid gather (statistics)
+};
 
 struct shx_system {
     shx_ctx* ctx = nullptr;
@@ -77,6 +94,7 @@ struct shx_system {
     bool classes_ready = false;
     int width = 5;  // interior stencil width shared by the three derivatives
     shx_adjoint_ws* adj = nullptr;  // adjoint workspace, allocated on first use
+    HostMms* hm = nullptr;          // SHX_MMS_HOST: the solution's functions and their sites
 };
 
 namespace shx {

@@ -180,6 +180,29 @@ __global__ void k_gather_fill(long n, const int* __restrict__ sel, double* __res
     if (q < n) dst[sel[q]] = v;
 }
 
+// values computed on the host (SHX_MMS_HOST) into their places: point sel[q]
+// of D takes v[q]; cut point i takes v[i] where it is Dirichlet (set) or adds
+// it where it is not (add)
+__global__ void k_scatter(long n, const int* __restrict__ sel, const double* __restrict__ v, double* __restrict__ dst)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) dst[sel[q]] = v[q];
+}
+
+__global__ void k_r_set_dirichlet(long n, const signed char* __restrict__ dirich, const double* __restrict__ v,
+                                  double* __restrict__ R)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && dirich[i]) R[i] = v[i];
+}
+
+__global__ void k_r_add_free(long n, const signed char* __restrict__ dirich, const double* __restrict__ v,
+                             double* __restrict__ R)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !dirich[i]) R[i] += v[i];
+}
+
 // heat.cpp:140-146: the source at the cut points of non-Dirichlet bodies
 __global__ void k_src_R_add(DevSys s, long n, const double* __restrict__ pos,
                             const signed char* __restrict__ dirich, double* __restrict__ R, double t)
@@ -232,7 +255,8 @@ __global__ void k_copy(long n, double* __restrict__ d, const double* __restrict_
 // compute_scalar_stats partials over the fluid gather: per workgroup
 // (max |u - exact|, its index, min u, max u)
 __global__ __launch_bounds__(256) void k_stats(DevSys s, long n, const int* __restrict__ fluid,
-                                               const double* __restrict__ u, double t,
+                                               const double* __restrict__ u, const double* __restrict__ exF,
+                                               double t,
                                                double* __restrict__ part_err,
                                                long* __restrict__ part_idx,
                                                double* __restrict__ part_min,
@@ -247,7 +271,7 @@ __global__ __launch_bounds__(256) void k_stats(DevSys s, long n, const int* __re
         int i, j, k;
         point_coords(idx, s.n1, s.n2, i, j, k);
         const double v = u[idx];
-        const double err = fabs(v - exact_at(s, s.x[i], s.y[j], s.z[k], t));
+        const double err = fabs(v - (exF ? exF[q] : exact_at(s, s.x[i], s.y[j], s.z[k], t)));
         // strict comparisons, as the reference's MaxLoc: a NaN error never
         // wins (scalar_system_utils.hpp:102-113); ties keep the first point
         if (err > e || (err == e && idx < ei)) {
@@ -310,8 +334,68 @@ RPtrs rptrs(const shx_field* f)
     return r;
 }
 
+// ---- SHX_MMS_HOST: the manufactured solution's functions called on the host
+// (lua_mms.hpp:12-79), their values copied to the device
+enum { kHmCall = 0, kHmSource = 1, kHmGrad = 2 };  // kHmGrad + d: component d of the gradient
+
+double hm_value(const shx_system* s, int what, double t, const double loc[3])
+{
+    const shx_mms_fns& f = s->hm->f;
+    if (what == kHmCall) return f.call(f.ctx, t, loc);
+    if (what == kHmSource) return f.ddt(f.ctx, t, loc) - s->dev.k * f.lap(f.ctx, t, loc);  // heat.cpp:111-117
+    double g[3] = {0.0, 0.0, 0.0};
+    f.grad(f.ctx, t, loc, g);
+    return g[what - kHmGrad];
+}
+
+// values at the mesh points sel[0..n) (all N points in index order when sel is
+// null) into the host staging
+void hm_mesh(shx_system* s, int what, double t, const int* sel, long n)
+{
+    HostMms& h = *s->hm;
+    h.h.resize((size_t)n);
+    const long plane = (long)s->n[1] * s->n[2];
+    for (long q = 0; q < n; ++q) {
+        const long idx = sel ? sel[q] : q;
+        const int i = (int)(idx / plane), rem = (int)(idx - (long)i * plane);
+        const double loc[3] = {h.xyz[0][i], h.xyz[1][rem / s->n[2]], h.xyz[2][rem % s->n[2]]};
+        h.h[(size_t)q] = hm_value(s, what, t, loc);
+    }
+}
+
+// values at the cut points of R(c) into the host staging
+void hm_cut(shx_system* s, int c, int what, double t)
+{
+    HostMms& h = *s->hm;
+    h.h.resize((size_t)s->nR[c]);
+    for (long i = 0; i < s->nR[c]; ++i) h.h[(size_t)i] = hm_value(s, what, t, &h.pos[c][(size_t)(3 * i)]);
+}
+
+// the host staging to dst (device), completed before returning
+int hm_put(shx_system* s, double* dst)
+{
+    HostMms& h = *s->hm;
+    if (h.h.empty()) return SHX_OK;
+    hipError_t e = hipMemcpyAsync(dst, h.h.data(), h.h.size() * sizeof(double), hipMemcpyHostToDevice, s->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+    return e == hipSuccess ? SHX_OK : hip_fail(s->ctx, e, "host solution values h2d");
+}
+
 int r_exact_all(shx_system* s, double* const R[3], double t)
 {
+    if (s->hm) {
+        for (int c = 0; c < 3; ++c) {
+            if (s->nR[c] == 0) continue;
+            hm_cut(s, c, kHmCall, t);
+            int rc = hm_put(s, s->hm->d);
+            if (rc != SHX_OK) return rc;
+            k_r_set_dirichlet<<<grid_for(s->nR[c]), 256, 0, s->ctx->stream>>>(s->nR[c], s->rdir[c], s->hm->d, R[c]);
+            rc = check(s->ctx, "cut-point boundary kernel");
+            if (rc == SHX_OK) rc = sync(s->ctx);
+            if (rc != SHX_OK) return rc;
+        }
+        return SHX_OK;
+    }
     for (int c = 0; c < 3; ++c) {
         if (s->nR[c] == 0) continue;
         k_r_exact<<<grid_for(s->nR[c]), 256, 0, s->ctx->stream>>>(s->dev, s->nR[c], s->rpos[c],
@@ -323,6 +407,14 @@ int r_exact_all(shx_system* s, double* const R[3], double t)
 int face_exact(shx_system* s, double* D, double t)
 {
     if (s->nfaces == 0) return SHX_OK;
+    if (s->hm) {
+        hm_mesh(s, kHmCall, t, s->hm->faces.data(), s->nfaces);
+        int rc = hm_put(s, s->hm->d);
+        if (rc != SHX_OK) return rc;
+        k_scatter<<<grid_for(s->nfaces), 256, 0, s->ctx->stream>>>(s->nfaces, s->faces, s->hm->d, D);
+        rc = check(s->ctx, "face boundary kernel");
+        return rc == SHX_OK ? sync(s->ctx) : rc;
+    }
     k_face_exact<<<grid_for(s->nfaces), 256, 0, s->ctx->stream>>>(s->dev, s->nfaces, s->faces, D, t);
     return check(s->ctx, "face boundary kernel");
 }
@@ -430,7 +522,7 @@ int rhs_fused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double t
 {
     // the point-wise form covers Dirichlet bodies on grids without Neumann
     // faces; other systems take the reference's sequence
-    if (s->general) return rhs_unfused_launch(s, u, rhs, t);
+    if (s->general || s->hm) return rhs_unfused_launch(s, u, rhs, t);
     const Tiling& tl = s->tiling;
     hipStream_t st = s->ctx->stream;
     switch (s->family * 10 + s->width) {
@@ -500,11 +592,28 @@ int rhs_unfused_launch(shx_system* s, const shx_field* u, shx_field* rhs, double
     k_scale<<<grid_for(s->N), 256, 0, st>>>(s->N, rhs->D, s->dev.k);
     for (int r = 0; r < 3; ++r)
         if (s->nR[r] > 0) k_scale<<<grid_for(s->nR[r]), 256, 0, st>>>(s->nR[r], rhs->R[r], s->dev.k);
-    k_src_D<<<grid_for(s->N), 256, 0, st>>>(s->dev, s->N, s->srcD, t);
+    if (s->hm) {
+        hm_mesh(s, kHmSource, t, nullptr, s->N);
+        rc = hm_put(s, s->srcD);
+        if (rc != SHX_OK) return rc;
+    } else {
+        k_src_D<<<grid_for(s->N), 256, 0, st>>>(s->dev, s->N, s->srcD, t);
+    }
     if (s->nfluid) k_gather_add<<<grid_for(s->nfluid), 256, 0, st>>>(s->nfluid, s->fluid, s->srcD, rhs->D);
-    for (int r = 0; r < 3; ++r)
-        if (s->nR[r] > 0 && s->any_free_r)
+    for (int r = 0; r < 3; ++r) {
+        if (s->nR[r] == 0 || !s->any_free_r) continue;
+        if (s->hm) {
+            hm_cut(s, r, kHmSource, t);
+            rc = hm_put(s, s->hm->d);
+            if (rc != SHX_OK) return rc;
+            k_r_add_free<<<grid_for(s->nR[r]), 256, 0, st>>>(s->nR[r], s->rdir[r], s->hm->d, rhs->R[r]);
+            rc = check(ctx, "heat source kernels");
+            if (rc == SHX_OK) rc = sync(ctx);
+            if (rc != SHX_OK) return rc;
+        } else {
             k_src_R_add<<<grid_for(s->nR[r]), 256, 0, st>>>(s->dev, s->nR[r], s->rpos[r], s->rdir[r], rhs->R[r], t);
+        }
+    }
     if (s->nfaces) k_gather_fill<<<grid_for(s->nfaces), 256, 0, st>>>(s->nfaces, s->faces, rhs->D, 0.0);
     for (int r = 0; r < 3; ++r)
         if (s->nR[r] > 0) k_zero_dirichlet<<<grid_for(s->nR[r]), 256, 0, st>>>(s->nR[r], s->rdir[r], rhs->R[r]);
@@ -520,9 +629,18 @@ int update_boundary_launch(shx_system* s, shx_field* u, double t)
     if (rc == SHX_OK && s->neuD) {
         for (int f = 0; f < 6; ++f) {
             const long n = s->neu_off[f + 1] - s->neu_off[f];
-            if (n > 0)
+            if (n > 0 && s->hm) {
+                hm_mesh(s, kHmGrad + f / 2, t, s->hm->neu.data() + s->neu_off[f], n);
+                rc = hm_put(s, s->hm->d);
+                if (rc != SHX_OK) return rc;
+                k_scatter<<<grid_for(n), 256, 0, s->ctx->stream>>>(n, s->neu_faces + s->neu_off[f], s->hm->d, s->neuD);
+                rc = check(s->ctx, "neumann data kernel");
+                if (rc == SHX_OK) rc = sync(s->ctx);
+                if (rc != SHX_OK) return rc;
+            } else if (n > 0) {
                 k_neumann_data<<<grid_for(n), 256, 0, s->ctx->stream>>>(s->dev, n, s->neu_faces + s->neu_off[f],
                                                                        s->neuD, f / 2, t);
+            }
         }
         rc = check(s->ctx, "neumann data kernel");
     }
@@ -1021,6 +1139,9 @@ int rk4_fused_step_on(shx_system* s, const FusedRegs& g, shx_field* u0, double t
 // Whether the fused form can step this system at all.
 int fused_supported(shx_system* s)
 {
+    if (s->hm)
+        return fail(s->ctx, SHX_EUNSUPPORTED,
+                    "fused form: a host-evaluated manufactured solution takes the reference schedule (shx_rk4_step)");
     if (s->general && !s->fused_free)
         return fail(s->ctx, SHX_EUNSUPPORTED,
                     "fused form: Neumann grid faces take the reference schedule (shx_rk4_step)");
@@ -1188,6 +1309,40 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
         if (rc == SHX_OK) rc = alloc_d(ctx, &s->neuD, s->N);
         if (rc == SHX_OK) rc = launch_fill(ctx, s->neuD, 0.0, s->N);
     }
+    // SHX_MMS_HOST: host copies of every evaluation site, device staging
+    if (rc == SHX_OK && desc->kind == SHX_SYS_HEAT && desc->mms_kind == SHX_MMS_HOST) {
+        const shx_mms_fns* f = desc->mms_fns;
+        if (!f || !f->call || !f->ddt || !f->grad || !f->lap) {
+            rc = fail(ctx, SHX_EINVAL, "system: a host-evaluated manufactured solution needs call, ddt, grad and lap");
+        } else {
+            s->hm = new (std::nothrow) HostMms();
+            if (!s->hm) rc = SHX_ENOMEM;
+        }
+        if (rc == SHX_OK) {
+            HostMms& h = *s->hm;
+            h.f = *f;
+            h.xyz[0].assign(desc->x, desc->x + s->n[0]);
+            h.xyz[1].assign(desc->y, desc->y + s->n[1]);
+            h.xyz[2].assign(desc->z, desc->z + s->n[2]);
+            long most = s->N;
+            for (int c = 0; c < 3; ++c) {
+                h.pos[c].assign(desc->r_pos[c], desc->r_pos[c] + 3 * s->nR[c]);
+                h.dirich[c].resize((size_t)s->nR[c]);
+                for (long i = 0; i < s->nR[c]; ++i) h.dirich[c][(size_t)i] = desc->r_dirichlet[c][i] ? 1 : 0;
+                most = std::max(most, s->nR[c]);
+            }
+            h.fluid = fl;
+            h.faces = faces;
+            if (s->neu_faces) {
+                for (int f6 = 0; f6 < 7; ++f6) h.neu_off[f6] = s->neu_off[f6];
+                h.neu.resize((size_t)s->neu_off[6]);
+                const hipError_t e = hipMemcpy(h.neu.data(), s->neu_faces, h.neu.size() * sizeof(int), hipMemcpyDeviceToHost);
+                if (e != hipSuccess) rc = hip_fail(ctx, e, "neumann faces d2h");
+            }
+            if (rc == SHX_OK) rc = alloc_d(ctx, &h.d, most);
+            if (rc == SHX_OK) rc = alloc_d(ctx, &h.exD, s->nfluid);
+        }
+    }
     for (int c = 0; c < 3 && rc == SHX_OK; ++c)
         if (free_r && s->nR[c] > 0) rc = alloc_d(ctx, &s->exR[c], s->nR[c]);
     for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
@@ -1283,6 +1438,11 @@ int shx_system_destroy(shx_system* s)
         if (s->Bfull[c]) shx_csr_destroy(s->Bfull[c]);
     }
     destroy_adjoint_ws(s->adj);
+    if (s->hm) {
+        hipFree(s->hm->d);
+        hipFree(s->hm->exD);
+        delete s->hm;
+    }
     hipFree(s->stageD[0]);
     hipFree(s->stageD[1]);
     hipFree(s->stageF[0]);
@@ -1332,6 +1492,15 @@ int shx_system_exact(shx_system* s, shx_field* out, double t)
 {
     if (!s || !out) return SHX_EINVAL;
     hipStream_t st = s->ctx->stream;
+    if (s->hm) {
+        hm_mesh(s, kHmCall, t, nullptr, s->N);
+        int rc = hm_put(s, out->D);
+        for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+            hm_cut(s, c, kHmCall, t);
+            rc = hm_put(s, out->R[c]);
+        }
+        return rc;
+    }
     k_exact_D<<<grid_for(s->N), 256, 0, st>>>(s->dev, s->N, out->D, t);
     for (int c = 0; c < 3; ++c)
         if (s->nR[c] > 0) k_exact_R<<<grid_for(s->nR[c]), 256, 0, st>>>(s->dev, s->nR[c], s->rpos[c], out->R[c], t);
@@ -1344,6 +1513,20 @@ int shx_system_initialize(shx_system* s, shx_field* u, double t)
     if (!s || !u) return SHX_EINVAL;
     hipStream_t st = s->ctx->stream;
     int rc = launch_fill(s->ctx, u->D, 0.0, s->N);
+    if (s->hm) {
+        if (rc == SHX_OK && s->nfluid) {
+            hm_mesh(s, kHmCall, t, s->hm->fluid.data(), s->nfluid);
+            rc = hm_put(s, s->hm->d);
+            if (rc == SHX_OK) k_scatter<<<grid_for(s->nfluid), 256, 0, st>>>(s->nfluid, s->fluid, s->hm->d, u->D);
+            if (rc == SHX_OK) rc = check(s->ctx, "initialize kernels");
+            if (rc == SHX_OK) rc = sync(s->ctx);
+        }
+        for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+            hm_cut(s, c, kHmCall, t);
+            rc = hm_put(s, u->R[c]);
+        }
+        return rc;
+    }
     if (rc == SHX_OK && s->nfluid)
         k_init_fluid<<<grid_for(s->nfluid), 256, 0, st>>>(s->dev, s->nfluid, s->fluid, u->D, t);
     for (int c = 0; c < 3 && rc == SHX_OK; ++c)
@@ -1362,7 +1545,14 @@ int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[1
     for (int q = 0; q < 11; ++q) stats[q] = 0.0;
     if (s->nfluid > 0) {
         const unsigned grid = (unsigned)std::min<long>(nparts, (s->nfluid + 255) / 256);
-        k_stats<<<grid, 256, 0, s->ctx->stream>>>(s->dev, s->nfluid, s->fluid, u->D, t, pe, s->part_idx,
+        const double* exF = nullptr;
+        if (s->hm) {  // the exact values on the fluid gather, from the host
+            hm_mesh(s, kHmCall, t, s->hm->fluid.data(), s->nfluid);
+            const int rc0 = hm_put(s, s->hm->exD);
+            if (rc0 != SHX_OK) return rc0;
+            exF = s->hm->exD;
+        }
+        k_stats<<<grid, 256, 0, s->ctx->stream>>>(s->dev, s->nfluid, s->fluid, u->D, exF, t, pe, s->part_idx,
                                                    pmin, pmax);
         int rc = check(s->ctx, "stats kernel");
         if (rc != SHX_OK) return rc;
@@ -1405,8 +1595,15 @@ int shx_system_stats(shx_system* s, const shx_field* u, double t, double stats[1
         for (int c = 0; c < 3; ++c) {
             const long n = s->nR[c];
             if (n == 0) continue;
-            k_exact_R<<<grid_for(n), 256, 0, s->ctx->stream>>>(s->dev, n, s->rpos[c], s->exR[c], t);
-            int rc = check(s->ctx, "exact kernel");
+            int rc = SHX_OK;
+            if (s->hm) {
+                hm_cut(s, c, kHmCall, t);
+                rc = hm_put(s, s->exR[c]);
+                if (rc != SHX_OK) return rc;
+            } else {
+                k_exact_R<<<grid_for(n), 256, 0, s->ctx->stream>>>(s->dev, n, s->rpos[c], s->exR[c], t);
+                rc = check(s->ctx, "exact kernel");
+            }
             std::vector<double> hu(n), hx(n);
             std::vector<signed char> fl(n);
             hipError_t e = hipMemcpyAsync(hu.data(), u->R[c], n * sizeof(double), hipMemcpyDeviceToHost, s->ctx->stream);
@@ -1739,7 +1936,7 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
     if (!(march_ty == 4 || march_ty == 8)) return SHX_EINVAL;
     *out = nullptr;
     for (int p = 0; p < n; ++p)
-        if (!systems[p] || systems[p]->general)
+        if (!systems[p] || systems[p]->general || systems[p]->hm)
             return fail(ctx, SHX_EUNSUPPORTED, "batch: every problem needs Dirichlet bodies on grids without Neumann faces");
     auto b = std::unique_ptr<shx_batch>(new (std::nothrow) shx_batch());
     if (!b) return SHX_ENOMEM;

@@ -7,9 +7,13 @@
 // ({a, b}, {k = v}, {[e] = v}), + - * / // % ^ .. unary - not #, comparisons
 // and `and`/`or`, parentheses, field and index access, and the math library
 // (sqrt, sin, cos, tan, asin, acos, atan, exp, log, abs, floor, ceil, min,
-// max, fmod, pi, huge); comments `--` and `--[[ ]]`.  Function values
-// (`function ... end`) are parsed and kept as opaque values; calling them is
-// not supported.
+// max, fmod, pi, huge); comments `--` and `--[[ ]]`.  Functions
+// (`function (params) body end`, and the statements `[local] function name
+// (params) body end`) are values that can be called: their bodies run
+// `local` and assignment statements, `if ... then ... elseif ... else ...
+// end` and `return explist` (a call in an expression gives its first
+// value).  A function sees the locals in scope where it was defined, as they
+// were then, and the globals.
 //
 // The keys read follow the reference's from_lua readers (file:line at each).
 #include <cmath>
@@ -28,12 +32,14 @@ namespace {
 
 // ------------------------------------------------------------------- values
 struct Table;
+struct FuncDef;
 struct Value {
     enum Kind { Nil, Bool, Num, Str, Tab, Fun, Builtin } k = Nil;
     double n = 0.0;
     bool b = false;
     std::string s;
     std::shared_ptr<Table> t;
+    std::shared_ptr<FuncDef> f;
     static Value num(double v)
     {
         Value x;
@@ -224,6 +230,14 @@ class Lexer {
     }
 };
 
+// A function value: its parameters, the lexer at the first token of its body,
+// and the locals in scope where it was defined.
+struct FuncDef {
+    std::vector<std::string> params;
+    Lexer body;
+    std::map<std::string, Value> upvalues;
+};
+
 // --------------------------------------------------------------- evaluator
 class Eval {
   public:
@@ -256,10 +270,50 @@ class Eval {
         return it == globals_.end() ? Value{} : it->second;
     }
 
+    // Calls a function value (or a math builtin) with args; its return values.
+    std::vector<Value> invoke(const Value& fv, const std::vector<Value>& args)
+    {
+        if (fv.k == Value::Builtin) return {call(fv, args)};
+        if (fv.k != Value::Fun || !fv.f) fail("attempt to call a non-function value");
+        if (depth_ >= 200) fail("call depth above 200");
+        // the caller's lexer, scope and return state come back on every exit
+        struct Restore {
+            Eval& e;
+            Lexer lx;
+            std::map<std::string, Value> locals;
+            std::vector<Value> ret;
+            bool returned, in_fn;
+            ~Restore()
+            {
+                e.lx_ = lx;
+                e.locals_ = std::move(locals);
+                e.ret_ = std::move(ret);
+                e.returned_ = returned;
+                e.in_fn_ = in_fn;
+                --e.depth_;
+            }
+        } guard{*this, lx_, std::move(locals_), std::move(ret_), returned_, in_fn_};
+        ++depth_;
+        const FuncDef& fd = *fv.f;
+        locals_ = fd.upvalues;
+        for (size_t i = 0; i < fd.params.size(); ++i) locals_[fd.params[i]] = i < args.size() ? args[i] : Value{};
+        lx_ = fd.body;
+        ret_.clear();
+        returned_ = false;
+        in_fn_ = true;
+        block();
+        if (!returned_ && !is_kw("end")) fail("'end' expected");
+        return returned_ ? ret_ : std::vector<Value>{};
+    }
+
   private:
     Lexer lx_;
     std::map<std::string, Value> globals_;
-    std::map<std::string, Value> locals_;  // one chunk-level scope
+    std::map<std::string, Value> locals_;  // the chunk's, or the running function's, scope
+    std::vector<Value> ret_;               // values of the return statement that ran
+    bool returned_ = false;                // a return statement ran in the current call
+    bool in_fn_ = false;                   // running a function body
+    int depth_ = 0;
 
     [[noreturn]] void fail(const std::string& m) const
     {
@@ -292,14 +346,116 @@ class Eval {
         Value key;
     };
 
+    // statements up to the end of the block (`end`, `else`, `elseif`, end of
+    // the chunk) or a return statement
+    void block()
+    {
+        while (!returned_ && lx_.peek().t != TEOF && !is_kw("end") && !is_kw("else") && !is_kw("elseif"))
+            statement();
+    }
+
+    static bool opens(const Token& t)
+    {
+        return t.t == TName && (t.s == "if" || t.s == "function" || t.s == "do" || t.s == "repeat");
+    }
+    static bool closes(const Token& t) { return t.t == TName && (t.s == "end" || t.s == "until"); }
+
+    // skips a block not taken, to its `else`, `elseif` or `end` (not consumed)
+    void skip_block()
+    {
+        int depth = 0;
+        for (;;) {
+            const Token& t = lx_.peek();
+            if (t.t == TEOF) fail("'end' expected");
+            if (depth == 0 && (is_kw("end") || is_kw("else") || is_kw("elseif"))) return;
+            if (opens(t)) ++depth;
+            else if (closes(t)) --depth;
+            lx_.take();
+        }
+    }
+
+    void expect_kw(const char* kw)
+    {
+        if (!is_kw(kw)) fail(std::string("'") + kw + "' expected");
+        lx_.take();
+    }
+
+    // if exp then block {elseif exp then block} [else block] end
+    void if_statement()
+    {
+        bool done = false;
+        for (;;) {
+            // at `if` or `elseif`
+            lx_.take();
+            if (done) {
+                while (!is_kw("then")) {
+                    if (lx_.peek().t == TEOF) fail("'then' expected");
+                    lx_.take();
+                }
+                lx_.take();
+                skip_block();
+            } else {
+                const bool c = expr().truthy();
+                expect_kw("then");
+                if (c) {
+                    block();
+                    if (returned_) return;
+                    done = true;
+                } else {
+                    skip_block();
+                }
+            }
+            if (is_kw("elseif")) continue;
+            if (is_kw("else")) {
+                lx_.take();
+                if (done) {
+                    skip_block();
+                } else {
+                    block();
+                    if (returned_) return;
+                }
+            }
+            expect_kw("end");
+            return;
+        }
+    }
+
     void statement()
     {
         if (is_op(";")) {
             lx_.take();
             return;
         }
+        if (is_kw("return")) {
+            lx_.take();
+            if (!in_fn_) fail("return outside a function");
+            ret_.clear();
+            if (lx_.peek().t != TEOF && !is_kw("end") && !is_kw("else") && !is_kw("elseif") && !is_op(";"))
+                ret_ = explist();
+            returned_ = true;
+            return;
+        }
+        if (is_kw("if")) {
+            if_statement();
+            return;
+        }
+        if (is_kw("function")) {  // function name (params) body end
+            lx_.take();
+            const std::string n = name();
+            Value v = function_value();
+            if (locals_.count(n)) locals_[n] = v;
+            else globals_[n] = v;
+            return;
+        }
         if (is_kw("local")) {
             lx_.take();
+            if (is_kw("function")) {  // local function name (params) body end
+                lx_.take();
+                const std::string n = name();
+                locals_[n] = Value{};
+                locals_[n] = function_value();
+                return;
+            }
             std::vector<std::string> names{name()};
             while (is_op(",")) {
                 lx_.take();
@@ -312,6 +468,31 @@ class Eval {
             }
             for (size_t i = 0; i < names.size(); ++i) locals_[names[i]] = i < vals.size() ? vals[i] : Value{};
             return;
+        }
+        // a call statement (name, fields, then a call) is evaluated for its effects
+        {
+            Lexer save = lx_;
+            name();
+            while (is_op(".") || is_op("[")) {
+                if (is_op(".")) {
+                    lx_.take();
+                    name();
+                } else {
+                    int depth = 0;
+                    do {
+                        if (lx_.peek().t == TEOF) fail("']' expected");
+                        if (is_op("[")) ++depth;
+                        else if (is_op("]")) --depth;
+                        lx_.take();
+                    } while (depth > 0);
+                }
+            }
+            const bool is_call = is_op("(");
+            lx_ = save;
+            if (is_call) {
+                primary();
+                return;
+            }
         }
         std::vector<Target> targets{target()};
         while (is_op(",")) {
@@ -508,32 +689,31 @@ class Eval {
         return r;
     }
 
-    // skips a function body to its matching `end`; the value is opaque
+    // a function value: `(` params `)` then the body, skipped to its `end`
     Value function_value()
     {
+        expect("(");
+        std::vector<std::string> params;
+        while (!is_op(")")) {
+            if (is_op("...")) lx_.take();
+            else params.push_back(name());
+            if (is_op(",")) lx_.take();
+            else if (!is_op(")")) fail("')' expected");
+        }
+        lx_.take();
+        auto fd = std::make_shared<FuncDef>(FuncDef{std::move(params), lx_, locals_});
         int depth = 1;
         while (depth > 0) {
             const Token t = lx_.take();
             if (t.t == TEOF) fail("'end' expected");
-            if (t.t != TName) continue;
-            if (t.s == "function" || t.s == "do" || t.s == "then" || t.s == "repeat") {
-                // `then` opens an if-block closed by `end`; `elseif ... then`
-                // does not open another one
-                if (t.s != "then" || !in_elseif_) ++depth;
-                in_elseif_ = false;
-            } else if (t.s == "elseif") {
-                in_elseif_ = true;
-            } else if (t.s == "end" || t.s == "until") {
-                --depth;
-            } else if (t.s == "while" || t.s == "for") {
-                // their `do` is counted
-            }
+            if (opens(t)) ++depth;
+            else if (closes(t)) --depth;
         }
         Value v;
         v.k = Value::Fun;
+        v.f = fd;
         return v;
     }
-    bool in_elseif_ = false;
 
     Value call(const Value& f, const std::vector<Value>& args) const
     {
@@ -606,8 +786,14 @@ class Eval {
                 std::vector<Value> args;
                 if (!is_op(")")) args = explist();
                 expect(")");
-                if (v.k != Value::Builtin) fail("cannot call " + path + ": only the math library can be called");
-                v = call(v, args);
+                if (v.k == Value::Builtin) {
+                    v = call(v, args);
+                } else if (v.k == Value::Fun) {
+                    std::vector<Value> r = invoke(v, args);
+                    v = r.empty() ? Value{} : r[0];
+                } else {
+                    fail("cannot call " + path + ": not a function");
+                }
                 path += "(...)";
             } else {
                 return v;
@@ -652,9 +838,76 @@ double idx_or(const Value& t, int i, double d)
     return v.k == Value::Num ? v.n : d;
 }
 
+// manufactured_solution.type = "lua" (mms.cpp:65-84, lua_mms.hpp:12-79): the
+// functions call, ddt, grad, lap of (time, loc), loc = {x, y, z}, called
+// through shx_mms_fns.  An error in a function gives NaN (the statistics then
+// end the run) and keeps the first message.
+struct LuaMms {
+    Eval* ev = nullptr;
+    Value call, ddt, grad, lap;
+    std::string err;
+
+    std::vector<Value> run(const Value& f, double t, const double* loc)
+    {
+        auto tab = std::make_shared<Table>();
+        for (int a = 0; a < 3; ++a) tab->set(Value::num(a + 1), Value::num(loc[a]));
+        Value lv;
+        lv.k = Value::Tab;
+        lv.t = tab;
+        return ev->invoke(f, {Value::num(t), lv});
+    }
+    double scalar(const Value& f, double t, const double* loc)
+    {
+        try {
+            const std::vector<Value> r = run(f, t, loc);
+            if (r.empty() || r[0].k != Value::Num) throw std::runtime_error("a manufactured_solution function did not return a number");
+            return r[0].n;
+        } catch (const std::exception& e) {
+            if (err.empty()) err = e.what();
+            return std::numeric_limits<double>::quiet_NaN();
+        }
+    }
+    static double cb_call(void* c, double t, const double* loc)
+    {
+        LuaMms* m = static_cast<LuaMms*>(c);
+        return m->scalar(m->call, t, loc);
+    }
+    static double cb_ddt(void* c, double t, const double* loc)
+    {
+        LuaMms* m = static_cast<LuaMms*>(c);
+        return m->scalar(m->ddt, t, loc);
+    }
+    static double cb_lap(void* c, double t, const double* loc)
+    {
+        LuaMms* m = static_cast<LuaMms*>(c);
+        return m->scalar(m->lap, t, loc);
+    }
+    static void cb_grad(void* c, double t, const double* loc, double* g)
+    {
+        LuaMms* m = static_cast<LuaMms*>(c);
+        try {
+            const std::vector<Value> r = m->run(m->grad, t, loc);
+            for (int a = 0; a < 3; ++a) {
+                if (a >= (int)r.size() || r[(size_t)a].k != Value::Num)
+                    throw std::runtime_error("manufactured_solution.grad must return three numbers");
+                g[a] = r[(size_t)a].n;
+            }
+        } catch (const std::exception& e) {
+            if (m->err.empty()) m->err = e.what();
+            g[0] = g[1] = g[2] = std::numeric_limits<double>::quiet_NaN();
+        }
+    }
+};
+
 }  // namespace
 
 struct shoccs_config {
+    // the chunk and its evaluator live as long as the configuration: the
+    // manufactured solution's functions run in them
+    std::string src;
+    std::unique_ptr<Eval> ev;
+    LuaMms mms;
+    shx_mms_fns mms_fns{};
     shoccs_problem_desc desc{};
     std::vector<double> spheres;
     std::vector<int32_t> shape_bc;
@@ -800,9 +1053,21 @@ int build(const Value& sim, shoccs_config& c, std::string& err)
             // this library's extension: sin(2 pi x) sin(2 pi y) [sin(2 pi z)] exp(-t)
             c.params.push_back(SHX_MMS_SINPROD);
         } else if (mt == "lua") {
-            err = "manufactured_solution.type = 'lua' needs a Lua interpreter for its functions; "
-                  "use 'gaussian' or 'sin_product'";
-            return SHX_EUNSUPPORTED;
+            // mms.cpp:65-84: call, ddt, grad, div and lap are all required
+            const char* fns[] = {"call", "ddt", "grad", "div", "lap"};
+            for (const char* f : fns)
+                if (sub(ms, f).k != Value::Fun) {
+                    err = "`lua` manufactured solution requires call, ddt, grad, div, lap functions";
+                    return SHX_EINVAL;
+                }
+            c.mms.ev = c.ev.get();
+            c.mms.call = sub(ms, "call");
+            c.mms.ddt = sub(ms, "ddt");
+            c.mms.grad = sub(ms, "grad");
+            c.mms.lap = sub(ms, "lap");
+            c.mms_fns = shx_mms_fns{&c.mms, &LuaMms::cb_call, &LuaMms::cb_ddt, &LuaMms::cb_grad, &LuaMms::cb_lap};
+            c.desc.mms_fns = &c.mms_fns;
+            c.params.push_back(SHX_MMS_HOST);
         } else {
             err = "manufactured_solution.type = '" + mt + "': expected gaussian, lua or sin_product";
             return SHX_EINVAL;
@@ -896,12 +1161,12 @@ int shoccs_config_parse(const char* text, shoccs_config** out, char* err, int64_
     if (!text || !out) return SHX_EINVAL;
     *out = nullptr;
     try {
-        const std::string src(text);
-        Eval ev(src);
-        ev.run();
         auto c = std::make_unique<shoccs_config>();
+        c->src = text;
+        c->ev = std::make_unique<Eval>(c->src);
+        c->ev->run();
         std::string msg;
-        const int rc = build(ev.global("simulation"), *c, msg);
+        const int rc = build(c->ev->global("simulation"), *c, msg);
         if (rc != SHX_OK) {
             put_err(msg, err, errlen);
             return rc;
@@ -929,6 +1194,11 @@ int shoccs_config_io(const shoccs_config* c, shoccs_io_desc* io)
     io->dir = c->io_dir.c_str();
     io->xdmf_filename = c->io_xmf.c_str();
     return c->has_io ? 1 : 0;
+}
+
+const char* shoccs_config_error(const shoccs_config* c)
+{
+    return c ? c->mms.err.c_str() : "";
 }
 
 int shoccs_config_destroy(shoccs_config* c)

@@ -150,6 +150,7 @@ int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_
             mms.assign(pr + 3, pr + 3 + 8 * sd.mms_n);
             sd.mms = mms.data();
         }
+        if (sd.mms_kind == SHX_MMS_HOST) sd.mms_fns = desc->mms_fns;
     }
     // a NULL context builds the host structures only (no device, no system)
     int rc = ctx ? shx_system_create(ctx, &sd, &p->sys) : SHX_OK;

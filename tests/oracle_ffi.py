@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 DIRICHLET, FLOATING, NEUMANN, INFLOW = 0, 1, 2, 3
 E2_2, E4_2, E4_1, E4U_1, E2P_1 = 0, 1, 2, 3, 4
 SYS_WAVE_RADIAL, SYS_WAVE_LINEAR, SYS_HEAT = 0, 1, 2
-MMS_GAUSS, MMS_SINPROD = 0, 1
+MMS_GAUSS, MMS_SINPROD, MMS_HOST = 0, 1, 2
 
 dp = C.POINTER(C.c_double)
 lp = C.POINTER(C.c_long)
@@ -118,6 +118,29 @@ class Field(C.Structure):
     _fields_ = [("D", dp), ("R", dp * 3)]
 
 
+# orc_mms_fns: a manufactured solution given by functions of (time, location)
+MMS_SCALAR = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double, dp)
+MMS_GRAD = C.CFUNCTYPE(None, C.c_void_p, C.c_double, dp, dp)
+
+
+class MmsFns(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("call", MMS_SCALAR), ("ddt", MMS_SCALAR), ("grad", MMS_GRAD),
+                ("lap", MMS_SCALAR)]
+
+
+def mms_fns(call, ddt, grad, lap) -> MmsFns:
+    """MmsFns from Python functions of (t, (x, y, z)); grad returns three
+    values.  The caller keeps the result alive while it is used."""
+    def scalar(f):
+        return MMS_SCALAR(lambda _c, t, loc: float(f(t, (loc[0], loc[1], loc[2]))))
+
+    def vec(_c, t, loc, g):
+        gx, gy, gz = grad(t, (loc[0], loc[1], loc[2]))
+        g[0], g[1], g[2] = float(gx), float(gy), float(gz)
+
+    return MmsFns(None, scalar(call), scalar(ddt), MMS_GRAD(vec), scalar(lap))
+
+
 _lib = None
 
 
@@ -151,6 +174,7 @@ def lib():
     L.orc_system_create.argtypes = [C.c_int, ip, dp, dp, C.c_int, dp, ip, ip,
                                     C.POINTER(Stencil), dp, C.c_int]
     L.orc_system_destroy.argtypes = [C.c_void_p]
+    L.orc_system_set_mms.argtypes = [C.c_void_p, C.POINTER(MmsFns)]
     L.orc_system_mesh.restype = C.POINTER(Mesh)
     L.orc_system_mesh.argtypes = [C.c_void_p]
     L.orc_system_deriv.restype = C.POINTER(Deriv)
@@ -377,6 +401,11 @@ class OracleSystem:
 
     def deriv(self, d):
         return lib().orc_system_deriv(self.p, d)
+
+    def set_mms(self, fns: MmsFns):
+        """Functions of a MMS_HOST manufactured solution (kept alive here)."""
+        self._mms = fns
+        lib().orc_system_set_mms(self.p, C.byref(fns))
 
     def initialize(self, u: OracleField, t: float):
         lib().orc_system_initialize(self.p, u.ref(), t)
