@@ -36,7 +36,7 @@ typedef struct shoccs_problem_desc {
     int32_t n[3];          /* index_extents (1 for unused axes) */
     double min[3], max[3]; /* domain_bounds */
     int32_t nshapes;
-    const double* spheres; /* nshapes x {cx, cy, cz, r} */
+    const double* spheres; /* nshapes x {cx, cy, cz, r} (when shapes8 is NULL) */
     const int32_t* shape_bc; /* per shape boundary_condition */
     int32_t grid_bc[6];    /* xmin xmax ymin ymax zmin zmax */
     int32_t scheme;
@@ -57,7 +57,18 @@ typedef struct shoccs_problem_desc {
     /* heat with mms_kind SHX_MMS_HOST: the manufactured solution's functions
      * (they must outlive the problem) */
     const shx_mms_fns* mms_fns;
+    /* NULL, or nshapes shape records of 8 doubles in place of spheres:
+     * a sphere {SHOCCS_SPHERE, cx, cy, cz, r, 0, 0, 0}; a rectangle normal to
+     * axis a {SHOCCS_RECT_X + a, plane coordinate, c0_slow, c0_fast, c1_slow,
+     * c1_fast, fluid_normal, 0} (src/mesh/rect.hpp:11-44, rect.cpp:8-33:
+     * yz_rect, xz_rect, xy_rect; bounds in the plane's (slow, fast) axes,
+     * y z for a yz rectangle; the fluid on the side fluid_normal points to) */
+    const double* shapes8;
 } shoccs_problem_desc;
+#define SHOCCS_SPHERE 0
+#define SHOCCS_RECT_X 1 /* yz_rect */
+#define SHOCCS_RECT_Y 2 /* xz_rect */
+#define SHOCCS_RECT_Z 3 /* xy_rect */
 
 /* ctx == NULL builds the host structures only.  Problems with floating or
  * Neumann bodies, or Neumann grid faces, step with the reference schedule
@@ -149,11 +160,13 @@ int shoccs_run(shoccs_problem* p, const shoccs_run_desc* d, shx_field* u, shoccs
  * `[local] name = exp`, `[local] function name (...) ... end`, call
  * statements; numbers, strings, booleans, tables, functions, arithmetic,
  * comparisons, `..`, and the math library; comments.  Function bodies run
- * locals, assignments, if/elseif/else and return.  Keys read: mesh{index_extents, domain_bounds{min, max} | {...}},
+ * locals, assignments, if/elseif/else and return.  Keys read:
+ * mesh{index_extents, domain_bounds{min, max} | {...}},
  * domain_boundaries{xmin..zmax}, shapes{{type = "sphere", center, radius,
- * boundary_condition}}, scheme{order, type, alpha} (E2/E4 order 2, E4/E4u
+ * boundary_condition} | {type = "yz_rect", lower_corner, upper_corner, normal,
+ * psi, boundary_condition}}, scheme{order, type, alpha} (E2/E4 order 2, E4/E4u
  * order 1), system{type = "heat" (diffusivity) | "scalar wave" (center,
- * radius, max_error)}, manufactured_solution{type = "gaussian", {center,
+ * radius, max_error) | "eigenvalues"}, manufactured_solution{type = "gaussian", {center,
  * variance, amplitude, frequency}...} | {type = "lua", call, ddt, grad,
  * div, lap} (functions of (time, loc), called on the host through
  * shx_mms_fns, src/mms/lua_mms.hpp:12-79), integrator{type = "rk4" | "euler"},
@@ -169,6 +182,12 @@ int shoccs_config_parse(const char* text, shoccs_config** out, char* err, int64_
 /* the problem and run descriptors of a parsed configuration (the problem
  * descriptor's arrays belong to the configuration) */
 int shoccs_config_desc(const shoccs_config* c, shoccs_problem_desc* desc, shoccs_run_desc* run);
+/* 1 when the configuration's system.type is "eigenvalues"
+ * (hyperbolic_eigenvalues.cpp: the problem then carries the mesh, conditions
+ * and scheme on a constant-velocity wave along x that is never stepped; the
+ * statistic -h_x min Re(eigenvalues of the x derivative) is
+ * shoccs_amd.spectral.hyperbolic_eigenvalue_stat), else 0 */
+int shoccs_config_eigenvalues(const shoccs_config* c);
 /* the first error raised by a function of a "lua" manufactured solution
  * ("" when none; such an error makes the function's value NaN) */
 const char* shoccs_config_error(const shoccs_config* c);

@@ -62,13 +62,38 @@ static int sphere_hit(const double* sph, int id, const double* o, const double* 
     return 0;
 }
 
+/* rect<I>::hit (src/mesh/rect.hpp:21-36): the plane normal to axis I at
+ * rc[1], bounded by rc[2..3] - rc[4..5] in its (slow, fast) axes, fluid on
+ * the side of rc[6].  Bounds are inclusive, as are t_min and t_max.  A line
+ * lying in the plane (t = 0/0) takes no hit here; the reference computes
+ * with a NaN t there. */
+static int rect_hit(const double* rc, int id, const double* o, const double* d, double tmin,
+                    double tmax, hit_t* h)
+{
+    const int I = (int)rc[0] - 1, S = SLOW[I], F = FAST[I];
+    const double t = (rc[1] - o[I]) / d[I];
+    if (t < tmin || t > tmax || isnan(t)) return 0;
+    double p[3];
+    for (int q = 0; q < 3; q++) p[q] = o[q] + t * d[q];
+    if (p[S] < rc[2] || p[S] > rc[4] || p[F] < rc[3] || p[F] > rc[5]) return 0;
+    h->t = t;
+    memcpy(h->p, p, sizeof p);
+    h->ray_outside = rc[6] * d[I] < 0;
+    h->id = id;
+    return 1;
+}
+
+/* object_geometry.cpp:17-30: the closest hit over the shapes in order */
 static int closest_hit(const orc_mesh* m, const double* o, const double* d, double tmin,
                        double tmax, hit_t* out)
 {
     int found = 0;
     for (int s = 0; s < m->nshapes; s++) {
         hit_t h;
-        if (sphere_hit(m->spheres + 4 * s, s, o, d, tmin, tmax, &h)) {
+        const double* sh = m->shapes8 + 8 * s;
+        const int hit = sh[0] == 0 ? sphere_hit(sh + 1, s, o, d, tmin, tmax, &h)
+                                   : rect_hit(sh, s, o, d, tmin, tmax, &h);
+        if (hit) {
             *out = h;
             found = 1;
             tmax = h.t;
@@ -77,8 +102,14 @@ static int closest_hit(const orc_mesh* m, const double* o, const double* d, doub
     return found;
 }
 
-static void sphere_normal(const double* sph, const double* pos, double* n)
+static void shape_normal(const double* sh, const double* pos, double* n)
 {
+    if (sh[0] != 0) { /* rect<I>::normal (rect.hpp:38-43) */
+        n[0] = n[1] = n[2] = 0.0;
+        n[(int)sh[0] - 1] = sh[6];
+        return;
+    }
+    const double* sph = sh + 1;
     double r[3] = {pos[0] - sph[0], pos[1] - sph[1], pos[2] - sph[2]};
     double len = sqrt(dot3(r, r));
     for (int q = 0; q < 3; q++) n[q] = r[q] / len;
@@ -129,7 +160,7 @@ static void geometry_init_line(orc_mesh* m, int I)
                 orc_rinfo r;
                 r.psi = psi;
                 memcpy(r.pos, hit.p, sizeof r.pos);
-                sphere_normal(m->spheres + 4 * hit.id, hit.p, r.normal);
+                shape_normal(m->shapes8 + 8 * hit.id, hit.p, r.normal);
                 r.ray_outside = hit.ray_outside;
                 memcpy(r.solid, coord, sizeof r.solid);
                 r.shape_id = hit.id;
@@ -243,6 +274,16 @@ static void mesh_init_fluid(orc_mesh* m)
 orc_mesh* orc_mesh_create(const int n[3], const double min[3], const double max[3],
                           int nshapes, const double* spheres)
 {
+    double* sh = (double*)calloc((size_t)(nshapes > 0 ? nshapes : 1) * 8, sizeof(double));
+    for (int s = 0; s < nshapes; s++) memcpy(sh + 8 * s + 1, spheres + 4 * s, 4 * sizeof(double));
+    orc_mesh* m = orc_mesh_create8(n, min, max, nshapes, sh);
+    free(sh);
+    return m;
+}
+
+orc_mesh* orc_mesh_create8(const int n[3], const double min[3], const double max[3],
+                           int nshapes, const double* shapes8)
+{
     orc_mesh* m = (orc_mesh*)calloc(1, sizeof(orc_mesh));
     m->dims = 0;
     for (int i = 0; i < 3; i++) {
@@ -257,9 +298,14 @@ orc_mesh* orc_mesh_create(const int n[3], const double min[3], const double max[
                                ? m->min[i] + k * (m->max[i] - m->min[i]) / (m->n[i] - 1)
                                : m->min[i];
     }
+    if (nshapes < 0) nshapes = 0;
     m->nshapes = nshapes;
-    m->spheres = (double*)malloc((size_t)(nshapes > 0 ? nshapes : 1) * 4 * sizeof(double));
-    if (nshapes) memcpy(m->spheres, spheres, (size_t)nshapes * 4 * sizeof(double));
+    m->shapes8 = (double*)malloc((size_t)(nshapes > 0 ? nshapes : 1) * 8 * sizeof(double));
+    if (nshapes > 0) memcpy(m->shapes8, shapes8, (size_t)nshapes * 8 * sizeof(double));
+    /* the spheres' {cx, cy, cz, r} (zero for other shapes) */
+    m->spheres = (double*)calloc((size_t)(nshapes > 0 ? nshapes : 1) * 4, sizeof(double));
+    for (int s = 0; s < nshapes; s++)
+        if (shapes8[8 * s] == 0) memcpy(m->spheres + 4 * s, shapes8 + 8 * s + 1, 4 * sizeof(double));
     for (int I = 0; I < 3; I++) geometry_init_line(m, I);
     for (int I = 0; I < 3; I++) mesh_init_line(m, I);
     mesh_init_fluid(m);
@@ -275,6 +321,7 @@ void orc_mesh_destroy(orc_mesh* m)
         free(m->lines[i]);
     }
     free(m->spheres);
+    free(m->shapes8);
     free(m->fluid);
     free(m);
 }

@@ -91,6 +91,10 @@ typedef struct {
     long nlines[3];
     long* fluid;
     long nfluid;
+    /* shape records, 8 doubles each: sphere {0, cx, cy, cz, r}; rectangle
+     * normal to axis a {1 + a, plane, c0_slow, c0_fast, c1_slow, c1_fast,
+     * fluid_normal} (src/mesh/rect.hpp:11-44, rect.cpp:8-33) */
+    double* shapes8;
 } orc_mesh;
 
 /* one inner_block in block-kernel (matvec_functor) terms, block.hpp:30-94 */
@@ -179,6 +183,9 @@ void orc_set_threads(int n);
 
 orc_mesh* orc_mesh_create(const int n[3], const double min[3], const double max[3],
                           int nshapes, const double* spheres);
+/* the same over shape records (orc_mesh.shapes8), spheres and rectangles */
+orc_mesh* orc_mesh_create8(const int n[3], const double min[3], const double max[3],
+                           int nshapes, const double* shapes8);
 void orc_mesh_destroy(orc_mesh*);
 long orc_mesh_size(const orc_mesh*);
 int orc_dirichlet_line(const orc_mesh* m, const int start[3], int dir, const int bc[6]);
@@ -217,6 +224,11 @@ orc_system* orc_system_create(int kind, const int n[3], const double min[3],
                               const double max[3], int nshapes, const double* spheres,
                               const int* obj_bc, const int grid_bc[6],
                               const orc_stencil* st, const double* params, int nparams);
+/* the same over shape records (orc_mesh.shapes8) */
+orc_system* orc_system_create8(int kind, const int n[3], const double min[3],
+                               const double max[3], int nshapes, const double* shapes8,
+                               const int* obj_bc, const int grid_bc[6],
+                               const orc_stencil* st, const double* params, int nparams);
 void orc_system_destroy(orc_system*);
 void orc_system_set_mms(orc_system*, const orc_mms_fns*);
 orc_mesh* orc_system_mesh(orc_system*);

@@ -276,9 +276,21 @@ orc_system* orc_system_create(int kind, const int n[3], const double min[3],
                               const int* obj_bc, const int grid_bc[6],
                               const orc_stencil* st, const double* params, int nparams)
 {
+    double* sh = (double*)calloc((size_t)(nshapes > 0 ? nshapes : 1) * 8, sizeof(double));
+    for (int q = 0; q < nshapes; q++) memcpy(sh + 8 * q + 1, spheres + 4 * q, 4 * sizeof(double));
+    orc_system* s = orc_system_create8(kind, n, min, max, nshapes, sh, obj_bc, grid_bc, st, params, nparams);
+    free(sh);
+    return s;
+}
+
+orc_system* orc_system_create8(int kind, const int n[3], const double min[3],
+                               const double max[3], int nshapes, const double* shapes8,
+                               const int* obj_bc, const int grid_bc[6],
+                               const orc_stencil* st, const double* params, int nparams)
+{
     orc_system* s = (orc_system*)calloc(1, sizeof(orc_system));
     s->kind = kind;
-    s->m = orc_mesh_create(n, min, max, nshapes, spheres);
+    s->m = orc_mesh_create8(n, min, max, nshapes, shapes8);
     memcpy(s->grid_bc, grid_bc, sizeof s->grid_bc);
     s->obj_bc = (int*)malloc((size_t)(nshapes + 1) * sizeof(int));
     if (nshapes) memcpy(s->obj_bc, obj_bc, (size_t)nshapes * sizeof(int));

@@ -25,7 +25,8 @@ import weakref
 import numpy as np
 
 from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_INFLOW, BC_NEUMANN, E2_2, E2P_1, E4_1, E4_2, E4U_1, EULER,
-                   F32_COMP, F64, MMS_GAUSS, MMS_HOST, MMS_SINPROD, RK4, RK4_FUSED, SYS_HEAT,
+                   F32_COMP, F64, MMS_GAUSS, MMS_HOST, MMS_SINPROD, RK4, RK4_FUSED, SHAPE_RECT_X,
+                   SHAPE_RECT_Y, SHAPE_RECT_Z, SHAPE_SPHERE, SYS_HEAT,
                    SYS_WAVE_LINEAR, SYS_WAVE_RADIAL, IODesc, MMS_GRAD, MMS_SCALAR, MmsFns, ProblemDesc, RunDesc,
                    RunResult, ShoccsError,
                    SHX_OK, ShxField, ShxLineBlock, check, lib)
@@ -34,8 +35,22 @@ __all__ = [
     "Context", "Problem", "Field", "Batch", "FieldIO", "Euler", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
     "BC_INFLOW", "E2_2", "E4_2", "E4_1", "E4U_1", "E2P_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
     "MMS_GAUSS", "MMS_SINPROD", "MMS_HOST", "F64", "F32_COMP", "RK4", "RK4_FUSED", "EULER", "eq", "plus_eq",
-    "mms_functions",
+    "mms_functions", "sphere_shape", "rect_shape", "SHAPE_SPHERE", "SHAPE_RECT_X", "SHAPE_RECT_Y", "SHAPE_RECT_Z",
 ]
+
+
+def sphere_shape(center, radius) -> list:
+    """A shape record (shoccs_problem_desc.shapes8) of a sphere."""
+    c = list(center) + [0.0] * (3 - len(center))
+    return [float(SHAPE_SPHERE), c[0], c[1], c[2], float(radius), 0.0, 0.0, 0.0]
+
+
+def rect_shape(axis: int, plane: float, c0, c1, normal: float) -> list:
+    """A shape record of the rectangle normal to `axis` (0: yz_rect, 1:
+    xz_rect, 2: xy_rect) at coordinate `plane`, bounded by c0 - c1 in the
+    plane's (slow, fast) axes, fluid on the side of `normal` (its sign)."""
+    return [float(SHAPE_RECT_X + axis), float(plane), c0[0], c0[1], c1[0], c1[1],
+            1.0 if normal > 0 else -1.0, 0.0]
 
 
 def mms_functions(call, ddt, grad, lap) -> MmsFns:
@@ -176,11 +191,15 @@ class Config:
             # a "lua" manufactured solution runs in the configuration's
             # evaluator: the configuration then lives as long as this object
             self.mms = d.mms_fns if d.mms_fns else None
+            # system.type: "eigenvalues" carries only the operators
+            self.eigenvalues = L.shoccs_config_eigenvalues(h) == 1
             self.kind = d.kind
             self.n = tuple(d.n[a] for a in range(3))
             self.lo = tuple(d.min[a] for a in range(3))
             self.hi = tuple(d.max[a] for a in range(3))
             self.spheres = [[d.spheres[4 * i + c] for c in range(4)] for i in range(d.nshapes)]
+            self.shapes = [[d.shapes8[8 * i + c] for c in range(8)] for i in range(d.nshapes)] \
+                if d.shapes8 else None
             self.shape_bc = [d.shape_bc[i] for i in range(d.nshapes)]
             self.grid_bc = tuple(d.grid_bc[f] for f in range(6))
             self.scheme = d.scheme
@@ -221,7 +240,7 @@ class Config:
     def problem(self, ctx: "Context | None") -> "Problem":
         p = Problem(ctx, self.kind, self.n, self.lo, self.hi, spheres=self.spheres,
                     shape_bc=self.shape_bc, grid_bc=self.grid_bc, scheme=self.scheme,
-                    alpha=self.alpha, params=self.params, mms=self.mms)
+                    alpha=self.alpha, params=self.params, mms=self.mms, shapes=self.shapes)
         p._config = self  # the solution's functions live in this configuration
         return p
 
@@ -280,10 +299,12 @@ class Problem:
 
     def __init__(self, ctx: Context | None, kind: int, n, lo, hi, spheres=(), shape_bc=(),
                  grid_bc=(BC_FLOATING,) * 6, scheme: int = E4_1, alpha=(0.0, 0.0), params=(),
-                 mesh_only: bool = False, mms=None):
+                 mesh_only: bool = False, mms=None, shapes=None):
         """mesh_only: the mesh and cut points alone (no operators or system).
         mms: for heat with params [k, MMS_HOST], the solution's functions
-        (mms_functions(...), or a configuration's pointer)."""
+        (mms_functions(...), or a configuration's pointer).
+        shapes: shape records in place of spheres (sphere_shape(),
+        rect_shape()); shape_bc then has one condition per record."""
         L = lib()
         self.ctx = ctx
         d = ProblemDesc()
@@ -299,6 +320,11 @@ class Problem:
         self._keep = (sp, bc, pr)
         d.nshapes = len(sp)
         d.spheres = sp.ctypes.data_as(C.POINTER(C.c_double))
+        if shapes is not None:
+            s8 = np.ascontiguousarray(np.array(shapes, dtype=np.float64).reshape(-1, 8))
+            self._keep = self._keep + (s8,)
+            d.nshapes = len(s8)
+            d.shapes8 = s8.ctypes.data_as(C.POINTER(C.c_double))
         d.shape_bc = bc.ctypes.data_as(C.POINTER(C.c_int32))
         for f in range(6):
             d.grid_bc[f] = int(grid_bc[f])

@@ -35,6 +35,19 @@ def main(argv=None) -> int:
         return 0
     ctx = S.Context(0)
     prob = cfg.problem(ctx)
+    if cfg.eigenvalues:
+        # hyperbolic_eigenvalues::stats and its log (Timestamp,MaxEigen)
+        from . import spectral
+
+        stat = spectral.hyperbolic_eigenvalue_stat(prob)
+        if args.csv:
+            os.makedirs(os.path.dirname(args.csv) or ".", exist_ok=True)
+            with open(args.csv, "w") as f:
+                f.write(f"Timestamp,MaxEigen\n{time.strftime('%Y-%m-%d %H:%M:%S')},{stat:.17g}\n")
+        print(f"max eigen statistic {stat:.6e}")
+        prob.close()
+        ctx.close()
+        return 0
     u = prob.field()
     kw = cfg.run_kwargs()
     if args.fused and kw["integrator"] == S.RK4:

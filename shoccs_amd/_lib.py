@@ -16,6 +16,7 @@ SHX_OK = 0
 BC_DIRICHLET, BC_FLOATING, BC_NEUMANN, BC_INFLOW = 0, 1, 2, 3
 SYS_WAVE_RADIAL, SYS_WAVE_LINEAR, SYS_HEAT = 0, 1, 2
 MMS_GAUSS, MMS_SINPROD, MMS_HOST = 0, 1, 2
+SHAPE_SPHERE, SHAPE_RECT_X, SHAPE_RECT_Y, SHAPE_RECT_Z = 0, 1, 2, 3
 E2_2, E4_2, E4_1, E4U_1, E2P_1 = 0, 1, 2, 3, 4
 F64, F32_COMP = 0, 1
 
@@ -60,6 +61,7 @@ class ProblemDesc(C.Structure):
         ("nparams", C.c_int32),
         ("mesh_only", C.c_int32),
         ("mms_fns", C.POINTER(MmsFns)),
+        ("shapes8", dp),
     ]
 
 
@@ -213,6 +215,7 @@ _SIGS = [
     ("shoccs_problem_geometry", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), dp, dp]),
     ("shoccs_config_destroy", C.c_int, [C.c_void_p]),
     ("shoccs_config_error", C.c_char_p, [C.c_void_p]),
+    ("shoccs_config_eigenvalues", C.c_int, [C.c_void_p]),
     ("shx_derivative_apply_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
                                          C.c_int]),
     ("shx_system_rhs_T", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField)]),

@@ -910,6 +910,8 @@ struct shoccs_config {
     shx_mms_fns mms_fns{};
     shoccs_problem_desc desc{};
     std::vector<double> spheres;
+    std::vector<double> shapes8;  // shoccs_problem_desc.shapes8
+    int eigenvalues = 0;          // system.type = "eigenvalues"
     std::vector<int32_t> shape_bc;
     std::vector<double> params;
     shoccs_run_desc run{};
@@ -975,13 +977,38 @@ int build(const Value& sim, shoccs_config& c, std::string& err)
         for (int i = 1; shapes.t->get(i).k != Value::Nil; ++i) {
             const Value s = shapes.t->get(i);
             const std::string type = str_or(s, "type", "");
-            if (type != "sphere") {
-                err = "shape type '" + type + "' is not supported (spheres only)";
-                return SHX_EUNSUPPORTED;
+            if (type == "sphere") {
+                const Value ctr = sub(s, "center");
+                double q[8] = {SHOCCS_SPHERE, 0, 0, 0, 0, 0, 0, 0};
+                for (int a = 0; a < 3; ++a) q[1 + a] = idx_or(ctr, a + 1, 0.0);
+                q[4] = num_or(s, "radius", 0.0);
+                c.spheres.insert(c.spheres.end(), q + 1, q + 5);
+                c.shapes8.insert(c.shapes8.end(), q, q + 8);
+            } else if (type == "yz_rect") {
+                // object_geometry.cpp:268-290: corners default to the x = min
+                // plane, one cell beyond the domain in y and z; psi places the
+                // plane (1 - psi) h inside the left (normal > 0) or right wall
+                const double h = (d.max[0] - d.min[0]) / (d.n[0] - 1);
+                const Value lcv = sub(s, "lower_corner"), ucv = sub(s, "upper_corner");
+                double lc[3] = {idx_or(lcv, 1, d.min[0]), idx_or(lcv, 2, d.min[1] - h), idx_or(lcv, 3, d.min[2] - h)};
+                double uc[3] = {idx_or(ucv, 1, d.min[0]), idx_or(ucv, 2, d.max[1] + h), idx_or(ucv, 3, d.max[2] + h)};
+                const double nrm = num_or(s, "normal", 1.0);
+                Value tmp;
+                if (const Value* psi = field(s, "psi", tmp)) {
+                    if (psi->k != Value::Num) {
+                        err = "shapes.psi must be a number";
+                        return SHX_EINVAL;
+                    }
+                    lc[0] = uc[0] = nrm > 0.0 ? d.min[0] + (1 - psi->n) * h : d.max[0] - (1 - psi->n) * h;
+                }
+                // make_yz_rect (rect.cpp:8-31): plane at lc[0], bounds in (y, z)
+                const double q[8] = {SHOCCS_RECT_X, lc[0], lc[1], lc[2], uc[1], uc[2], nrm > 0 ? 1.0 : -1.0, 0};
+                c.spheres.insert(c.spheres.end(), 4, 0.0);
+                c.shapes8.insert(c.shapes8.end(), q, q + 8);
+            } else {
+                err = "shape type must be one of: sphere, yz_rect (got '" + type + "')";
+                return SHX_EINVAL;
             }
-            const Value ctr = sub(s, "center");
-            for (int a = 0; a < 3; ++a) c.spheres.push_back(idx_or(ctr, a + 1, 0.0));
-            c.spheres.push_back(num_or(s, "radius", 0.0));
             const std::string b = str_or(s, "boundary_condition", "floating");
             const int32_t bc = bc_of(b, 2, err, "shapes.boundary_condition");
             if (bc < 0) return SHX_EINVAL;
@@ -1093,7 +1120,7 @@ int build(const Value& sim, shoccs_config& c, std::string& err)
             if (sc.k == Value::Tab && sub(sys, "radius").k == Value::Num) {
                 for (int a = 0; a < 3; ++a) ctr[a] = idx_or(sc, a + 1, 0.0);
                 rad = num_or(sys, "radius", 0.0);
-            } else if (!c.spheres.empty()) {
+            } else if (!c.shapes8.empty() && c.shapes8[0] == SHOCCS_SPHERE) {
                 for (int a = 0; a < 3; ++a) ctr[a] = c.spheres[a];
                 rad = c.spheres[3];
             } else {
@@ -1104,11 +1131,21 @@ int build(const Value& sim, shoccs_config& c, std::string& err)
             c.params.push_back(rad);
             c.params.push_back(max_error);
         }
+    } else if (type == "eigenvalues") {
+        // hyperbolic_eigenvalues (hyperbolic_eigenvalues.cpp:39-52): the
+        // mesh, boundaries and scheme only; its statistic is computed from the
+        // x derivative (shoccs_amd.spectral).  The operators are carried by a
+        // constant-velocity wave along x, which is never stepped.
+        d.kind = SHX_SYS_WAVE_LINEAR;
+        c.eigenvalues = 1;
+        const double p[9] = {1.0, 0.0, 0.0, d.min[0], d.min[1], d.min[2], 1.0, 1.0, 100.0};
+        c.params.assign(p, p + 9);
     } else {
-        err = "system.type '" + type + "' not built (heat, scalar wave)";
+        err = "system.type '" + type + "' not built (heat, scalar wave, eigenvalues)";
         return SHX_EUNSUPPORTED;
     }
     d.spheres = c.spheres.data();
+    d.shapes8 = c.shapes8.empty() ? nullptr : c.shapes8.data();
     d.shape_bc = c.shape_bc.data();
     d.params = c.params.data();
     d.nparams = (int32_t)c.params.size();
@@ -1194,6 +1231,11 @@ int shoccs_config_io(const shoccs_config* c, shoccs_io_desc* io)
     io->dir = c->io_dir.c_str();
     io->xdmf_filename = c->io_xmf.c_str();
     return c->has_io ? 1 : 0;
+}
+
+int shoccs_config_eigenvalues(const shoccs_config* c)
+{
+    return c ? c->eigenvalues : SHX_EINVAL;
 }
 
 const char* shoccs_config_error(const shoccs_config* c)

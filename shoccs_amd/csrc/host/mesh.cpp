@@ -3,6 +3,7 @@
 //   cartesian ctor / linear_distribute  src/mesh/cartesian.cpp:13-44,
 //                                       src/fields/lazy_views.hpp:563-569
 //   sphere::hit / normal                src/mesh/sphere.cpp:15-45
+//   rect<I>::hit / normal               src/mesh/rect.hpp:21-43
 //   dot (unary right fold)              src/real3_operators.hpp:62-72
 //   ray casting, snap and psi clamp     src/mesh/object_geometry.cpp:17-116
 //   line segments                       src/mesh/mesh.cpp:29-87
@@ -28,7 +29,7 @@ struct hit_info {
     int shape_id;
 };
 
-std::optional<hit_info> sphere_hit(const sphere_shape& s, int id, const real3& origin,
+std::optional<hit_info> sphere_hit(const body_shape& s, int id, const real3& origin,
                                    const real3& direction, real t_min, real t_max)
 {
     const real3 oc = sub(origin, s.center);
@@ -49,8 +50,28 @@ std::optional<hit_info> sphere_hit(const sphere_shape& s, int id, const real3& o
     return std::nullopt;
 }
 
-real3 sphere_normal(const sphere_shape& s, const real3& pos)
+// rect<I>::hit (src/mesh/rect.hpp:21-36): bounds and t_min, t_max inclusive.
+// A line lying in the plane (t = 0/0) takes no hit; the reference computes
+// with a NaN t there.
+std::optional<hit_info> rect_hit(const body_shape& s, int id, const real3& origin, const real3& direction,
+                                 real t_min, real t_max)
 {
+    const int I = s.axis, S = slow_axis(I), F = fast_axis(I);
+    const real t = (s.plane - origin[I]) / direction[I];
+    if (t < t_min || t > t_max || std::isnan(t)) return std::nullopt;
+    real3 p;
+    for (int q = 0; q < 3; ++q) p[q] = origin[q] + t * direction[q];
+    if (p[S] < s.c0[0] || p[S] > s.c1[0] || p[F] < s.c0[1] || p[F] > s.c1[1]) return std::nullopt;
+    return hit_info{t, p, s.fluid_normal * direction[I] < 0, id};
+}
+
+real3 shape_normal(const body_shape& s, const real3& pos)
+{
+    if (s.kind == 1) {  // rect<I>::normal (rect.hpp:38-43)
+        real3 n{0, 0, 0};
+        n[s.axis] = s.fluid_normal;
+        return n;
+    }
     const real3 r = sub(pos, s.center);
     const real d = std::sqrt(dot(r, r));
     return {r[0] / d, r[1] / d, r[2] / d};
@@ -58,7 +79,7 @@ real3 sphere_normal(const sphere_shape& s, const real3& pos)
 
 }  // namespace
 
-mesh::mesh(const int3& n, const real3& min, const real3& max, const std::vector<sphere_shape>& shapes)
+mesh::mesh(const int3& n, const real3& min, const real3& max, const std::vector<body_shape>& shapes)
     : shapes_(shapes)
 {
     for (int i = 0; i < 3; ++i) {
@@ -105,7 +126,8 @@ void mesh::cast_rays(int I)
                 std::optional<hit_info> hit;
                 real tm = t_max;
                 for (int id = 0; id < (int)shapes_.size(); ++id)
-                    if (auto h = sphere_hit(shapes_[id], id, origin, direction, t_min, tm)) {
+                    if (auto h = shapes_[id].kind == 0 ? sphere_hit(shapes_[id], id, origin, direction, t_min, tm)
+                                                       : rect_hit(shapes_[id], id, origin, direction, t_min, tm)) {
                         hit = h;
                         tm = h->t;
                     }
@@ -120,7 +142,7 @@ void mesh::cast_rays(int I)
                 real psi = off * (fluid_pos - hit->position[I]) / h_[I];
                 psi = std::clamp(psi, snap_tol, 1.0 - snap_tol);
                 out.push_back(mesh_object_info{psi, hit->position,
-                                               sphere_normal(shapes_[hit->shape_id], hit->position),
+                                               shape_normal(shapes_[hit->shape_id], hit->position),
                                                hit->ray_outside, coord, hit->shape_id});
                 t_min = std::nextafter(hit->t, t_max);
             }

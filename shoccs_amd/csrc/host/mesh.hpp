@@ -1,4 +1,4 @@
-// mesh.hpp -- Cartesian mesh with embedded spheres, found by ray casting.
+// mesh.hpp -- Cartesian mesh with embedded spheres and rectangles, found by ray casting.
 //
 // Mirrors ccs::mesh (src/mesh/mesh.hpp:15-119) and ccs::object_geometry
 // (src/mesh/object_geometry.hpp): coordinates, cut points R(dir) ordered by
@@ -38,9 +38,18 @@ using Points = std::array<std::vector<type>, 3>;  // one per cut point of R(x), 
 constexpr int slow_axis(int dir) { return dir == 0 ? 1 : 0; }
 constexpr int fast_axis(int dir) { return dir == 2 ? 1 : 2; }
 
-struct sphere_shape {
-    real3 center;
-    real radius;
+// A body: a sphere (src/mesh/sphere.cpp:9-45), or a rectangle in the plane
+// normal to `axis` at coordinate `plane`, bounded by c0 - c1 in that plane's
+// (slow, fast) axes, with the fluid on the side of fluid_normal (+1 or -1;
+// src/mesh/rect.hpp:11-44, rect.cpp:8-19).
+struct body_shape {
+    int kind = 0;  // 0 sphere, 1 rectangle
+    real3 center{};
+    real radius = 0;
+    int axis = 0;
+    real plane = 0;
+    real c0[2] = {0, 0}, c1[2] = {0, 0};
+    real fluid_normal = 1;
 };
 
 // src/mesh/mesh_types.hpp:30-38
@@ -84,7 +93,7 @@ class mesh
 {
 public:
     mesh() = default;
-    mesh(const int3& n, const real3& min, const real3& max, const std::vector<sphere_shape>& shapes);
+    mesh(const int3& n, const real3& min, const real3& max, const std::vector<body_shape>& shapes);
 
     const int3& extents() const { return n_; }
     int64_t size() const { return (int64_t)n_[0] * n_[1] * n_[2]; }
@@ -97,7 +106,7 @@ public:
     const std::vector<int64_t>& fluid() const { return fluid_; }
     const real3& lo() const { return min_; }
     const real3& hi() const { return max_; }
-    const std::vector<sphere_shape>& shapes() const { return shapes_; }
+    const std::vector<body_shape>& shapes() const { return shapes_; }
     int64_t stride(int dir) const;
     int64_t ic(const int3& ijk) const
     {
@@ -115,7 +124,7 @@ private:
     real3 min_{}, max_{}, h_{};
     int dims_ = 0;
     std::array<std::vector<real>, 3> xyz_;
-    std::vector<sphere_shape> shapes_;
+    std::vector<body_shape> shapes_;
     std::array<std::vector<mesh_object_info>, 3> R_;
     std::array<std::vector<line>, 3> lines_;
     std::vector<int64_t> fluid_;

@@ -68,10 +68,34 @@ int shoccs_problem_create(shx_ctx* ctx, const shoccs_problem_desc* desc, shoccs_
     if (!p) return SHX_ENOMEM;
     p->ctx = ctx;
     p->kind = desc->kind;
-    std::vector<sphere_shape> shapes;
+    std::vector<body_shape> shapes;
     for (int s = 0; s < desc->nshapes; ++s) {
-        const double* q = desc->spheres + 4 * s;
-        shapes.push_back(sphere_shape{{q[0], q[1], q[2]}, q[3]});
+        body_shape b;
+        if (desc->shapes8) {
+            const double* q = desc->shapes8 + 8 * s;
+            const int kind = (int)q[0];
+            if (kind < SHOCCS_SPHERE || kind > SHOCCS_RECT_Z || (double)kind != q[0])
+                return host_fail(SHX_EINVAL, "shape " + std::to_string(s) + ": unknown kind");
+            if (kind == SHOCCS_SPHERE) {
+                b.center = {q[1], q[2], q[3]};
+                b.radius = q[4];
+            } else {
+                // make_rect (rect.cpp:8-19): the fluid normal is its sign
+                b.kind = 1;
+                b.axis = kind - SHOCCS_RECT_X;
+                b.plane = q[1];
+                b.c0[0] = q[2];
+                b.c0[1] = q[3];
+                b.c1[0] = q[4];
+                b.c1[1] = q[5];
+                b.fluid_normal = q[6] > 0 ? 1.0 : -1.0;
+            }
+        } else {
+            const double* q = desc->spheres + 4 * s;
+            b.center = {q[0], q[1], q[2]};
+            b.radius = q[3];
+        }
+        shapes.push_back(b);
         p->obj.push_back((bcs::type)desc->shape_bc[s]);
     }
     for (int a = 0; a < 3; ++a) {

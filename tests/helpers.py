@@ -107,17 +107,33 @@ def heat3d_cfg(n=(20, 24, 28), center=(0.5, 0.45, 0.55), radius=0.22, scheme=O.E
 
 
 def oracle_system(cfg):
+    """cfg["shapes"] (shape records), when present, replaces cfg["spheres"]."""
     st = O.stencil(cfg["scheme"], cfg["alpha"])
-    return O.OracleSystem(cfg["kind"], cfg["n"], cfg["lo"], cfg["hi"], cfg["spheres"], cfg["obj_bc"],
-                          cfg["grid_bc"], st, cfg["params"])
+    return O.OracleSystem(cfg["kind"], cfg["n"], cfg["lo"], cfg["hi"], cfg.get("spheres", ()), cfg["obj_bc"],
+                          cfg["grid_bc"], st, cfg["params"], shapes=cfg.get("shapes"))
 
 
 def product_problem(ctx, cfg, mesh_only=False):
     import shoccs_amd as S
 
-    return S.Problem(ctx, cfg["kind"], cfg["n"], cfg["lo"], cfg["hi"], spheres=cfg["spheres"],
+    return S.Problem(ctx, cfg["kind"], cfg["n"], cfg["lo"], cfg["hi"], spheres=cfg.get("spheres", ()),
                      shape_bc=cfg["obj_bc"], grid_bc=cfg["grid_bc"], scheme=cfg["scheme"],
-                     alpha=cfg["alpha"], params=cfg["params"], mesh_only=mesh_only)
+                     alpha=cfg["alpha"], params=cfg["params"], mesh_only=mesh_only, shapes=cfg.get("shapes"))
+
+
+def rect_1d_cfg(psi_left, psi_right, n=21, scheme=O.E2P_1, alpha=None, bc=(O.DIRICHLET, O.FLOATING)):
+    """The reference's one-dimensional eigenvalue problems
+    (hyperbolic_eigenvalues.t.cpp:17-60, eigenvalues.lua): n points on [0, 1]
+    and two yz_rect walls placed psi inside either end (object_geometry.cpp:
+    268-290): the left one with normal +1 (fluid to the right), the right one
+    with normal -1.  A constant-velocity wave (a = 1) carries the operators."""
+    h = 1.0 / (n - 1)
+    shapes = [O.rect(0, 0.0 + (1 - psi_left) * h, (-h, -h), (h, h), 1),
+              O.rect(0, 1.0 - (1 - psi_right) * h, (-h, -h), (h, h), -1)]
+    return dict(kind=O.SYS_WAVE_LINEAR, n=(n, 1, 1), lo=(0.0, 0.0, 0.0), hi=(1.0, 0.0, 0.0), shapes=shapes,
+                obj_bc=list(bc), grid_bc=[O.FLOATING] * 6, scheme=scheme,
+                alpha=alpha if alpha is not None else E2P_LUA,
+                params=[1.0, 0.0, 0.0, 0.5, 0.0, 0.0, 0.1, 1.0])
 
 
 # E2-poly parameters of the reference's scalar_wave.lua:29-40 (floating,

@@ -75,6 +75,7 @@ class Mesh(C.Structure):
         ("nlines", C.c_long * 3),
         ("fluid", lp),
         ("nfluid", C.c_long),
+        ("shapes8", dp),
     ]
 
 
@@ -157,6 +158,10 @@ def lib():
     L = C.CDLL(LIB_PATH)
     L.orc_mesh_create.restype = C.POINTER(Mesh)
     L.orc_mesh_create.argtypes = [ip, dp, dp, C.c_int, dp]
+    L.orc_mesh_create8.restype = C.POINTER(Mesh)
+    L.orc_mesh_create8.argtypes = [ip, dp, dp, C.c_int, dp]
+    L.orc_system_create8.restype = C.c_void_p
+    L.orc_system_create8.argtypes = [C.c_int, ip, dp, dp, C.c_int, dp, ip, ip, C.POINTER(Stencil), dp, C.c_int]
     L.orc_mesh_destroy.argtypes = [C.POINTER(Mesh)]
     L.orc_query.restype = Info
     L.orc_query.argtypes = [C.POINTER(Stencil), C.c_int]
@@ -248,16 +253,40 @@ class OracleField:
         return np.concatenate([self.D] + [self.R[d][: self.nR[d]] for d in range(3)])
 
 
+def sphere(center, radius):
+    """A shape record (orc_mesh.shapes8) of a sphere."""
+    c = list(center) + [0.0] * (3 - len(center))
+    return [0.0, c[0], c[1], c[2], radius, 0.0, 0.0, 0.0]
+
+
+def rect(axis, plane, c0, c1, normal):
+    """A shape record of the rectangle normal to `axis` at `plane`, bounded
+    by c0 - c1 in that plane's (slow, fast) axes, fluid on the side of
+    `normal` (+1 or -1; rect.cpp:8-19)."""
+    return [1.0 + axis, plane, c0[0], c0[1], c1[0], c1[1], 1.0 if normal > 0 else -1.0, 0.0]
+
+
+def _shapes8(shapes):
+    return np.ascontiguousarray(np.array(shapes, dtype=np.float64).reshape(-1, 8)) if len(shapes) \
+        else np.zeros((0, 8))
+
+
 class OracleMesh:
-    def __init__(self, n, lo, hi, spheres=()):
+    def __init__(self, n, lo, hi, spheres=(), shapes=None):
+        """spheres: n x {cx, cy, cz, r}; or shapes: shape records (sphere(), rect())."""
         L = lib()
         self.n = np.array(list(n) + [1] * (3 - len(n)), dtype=np.intc)
         self.lo = np.array(lo, dtype=np.float64)
         self.hi = np.array(hi, dtype=np.float64)
-        sp = np.array(spheres, dtype=np.float64).reshape(-1, 4) if len(spheres) else np.zeros((0, 4))
-        self._sp = np.ascontiguousarray(sp)
-        self.p = L.orc_mesh_create(_arr(self.n, C.c_int), _arr(self.lo, C.c_double),
-                                   _arr(self.hi, C.c_double), len(sp), _arr(self._sp, C.c_double))
+        if shapes is not None:
+            self._sp = _shapes8(shapes)
+            self.p = L.orc_mesh_create8(_arr(self.n, C.c_int), _arr(self.lo, C.c_double),
+                                        _arr(self.hi, C.c_double), len(self._sp), _arr(self._sp, C.c_double))
+        else:
+            sp = np.array(spheres, dtype=np.float64).reshape(-1, 4) if len(spheres) else np.zeros((0, 4))
+            self._sp = np.ascontiguousarray(sp)
+            self.p = L.orc_mesh_create(_arr(self.n, C.c_int), _arr(self.lo, C.c_double),
+                                       _arr(self.hi, C.c_double), len(sp), _arr(self._sp, C.c_double))
         self.own = True
 
     @classmethod
@@ -372,20 +401,27 @@ def deriv_blocks(dp_) -> list:
 class OracleSystem:
     """A heat or scalar-wave system restated on the CPU."""
 
-    def __init__(self, kind, n, lo, hi, spheres, obj_bc, grid_bc, st: Stencil, params):
+    def __init__(self, kind, n, lo, hi, spheres, obj_bc, grid_bc, st: Stencil, params, shapes=None):
+        """spheres: n x {cx, cy, cz, r}; or shapes: shape records (sphere(), rect())."""
         L = lib()
         self.kind = kind
         n = np.array(list(n) + [1] * (3 - len(n)), dtype=np.intc)
         lo = np.array(lo, dtype=np.float64)
         hi = np.array(hi, dtype=np.float64)
-        sp = np.ascontiguousarray(np.array(spheres, dtype=np.float64).reshape(-1, 4))
         ob = np.array(list(obj_bc) + [0], dtype=np.intc)
         gb = np.array(grid_bc, dtype=np.intc)
         pr = np.array(params, dtype=np.float64)
+        if shapes is not None:
+            sp = _shapes8(shapes)
+            self.p = L.orc_system_create8(kind, _arr(n, C.c_int), _arr(lo, C.c_double), _arr(hi, C.c_double),
+                                          len(sp), _arr(sp, C.c_double), _arr(ob, C.c_int), _arr(gb, C.c_int),
+                                          C.byref(st), _arr(pr, C.c_double), len(pr))
+        else:
+            sp = np.ascontiguousarray(np.array(spheres, dtype=np.float64).reshape(-1, 4))
+            self.p = L.orc_system_create(kind, _arr(n, C.c_int), _arr(lo, C.c_double), _arr(hi, C.c_double),
+                                         len(sp), _arr(sp, C.c_double), _arr(ob, C.c_int), _arr(gb, C.c_int),
+                                         C.byref(st), _arr(pr, C.c_double), len(pr))
         self._keep = (n, lo, hi, sp, ob, gb, pr, st)
-        self.p = L.orc_system_create(kind, _arr(n, C.c_int), _arr(lo, C.c_double), _arr(hi, C.c_double),
-                                     len(sp), _arr(sp, C.c_double), _arr(ob, C.c_int), _arr(gb, C.c_int),
-                                     C.byref(st), _arr(pr, C.c_double), len(pr))
         if not self.p:
             raise ValueError("oracle rejected the configuration")
         self.mesh = OracleMesh.borrowed(L.orc_system_mesh(self.p))

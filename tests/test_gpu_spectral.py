@@ -85,3 +85,67 @@ def test_hyperbolic_eigenvalue_statistic(ctx):
         prob.close()
     assert stats[0] <= 0.0 and stats[1] <= 0.0
     assert abs(stats[0]) < 0.05 and abs(stats[1]) < 0.25 * abs(stats[0])
+
+
+HE = [13 / 100, 7 / 50, 3 / 20, 4 / 25, 17 / 100, 9 / 50, 3 / 25, 13 / 100, 7 / 50, 0, 0, 0, 0]
+
+
+def test_hyperbolic_eigenvalues_known_answer_with_rects(ctx):
+    """hyperbolic_eigenvalues.t.cpp:17-60: E2-poly between a Dirichlet yz_rect
+    wall (psi 0.001, normal +1) and a floating one (psi 0.9, normal -1) on 21
+    points; the statistic is within 1.2e-5 of zero.  The device operator
+    equals the oracle's over the same unknowns, bit for bit."""
+    import shoccs_amd.spectral as SP
+
+    cfg = H.rect_1d_cfg(0.001, 0.9, n=21, alpha=HE)
+    prob = H.product_problem(ctx, cfg)
+    stat = SP.hyperbolic_eigenvalue_stat(prob)
+    assert stat + 1.0 == pytest.approx(1.0, rel=1.2e-5)
+    pos = _positions(prob, 0)
+    np.testing.assert_array_equal(SP.operator_matrix(prob, 0), H.oracle_operator_matrix(cfg, 0, pos))
+    prob.close()
+
+
+EIGEN_LUA = """
+-- the configuration of the reference's eigenvalues.lua: 21 points, two
+-- yz_rect walls at psi = 1 - 1e-8 (Dirichlet, normal +1; floating, normal
+-- -1), E2-poly with its floating and Dirichlet parameters
+local psi = 1 - 1e-8
+simulation = {
+    mesh = { index_extents = {21}, domain_bounds = {1} },
+    shapes = {
+        { type = "yz_rect", psi = psi, normal = 1, boundary_condition = "dirichlet" },
+        { type = "yz_rect", psi = psi, normal = -1, boundary_condition = "floating" },
+    },
+    scheme = { order = 1, type = "E2-poly",
+               floating_alpha = { 0.7039278390946743, 0.5390086175376538, -0.647109821986589,
+                                  0.2051508287133347, 0.6062051039572746, 0.8148425279273044 },
+               dirichlet_alpha = { -0.10739761225713096, 0.8736492896991024, 0.40413606410467495 } },
+    system = { type = "eigenvalues" },
+}
+"""
+
+
+def test_eigenvalues_config_through_the_cli(ctx, tmp_path):
+    """system.type "eigenvalues" run by python -m shoccs_amd: one MaxEigen
+    row holding the statistic of the device operator, which equals the one
+    computed from the oracle's assembled operator."""
+    import shoccs_amd as S
+    import shoccs_amd.__main__ as M
+    import shoccs_amd.spectral as SP
+
+    path = tmp_path / "eigenvalues.lua"
+    path.write_text(EIGEN_LUA)
+    csv = tmp_path / "system.csv"
+    assert M.main([str(path), "--csv", str(csv)]) == 0
+    lines = csv.read_text().splitlines()
+    assert lines[0] == "Timestamp,MaxEigen" and len(lines) == 2
+    cfg = S.Config(EIGEN_LUA)
+    prob = cfg.problem(ctx)
+    stat = SP.hyperbolic_eigenvalue_stat(prob)
+    assert float(lines[1].split(",")[1]) == stat
+    ocfg = H.rect_1d_cfg(1 - 1e-8, 1 - 1e-8, n=21, alpha=H.E2P_LUA[:9] + [0, 0, 0, 0])
+    A = H.oracle_operator_matrix(ocfg, 0, _positions(prob, 0))
+    lam = np.linalg.eigvals(A)
+    assert stat == pytest.approx(float(-prob.h()[0] * np.min(lam.real)), rel=1e-12, abs=1e-14)
+    prob.close()

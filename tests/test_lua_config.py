@@ -158,6 +158,15 @@ def test_reference_configs():
     p = w.problem(None)  # the floating circle carries cut-point rows
     assert all(len(p.cut_rows(d)["rw"]) > 0 for d in range(2))
     p.close()
+    e = S.Config.from_file(os.path.join(REF, "eigenvalues.lua"))  # two yz_rect walls, E2-poly
+    assert e.eigenvalues and e.n == (21, 1, 1) and e.scheme == S.E2P_1
+    assert [int(sh[0]) for sh in e.shapes] == [S.SHAPE_RECT_X, S.SHAPE_RECT_X]
+    assert e.shape_bc == [S.BC_DIRICHLET, S.BC_FLOATING]
+    p = e.problem(None)
+    assert p.nR == (2, 0, 0)
+    p.close()
+    with pytest.raises(S.ShoccsError, match="E2"):  # scheme order 1 "E2" (E2_1) is not built
+        S.Config.from_file(os.path.join(REF, "scalar_wave_1d.lua"))
     h = S.Config.from_file(os.path.join(REF, "heat.lua"))
     assert h.kind == S.SYS_HEAT and h.shape_bc == [S.BC_FLOATING]
     assert h.params[:3] == [1 / 30, S.MMS_GAUSS, 1]  # the commented-out gaussians are skipped
