@@ -365,6 +365,20 @@ int shx_batch_launch_async(shx_batch* b);
 /* current time of every problem */
 int shx_batch_times(shx_batch* b, double* t);
 
+/* One problem's fused RK4 steps captured as a hipGraph (SURVEY's
+ * shx_rk4_graph; the reference's rk4 has no graph form): nsteps steps from
+ * t0 with step dt on u0 (kept by the graph; the system's fused registers and
+ * tuning as shx_rk4_fused_steps_async uses them), the clock on the device.
+ * Each shx_rk4_graph_launch replays the nsteps steps asynchronously on the
+ * context's stream and advances the clock; the values equal nsteps calls of
+ * shx_rk4_fused_steps_async.  Systems the batch refuses (Neumann grid faces,
+ * host-evaluated solutions) are refused. */
+typedef struct shx_rk4_graph shx_rk4_graph;
+int shx_rk4_graph_create(shx_system* s, shx_field* u0, double t0, double dt, int nsteps, shx_rk4_graph** out);
+int shx_rk4_graph_launch(shx_rk4_graph* g);
+int shx_rk4_graph_time(shx_rk4_graph* g, double* t);
+int shx_rk4_graph_destroy(shx_rk4_graph* g);
+
 /* ----------------------------------------------------------------- adjoint
  * No reference counterpart: the reference has no adjoint.  Transposes of the
  * operators above and the discrete adjoint of the RK4 step, for gradients of a

@@ -32,7 +32,7 @@ from ._lib import (BC_DIRICHLET, BC_FLOATING, BC_INFLOW, BC_NEUMANN, E2_2, E2P_1
                    SHX_OK, ShxField, ShxLineBlock, check, lib)
 
 __all__ = [
-    "Context", "Problem", "Field", "Batch", "FieldIO", "Euler", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
+    "Context", "Problem", "Field", "Batch", "Rk4Graph", "FieldIO", "Euler", "ShoccsError", "BC_DIRICHLET", "BC_FLOATING", "BC_NEUMANN",
     "BC_INFLOW", "E2_2", "E4_2", "E4_1", "E4U_1", "E2P_1", "SYS_WAVE_RADIAL", "SYS_WAVE_LINEAR", "SYS_HEAT",
     "MMS_GAUSS", "MMS_SINPROD", "MMS_HOST", "F64", "F32_COMP", "RK4", "RK4_FUSED", "EULER", "eq", "plus_eq",
     "mms_functions", "sphere_shape", "rect_shape", "SHAPE_SPHERE", "SHAPE_RECT_X", "SHAPE_RECT_Y", "SHAPE_RECT_Z",
@@ -829,3 +829,40 @@ class Batch:
         except Exception:
             pass
 
+
+class Rk4Graph:
+    """One problem's fused RK4 steps captured as a hipGraph
+    (shx_rk4_graph_*): ``launch()`` replays `nsteps` steps of ``field`` in
+    place, the clock on the device advancing with every replay."""
+
+    def __init__(self, problem: Problem, field: Field, t0: float, dt: float, nsteps: int):
+        problem._need_device()
+        self.ctx = problem.ctx
+        self.problems = [problem]
+        self.field = field  # the graph refers to its device memory
+        h = C.c_void_p()
+        check(lib().shx_rk4_graph_create(problem.sys, C.byref(field.c), t0, dt, nsteps, C.byref(h)),
+              self.ctx.p, "rk4 graph create")
+        self.h = h
+        self.ctx._batches.add(self)
+
+    def launch(self, sync: bool = True):
+        check(lib().shx_rk4_graph_launch(self.h), self.ctx.p, "rk4 graph launch")
+        if sync:
+            self.ctx.sync()
+
+    def time(self) -> float:
+        t = C.c_double()
+        check(lib().shx_rk4_graph_time(self.h, C.byref(t)), self.ctx.p, "rk4 graph time")
+        return t.value
+
+    def close(self):
+        if getattr(self, "h", None) and getattr(self.ctx, "p", None):
+            lib().shx_rk4_graph_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -190,6 +190,11 @@ _SIGS = [
     ("shx_batch_capture", C.c_int, [C.c_void_p, C.c_int]),
     ("shx_batch_launch_async", C.c_int, [C.c_void_p]),
     ("shx_batch_times", C.c_int, [C.c_void_p, dp]),
+    ("shx_rk4_graph_create", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double, C.c_int,
+                                       C.POINTER(C.c_void_p)]),
+    ("shx_rk4_graph_launch", C.c_int, [C.c_void_p]),
+    ("shx_rk4_graph_time", C.c_int, [C.c_void_p, dp]),
+    ("shx_rk4_graph_destroy", C.c_int, [C.c_void_p]),
     ("shx_time_derivative_apply", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
                                             C.c_int, C.c_int, dp]),
     ("shx_time_batch", C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, C.POINTER(C.c_int64)]),

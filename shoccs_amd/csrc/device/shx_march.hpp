@@ -678,9 +678,9 @@ __global__ __launch_bounds__(64 * TY) void k_batch_stage(const BatchEntry* __res
     ss.c_acc = dt * kStageWeight[STAGE];
     ss.c_next = STAGE < 3 ? dt * kStageTime[STAGE + 1] : 1.0;
     ss.t_next = STAGE < 3 ? t + dt * kStageTime[STAGE + 1] : t + dt;
-    // batches refuse systems with free cut points: that part of the special
-    // work is compiled out, so its arguments are never loaded
-    stage_block<KIND, STAGE, W, TY, VEC, MIXED, false>(m, sp, ss, blockIdx.x - start[lo], sg, mg, mode);
+    // problems with free cut points (floating bodies) carry them in the
+    // special work, as the single-problem stage does
+    stage_block<KIND, STAGE, W, TY, VEC, MIXED, true>(m, sp, ss, blockIdx.x - start[lo], sg, mg, mode);
 }
 
 template <int KIND, int STAGE, int W, bool MIXED>

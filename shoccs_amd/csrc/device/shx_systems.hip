@@ -1936,8 +1936,9 @@ int shx_batch_create(shx_ctx* ctx, shx_system* const* systems, const shx_field* 
     if (!(march_ty == 4 || march_ty == 8)) return SHX_EINVAL;
     *out = nullptr;
     for (int p = 0; p < n; ++p)
-        if (!systems[p] || systems[p]->general || systems[p]->hm)
-            return fail(ctx, SHX_EUNSUPPORTED, "batch: every problem needs Dirichlet bodies on grids without Neumann faces");
+        if (!systems[p] || (systems[p]->general && !systems[p]->fused_free) || systems[p]->hm)
+            return fail(ctx, SHX_EUNSUPPORTED,
+                        "batch: Neumann grid faces and host-evaluated solutions take the reference schedule");
     auto b = std::unique_ptr<shx_batch>(new (std::nothrow) shx_batch());
     if (!b) return SHX_ENOMEM;
     b->ctx = ctx;
@@ -2106,6 +2107,45 @@ int shx_time_batch(shx_batch* b, int use_graph, int reps, double* ms_per_step, i
     const int steps = use_graph ? b->graph_steps : 1;
     *ms_per_step = (double)ms / ((double)reps * steps);
     if (points) *points = b->points;
+    return rc;
+}
+
+}  // extern "C"
+
+// ---- one problem's fused RK4 steps as a hipGraph: a batch of that problem
+// alone, captured (the clock lives on the device, so every replay advances)
+struct shx_rk4_graph {
+    shx_batch* b = nullptr;
+};
+
+extern "C" {
+
+int shx_rk4_graph_create(shx_system* s, shx_field* u0, double t0, double dt, int nsteps, shx_rk4_graph** out)
+{
+    if (!s || !u0 || nsteps < 1 || !out) return SHX_EINVAL;
+    *out = nullptr;
+    auto g = std::unique_ptr<shx_rk4_graph>(new (std::nothrow) shx_rk4_graph());
+    if (!g) return SHX_ENOMEM;
+    shx_system* const sys[1] = {s};
+    int rc = shx_batch_create(s->ctx, sys, u0, &t0, &dt, 1, s->march_ty == 8 ? 8 : 4, &g->b);
+    if (rc == SHX_OK) rc = shx_batch_capture(g->b, nsteps);
+    if (rc != SHX_OK) {
+        if (g->b) shx_batch_destroy(g->b);
+        return rc;
+    }
+    *out = g.release();
+    return SHX_OK;
+}
+
+int shx_rk4_graph_launch(shx_rk4_graph* g) { return g ? shx_batch_launch_async(g->b) : SHX_EINVAL; }
+
+int shx_rk4_graph_time(shx_rk4_graph* g, double* t) { return g && t ? shx_batch_times(g->b, t) : SHX_EINVAL; }
+
+int shx_rk4_graph_destroy(shx_rk4_graph* g)
+{
+    if (!g) return SHX_EINVAL;
+    const int rc = shx_batch_destroy(g->b);
+    delete g;
     return rc;
 }
 

@@ -243,8 +243,17 @@ def test_fused_batch_and_adjoint_refuse(ctx):
     prob2.set_tuning(0)
     with pytest.raises(S.ShoccsError, match="x-marching"):
         prob2.rk4_fused(f2, 0.0, 1e-4, 1)
-    with pytest.raises(S.ShoccsError):
-        S.Batch(ctx, [prob2], [f2], [0.0], [1e-4])
+    # a floating body without Neumann faces steps in a batch (its free cut
+    # points in the special work, as the fused single-problem stage does)
+    prob2.initialize(f2, 0.0)
+    prob2.update_boundary(f2, 0.0)
+    f3 = prob2.field()
+    f3.upload(f2.download())
+    bat = S.Batch(ctx, [prob2], [f2], [0.0], [1e-4])
+    bat.steps(2)
+    prob2.rk4_fused(f3, 0.0, 1e-4, 2)
+    np.testing.assert_array_equal(f2.download(), f3.download())
+    bat.close()
     prob2.close()
     with pytest.raises(S.ShoccsError):
         prob.rhs_T(f, g)
