@@ -394,6 +394,28 @@ int shx_rk4_graph_destroy(shx_rk4_graph* g);
  * with cut-point rows (non-Dirichlet bodies) return SHX_EUNSUPPORTED. */
 int shx_derivative_apply_T(shx_derivative* d, const shx_field* dbar, shx_field* ubar, int accumulate);
 
+/* One fused FP64 RK4 step from u (read only) to out (a different field),
+ * writing the stage inputs w1, w2, w3 (BU(u + dt rki[s] k_{s-1}) of
+ * rk4.cpp:25-41, as the fused stages form them): the linearization points of
+ * shx_rk4_adjoint_stages_async.  Asynchronous; same refusals as
+ * shx_rk4_fused_steps_async, and FP64 only.  No reference counterpart. */
+int shx_rk4_fused_keep_async(shx_system* s, const shx_field* u, shx_field* w1, shx_field* w2, shx_field* w3,
+                             shx_field* out, double t, double dt);
+/* The discrete adjoint of one RK4 step over its kept stage inputs
+ * w[0..3] (w[0] the step's input, w[1..3] from shx_rk4_fused_keep_async): on
+ * entry ubar holds dJ/du_{n+1}, on exit dJ/du_n, as shx_rk4_adjoint computes
+ * it without recomputing the stages.  With with_coef the step's coefficient
+ * gradient is added to a 27-entry accumulator on the device
+ * (shx_adjoint_coef_set / shx_adjoint_coef_get, both synchronous).
+ * Asynchronous.  No reference counterpart. */
+int shx_rk4_adjoint_stages_async(shx_system* s, const shx_field* w, shx_field* ubar, double dt, int with_coef);
+/* Builds the system's adjoint structures (workspace, the operators'
+ * transposes, the merged closure lists), which the calls above otherwise
+ * build on first use; synchronous. */
+int shx_adjoint_prepare(shx_system* s);
+int shx_adjoint_coef_set(shx_system* s, const double* g);
+int shx_adjoint_coef_get(shx_system* s, double* g);
+
 /* wbar = (d rhs / d u)^T kbar for the system's right-hand side (rhs rows on
  * D; the rows of the cut points are zero for Dirichlet bodies): scalar wave
  * sum_c D_c^T (gG_c kbar), heat sum_c D_c^T (k kbar), zero weight on

@@ -550,6 +550,39 @@ class Problem:
         check(lib().shx_rk4_adjoint(self.sys, u.ref(), ubar.ref(), t, dt, gp), self.ctx.p,
               "rk4_adjoint")
 
+    def rk4_fused_keep(self, u: Field, w1: Field, w2: Field, w3: Field, out: Field, t: float, dt: float):
+        """One fused FP64 step u -> out keeping the stage inputs w1..w3
+        (shx_rk4_fused_keep_async; asynchronous)."""
+        self._need_device()
+        check(lib().shx_rk4_fused_keep_async(self.sys, u.ref(), w1.ref(), w2.ref(), w3.ref(), out.ref(), t, dt),
+              self.ctx.p, "rk4_fused_keep")
+
+    def rk4_adjoint_stages(self, w, ubar: Field, dt: float, with_coef: bool = True):
+        """Adjoint of one RK4 step over its kept stage inputs w = (u, w1, w2,
+        w3) (shx_rk4_adjoint_stages_async; asynchronous); the coefficient
+        gradient accumulates on the device (adjoint_coef_set / _get)."""
+        self._need_device()
+        arr = (ShxField * 4)(*[f.c for f in w])
+        check(lib().shx_rk4_adjoint_stages_async(self.sys, arr, ubar.ref(), dt, int(with_coef)), self.ctx.p,
+              "rk4_adjoint_stages")
+
+    def adjoint_prepare(self):
+        """Builds the adjoint structures now (shx_adjoint_prepare) instead of
+        on the first adjoint call."""
+        self._need_device()
+        check(lib().shx_adjoint_prepare(self.sys), self.ctx.p, "adjoint_prepare")
+
+    def adjoint_coef_set(self, g: np.ndarray):
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(27)
+        check(lib().shx_adjoint_coef_set(self.sys, g.ctypes.data_as(C.POINTER(C.c_double))), self.ctx.p,
+              "adjoint_coef_set")
+
+    def adjoint_coef_get(self) -> np.ndarray:
+        g = np.zeros(27)
+        check(lib().shx_adjoint_coef_get(self.sys, g.ctypes.data_as(C.POINTER(C.c_double))), self.ctx.p,
+              "adjoint_coef_get")
+        return g
+
     def device_interior(self, d: int) -> np.ndarray:
         """Interior stencil coefficients that direction d's device operator
         applies (those of the assembly unless replaced by set_interior)."""

@@ -190,6 +190,14 @@ _SIGS = [
     ("shx_batch_capture", C.c_int, [C.c_void_p, C.c_int]),
     ("shx_batch_launch_async", C.c_int, [C.c_void_p]),
     ("shx_batch_times", C.c_int, [C.c_void_p, dp]),
+    ("shx_rk4_fused_keep_async", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField),
+                                           C.POINTER(ShxField), C.POINTER(ShxField), C.POINTER(ShxField),
+                                           C.c_double, C.c_double]),
+    ("shx_rk4_adjoint_stages_async", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.POINTER(ShxField), C.c_double,
+                                               C.c_int]),
+    ("shx_adjoint_prepare", C.c_int, [C.c_void_p]),
+    ("shx_adjoint_coef_set", C.c_int, [C.c_void_p, dp]),
+    ("shx_adjoint_coef_get", C.c_int, [C.c_void_p, dp]),
     ("shx_rk4_graph_create", C.c_int, [C.c_void_p, C.POINTER(ShxField), C.c_double, C.c_double, C.c_int,
                                        C.POINTER(C.c_void_p)]),
     ("shx_rk4_graph_launch", C.c_int, [C.c_void_p]),

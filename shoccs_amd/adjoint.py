@@ -3,10 +3,11 @@
 J(u0, coef) = 1/2 ||u(T) - u_obs||^2 over every entry of the state (D, Rx, Ry,
 Rz), where u(T) is nsteps fused FP64 RK4 steps of a system from u0 with the
 interior stencil coefficients `coef` of its three operators.  The gradient
-comes from the discrete adjoint of the step (shx_rk4_adjoint): the forward
-run keeps a checkpoint every `every` steps; the backward run replays each
-segment from its checkpoint, keeping the segment's states, and runs the step
-adjoints in reverse.  No reference counterpart (the reference has no
+comes from the discrete adjoint of the step: the forward run keeps a
+checkpoint every `every` steps; the backward run replays each segment from
+its checkpoint, keeping every step's input and stage inputs
+(shx_rk4_fused_keep_async), and runs the step adjoints in reverse over them
+(shx_rk4_adjoint_stages_async).  No reference counterpart (the reference has no
 adjoint).
 
 `Solve` works on numpy arrays at its boundary; `solve_fn` wraps it as a
@@ -26,7 +27,11 @@ class Solve:
         self.p = problem
         self.t0, self.dt, self.nsteps, self.every = float(t0), float(dt), int(nsteps), int(every)
         self._ckpt = []  # (step, Field)
+        self._seg = []  # the backward sweep's segment buffers
         self._work = problem.field()
+        # the transposed operators are assembled once, with the solver (as the
+        # forward operators are with the problem)
+        problem.adjoint_prepare()
 
     # ---------------------------------------------------------- helpers
     def set_coef(self, coef: np.ndarray) -> None:
@@ -75,34 +80,42 @@ class Solve:
     # --------------------------------------------------------- backward
     def gradient(self, uT_bar: np.ndarray, with_coef: bool = True):
         """dJ/du0 and dJ/dcoef (3 x 9, per direction's stored interior
-        coefficients) given dJ/du(T), after `run`."""
+        coefficients) given dJ/du(T), after `run`.  Each segment is replayed
+        from its checkpoint keeping every step's input and stage inputs
+        (shx_rk4_fused_keep_async), then the step adjoints run in reverse over
+        them (shx_rk4_adjoint_stages_async), all on the device; the
+        coefficient gradient is read back once."""
         if len(self._ckpt) * self.every < self.nsteps:
             raise RuntimeError("gradient() needs a preceding run()")
         ubar = self.p.field()
         ubar.upload(uT_bar)
-        grad = np.zeros(27) if with_coef else None
-        states = []
+        if with_coef:
+            self.p.adjoint_coef_set(np.zeros(27))
+        seg = self._seg  # per step of a segment: (input, w1, w2, w3), kept for later calls
         try:
             for s0, c in reversed(self._ckpt):
                 n = min(self.every, self.nsteps - s0)
-                # replay the segment: states[k] = u at step s0 + k
-                while len(states) < n:
-                    states.append(self.p.field())
-                states[0].copy_from(c)
-                for k in range(1, n):
-                    states[k].copy_from(states[k - 1])
-                    self.p.rk4_fused(states[k], self._time(s0 + k - 1), self.dt, 1)
+                while len(seg) < n:
+                    seg.append(tuple(self.p.field() for _ in range(4)))
+                seg[0][0].copy_from(c)
+                for k in range(n):
+                    u, w1, w2, w3 = seg[k]
+                    nxt = seg[k + 1][0] if k + 1 < n else self._work
+                    self.p.rk4_fused_keep(u, w1, w2, w3, nxt, self._time(s0 + k), self.dt)
                 for k in range(n - 1, -1, -1):
-                    self.p.rk4_adjoint(states[k], ubar, self._time(s0 + k), self.dt, grad)
+                    self.p.rk4_adjoint_stages(seg[k], ubar, self.dt, with_coef)
             g_u0 = ubar.download()
+            grad = self.p.adjoint_coef_get().reshape(3, 9) if with_coef else None
         finally:
             ubar.free()
-            for f in states:
-                f.free()
-        return g_u0, (grad.reshape(3, 9) if with_coef else None)
+        return g_u0, grad
 
     def close(self):
         self._free_ckpt()
+        for fs in self._seg:
+            for f in fs:
+                f.free()
+        self._seg = []
         self._work.free()
 
 

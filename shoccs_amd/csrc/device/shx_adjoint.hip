@@ -39,17 +39,35 @@ struct shx_deriv_T {
     int* b_ptr = nullptr;  // per cut point of the direction
     int* b_row = nullptr;
     double* b_w = nullptr;
+    // host copies of the irregular columns' lists (the system merges the
+    // three directions' lists into one)
+    std::vector<int> h_irr_idx, h_irr_ptr, h_irr_row;
+    std::vector<double> h_irr_w;
+    // host copies of the closure rows' entries (column, row, weight): with the
+    // circulant rows, whose columns every point can form by itself, they make
+    // up O; the system merges the three directions' entries by column
+    std::vector<int> h_cl_col, h_cl_row;
+    std::vector<double> h_cl_w;
 };
 
-// Adjoint workspace of a system: stage inputs w1..w3, the stage-RHS scratch,
-// the adjoint registers and the per-direction weighted adjoints.
+// Adjoint workspace of a system: stage inputs w1..w3, the stage-RHS scratch
+// and the adjoint registers.
 struct shx_adjoint_ws {
     shx_field w[3];
     shx_field k, g, kb, wb;
-    double* ybar[3] = {nullptr, nullptr, nullptr};
-    double* part = nullptr;  // coefficient-gradient partials
-    double* gacc = nullptr;  // coefficient gradient accumulated on the device, 3 x 9
+    double* part = nullptr;  // coefficient-gradient partials, 15 per workgroup, accumulated
+    double* gacc = nullptr;  // coefficient gradient on the device, 3 x 9 (partials added on read)
     long npart = 0;
+    int coef_w = 0;          // stencil width of the accumulated partials
+    // the irregular columns of the three directions in one list: column,
+    // entries (row, weight, direction), directions ascending then rows
+    long nirr = 0;
+    int* irr_col = nullptr;
+    int* irr_ptr = nullptr;
+    int* irr_row = nullptr;
+    double* irr_w = nullptr;
+    int8_t* irr_dir = nullptr;
+    bool lists_ready = false;
 };
 
 namespace {
@@ -107,119 +125,238 @@ __global__ void k_lc(long n, const double* __restrict__ a, const double* __restr
     if (e < n) out[e] = a[e] + c * b[e];
 }
 
-// out = c a
-__global__ void k_scale(long n, double c, const double* __restrict__ a, double* __restrict__ out)
+// ---------------------------------------------- fused transposed rhs (all directions)
+// The rhs weight of direction c at mesh point (i, j, k), by which the
+// adjoint kbar is multiplied before that direction's transpose: heat k,
+// scalar wave gG_c; zero on Dirichlet grid faces (heat.cpp:119-162,
+// scalar_wave.cpp:208-238).
+__device__ __forceinline__ double adj_weight(const DevSys& s, int c, int i, int j, int k)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) out[e] = c * a[e];
+    if (on_dirichlet_face(s, i, j, k)) return 0.0;
+    if (s.kind == SHX_SYS_HEAT) return s.k;
+    if (s.kind == SHX_SYS_WAVE_LINEAR) return -s.vel[c];
+    return wave_neg_g(s, c, s.x[i], s.y[j], s.z[k]);
 }
 
-// out = ca a + cb b
-__global__ void k_lc2(long n, double ca, const double* __restrict__ a, double cb,
-                      const double* __restrict__ b, double* __restrict__ out)
+// Per direction of the fused transposes: the stored interior stencil, the
+// stride, the extent and the mask of the points whose row is a circulant row.
+struct AdjDir {
+    const double* coef;
+    long st;
+    const unsigned* intr;
+    int present;
+};
+struct AdjDirs {
+    AdjDir d[3];
+};
+
+// Dirichlet grid face of axis a at coordinate q along it.
+__device__ __forceinline__ bool face_of(const DevSys& s, int a, int q, int n)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) out[e] = ca * a[e] + cb * b[e];
+    return (s.bc[2 * a] == SHX_BC_DIRICHLET && q == 0) || (s.bc[2 * a + 1] == SHX_BC_DIRICHLET && q == n - 1);
 }
 
-// x += y
-__global__ void k_add(long n, const double* __restrict__ y, double* __restrict__ x)
+// The circulant-row part of (d rhs / d u)^T kbar at every column q: for each
+// direction c (in order) and tap m (in stencil order), the row p = q + (hw -
+// m) st_c contributes coef_c[m] y_c[p] when it is a circulant row, with y_c =
+// weight_c kbar (zero weight on Dirichlet faces).  The closure rows and the B
+// rows follow from k_adj_lists.  With GRAD, the same products give the
+// coefficient gradient: row p's term of coefficient m reads w at column q, so
+// y_c[p] w[q] is added to value c W + m; per workgroup the values are summed
+// in a fixed order and added to the workgroup's slot of `part` (one writer
+// per slot and launch: deterministic).
+template <int W, bool GRAD>
+__global__ __launch_bounds__(kT) void k_rhsT(DevSys s, AdjDirs ad, long N, const double* __restrict__ kbar,
+                                             double* __restrict__ wbar, const double* __restrict__ w,
+                                             double* __restrict__ part)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) x[e] += y[e];
-}
-
-__global__ void k_zero_list(long n, const int* __restrict__ idx, double* __restrict__ x)
-{
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) x[idx[e]] = 0.0;
-}
-
-__global__ void k_zero_flag(long n, const signed char* __restrict__ flag, double* __restrict__ x)
-{
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n && flag[e]) x[e] = 0.0;
-}
-
-// The rhs's derivative weights applied to an adjoint: for direction c,
-// ybar[c] = d rhs / d (D_c u) * kbar, zero on Dirichlet faces.  Scalar wave:
-// gG_c (scalar_wave.cpp:28-33 and the constant-velocity form); heat: k, the
-// same for every direction (heat.cpp:119-162).
-template <bool HEAT>
-__global__ void k_adj_weights(DevSys s, long N, const double* __restrict__ kbar, double* __restrict__ y0,
-                              double* __restrict__ y1, double* __restrict__ y2)
-{
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= N) return;
-    int i, j, k;
-    point_coords(idx, s.n1, s.n2, i, j, k);
-    const bool face = on_dirichlet_face(s, i, j, k);
-    const double kb = kbar[idx];
-    if (HEAT) {
-        y0[idx] = face ? 0.0 : s.k * kb;
-        return;
+    constexpr int hw = W / 2, NV = 3 * W;
+    static_assert(NV * 16 <= kT, "one thread per partial sum");
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = q < N;
+    double g[GRAD ? NV : 1];
+    if constexpr (GRAD)
+#pragma unroll
+        for (int v = 0; v < NV; ++v) g[v] = 0.0;
+    if (in) {
+        int ijk[3];
+        point_coords(q, s.n1, s.n2, ijk[0], ijk[1], ijk[2]);
+        const int nn[3] = {s.n0, s.n1, s.n2};
+        bool fa[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) fa[a] = face_of(s, a, ijk[a], nn[a]);
+        const double wq = GRAD ? w[q] : 0.0;
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const AdjDir& a = ad.d[c];
+            if (!a.present) continue;
+            // a row lies on a Dirichlet face when the point does along the
+            // other axes, or when its own coordinate along c does
+            const bool other = (c != 0 && fa[0]) || (c != 1 && fa[1]) || (c != 2 && fa[2]);
+            double sc = 0.0;
+#pragma unroll
+            for (int m = 0; m < W; ++m) {
+                const int o = hw - m;
+                const int qc = ijk[c] + o;
+                if (qc < 0 || qc >= nn[c]) continue;
+                const long p = q + (long)o * a.st;
+                if (!bit(a.intr, p)) continue;
+                double wgt;
+                if (other || face_of(s, c, qc, nn[c])) wgt = 0.0;
+                else if (s.kind == SHX_SYS_HEAT) wgt = s.k;
+                else if (s.kind == SHX_SYS_WAVE_LINEAR) wgt = -s.vel[c];
+                else wgt = adj_weight(s, c, ijk[0] + (c == 0 ? o : 0), ijk[1] + (c == 1 ? o : 0), ijk[2] + (c == 2 ? o : 0));
+                const double y = wgt * kbar[p];
+                sc += a.coef[m] * y;
+                if constexpr (GRAD) g[c * W + m] += y * wq;
+            }
+            acc += sc;
+        }
+        wbar[q] = acc;
     }
-    double g[3] = {0.0, 0.0, 0.0};
-    if (!face) {
-        const double x = s.x[i], y = s.y[j], z = s.z[k];
-        for (int c = 0; c < 3; ++c) g[c] = wave_neg_g(s, c, x, y, z);
-    }
-    if (y0) y0[idx] = g[0] * kb;
-    if (y1) y1[idx] = g[1] * kb;
-    if (y2) y2[idx] = g[2] * kb;
-}
-
-// Per-workgroup partial sums, over the points whose row is a circulant row,
-// of ybar[p] w[p + (m - hw) stride], m < W.
-template <int W>
-__global__ __launch_bounds__(kT) void k_coef_part(long N, long st, const unsigned* __restrict__ introw,
-                                                  const double* __restrict__ ybar,
-                                                  const double* __restrict__ w, double* __restrict__ part)
-{
-    constexpr int hw = W / 2;
-    __shared__ double red[W][kT];
-    double acc[W];
+    if constexpr (GRAD) {
+        // the workgroup's sums in a fixed order: lane values staged in LDS,
+        // 16 partial sums of 16 consecutive lanes per value, then those 16
+        __shared__ double red[NV][kT];
+        __shared__ double red2[NV][16];
 #pragma unroll
-    for (int m = 0; m < W; ++m) acc[m] = 0.0;
-    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < N; p += (long)gridDim.x * blockDim.x) {
-        if (!bit(introw, p)) continue;
-        const double yb = ybar[p];
+        for (int v = 0; v < NV; ++v) red[v][threadIdx.x] = g[v];
+        __syncthreads();
+        if ((int)threadIdx.x < NV * 16) {
+            const int v = threadIdx.x / 16, q0 = (threadIdx.x % 16) * 16;
+            double sum = 0.0;
 #pragma unroll
-        for (int m = 0; m < W; ++m) acc[m] += yb * w[p + (long)(m - hw) * st];
-    }
-#pragma unroll
-    for (int m = 0; m < W; ++m) red[m][threadIdx.x] = acc[m];
-    __syncthreads();
-    for (int h = kT / 2; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) {
-#pragma unroll
-            for (int m = 0; m < W; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + h];
+            for (int r = 0; r < 16; ++r) sum += red[v][q0 + r];
+            red2[v][threadIdx.x % 16] = sum;
         }
         __syncthreads();
+        if ((int)threadIdx.x < NV) {
+            double sum = 0.0;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) sum += red2[threadIdx.x][r];
+            part[(long)blockIdx.x * NV + threadIdx.x] += sum;
+        }
     }
-    if (threadIdx.x < W) part[(long)blockIdx.x * W + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// acc[m] += (sum over workgroups of part[b][m], from 0.0 in workgroup order).
-// The partials are staged through LDS a chunk at a time by the whole
-// workgroup (coalesced loads in flight together); lane m then adds its
-// column of the chunk in order, so the sum is the sequential one.
-constexpr int kReduceChunk = 256;
-__global__ __launch_bounds__(kT) void k_coef_reduce(int nb, int W, const double* __restrict__ part,
-                                                    double* __restrict__ acc)
+// The listed rows in one launch: items [0, ncol) are the columns read by
+// closure rows (wbar.D[col] += their entries' w y_dir[row], directions
+// ascending), then the cut points of R(0), R(1), R(2) (wbar.R[c][e] = B^T
+// entries' sum).
+struct AdjLists {
+    long ncol;
+    const int* col;
+    const int* ptr;
+    const int* row;
+    const double* w;
+    const int8_t* dir;
+    long off[4];  // cut points of R(c): items ncol + [off[c], off[c + 1])
+    const int* bptr[3];
+    const int* brow[3];
+    const double* bw[3];
+    double* xR[3];
+};
+__global__ void k_adj_lists(DevSys s, AdjLists L, const double* __restrict__ kbar, double* __restrict__ wbar)
 {
-    __shared__ double buf[kReduceChunk * 9];
-    const int m = threadIdx.x;
-    double s = 0.0;
-    for (int b0 = 0; b0 < nb; b0 += kReduceChunk) {
-        const int nc = min(kReduceChunk, nb - b0);
-        for (int e = threadIdx.x; e < nc * W; e += blockDim.x) buf[e] = part[(long)b0 * W + e];
-        __syncthreads();
-        if (m < W)
-            for (int b = 0; b < nc; ++b) s += buf[b * W + m];
-        __syncthreads();
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= L.ncol + L.off[3]) return;
+    double sum = 0.0;
+    if (g < L.ncol) {
+        for (int t = L.ptr[g]; t < L.ptr[g + 1]; ++t) {
+            const long r = L.row[t];
+            int i, j, k;
+            point_coords(r, s.n1, s.n2, i, j, k);
+            sum += L.w[t] * (adj_weight(s, L.dir[t], i, j, k) * kbar[r]);
+        }
+        wbar[L.col[g]] += sum;
+        return;
     }
-    if (m < W) acc[m] += s;
+    const long gr = g - L.ncol;
+    const int c = gr < L.off[1] ? 0 : gr < L.off[2] ? 1 : 2;
+    const long e = gr - L.off[c];
+    const int* ptr = c == 0 ? L.bptr[0] : c == 1 ? L.bptr[1] : L.bptr[2];
+    const int* row = c == 0 ? L.brow[0] : c == 1 ? L.brow[1] : L.brow[2];
+    const double* w = c == 0 ? L.bw[0] : c == 1 ? L.bw[1] : L.bw[2];
+    double* x = c == 0 ? L.xR[0] : c == 1 ? L.xR[1] : L.xR[2];
+    if (ptr)
+        for (int t = ptr[e]; t < ptr[e + 1]; ++t) {
+            const long r = row[t];
+            int i, j, k;
+            point_coords(r, s.n1, s.n2, i, j, k);
+            sum += w[t] * (adj_weight(s, c, i, j, k) * kbar[r]);
+        }
+    x[e] = sum;
+}
+
+// gacc[9 c + m] += sum over the nb workgroup slots of value c W + m (fixed
+// order: lane l sums slots l, l + 64, ..., then a butterfly), and the slots
+// are cleared; one wave per value.
+__global__ void k_coef_reduce3(int nb, int W, double* __restrict__ part, double* __restrict__ gacc)
+{
+    const int v = blockIdx.x, lane = threadIdx.x;
+    double sum = 0.0;
+    for (int b = lane; b < nb; b += 64) {
+        sum += part[(long)b * 3 * W + v];
+        part[(long)b * 3 * W + v] = 0.0;
+    }
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (lane == 0) gacc[9 * (v / W) + v % W] += sum;
+}
+
+// A field's D and R parts as one index space of N + nR[0] + nR[1] + nR[2]
+// items (the R sets in order).
+struct FieldParts {
+    long N;
+    long off[4];  // R(c): items N + [off[c], off[c + 1])
+    const signed char* dirich[3];
+};
+struct PartPtr {
+    double* D;
+    double* R[3];
+};
+// item g of the parts: the element pointer, and whether P zeroes it
+// (Dirichlet grid face of D, Dirichlet cut point)
+__device__ __forceinline__ double* part_at(const DevSys& s, const FieldParts& f, const PartPtr& p, long g, bool& zero)
+{
+    if (g < f.N) {
+        int i, j, k;
+        point_coords(g, s.n1, s.n2, i, j, k);
+        zero = on_dirichlet_face(s, i, j, k);
+        return p.D + g;
+    }
+    const long gr = g - f.N;
+    const int c = gr < f.off[1] ? 0 : gr < f.off[2] ? 1 : 2;
+    const long e = gr - f.off[c];
+    zero = (c == 0 ? f.dirich[0] : c == 1 ? f.dirich[1] : f.dirich[2])[e] != 0;
+    return (c == 0 ? p.R[0] : c == 1 ? p.R[1] : p.R[2]) + e;
+}
+
+// End of a backward stage: pw = P wbar, ubar += pw, kb = ca g + cb pw (P is
+// the identity when proj is false: the last stage, whose kb is not used).
+__global__ void k_adj_stage_end(DevSys s, FieldParts f, PartPtr wbar, PartPtr g, PartPtr ubar, PartPtr kb, double ca,
+                                double cb, int proj)
+{
+    const long gi = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= f.N + f.off[3]) return;
+    bool zero;
+    const double* wp = part_at(s, f, wbar, gi, zero);
+    const double pw = proj && zero ? 0.0 : *wp;
+    *part_at(s, f, ubar, gi, zero) += pw;
+    if (proj) *part_at(s, f, kb, gi, zero) = ca * *part_at(s, f, g, gi, zero) + cb * pw;
+}
+
+// Start of a step's backward sweep: g = P ubar, ubar = g, kb = c g.
+__global__ void k_adj_step_begin(DevSys s, FieldParts f, PartPtr ubar, PartPtr g, PartPtr kb, double c)
+{
+    const long gi = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= f.N + f.off[3]) return;
+    bool zero;
+    double* up = part_at(s, f, ubar, gi, zero);
+    const double v = zero ? 0.0 : *up;
+    *up = v;
+    *part_at(s, f, g, gi, zero) = v;
+    *part_at(s, f, kb, gi, zero) = c * v;
 }
 
 // ------------------------------------------------------------ transpose build
@@ -300,7 +437,7 @@ int build_transpose(shx_derivative* d)
         long row;
         double w;
     };
-    std::vector<Ent> ent, bent;
+    std::vector<Ent> ent, bent, cl;
     for (const shx_line_block& b : d->h_blocks) {
         const long base = line_base(d, b.pair);
         const int total = b.nl + b.ni + b.nr;
@@ -315,7 +452,10 @@ int build_transpose(shx_derivative* d)
                 int col0, ncols;
                 const double* a;
                 closure(b, lr, col0, ncols, a);
-                for (int jj = 0; jj < ncols; ++jj) ent.push_back({base + (long)(col0 + jj) * st, p, a[jj]});
+                for (int jj = 0; jj < ncols; ++jj) {
+                    ent.push_back({base + (long)(col0 + jj) * st, p, a[jj]});
+                    cl.push_back({base + (long)(col0 + jj) * st, p, a[jj]});
+                }
             }
             if (b.lobj >= 0 && lr < b.nl) bent.push_back({b.lobj, p, d->h_pool[(size_t)b.lbw + lr]});
             if (b.robj >= 0 && lr >= total - b.nr)
@@ -358,6 +498,15 @@ int build_transpose(shx_derivative* d)
 
     auto* T = new (std::nothrow) shx_deriv_T();
     if (!T) return SHX_ENOMEM;
+    for (const Ent& x : cl) {
+        T->h_cl_col.push_back((int)x.col);
+        T->h_cl_row.push_back((int)x.row);
+        T->h_cl_w.push_back(x.w);
+    }
+    T->h_irr_idx = irr_idx;
+    T->h_irr_ptr = irr_ptr;
+    T->h_irr_row = irr_row;
+    T->h_irr_w = irr_w;
     T->N = N;
     T->nR = nR;
     T->nirr = (long)irr_idx.size();
@@ -427,14 +576,12 @@ int ensure_ws(shx_system* s)
     for (int i = 0; i < 3 && rc == SHX_OK; ++i) rc = alloc_field(s, w->w[i]);
     for (shx_field* f : {&w->k, &w->g, &w->kb, &w->wb})
         if (rc == SHX_OK) rc = alloc_field(s, *f);
-    for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
-        hipError_t e = hipMalloc((void**)&w->ybar[c], sizeof(double) * (size_t)std::max<long>(s->N, 1));
-        if (e != hipSuccess) rc = hip_fail(s->ctx, e, "adjoint workspace");
-    }
     if (rc == SHX_OK) {
-        w->npart = std::min<long>(2048, std::max<long>(1, (s->N + kT - 1) / kT));
-        hipError_t e = hipMalloc((void**)&w->part, sizeof(double) * (size_t)w->npart * 9);
+        w->npart = std::max<long>(1, (s->N + kT - 1) / kT);  // one slot per workgroup of k_rhsT
+        hipError_t e = hipMalloc((void**)&w->part, sizeof(double) * (size_t)w->npart * 27);
         if (e == hipSuccess) e = hipMalloc((void**)&w->gacc, sizeof(double) * 27);
+        if (e == hipSuccess) e = hipMemset(w->part, 0, sizeof(double) * (size_t)w->npart * 27);
+        if (e == hipSuccess) e = hipMemset(w->gacc, 0, sizeof(double) * 27);
         if (e != hipSuccess) rc = hip_fail(s->ctx, e, "adjoint workspace");
     }
     if (rc != SHX_OK) {
@@ -453,83 +600,163 @@ void field_lc(shx_system* s, const shx_field& a, const shx_field& b, double c, s
         if (s->nR[r]) k_lc<<<blocks_for(s->nR[r]), kT, 0, st>>>(s->nR[r], a.R[r], b.R[r], c, out.R[r]);
 }
 
-void field_lc2(shx_system* s, double ca, const shx_field& a, double cb, const shx_field& b, shx_field& out)
-{
-    hipStream_t st = s->ctx->stream;
-    k_lc2<<<blocks_for(s->N), kT, 0, st>>>(s->N, ca, a.D, cb, b.D, out.D);
-    for (int r = 0; r < 3; ++r)
-        if (s->nR[r]) k_lc2<<<blocks_for(s->nR[r]), kT, 0, st>>>(s->nR[r], ca, a.R[r], cb, b.R[r], out.R[r]);
-}
-
-void field_scale(shx_system* s, double c, const shx_field& a, shx_field& out)
-{
-    hipStream_t st = s->ctx->stream;
-    k_scale<<<blocks_for(s->N), kT, 0, st>>>(s->N, c, a.D, out.D);
-    for (int r = 0; r < 3; ++r)
-        if (s->nR[r]) k_scale<<<blocks_for(s->nR[r]), kT, 0, st>>>(s->nR[r], c, a.R[r], out.R[r]);
-}
-
-void field_add(shx_system* s, const shx_field& y, shx_field& x)
-{
-    hipStream_t st = s->ctx->stream;
-    k_add<<<blocks_for(s->N), kT, 0, st>>>(s->N, y.D, x.D);
-    for (int r = 0; r < 3; ++r)
-        if (s->nR[r]) k_add<<<blocks_for(s->nR[r]), kT, 0, st>>>(s->nR[r], y.R[r], x.R[r]);
-}
-
-// P: zero the entries update_boundary overwrites (Dirichlet grid faces of D,
-// Dirichlet cut points)
-void project(shx_system* s, shx_field& f)
-{
-    hipStream_t st = s->ctx->stream;
-    if (s->nfaces) k_zero_list<<<blocks_for(s->nfaces), kT, 0, st>>>(s->nfaces, s->faces, f.D);
-    for (int r = 0; r < 3; ++r)
-        if (s->nR[r]) k_zero_flag<<<blocks_for(s->nR[r]), kT, 0, st>>>(s->nR[r], s->rdir[r], f.R[r]);
-}
-
-// wbar = (d rhs / d u)^T kbar, and, when gacc (device, 3 x 9) is given, the
-// stage's coefficient-gradient terms with stage input w added to
-// gacc[9 c + m]
-int rhs_T(shx_system* s, const shx_field& kbar, shx_field& wbar, const shx_field* w, double* gacc)
+// The closure-row entries of the system's directions merged by column
+// (built once, after the directions' transposes).
+int ensure_lists(shx_system* s)
 {
     shx_adjoint_ws* ws = s->adj;
-    hipStream_t st = s->ctx->stream;
-    const bool heat = s->kind == SHX_SYS_HEAT;
-    if (heat)
-        k_adj_weights<true><<<blocks_for(s->N), kT, 0, st>>>(s->dev, s->N, kbar.D, ws->ybar[0], nullptr, nullptr);
-    else
-        k_adj_weights<false><<<blocks_for(s->N), kT, 0, st>>>(s->dev, s->N, kbar.D, s->d[0] ? ws->ybar[0] : nullptr,
-                                                            s->d[1] ? ws->ybar[1] : nullptr,
-                                                            s->d[2] ? ws->ybar[2] : nullptr);
-    int rc = check(s->ctx, "adjoint weights");
-    bool first = true;
-    for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
-        if (s->nR[c]) k_fill0<<<blocks_for(s->nR[c]), kT, 0, st>>>(s->nR[c], wbar.R[c]);
+    if (ws->lists_ready) return SHX_OK;
+    struct Ent {
+        int col, c, row;
+        double w;
+    };
+    std::vector<Ent> ent;
+    for (int c = 0; c < 3; ++c) {
         shx_derivative* d = s->d[c];
-        if (!d || !d->dev.present) continue;
-        const double* y = heat ? ws->ybar[0] : ws->ybar[c];
-        rc = apply_T(d, y, wbar.D, wbar.R[c], !first);
-        first = false;
-        if (rc != SHX_OK || !gacc || !w) continue;
-        const int W = d->dev.width;
-        const int nb = (int)ws->npart;
-        switch (W) {
-        case 3: k_coef_part<3><<<nb, kT, 0, st>>>(s->N, d->dev.stride, d->T->int_mask, y, w->D, ws->part); break;
-        case 5: k_coef_part<5><<<nb, kT, 0, st>>>(s->N, d->dev.stride, d->T->int_mask, y, w->D, ws->part); break;
-        default: return fail(s->ctx, SHX_EUNSUPPORTED, "adjoint: coefficient gradient for widths 3 and 5");
+        if (!d || !d->dev.present || !d->T) continue;
+        const shx_deriv_T* T = d->T;
+        for (size_t e = 0; e < T->h_cl_col.size(); ++e) ent.push_back({T->h_cl_col[e], c, T->h_cl_row[e], T->h_cl_w[e]});
+    }
+    // by column, then direction; rows ascending within (stable: the entries
+    // of a direction come row by row)
+    std::stable_sort(ent.begin(), ent.end(),
+                     [](const Ent& a, const Ent& b) { return a.col != b.col ? a.col < b.col : a.c < b.c; });
+    std::vector<int> col, ptr{0}, row;
+    std::vector<double> w;
+    std::vector<int8_t> dir;
+    for (size_t e = 0; e < ent.size(); ++e) {
+        if (e == 0 || ent[e].col != ent[e - 1].col) {
+            if (e) ptr.push_back((int)row.size());
+            col.push_back(ent[e].col);
         }
-        k_coef_reduce<<<1, kT, 0, st>>>(nb, W, ws->part, gacc + 9 * c);
-        rc = check(s->ctx, "adjoint: coefficient gradient");
+        row.push_back(ent[e].row);
+        w.push_back(ent[e].w);
+        dir.push_back((int8_t)ent[e].c);
     }
-    if (rc == SHX_OK && first) {  // no direction: the rhs does not depend on u
-        k_fill0<<<blocks_for(s->N), kT, 0, st>>>(s->N, wbar.D);
-        rc = check(s->ctx, "adjoint");
-    }
+    if (!ent.empty()) ptr.push_back((int)row.size());
+    ws->nirr = (long)col.size();
+    int rc = up(s->ctx, &ws->irr_col, col);
+    if (rc == SHX_OK) rc = up(s->ctx, &ws->irr_ptr, ptr);
+    if (rc == SHX_OK) rc = up(s->ctx, &ws->irr_row, row);
+    if (rc == SHX_OK) rc = up(s->ctx, &ws->irr_w, w);
+    if (rc == SHX_OK) rc = up(s->ctx, &ws->irr_dir, dir);
+    ws->lists_ready = rc == SHX_OK;
     return rc;
 }
 
+// wbar = (d rhs / d u)^T kbar, and, when grad is set, the stage's
+// coefficient-gradient terms with stage input w accumulated in the partials:
+// one pass over the columns (circulant rows of every direction, and the
+// gradient), one over the listed rows (closure columns, B^T).
+int rhs_T(shx_system* s, const shx_field& kbar, shx_field& wbar, const shx_field* w, bool grad)
+{
+    shx_adjoint_ws* ws = s->adj;
+    hipStream_t st = s->ctx->stream;
+    AdjDirs ad{};
+    int W = 0;
+    int rc = SHX_OK;
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c) {
+        shx_derivative* d = s->d[c];
+        if (!d || !d->dev.present) continue;
+        rc = build_transpose(d);
+        if (rc != SHX_OK) break;
+        if (W && d->dev.width != W) return fail(s->ctx, SHX_EUNSUPPORTED, "adjoint: directions differ in stencil width");
+        W = d->dev.width;
+        ad.d[c] = AdjDir{d->coef, d->dev.stride, d->T->int_mask, 1};
+    }
+    if (rc != SHX_OK) return rc;
+    for (int c = 0; c < 3; ++c)
+        if (s->nR[c] && !ad.d[c].present) k_fill0<<<blocks_for(s->nR[c]), kT, 0, st>>>(s->nR[c], wbar.R[c]);
+    if (W == 0) {  // no direction: the rhs does not depend on u
+        k_fill0<<<blocks_for(s->N), kT, 0, st>>>(s->N, wbar.D);
+        return check(s->ctx, "adjoint");
+    }
+    if (grad && ws->coef_w && ws->coef_w != W)
+        return fail(s->ctx, SHX_EUNSUPPORTED, "adjoint: stencil width changed");
+    grad = grad && w;
+    if (grad) ws->coef_w = W;
+    const unsigned nb = blocks_for(s->N);
+#define SHX_RHST(WW)                                                                                      \
+    if (grad) k_rhsT<WW, true><<<nb, kT, 0, st>>>(s->dev, ad, s->N, kbar.D, wbar.D, w->D, ws->part);      \
+    else k_rhsT<WW, false><<<nb, kT, 0, st>>>(s->dev, ad, s->N, kbar.D, wbar.D, nullptr, nullptr);
+    switch (W) {
+    case 3: SHX_RHST(3) break;
+    case 5: SHX_RHST(5) break;
+    default: return fail(s->ctx, SHX_EUNSUPPORTED, "adjoint: interior width must be 3 or 5");
+    }
+#undef SHX_RHST
+    rc = ensure_lists(s);
+    if (rc != SHX_OK) return rc;
+    AdjLists L{};
+    L.ncol = ws->nirr;
+    L.col = ws->irr_col;
+    L.ptr = ws->irr_ptr;
+    L.row = ws->irr_row;
+    L.w = ws->irr_w;
+    L.dir = ws->irr_dir;
+    for (int c = 0; c < 3; ++c) {
+        // cut points of an absent direction were zero-filled above
+        L.off[c + 1] = L.off[c] + (ad.d[c].present ? s->nR[c] : 0);
+        if (ad.d[c].present) {
+            const shx_deriv_T* T = s->d[c]->T;
+            L.bptr[c] = T->b_ptr;
+            L.brow[c] = T->b_row;
+            L.bw[c] = T->b_w;
+        }
+        L.xR[c] = ad.d[c].present ? wbar.R[c] : nullptr;
+    }
+    const long items = L.ncol + L.off[3];
+    if (items > 0) k_adj_lists<<<blocks_for(items), kT, 0, st>>>(s->dev, L, kbar.D, wbar.D);
+    return check(s->ctx, "transposed rhs");
+}
+
+// Adds the accumulated coefficient partials to gacc and clears them.
+int coef_flush(shx_system* s)
+{
+    shx_adjoint_ws* ws = s->adj;
+    if (ws->coef_w == 0) return SHX_OK;
+    k_coef_reduce3<<<3 * ws->coef_w, 64, 0, s->ctx->stream>>>((int)ws->npart, ws->coef_w, ws->part, ws->gacc);
+    return check(s->ctx, "adjoint: coefficient gradient");
+}
+
+// The backward sweep of one RK4 step over its stage inputs w[0..3] (w[0] the
+// step's input): on entry ubar = dJ/du_{n+1}, on exit dJ/du_n; the
+// coefficient gradient (when grad is set) accumulates on the device.
+int rk4_backward(shx_system* s, const shx_field* const w[4], shx_field& ubar, double dt, bool grad);
+
 constexpr double kRki[4] = {0.0, 0.5, 0.5, 1.0};
 constexpr double kRkf[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+
+int rk4_backward(shx_system* s, const shx_field* const w[4], shx_field& ubar, double dt, bool grad)
+{
+    shx_adjoint_ws* ws = s->adj;
+    hipStream_t st = s->ctx->stream;
+    FieldParts f{};
+    f.N = s->N;
+    for (int r = 0; r < 3; ++r) {
+        f.off[r + 1] = f.off[r] + s->nR[r];
+        f.dirich[r] = s->rdir[r];
+    }
+    auto pp = [](const shx_field& x) {
+        PartPtr p{x.D, {x.R[0], x.R[1], x.R[2]}};
+        return p;
+    };
+    const unsigned nb = blocks_for(f.N + f.off[3]);
+    // u' = BU(u + sum_s dt rkf[s] k_s): g = P ubar', ubar = g (u enters the
+    // final sum directly), kbar_3 = dt rkf[3] g
+    k_adj_step_begin<<<nb, kT, 0, st>>>(s->dev, f, pp(ubar), pp(ws->g), pp(ws->kb), dt * kRkf[3]);
+    int rc = check(s->ctx, "rk4 adjoint");
+    for (int stg = 3; stg >= 0 && rc == SHX_OK; --stg) {
+        rc = rhs_T(s, ws->kb, ws->wb, w[stg], grad);
+        if (rc != SHX_OK) break;
+        // stg > 0: ubar += P wbar, kbar_{s-1} = dt rkf[s-1] g + dt rki[s] P wbar;
+        // stg = 0: ubar += wbar (kb is not needed after the last stage)
+        const double ca = stg > 0 ? dt * kRkf[stg - 1] : 0.0, cb = stg > 0 ? dt * kRki[stg] : 0.0;
+        k_adj_stage_end<<<nb, kT, 0, st>>>(s->dev, f, pp(ws->wb), pp(ws->g), pp(ubar), pp(ws->kb), ca, cb, stg > 0);
+        rc = check(s->ctx, "rk4 adjoint");
+    }
+    return rc;
+}
 
 }  // namespace
 
@@ -558,9 +785,13 @@ void destroy_adjoint_ws(shx_adjoint_ws* w)
     free_field(w->g);
     free_field(w->kb);
     free_field(w->wb);
-    for (auto& y : w->ybar) hipFree(y);
     hipFree(w->part);
     hipFree(w->gacc);
+    hipFree(w->irr_col);
+    hipFree(w->irr_ptr);
+    hipFree(w->irr_row);
+    hipFree(w->irr_w);
+    hipFree(w->irr_dir);
     delete w;
 }
 
@@ -595,7 +826,7 @@ int shx_system_rhs_T(shx_system* s, const shx_field* kbar, shx_field* wbar)
         return fail(s->ctx, SHX_EUNSUPPORTED,
                     "adjoint: floating/Neumann bodies, Neumann grid faces and host-evaluated solutions are not supported");
     int rc = ensure_ws(s);
-    if (rc == SHX_OK) rc = rhs_T(s, *kbar, *wbar, nullptr, nullptr);
+    if (rc == SHX_OK) rc = rhs_T(s, *kbar, *wbar, nullptr, false);
     return rc == SHX_OK ? sync(s->ctx) : rc;
 }
 
@@ -617,43 +848,69 @@ int shx_rk4_adjoint(shx_system* s, const shx_field* u, shx_field* ubar, double t
         rc = shx_system_update_boundary(s, &ws->w[stg - 1], t + dt * kRki[stg]);
     }
     if (rc != SHX_OK) return rc;
-    // backward: u' = BU(u + sum_s dt rkf[s] k_s), k_s = rhs(w_s)
-    hipStream_t st = s->ctx->stream;
-    auto copy = [&](const shx_field& a, shx_field& b) {
-        hipMemcpyAsync(b.D, a.D, sizeof(double) * (size_t)s->N, hipMemcpyDeviceToDevice, st);
-        for (int c = 0; c < 3; ++c)
-            if (s->nR[c])
-                hipMemcpyAsync(b.R[c], a.R[c], sizeof(double) * (size_t)s->nR[c], hipMemcpyDeviceToDevice, st);
-    };
-    copy(*ubar, ws->g);
-    project(s, ws->g);  // g = P ubar'
-    copy(ws->g, *ubar);  // ubar = g (u enters the final sum directly)
-    field_scale(s, dt * kRkf[3], ws->g, ws->kb);  // kbar_3 = dt rkf[3] g
     // the coefficient gradient accumulates on the device through the four
     // stages (each stage's sum added to the running value, in stage order)
+    hipStream_t st = s->ctx->stream;
     if (coef_grad) {
         const hipError_t e = hipMemcpyAsync(ws->gacc, coef_grad, sizeof(double) * 27, hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return hip_fail(s->ctx, e, "rk4 adjoint: coefficient gradient");
     }
-    rc = check(s->ctx, "rk4 adjoint");
-    for (int stg = 3; stg >= 0 && rc == SHX_OK; --stg) {
-        rc = rhs_T(s, ws->kb, ws->wb, w[stg], coef_grad ? ws->gacc : nullptr);
-        if (rc != SHX_OK) break;
-        if (stg > 0) {
-            project(s, ws->wb);
-            field_add(s, ws->wb, *ubar);
-            // kbar_{s-1} = dt rkf[s-1] g + dt rki[s] P wbar
-            field_lc2(s, dt * kRkf[stg - 1], ws->g, dt * kRki[stg], ws->wb, ws->kb);
-        } else {
-            field_add(s, ws->wb, *ubar);
-        }
-        rc = check(s->ctx, "rk4 adjoint");
-    }
+    rc = rk4_backward(s, w, *ubar, dt, coef_grad != nullptr);
+    if (rc == SHX_OK && coef_grad) rc = coef_flush(s);
     if (rc == SHX_OK && coef_grad) {
         const hipError_t e = hipMemcpyAsync(coef_grad, ws->gacc, sizeof(double) * 27, hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) return hip_fail(s->ctx, e, "rk4 adjoint: coefficient gradient");
     }
     return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
+int shx_rk4_adjoint_stages_async(shx_system* s, const shx_field* w, shx_field* ubar, double dt, int with_coef)
+{
+    if (!s || !w || !ubar) return SHX_EINVAL;
+    if (s->general || s->hm)
+        return fail(s->ctx, SHX_EUNSUPPORTED,
+                    "adjoint: floating/Neumann bodies, Neumann grid faces and host-evaluated solutions are not supported");
+    int rc = ensure_ws(s);
+    if (rc != SHX_OK) return rc;
+    const shx_field* ws[4] = {&w[0], &w[1], &w[2], &w[3]};
+    return rk4_backward(s, ws, *ubar, dt, with_coef != 0);
+}
+
+int shx_adjoint_prepare(shx_system* s)
+{
+    if (!s) return SHX_EINVAL;
+    if (s->general || s->hm)
+        return fail(s->ctx, SHX_EUNSUPPORTED,
+                    "adjoint: floating/Neumann bodies, Neumann grid faces and host-evaluated solutions are not supported");
+    int rc = ensure_ws(s);
+    for (int c = 0; c < 3 && rc == SHX_OK; ++c)
+        if (s->d[c] && s->d[c]->dev.present) rc = build_transpose(s->d[c]);
+    if (rc == SHX_OK) rc = ensure_lists(s);
+    return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
+int shx_adjoint_coef_set(shx_system* s, const double* g)
+{
+    if (!s || !g) return SHX_EINVAL;
+    int rc = ensure_ws(s);
+    if (rc != SHX_OK) return rc;
+    rc = coef_flush(s);  // partials not yet added are dropped by the overwrite
+    if (rc != SHX_OK) return rc;
+    const hipError_t e = hipMemcpyAsync(s->adj->gacc, g, sizeof(double) * 27, hipMemcpyHostToDevice, s->ctx->stream);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "adjoint coefficient gradient");
+    return sync(s->ctx);
+}
+
+int shx_adjoint_coef_get(shx_system* s, double* g)
+{
+    if (!s || !g) return SHX_EINVAL;
+    int rc = ensure_ws(s);
+    if (rc != SHX_OK) return rc;
+    rc = coef_flush(s);
+    if (rc != SHX_OK) return rc;
+    const hipError_t e = hipMemcpyAsync(g, s->adj->gacc, sizeof(double) * 27, hipMemcpyDeviceToHost, s->ctx->stream);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "adjoint coefficient gradient");
+    return sync(s->ctx);
 }
 
 int shx_system_set_interior(shx_system* s, int dir, const double* coef, int width)

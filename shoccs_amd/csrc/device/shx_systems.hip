@@ -1045,6 +1045,10 @@ struct FusedRegs {
     double* stageD[2] = {nullptr, nullptr};
     float* stageF[2] = {nullptr, nullptr};
     double* stageR[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    // when keep[0] is set (FP64): stage s < 3 writes keep[s], which stage s + 1
+    // reads, and stage 3 writes out; the step's input is only read
+    shx_field* keep[3] = {nullptr, nullptr, nullptr};
+    shx_field* out = nullptr;
 };
 
 FusedRegs own_regs(const shx_system* s)
@@ -1063,6 +1067,15 @@ FusedRegs own_regs(const shx_system* s)
 void stage_registers(shx_system* s, const FusedRegs& g, shx_field* u0, int stage, const void*& usV,
                      void*& outV, RPtrs& usR, double* outR[3])
 {
+    if (g.keep[0]) {
+        usV = stage == 0 ? (const void*)u0->D : (const void*)g.keep[stage - 1]->D;
+        outV = stage == 3 ? (void*)g.out->D : (void*)g.keep[stage]->D;
+        for (int c = 0; c < 3; ++c) {
+            usR.r[c] = stage == 0 ? u0->R[c] : g.keep[stage - 1]->R[c];
+            outR[c] = stage == 3 ? g.out->R[c] : g.keep[stage]->R[c];
+        }
+        return;
+    }
     const bool mixed = s->precision == SHX_F32_COMP;
     usV = stage == 0 ? (const void*)u0->D
                      : mixed ? (const void*)g.stageF[(stage - 1) & 1] : (const void*)g.stageD[(stage - 1) & 1];
@@ -1162,6 +1175,25 @@ int rk4_fused_step(shx_system* s, shx_field* u0, double t, double dt, hipEvent_t
     if (rc == SHX_OK && s->variant >= 1) rc = ensure_classes(s);
     if (rc == SHX_OK) rc = rk4_fused_step_on(s, own_regs(s), u0, t, dt, ev);
     return rc;
+}
+
+// One fused FP64 step from u (read only) to out, keeping the stage inputs
+// w1, w2, w3 (the adjoint's linearization points).
+int rk4_fused_keep(shx_system* s, const shx_field* u, shx_field* w1, shx_field* w2, shx_field* w3, shx_field* out,
+                   double t, double dt)
+{
+    if (s->precision != SHX_F64)
+        return fail(s->ctx, SHX_EUNSUPPORTED, "kept stage inputs: FP64 systems only");
+    int rc = fused_supported(s);
+    if (rc == SHX_OK) rc = ensure_fused(s);
+    if (rc == SHX_OK && s->variant >= 1) rc = ensure_classes(s);
+    if (rc != SHX_OK) return rc;
+    FusedRegs g = own_regs(s);
+    g.keep[0] = w1;
+    g.keep[1] = w2;
+    g.keep[2] = w3;
+    g.out = out;
+    return rk4_fused_step_on(s, g, const_cast<shx_field*>(u), t, dt, nullptr);
 }
 
 // The fused registers carved from the per-device workspace, 256-byte aligned.
@@ -1659,6 +1691,13 @@ int shx_rk4_step(shx_system* s, shx_field* u0, shx_field* out, shx_field* rk, sh
     if (!s || !u0 || !out || !rk || !srhs) return SHX_EINVAL;
     int rc = rk4_unfused(s, u0, out, rk, srhs, t, dt);
     return rc == SHX_OK ? sync(s->ctx) : rc;
+}
+
+int shx_rk4_fused_keep_async(shx_system* s, const shx_field* u, shx_field* w1, shx_field* w2, shx_field* w3,
+                             shx_field* out, double t, double dt)
+{
+    if (!s || !u || !w1 || !w2 || !w3 || !out || out == u) return SHX_EINVAL;
+    return rk4_fused_keep(s, u, w1, w2, w3, out, t, dt);
 }
 
 int shx_rk4_fused_steps_async(shx_system* s, shx_field* u0, double t, double dt, int nsteps)
