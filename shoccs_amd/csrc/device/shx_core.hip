@@ -769,6 +769,9 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     if (d->nrrow > 0) {
         d->h_rset.assign(desc->rset, desc->rset + d->nrrow);
         d->h_rrow.assign(desc->rrow, desc->rrow + d->nrrow);
+        d->h_rptr.assign(desc->rptr, desc->rptr + d->nrrow + 1);
+        d->h_rcol.assign(desc->rcol, desc->rcol + d->h_rptr[(size_t)d->nrrow]);
+        d->h_rw.assign(desc->rw, desc->rw + d->h_rptr[(size_t)d->nrrow]);
     }
     d->h_pair_ptr = std::move(pair_ptr);
     for (int m = 0; m < 9; ++m) d->h_coef[m] = desc->interior[m];

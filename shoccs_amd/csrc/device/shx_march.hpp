@@ -284,18 +284,21 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
             if (zslot >= 0) t[ty + hw][zslot] = ld.z;
             __syncthreads();
             double vout[VEC], vacc[VEC];
+            double sxa[VEC], sya[VEC], sza[VEC];
+            unsigned char cl[VEC];
             bool ok[VEC];
             bool all = true;
+            // interior circulant rows of every present direction, for every
+            // lane (values of lanes that do not use them are discarded): dots
+            // from 0.0 in tap order
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
                 const unsigned char c = ld.c[e];
+                cl[e] = c;
                 ok[e] = active && c != kSpecial;
                 all = all && ok[e];
-                const int ke = k + e, ze = zc + e;
+                const int ze = zc + e;
                 auto qv = [&](int m) -> TI { return m < QW - 1 ? q[m][e] : ld.q[e]; };
-                // interior circulant rows of every present direction, for
-                // every lane (values of lanes that do not use them are
-                // discarded): dots from 0.0 in tap order
                 double sx = 0.0, sy = 0.0, sz = 0.0;
                 if constexpr (std::is_same<TI, double>::value) {
 #pragma unroll
@@ -316,12 +319,76 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
                     sy = idot<W>(cy, vy);
                     sz = idot<W>(cz, vz);
                 }
-                // closure rows of a pattern point replace the interior rows
-                // of their directions
-                if (ok[e] && c >= kPattern0) {
-                    const unsigned pd = patdec[c - kPattern0];
-                    const int ix = (int)(pd & 0xffu), iy = (int)((pd >> 8) & 0xffu), iz = (int)(pd >> 16);
-                    if (ix != 0) {
+                sxa[e] = sx;
+                sya[e] = sy;
+                sza[e] = sz;
+            }
+            // closure rows of pattern points replace the interior rows of
+            // their directions; each direction's rows of the lane's points in
+            // one region (skipped by a wave without such a row).  A row of W
+            // columns is summed in a fixed sequence, column by column from
+            // 0.0 like the variable-length sum
+            unsigned pd[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e)
+                pd[e] = ok[e] && cl[e] >= kPattern0 ? patdec[cl[e] - kPattern0] : 0u;
+            {
+                bool any = false;
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) any = any || (pd[e] >> 16) != 0u;
+                if (any) {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) {
+                        const int iz = (int)(pd[e] >> 16);
+                        const ClosureRow& cr = crow[2][iz > 0 ? iz - 1 : 0];
+                        const TI* c0 = &t[ty + hw][zc + e];
+                        double s;
+                        if (a.fixw[2]) {
+                            // lanes without a z row read the interior frame
+                            const int off = iz != 0 ? cr.off : -hw;
+                            s = 0.0;
+#pragma unroll
+                            for (int m = 0; m < W; ++m) s += cr.w[m] * (double)c0[off + m];
+                        } else {
+                            s = tile_row(cr, c0, 1);
+                        }
+                        if (iz != 0) sza[e] = s;
+                    }
+                }
+            }
+            {
+                bool any = false;
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) any = any || ((pd[e] >> 8) & 0xffu) != 0u;
+                if (any) {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) {
+                        const int iy = (int)((pd[e] >> 8) & 0xffu);
+                        const ClosureRow& cr = crow[1][iy > 0 ? iy - 1 : 0];
+                        const TI* c0 = &t[ty + hw][zc + e];
+                        double s;
+                        if (a.fixw[1]) {
+                            const int off = iy != 0 ? cr.off : -hw;
+                            s = 0.0;
+#pragma unroll
+                            for (int m = 0; m < W; ++m) s += cr.w[m] * (double)c0[(off + m) * LZ];
+                        } else {
+                            s = tile_row(cr, c0, LZ);
+                        }
+                        if (iy != 0) sya[e] = s;
+                    }
+                }
+            }
+            {
+                bool any = false;
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) any = any || (pd[e] & 0xffu) != 0u;
+                if (any) {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) {
+                        const int ix = (int)(pd[e] & 0xffu);
+                        if (ix == 0) continue;
+                        auto qv = [&](int m) -> TI { return m < QW - 1 ? q[m][e] : ld.q[e]; };
                         // over the queue when the row's columns lie in it,
                         // else from the stage input (the wall planes of a
                         // chunk, uniform over the workgroup); the row's own
@@ -339,17 +406,20 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
 #pragma unroll 1
                             for (int m = 0; m < cr.n; ++m) s += cr.w[m] * (double)v[(long)m * plane];
                         }
-                        sx = s;
+                        sxa[e] = s;
                     }
-                    if (iy != 0) sy = tile_row(crow[1][iy - 1], &t[ty + hw][ze], LZ);
-                    if (iz != 0) sz = tile_row(crow[2][iz - 1], &t[ty + hw][ze], 1);
                 }
+            }
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const unsigned char c = cl[e];
+                const int ke = k + e;
                 // a solid point (kNoRows) has no rows: zero derivatives, no
                 // source
                 const bool rows = c != kNoRows;
-                const double d0 = px && rows ? sx : 0.0;
-                const double d1 = py && rows ? sy : 0.0;
-                const double d2 = pz && rows ? sz : 0.0;
+                const double d0 = px && rows ? sxa[e] : 0.0;
+                const double d1 = py && rows ? sya[e] : 0.0;
+                const double d2 = pz && rows ? sza[e] : 0.0;
                 double rhs;
                 if (kIsHeat<KIND>) {
                     // laplacian rows added in direction order (plus_eq)
@@ -395,12 +465,17 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
     }
 }
 
-// Row of direction d at a special point from its record (the same sums as
-// row_at: interior rows tap by tap, other rows as listed, B terms after).
+// Row of direction d at a listed point (the same sums as row_at: interior
+// rows tap by tap, other rows as listed, B terms after).  `e` is the row's
+// table entry, fetched by the caller.  The loads of a listed row are issued
+// eight columns at a time before their products are summed in column order
+// (columns past the row's end repeat its last column and are not summed), so
+// that a row costs one round trip to memory rather than one per column; the
+// B-term loads are issued before them.
 template <int W, typename TI>
-__device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRow* __restrict__ rows,
-                                              unsigned ref, const TI* __restrict__ u,
-                                              const double* __restrict__ uR, long idx)
+__device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRow& e, unsigned ref,
+                                              const TI* __restrict__ u, const double* __restrict__ uR,
+                                              long idx)
 {
     RowVal r;
     r.dot = 0.0;
@@ -412,19 +487,41 @@ __device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRo
     double dot = 0.0;
     if (ref == kRowInterior) {
         constexpr int hw = W / 2;
+        double v[W];
 #pragma unroll
-        for (int m = 0; m < W; ++m) dot += d.coef[m] * (double)u[idx + (long)(m - hw) * st];
+        for (int m = 0; m < W; ++m) v[m] = (double)u[idx + (long)(m - hw) * st];
+#pragma unroll
+        for (int m = 0; m < W; ++m) dot += d.coef[m] * v[m];
         r.dot = dot;
         return r;
     }
-    const SpecialRow e = rows[ref];
+    double bw0 = 0.0, bv0 = 0.0, bw1 = 0.0, bv1 = 0.0;
+    if (e.nb > 0) {
+        bw0 = d.pool[e.bw0];
+        bv0 = uR[e.bobj0];
+    }
+    if (e.nb > 1) {
+        bw1 = d.pool[e.bw1];
+        bv1 = uR[e.bobj1];
+    }
     const double* __restrict__ w = d.pool + e.coef;
     const TI* __restrict__ v = u + idx + (long)e.off * st;
-    for (int c = 0; c < e.ncols; ++c) dot += w[c] * (double)v[(long)c * st];
+    for (int c0 = 0; c0 < e.ncols; c0 += 8) {
+        double wv[8], vv[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int c = min(c0 + m, e.ncols - 1);
+            wv[m] = w[c];
+            vv[m] = (double)v[(long)c * st];
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+            if (c0 + m < e.ncols) dot += wv[m] * vv[m];
+    }
     r.dot = dot;
     r.nb = e.nb;
-    if (e.nb > 0) r.b0 = d.pool[e.bw0] * uR[e.bobj0];
-    if (e.nb > 1) r.b1 = d.pool[e.bw1] * uR[e.bobj1];
+    if (e.nb > 0) r.b0 = bw0 * bv0;
+    if (e.nb > 1) r.b1 = bw1 * bv1;
     return r;
 }
 
@@ -434,44 +531,10 @@ __device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRo
 // running sum across the directions, the wave's gradient starts each from
 // zero), the rhs at the cut point (scalar_wave.cpp:208-238: gG . du with gG
 // at the point; heat.cpp:119-162: k lap + source) and the stage update of
-// its value and accumulator (rk4.cpp:27-46), all in FP64.
-// One R-space row, from the running value acc, entries in stored order:
-// column >= 0 names a point of D (stage value), -1 - c cut point c of the
-// row's set.  The loads of eight entries are issued before their products
-// are summed, so a row's gathers overlap; the sum keeps its order.
-template <typename TI>
-__device__ __forceinline__ double cut_row_sum(const long* __restrict__ rptr, const long* __restrict__ rcol,
-                                              const double* __restrict__ rw, long k, const TI* __restrict__ us,
-                                              const double* __restrict__ uR, double acc)
-{
-    long e = rptr[k];
-    const long e1 = rptr[k + 1];
-    for (; e + 8 <= e1; e += 8) {
-        double w[8], v[8];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const long col = rcol[e + m];
-            w[m] = rw[e + m];
-            v[m] = col >= 0 ? (double)us[col] : uR[-1 - col];
-        }
-#pragma unroll
-        for (int m = 0; m < 8; ++m) acc += w[m] * v[m];
-    }
-    for (; e < e1; ++e) {
-        const long col = rcol[e];
-        acc += rw[e] * (col >= 0 ? (double)us[col] : uR[-1 - col]);
-    }
-    return acc;
-}
-
-// A free cut point (of a floating or Neumann body): its rows of the three
-// derivatives (derivative.cpp:479-485: each row from the running value,
-// entries over D then R in stored order; the heat laplacian continues one
-// running sum across the directions, the wave's gradient starts each from
-// zero), the rhs at the cut point (scalar_wave.cpp:208-238: gG . du with gG
-// at the point; heat.cpp:119-162: k lap + source) and the stage update of
-// its value and accumulator (rk4.cpp:27-46), all in FP64.  Pointers are
-// picked by comparisons, never by a run-time array index, so that argument
+// its value and accumulator (rk4.cpp:27-46), all in FP64.  The rows'
+// entries are consecutive in one list; eight entries are loaded, then their
+// values, before their products are summed in order.  Pointers are picked
+// by comparisons, never by a run-time array index, so that argument
 // structures copied to registers stay out of scratch memory.
 template <int KIND, int STAGE, bool MIXED>
 __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScalars& ss, long f)
@@ -482,34 +545,86 @@ __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScala
     const int set = fr.set;
     const double* __restrict__ uR = set == 0 ? a.usR.r[0] : set == 1 ? a.usR.r[1] : a.usR.r[2];
     const double* pos = set == 0 ? a.pos0 : set == 1 ? a.pos1 : a.pos2;
+    const double* u0R = set == 0 ? a.u0R.r[0] : set == 1 ? a.u0R.r[1] : a.u0R.r[2];
     const double x = pos[3 * fr.q], y = pos[3 * fr.q + 1], z = pos[3 * fr.q + 2];
-    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
+    const double u0v = u0R[fr.q];
+    const double accv = STAGE == 0 ? 0.0 : a.accF[f];
+    const long e0 = fr.ent;
+    const long s1 = e0 + fr.n[0], s2 = s1 + fr.n[1], e1 = s2 + fr.n[2];
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;  // heat: r0 is the laplacian's running sum
+    for (long e = e0; e < e1; e += 8) {
+        FreeEnt x8[8];
+        double v[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) x8[m] = a.fents[min(e + m, e1 - 1)];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[m] = x8[m].col >= 0 ? (double)us[x8[m].col] : uR[-1 - x8[m].col];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const long em = e + m;
+            if (em >= e1) continue;
+            const double p = x8[m].w * v[m];
+            if (kIsHeat<KIND> || em < s1) r0 += p;
+            else if (em < s2) r1 += p;
+            else r2 += p;
+        }
+    }
     double rhs;
     if (kIsHeat<KIND>) {
-        double lap = 0.0;
-        if (fr.row[0] >= 0) lap = cut_row_sum(a.cut.rptr[0], a.cut.rcol[0], a.cut.rw[0], fr.row[0], us, uR, lap);
-        if (fr.row[1] >= 0) lap = cut_row_sum(a.cut.rptr[1], a.cut.rcol[1], a.cut.rw[1], fr.row[1], us, uR, lap);
-        if (fr.row[2] >= 0) lap = cut_row_sum(a.cut.rptr[2], a.cut.rcol[2], a.cut.rw[2], fr.row[2], us, uR, lap);
-        rhs = lap * a.s.k;
+        rhs = r0 * a.s.k;
         rhs += heat_source(a.s, ss.t_s, x, y, z);
     } else {
-        if (fr.row[0] >= 0) r0 = cut_row_sum(a.cut.rptr[0], a.cut.rcol[0], a.cut.rw[0], fr.row[0], us, uR, 0.0);
-        if (fr.row[1] >= 0) r1 = cut_row_sum(a.cut.rptr[1], a.cut.rcol[1], a.cut.rw[1], fr.row[1], us, uR, 0.0);
-        if (fr.row[2] >= 0) r2 = cut_row_sum(a.cut.rptr[2], a.cut.rcol[2], a.cut.rw[2], fr.row[2], us, uR, 0.0);
         rhs = wave_neg_g(a.s, 0, x, y, z) * r0 + wave_neg_g(a.s, 1, x, y, z) * r1 +
               wave_neg_g(a.s, 2, x, y, z) * r2;
     }
-    const double* u0R = set == 0 ? a.u0R.r[0] : set == 1 ? a.u0R.r[1] : a.u0R.r[2];
-    const double u0v = u0R[fr.q];
-    const double aa = (STAGE == 0 ? 0.0 : a.accF[f]) + ss.c_acc * rhs;
+    const double aa = accv + ss.c_acc * rhs;
     if (STAGE < 3) a.accF[f] = aa;
     double* outR = set == 0 ? a.outR0 : set == 1 ? a.outR1 : a.outR2;
     outR[fr.q] = STAGE < 3 ? u0v + ss.c_next * rhs : u0v + 1.0 * aa;
 }
 
-// Item q of the special work: a free cut point, a listed point, then the
-// cut points of the three directions.  Rows are evaluated in FP64 in both
-// forms.
+// Point (i, j, k) of face item q: the Dirichlet grid faces in the order
+// xmin, xmax, ymin, ymax, zmin, zmax each hold one item per point of their
+// plane; a point on several of them belongs to the first (false for the
+// items of the others).
+__device__ __forceinline__ bool face_point(const DevSys& s, long q, int& i, int& j, int& k)
+{
+    const int n0 = s.n0, n1 = s.n1, n2 = s.n2;
+    bool earlier[6] = {false, false, false, false, false, false};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        if (s.bc[f] != SHX_BC_DIRICHLET) continue;
+        const long size = f < 2 ? (long)n1 * n2 : f < 4 ? (long)n0 * n2 : (long)n0 * n1;
+        if (q >= size) {
+            q -= size;
+            earlier[f] = true;
+            continue;
+        }
+        if (f < 2) {
+            i = f == 0 ? 0 : n0 - 1;
+            j = (int)(q / n2);
+            k = (int)(q % n2);
+        } else if (f < 4) {
+            j = f == 2 ? 0 : n1 - 1;
+            i = (int)(q / n2);
+            k = (int)(q % n2);
+        } else {
+            k = f == 4 ? 0 : n2 - 1;
+            i = (int)(q / n1);
+            j = (int)(q % n1);
+        }
+        const bool dup = (earlier[0] && i == 0) || (earlier[1] && i == n0 - 1) || (earlier[2] && j == 0) ||
+                         (earlier[3] && j == n1 - 1) || (earlier[4] && k == 0);
+        return !dup;
+    }
+    return false;
+}
+
+// Item q of the special work: a free cut point, a listed point, a Dirichlet
+// grid-face point, then the cut points of the three directions.  Rows are
+// evaluated in FP64 in both forms.  A face point's right-hand side is zero
+// and its value exact(t_next); its accumulator is never read (the point's
+// value is exact at every stage), so only the value is written.
 template <int KIND, int STAGE, int W, bool MIXED, bool FREE = true>
 __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageScalars& ss, long q)
 {
@@ -531,44 +646,52 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
         int i, j, k;
         point_coords(idx, a.s.n1, a.s.n2, i, j, k);
         const DevOps& ops = a.ops;
+        // every load that does not depend on another is issued first: the
+        // stage-update operands and the three rows' table entries
+        const double u0v = a.u0[idx];
+        const double accv = STAGE == 0 ? 0.0 : a.acc[idx];
+        const unsigned f0 = ops.d[0].present ? rec.ref[0] : kRowNone;
+        const unsigned f1 = ops.d[1].present ? rec.ref[1] : kRowNone;
+        const unsigned f2 = ops.d[2].present ? rec.ref[2] : kRowNone;
+        SpecialRow e0{}, e1{}, e2{};
+        if (f0 < kRowInterior) e0 = a.rows0[f0];
+        if (f1 < kRowInterior) e1 = a.rows1[f1];
+        if (f2 < kRowInterior) e2 = a.rows2[f2];
         double rhs;
         if (kIsHeat<KIND>) {
             double lap = 0.0;
-            if (ops.d[0].present) {
-                const RowVal r = special_row<W>(ops.d[0], a.rows0, rec.ref[0], us, a.usR.r[0], idx);
-                if (r.row) lap = row_plus(lap, r);
-            }
-            if (ops.d[1].present) {
-                const RowVal r = special_row<W>(ops.d[1], a.rows1, rec.ref[1], us, a.usR.r[1], idx);
-                if (r.row) lap = row_plus(lap, r);
-            }
-            if (ops.d[2].present) {
-                const RowVal r = special_row<W>(ops.d[2], a.rows2, rec.ref[2], us, a.usR.r[2], idx);
-                if (r.row) lap = row_plus(lap, r);
-            }
+            const RowVal r0 = special_row<W>(ops.d[0], e0, f0, us, a.usR.r[0], idx);
+            if (r0.row) lap = row_plus(lap, r0);
+            const RowVal r1 = special_row<W>(ops.d[1], e1, f1, us, a.usR.r[1], idx);
+            if (r1.row) lap = row_plus(lap, r1);
+            const RowVal r2 = special_row<W>(ops.d[2], e2, f2, us, a.usR.r[2], idx);
+            if (r2.row) lap = row_plus(lap, r2);
             rhs = heat_finish<KIND>(a.s, on_dirichlet_face(a.s, i, j, k), is_fluid(a.s, idx), i, j, k, lap,
-                              ss.t_s);
+                                    ss.t_s);
         } else {
             double du0 = 0.0, du1 = 0.0, du2 = 0.0;
-            if (ops.d[0].present) {
-                const RowVal r = special_row<W>(ops.d[0], a.rows0, rec.ref[0], us, a.usR.r[0], idx);
-                if (r.row) du0 = row_eq(r);
-            }
-            if (ops.d[1].present) {
-                const RowVal r = special_row<W>(ops.d[1], a.rows1, rec.ref[1], us, a.usR.r[1], idx);
-                if (r.row) du1 = row_eq(r);
-            }
-            if (ops.d[2].present) {
-                const RowVal r = special_row<W>(ops.d[2], a.rows2, rec.ref[2], us, a.usR.r[2], idx);
-                if (r.row) du2 = row_eq(r);
-            }
+            const RowVal r0 = special_row<W>(ops.d[0], e0, f0, us, a.usR.r[0], idx);
+            if (r0.row) du0 = row_eq(r0);
+            const RowVal r1 = special_row<W>(ops.d[1], e1, f1, us, a.usR.r[1], idx);
+            if (r1.row) du1 = row_eq(r1);
+            const RowVal r2 = special_row<W>(ops.d[2], e2, f2, us, a.usR.r[2], idx);
+            if (r2.row) du2 = row_eq(r2);
             rhs = wave_combine<KIND>(a.s, on_dirichlet_face(a.s, i, j, k), i, j, k, du0, du1, du2);
         }
-        stage_store<KIND, STAGE>(a.s, rhs, a.u0, a.acc, static_cast<TO*>(a.out), idx, i, j, k, ss.c_acc,
-                                 ss.c_next, ss.t_next);
+        double v = stage_update<STAGE>(rhs, accv, u0v, a.acc, idx, ss.c_acc, ss.c_next);
+        if (on_dirichlet_face(a.s, i, j, k)) v = exact_ijk<KIND>(a.s, i, j, k, ss.t_next);
+        static_cast<TO*>(a.out)[idx] = (TO)v;
         return;
     }
     q -= a.nspecial;
+    if (q < a.nface) {
+        int i, j, k;
+        if (face_point(a.s, q, i, j, k))
+            static_cast<TO*>(a.out)[((long)i * a.s.n1 + j) * a.s.n2 + k] =
+                (TO)exact_ijk<KIND>(a.s, i, j, k, ss.t_next);
+        return;
+    }
+    q -= a.nface;
     if (q < a.nR0) {
         if (a.dir0[q])
             a.outR0[q] = exact_of<KIND>(a.s, a.pos0[3 * q], a.pos0[3 * q + 1], a.pos0[3 * q + 2], ss.t_next);
@@ -731,7 +854,7 @@ template <bool MIXED>
 int launch_stage_form(hipStream_t st, int kind, int stage, int width, int ty, int mode,
                       const MarchArgs& a, const SpecialArgs& sp, const StageScalars& ss)
 {
-    const long nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2 + sp.nfree;
+    const long nitems = sp.nspecial + sp.nface + sp.nR0 + sp.nR1 + sp.nR2 + sp.nfree;
     if (a.nblocks == 0 && nitems == 0) return 0;
     switch (kind * 10 + width) {
     case SHX_SYS_HEAT * 10 + 5: return stage_of<SHX_SYS_HEAT, 5, MIXED>(st, stage, ty, mode, a, sp, ss, nitems);

@@ -50,8 +50,10 @@ struct shx_derivative {
     std::vector<shx_line_block> h_blocks;
     std::vector<double> h_pool;
     std::vector<int> h_pair_ptr;
-    std::vector<int> h_rset;   // R-space rows: set and row of each
-    std::vector<long> h_rrow;
+    std::vector<int> h_rset;   // R-space rows: set and row of each, and
+    std::vector<long> h_rrow;  // their entries
+    std::vector<long> h_rptr, h_rcol;
+    std::vector<double> h_rw;
     double h_coef[9] = {};  // interior stencil
     shx_deriv_T* T = nullptr;
 };

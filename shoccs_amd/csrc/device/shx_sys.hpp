@@ -140,6 +140,7 @@ struct MarchArgs {
     const unsigned char* cls;
     const ClosureRow* rows;  // 3 x kMaxClosureRows
     int nrows[3];
+    int fixw[3];  // every table row of the direction has as many columns as the interior rows
     int nbz, nby, xchunk;
     int vec;  // points per lane along z: 2 (even z extent) or 1
     unsigned nblocks;
@@ -164,25 +165,28 @@ struct SpecialRec {
 };
 
 // One free cut point (a cut point of a non-Dirichlet body) of the fused
-// form: point q of R(set), and per direction the index of its row in that
-// derivative's R-space row list (-1: none).
+// form: point q of R(set), and the entries of its rows of the three
+// derivatives, consecutive from `ent` (n[0] of the x row, then n[1] of the y
+// row, n[2] of the z row; a missing row has none).
 struct FreeRec {
     long q;
-    long row[3];
+    long ent;
+    int n[3];
     int set;
-    int pad;
 };
 
-// The R-space rows of the three derivatives (shx_derivative_desc.rptr ...).
-struct CutRows {
-    const long* rptr[3];
-    const long* rcol[3];
-    const double* rw[3];
+// One entry of a free cut point's row (the derivative's R-space row entry
+// in stored order): column >= 0 names a point of D, -1 - c cut point c of
+// the row's set.
+struct FreeEnt {
+    long col;
+    double w;
 };
 
-// Arguments of the special-point kernel: the listed points of D, then the
-// cut points of the three directions (Dirichlet ones take exact(t_next)),
-// then the free cut points (their rows, the rhs and the stage update).
+// Arguments of the special-point work: the free cut points (their rows, the
+// rhs and the stage update), the listed points of D, the Dirichlet grid
+// faces (exact(t_next)), then the cut points of the three directions
+// (Dirichlet ones take exact(t_next)).
 struct SpecialArgs {
     DevSys s;
     DevOps ops;
@@ -200,7 +204,8 @@ struct SpecialArgs {
     double *outR0, *outR1, *outR2;
     long nfree;
     const FreeRec* frees;
-    CutRows cut;
+    const FreeEnt* fents;
+    long nface;  // face items: the sizes of the Dirichlet faces, summed
     RPtrs u0R;      // the step's u0 at the cut points
     double* accF;   // accumulator of the free cut points (nfree values)
 };

@@ -50,9 +50,11 @@ struct shx_system {
     bool general = false;
     bool any_free_r = false;      // some cut point of a non-Dirichlet body
     // the fused x-marching form also steps free cut points (no Neumann grid
-    // faces): one record each, with the rows of its three derivatives
+    // faces): one record each, with the entries of its three derivatives'
+    // rows in one list
     bool fused_free = false;
     shx::FreeRec* frees = nullptr;
+    shx::FreeEnt* fents = nullptr;
     long nfree = 0;
     double* accF = nullptr;       // their fused accumulator register
     double* neuD = nullptr;       // heat: Neumann face data (N values)
@@ -90,7 +92,10 @@ struct shx_system {
     shx::SpecialRow* srows[3] = {nullptr, nullptr, nullptr};
     shx::ClosureRow* crows = nullptr;  // 3 x kMaxClosureRows
     int nrows[3] = {0, 0, 0};
+    int fixw[3] = {0, 0, 0};  // every table row of a direction has `width` columns
     long nspecial = 0;
+    long nface_items = 0;   // special items of the Dirichlet grid faces (face sizes, summed)
+    long nface_points = 0;  // distinct points among them
     bool classes_ready = false;
     int width = 5;  // interior stencil width shared by the three derivatives
     shx_adjoint_ws* adj = nullptr;  // adjoint workspace, allocated on first use

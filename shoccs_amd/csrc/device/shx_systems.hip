@@ -924,22 +924,31 @@ int ensure_classes(shx_system* s)
     std::vector<ClosureRow> table((size_t)3 * kMaxClosureRows, ClosureRow{});
     for (int c = 0; c < 3; ++c) {
         s->nrows[c] = nr[c];
+        s->fixw[c] = 1;
         for (size_t r = 0; r < allrows[c].size(); ++r)
-            if (rank[c][r] <= nr[c]) table[(size_t)c * kMaxClosureRows + rank[c][r] - 1] = allrows[c][r];
+            if (rank[c][r] <= nr[c]) {
+                table[(size_t)c * kMaxClosureRows + rank[c][r] - 1] = allrows[c][r];
+                if (allrows[c][r].n != s->width) s->fixw[c] = 0;
+            }
     }
-    // records of the remaining special points, in index order; a Dirichlet
+    // records of the remaining special points, in index order.  A Dirichlet
     // face point needs no rows (its right-hand side is zero: gG = 0 or the
-    // heat face rule) and takes exact(t_next)
+    // heat face rule) and takes exact(t_next): the faces are special items
+    // of their own, enumerated plane by plane
     std::vector<SpecialRec> recs;
+    s->nface_points = 0;
+    s->nface_items = 0;
+    for (int f = 0; f < 6; ++f)
+        if (s->dev.bc[f] == SHX_BC_DIRICHLET)
+            s->nface_items += f < 2 ? (long)n1 * n2 : f < 4 ? (long)s->n[0] * n2 : (long)s->n[0] * n1;
     std::vector<SpecialRow> rows[3];
     std::map<std::array<int, 8>, unsigned> seen[3];
     for (long idx = 0; idx < s->N; ++idx) {
         if (h[(size_t)idx] != kSpecial) continue;
         SpecialRec r;
         r.idx = (unsigned)idx;
-        if (raw_face[(size_t)idx]) {
-            r.ref[0] = r.ref[1] = r.ref[2] = kRowNone;
-            recs.push_back(r);
+        if (raw_face[(size_t)idx]) {  // a face item, not a record
+            ++s->nface_points;
             continue;
         }
         int kind[3];
@@ -985,7 +994,10 @@ int march_args(shx_system* s, const void* usD, RPtrs usR, const double* u0, RPtr
     a.out = out;
     a.cls = s->cls;
     a.rows = s->crows;
-    for (int c = 0; c < 3; ++c) a.nrows[c] = s->nrows[c];
+    for (int c = 0; c < 3; ++c) {
+        a.nrows[c] = s->nrows[c];
+        a.fixw[c] = s->fixw[c];
+    }
     a.xchunk = s->march_xchunk;
     // two points per lane along z when every pair is aligned (even n2)
     a.vec = s->march_vec != 0 ? s->march_vec : (s->n[2] % 2 == 0 ? 2 : 1);
@@ -1019,15 +1031,11 @@ int march_args(shx_system* s, const void* usD, RPtrs usR, const double* u0, RPtr
     sp.outR2 = outR[2];
     sp.nfree = s->fused_free ? s->nfree : 0;
     sp.frees = s->frees;
-    for (int c = 0; c < 3; ++c) {
-        const shx_derivative* d = s->d[c];
-        sp.cut.rptr[c] = d ? d->rptr : nullptr;
-        sp.cut.rcol[c] = d ? d->rcol : nullptr;
-        sp.cut.rw[c] = d ? d->rw : nullptr;
-    }
+    sp.fents = s->fents;
+    sp.nface = s->nface_items;
     sp.u0R = u0R;
     sp.accF = accF;
-    nitems = sp.nspecial + sp.nR0 + sp.nR1 + sp.nR2 + sp.nfree;
+    nitems = sp.nspecial + sp.nface + sp.nR0 + sp.nR1 + sp.nR2 + sp.nfree;
     if (nb + nitems / 64 + 8 >= (1L << 31)) return fail(s->ctx, SHX_EUNSUPPORTED, "march: too many workgroups");
     a.nblocks = (unsigned)nb;
     return SHX_OK;
@@ -1386,18 +1394,20 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
                         reinterpret_cast<const int8_t*>(fl8.data()), fl8.size());
     }
     // the fused form's free cut points (no Neumann grid faces): point q of
-    // R(r) and the index of its row in each derivative's R-space row list
-    // (rows sorted by (set, row))
+    // R(r) and the entries of its row in each derivative's R-space row list
+    // (rows sorted by (set, row)), copied in direction order into one list
     if (rc == SHX_OK && free_r && !neumann) {
         std::vector<FreeRec> fr;
+        std::vector<FreeEnt> ents;
         for (int r = 0; r < 3; ++r)
             for (long q = 0; q < s->nR[r]; ++q) {
                 if (desc->r_dirichlet[r][q]) continue;
                 FreeRec e{};
                 e.q = q;
                 e.set = r;
+                e.ent = (long)ents.size();
                 for (int c = 0; c < 3; ++c) {
-                    e.row[c] = -1;
+                    e.n[c] = 0;
                     const shx_derivative* d = s->d[c];
                     if (!d || d->h_rset.empty()) continue;
                     long lo = 0, hi = (long)d->h_rset.size();
@@ -1406,12 +1416,16 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
                         if (d->h_rset[mid] < r || (d->h_rset[mid] == r && d->h_rrow[mid] < q)) lo = mid + 1;
                         else hi = mid;
                     }
-                    if (lo < (long)d->h_rset.size() && d->h_rset[lo] == r && d->h_rrow[lo] == q) e.row[c] = lo;
+                    if (lo >= (long)d->h_rset.size() || d->h_rset[lo] != r || d->h_rrow[lo] != q) continue;
+                    for (long k = d->h_rptr[(size_t)lo]; k < d->h_rptr[(size_t)lo + 1]; ++k)
+                        ents.push_back(FreeEnt{d->h_rcol[(size_t)k], d->h_rw[(size_t)k]});
+                    e.n[c] = (int)(d->h_rptr[(size_t)lo + 1] - d->h_rptr[(size_t)lo]);
                 }
                 fr.push_back(e);
             }
         s->nfree = (long)fr.size();
         rc = upload_vec(ctx, &s->frees, fr);
+        if (rc == SHX_OK && !ents.empty()) rc = upload_vec(ctx, &s->fents, ents);
         s->fused_free = rc == SHX_OK;
     }
     if (rc == SHX_OK) {
@@ -1482,6 +1496,7 @@ int shx_system_destroy(shx_system* s)
     hipFree(s->accD);
     hipFree(s->accF);
     hipFree(s->frees);
+    hipFree(s->fents);
     hipFree(s->cls);
     hipFree(s->special);
     for (int c = 0; c < 3; ++c) hipFree(s->srows[c]);
@@ -1832,7 +1847,7 @@ int shx_system_stage_points(shx_system* s, int64_t out[2])
         const int rc = ensure_classes(s);
         if (rc != SHX_OK) return rc;
         out[0] = s->N;
-        out[1] = s->nspecial;
+        out[1] = s->nspecial + s->nface_points;
     } else {
         out[0] = s->N;
         out[1] = 0;
