@@ -465,63 +465,86 @@ __device__ __forceinline__ void march_tile(const MarchArgs& a, const StageScalar
     }
 }
 
-// Row of direction d at a listed point (the same sums as row_at: interior
-// rows tap by tap, other rows as listed, B terms after).  `e` is the row's
-// table entry, fetched by the caller.  The loads of a listed row are issued
-// eight columns at a time before their products are summed in column order
-// (columns past the row's end repeat its last column and are not summed), so
-// that a row costs one round trip to memory rather than one per column; the
-// B-term loads are issued before them.
+// The operands of one row of direction d at a listed point, loaded without
+// waiting on each other: the first W columns' values (an interior row's
+// taps, a listed row's columns; columns past a row's end repeat its last
+// column and are not summed) and the B terms.  `e` is the row's table
+// entry.  The weights are read where the row is summed.
+template <int W>
+struct RowOps {
+    double v[W];
+    double bw0, bv0, bw1, bv1;
+    int n, nb;
+    bool row, listed;
+};
+
 template <int W, typename TI>
-__device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRow& e, unsigned ref,
-                                              const TI* __restrict__ u, const double* __restrict__ uR,
-                                              long idx)
+__device__ __forceinline__ void row_ops(RowOps<W>& o, const DevDeriv& d, const SpecialRow& e, unsigned ref,
+                                        const TI* __restrict__ u, const double* __restrict__ uR, long idx)
+{
+    constexpr int hw = W / 2;
+    o.listed = ref < kRowInterior;
+    o.row = ref != kRowNone;
+    o.n = 0;
+    o.nb = 0;
+    o.bw0 = o.bv0 = o.bw1 = o.bv1 = 0.0;
+#pragma unroll
+    for (int m = 0; m < W; ++m) o.v[m] = 0.0;
+    if (!o.row) return;  // (an absent direction's pointers may be null)
+    o.n = o.listed ? e.ncols : W;
+    o.nb = o.listed ? e.nb : 0;
+    const int off = o.listed ? e.off : -hw;
+    const long st = d.stride;
+    const int last = o.n - 1;
+    if (o.n > 0) {
+#pragma unroll
+        for (int m = 0; m < W; ++m) o.v[m] = (double)u[idx + (long)(off + min(m, last)) * st];
+    }
+    if (o.nb > 0) {
+        o.bw0 = d.pool[e.bw0];
+        o.bv0 = uR[e.bobj0];
+    }
+    if (o.nb > 1) {
+        o.bw1 = d.pool[e.bw1];
+        o.bv1 = uR[e.bobj1];
+    }
+}
+
+// The row from its operands (the same sums as row_at: the dot over the
+// row's columns in order from 0.0, B terms after); a listed row longer than
+// W continues from memory.
+template <int W, typename TI>
+__device__ __forceinline__ RowVal row_of(const RowOps<W>& o, const DevDeriv& d, const SpecialRow& e,
+                                         const TI* __restrict__ u, long idx)
 {
     RowVal r;
-    r.dot = 0.0;
-    r.b0 = r.b1 = 0.0;
-    r.nb = 0;
-    r.row = ref != kRowNone;
-    if (!r.row) return r;
-    const long st = d.stride;
+    r.row = o.row;
+    r.nb = o.nb;
+    r.dot = r.b0 = r.b1 = 0.0;
+    if (!o.row) return r;
     double dot = 0.0;
-    if (ref == kRowInterior) {
-        constexpr int hw = W / 2;
-        double v[W];
+    if (o.listed) {
+        const double* __restrict__ w = d.pool + e.coef;
+        const int last = o.n - 1;
+        double wv[W] = {};
+        if (o.n > 0) {
 #pragma unroll
-        for (int m = 0; m < W; ++m) v[m] = (double)u[idx + (long)(m - hw) * st];
-#pragma unroll
-        for (int m = 0; m < W; ++m) dot += d.coef[m] * v[m];
-        r.dot = dot;
-        return r;
-    }
-    double bw0 = 0.0, bv0 = 0.0, bw1 = 0.0, bv1 = 0.0;
-    if (e.nb > 0) {
-        bw0 = d.pool[e.bw0];
-        bv0 = uR[e.bobj0];
-    }
-    if (e.nb > 1) {
-        bw1 = d.pool[e.bw1];
-        bv1 = uR[e.bobj1];
-    }
-    const double* __restrict__ w = d.pool + e.coef;
-    const TI* __restrict__ v = u + idx + (long)e.off * st;
-    for (int c0 = 0; c0 < e.ncols; c0 += 8) {
-        double wv[8], vv[8];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const int c = min(c0 + m, e.ncols - 1);
-            wv[m] = w[c];
-            vv[m] = (double)v[(long)c * st];
+            for (int m = 0; m < W; ++m) wv[m] = w[min(m, last)];
         }
 #pragma unroll
-        for (int m = 0; m < 8; ++m)
-            if (c0 + m < e.ncols) dot += wv[m] * vv[m];
+        for (int m = 0; m < W; ++m)
+            if (m < o.n) dot += wv[m] * o.v[m];
+        if (o.n > W) {
+            const TI* __restrict__ v = u + idx + (long)e.off * d.stride;
+            for (int c = W; c < o.n; ++c) dot += w[c] * (double)v[(long)c * d.stride];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < W; ++m) dot += d.coef[m] * o.v[m];
     }
     r.dot = dot;
-    r.nb = e.nb;
-    if (e.nb > 0) r.b0 = bw0 * bv0;
-    if (e.nb > 1) r.b1 = bw1 * bv1;
+    r.b0 = o.bw0 * o.bv0;
+    r.b1 = o.bw1 * o.bv1;
     return r;
 }
 
@@ -531,16 +554,19 @@ __device__ __forceinline__ RowVal special_row(const DevDeriv& d, const SpecialRo
 // running sum across the directions, the wave's gradient starts each from
 // zero), the rhs at the cut point (scalar_wave.cpp:208-238: gG . du with gG
 // at the point; heat.cpp:119-162: k lap + source) and the stage update of
-// its value and accumulator (rk4.cpp:27-46), all in FP64.  The rows'
-// entries are consecutive in one list; eight entries are loaded, then their
-// values, before their products are summed in order.  Pointers are picked
-// by comparisons, never by a run-time array index, so that argument
-// structures copied to registers stay out of scratch memory.
+// its value and accumulator (rk4.cpp:27-46), all in FP64.  The entries in
+// the point's slots load with its record, then their values, before the
+// products are summed in order; entries past the slots continue one by one.
+// Pointers are picked by comparisons, never by a run-time array index, so
+// that argument structures copied to registers stay out of scratch memory.
 template <int KIND, int STAGE, bool MIXED>
 __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScalars& ss, long f)
 {
     using TI = typename StageTypes<MIXED, STAGE>::in;
     const TI* __restrict__ us = static_cast<const TI*>(a.us);
+    FreeEnt x8[kFreeSlots];
+#pragma unroll
+    for (int m = 0; m < kFreeSlots; ++m) x8[m] = a.fents[f * kFreeSlots + m];
     const FreeRec fr = a.frees[f];
     const int set = fr.set;
     const double* __restrict__ uR = set == 0 ? a.usR.r[0] : set == 1 ? a.usR.r[1] : a.usR.r[2];
@@ -549,25 +575,27 @@ __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScala
     const double x = pos[3 * fr.q], y = pos[3 * fr.q + 1], z = pos[3 * fr.q + 2];
     const double u0v = u0R[fr.q];
     const double accv = STAGE == 0 ? 0.0 : a.accF[f];
-    const long e0 = fr.ent;
-    const long s1 = e0 + fr.n[0], s2 = s1 + fr.n[1], e1 = s2 + fr.n[2];
+    const int s1 = fr.n[0], s2 = s1 + fr.n[1], ntot = s2 + fr.n[2];
+    double v[kFreeSlots];
+#pragma unroll
+    for (int m = 0; m < kFreeSlots; ++m) v[m] = x8[m].col >= 0 ? (double)us[x8[m].col] : uR[-1 - x8[m].col];
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;  // heat: r0 is the laplacian's running sum
-    for (long e = e0; e < e1; e += 8) {
-        FreeEnt x8[8];
-        double v[8];
+    // entry e's product goes to the sum of its row (selects on the values,
+    // so that the three sums stay in registers)
+    auto add = [&](int e, double p) {
+        const bool in0 = kIsHeat<KIND> || e < s1;
+        const bool in1 = !in0 && e < s2;
+        const bool in2 = !in0 && !in1;
+        r0 = in0 ? r0 + p : r0;
+        r1 = in1 ? r1 + p : r1;
+        r2 = in2 ? r2 + p : r2;
+    };
 #pragma unroll
-        for (int m = 0; m < 8; ++m) x8[m] = a.fents[min(e + m, e1 - 1)];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) v[m] = x8[m].col >= 0 ? (double)us[x8[m].col] : uR[-1 - x8[m].col];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const long em = e + m;
-            if (em >= e1) continue;
-            const double p = x8[m].w * v[m];
-            if (kIsHeat<KIND> || em < s1) r0 += p;
-            else if (em < s2) r1 += p;
-            else r2 += p;
-        }
+    for (int m = 0; m < kFreeSlots; ++m)
+        if (m < ntot) add(m, x8[m].w * v[m]);
+    for (int e = kFreeSlots; e < ntot; ++e) {
+        const FreeEnt xe = a.fents[fr.over + e - kFreeSlots];
+        add(e, xe.w * (xe.col >= 0 ? (double)us[xe.col] : uR[-1 - xe.col]));
     }
     double rhs;
     if (kIsHeat<KIND>) {
@@ -647,7 +675,8 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
         point_coords(idx, a.s.n1, a.s.n2, i, j, k);
         const DevOps& ops = a.ops;
         // every load that does not depend on another is issued first: the
-        // stage-update operands and the three rows' table entries
+        // stage-update operands and the three rows' table entries, then the
+        // operands of the three rows together
         const double u0v = a.u0[idx];
         const double accv = STAGE == 0 ? 0.0 : a.acc[idx];
         const unsigned f0 = ops.d[0].present ? rec.ref[0] : kRowNone;
@@ -657,24 +686,25 @@ __device__ __forceinline__ void special_item(const SpecialArgs& a, const StageSc
         if (f0 < kRowInterior) e0 = a.rows0[f0];
         if (f1 < kRowInterior) e1 = a.rows1[f1];
         if (f2 < kRowInterior) e2 = a.rows2[f2];
+        RowOps<W> o0, o1, o2;
+        row_ops<W>(o0, ops.d[0], e0, f0, us, a.usR.r[0], idx);
+        row_ops<W>(o1, ops.d[1], e1, f1, us, a.usR.r[1], idx);
+        row_ops<W>(o2, ops.d[2], e2, f2, us, a.usR.r[2], idx);
+        const RowVal r0 = row_of<W>(o0, ops.d[0], e0, us, idx);
+        const RowVal r1 = row_of<W>(o1, ops.d[1], e1, us, idx);
+        const RowVal r2 = row_of<W>(o2, ops.d[2], e2, us, idx);
         double rhs;
         if (kIsHeat<KIND>) {
             double lap = 0.0;
-            const RowVal r0 = special_row<W>(ops.d[0], e0, f0, us, a.usR.r[0], idx);
             if (r0.row) lap = row_plus(lap, r0);
-            const RowVal r1 = special_row<W>(ops.d[1], e1, f1, us, a.usR.r[1], idx);
             if (r1.row) lap = row_plus(lap, r1);
-            const RowVal r2 = special_row<W>(ops.d[2], e2, f2, us, a.usR.r[2], idx);
             if (r2.row) lap = row_plus(lap, r2);
             rhs = heat_finish<KIND>(a.s, on_dirichlet_face(a.s, i, j, k), is_fluid(a.s, idx), i, j, k, lap,
                                     ss.t_s);
         } else {
             double du0 = 0.0, du1 = 0.0, du2 = 0.0;
-            const RowVal r0 = special_row<W>(ops.d[0], e0, f0, us, a.usR.r[0], idx);
             if (r0.row) du0 = row_eq(r0);
-            const RowVal r1 = special_row<W>(ops.d[1], e1, f1, us, a.usR.r[1], idx);
             if (r1.row) du1 = row_eq(r1);
-            const RowVal r2 = special_row<W>(ops.d[2], e2, f2, us, a.usR.r[2], idx);
             if (r2.row) du2 = row_eq(r2);
             rhs = wave_combine<KIND>(a.s, on_dirichlet_face(a.s, i, j, k), i, j, k, du0, du1, du2);
         }

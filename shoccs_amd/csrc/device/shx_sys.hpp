@@ -166,11 +166,15 @@ struct SpecialRec {
 
 // One free cut point (a cut point of a non-Dirichlet body) of the fused
 // form: point q of R(set), and the entries of its rows of the three
-// derivatives, consecutive from `ent` (n[0] of the x row, then n[1] of the y
-// row, n[2] of the z row; a missing row has none).
+// derivatives (n[0] of the x row, then n[1] of the y row, n[2] of the z row;
+// a missing row has none).  Free cut point f's first kFreeSlots entries sit
+// at slots f kFreeSlots + e of the entry list (unused slots name D point 0),
+// so that they load together with its record; the rest continue from
+// `over`.
+constexpr int kFreeSlots = 12;
 struct FreeRec {
     long q;
-    long ent;
+    long over;
     int n[3];
     int set;
 };

@@ -1395,17 +1395,18 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
     }
     // the fused form's free cut points (no Neumann grid faces): point q of
     // R(r) and the entries of its row in each derivative's R-space row list
-    // (rows sorted by (set, row)), copied in direction order into one list
+    // (rows sorted by (set, row)), copied in direction order: the first
+    // kFreeSlots into the point's slots, the rest after every point's slots
     if (rc == SHX_OK && free_r && !neumann) {
         std::vector<FreeRec> fr;
-        std::vector<FreeEnt> ents;
+        std::vector<std::vector<FreeEnt>> per;
         for (int r = 0; r < 3; ++r)
             for (long q = 0; q < s->nR[r]; ++q) {
                 if (desc->r_dirichlet[r][q]) continue;
                 FreeRec e{};
                 e.q = q;
                 e.set = r;
-                e.ent = (long)ents.size();
+                std::vector<FreeEnt> ents;
                 for (int c = 0; c < 3; ++c) {
                     e.n[c] = 0;
                     const shx_derivative* d = s->d[c];
@@ -1422,10 +1423,19 @@ int shx_system_create(shx_ctx* ctx, const shx_system_desc* desc, shx_system** ou
                     e.n[c] = (int)(d->h_rptr[(size_t)lo + 1] - d->h_rptr[(size_t)lo]);
                 }
                 fr.push_back(e);
+                per.push_back(std::move(ents));
             }
         s->nfree = (long)fr.size();
+        std::vector<FreeEnt> list((size_t)s->nfree * kFreeSlots, FreeEnt{0, 0.0});
+        for (size_t f = 0; f < per.size(); ++f) {
+            fr[f].over = (long)list.size();
+            for (size_t m = 0; m < per[f].size(); ++m) {
+                if (m < (size_t)kFreeSlots) list[f * kFreeSlots + m] = per[f][m];
+                else list.push_back(per[f][m]);
+            }
+        }
         rc = upload_vec(ctx, &s->frees, fr);
-        if (rc == SHX_OK && !ents.empty()) rc = upload_vec(ctx, &s->fents, ents);
+        if (rc == SHX_OK && !list.empty()) rc = upload_vec(ctx, &s->fents, list);
         s->fused_free = rc == SHX_OK;
     }
     if (rc == SHX_OK) {
