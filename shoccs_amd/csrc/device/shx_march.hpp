@@ -557,8 +557,20 @@ __device__ __forceinline__ RowVal row_of(const RowOps<W>& o, const DevDeriv& d, 
 // its value and accumulator (rk4.cpp:27-46), all in FP64.  The entries in
 // the point's slots load with its record, then their values, before the
 // products are summed in order; entries past the slots continue one by one.
-// Pointers are picked by comparisons, never by a run-time array index, so
-// that argument structures copied to registers stay out of scratch memory.
+// Pointers are picked among values (pick3), never by a run-time array index
+// or a choice of addresses.
+// One of three pointers by a run-time choice, taken between the three
+// values: the values pass through an empty asm statement first, so that the
+// compiler cannot turn the choice into one between the addresses of the
+// fields that hold them (which would keep an argument structure copied into
+// a local variable, as the batch kernel's are, in scratch memory).
+template <typename T>
+__device__ __forceinline__ T* pick3(int set, T* p0, T* p1, T* p2)
+{
+    asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2));
+    return set == 0 ? p0 : set == 1 ? p1 : p2;
+}
+
 template <int KIND, int STAGE, bool MIXED>
 __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScalars& ss, long f)
 {
@@ -569,9 +581,9 @@ __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScala
     for (int m = 0; m < kFreeSlots; ++m) x8[m] = a.fents[f * kFreeSlots + m];
     const FreeRec fr = a.frees[f];
     const int set = fr.set;
-    const double* __restrict__ uR = set == 0 ? a.usR.r[0] : set == 1 ? a.usR.r[1] : a.usR.r[2];
-    const double* pos = set == 0 ? a.pos0 : set == 1 ? a.pos1 : a.pos2;
-    const double* u0R = set == 0 ? a.u0R.r[0] : set == 1 ? a.u0R.r[1] : a.u0R.r[2];
+    const double* __restrict__ uR = pick3(set, a.usR.r[0], a.usR.r[1], a.usR.r[2]);
+    const double* pos = pick3(set, a.pos0, a.pos1, a.pos2);
+    const double* u0R = pick3(set, a.u0R.r[0], a.u0R.r[1], a.u0R.r[2]);
     const double x = pos[3 * fr.q], y = pos[3 * fr.q + 1], z = pos[3 * fr.q + 2];
     const double u0v = u0R[fr.q];
     const double accv = STAGE == 0 ? 0.0 : a.accF[f];
@@ -607,7 +619,7 @@ __device__ __forceinline__ void free_item(const SpecialArgs& a, const StageScala
     }
     const double aa = accv + ss.c_acc * rhs;
     if (STAGE < 3) a.accF[f] = aa;
-    double* outR = set == 0 ? a.outR0 : set == 1 ? a.outR1 : a.outR2;
+    double* outR = pick3(set, a.outR0, a.outR1, a.outR2);
     outR[fr.q] = STAGE < 3 ? u0v + ss.c_next * rhs : u0v + 1.0 * aa;
 }
 
@@ -790,6 +802,18 @@ __global__ __launch_bounds__(64 * TY) void k_stage(MarchArgs a, SpecialArgs sp, 
     stage_block<KIND, STAGE, W, TY, VEC, MIXED>(a, sp, ss, blockIdx.x, sg, mg, mode);
 }
 
+// The same stage for the heat systems, held to four waves per SIMD: their
+// manufactured solutions' out-of-line evaluations (sines, cosines and
+// exponentials at cut points, reached only by special items) would
+// otherwise set the register count of the whole kernel; under the bound
+// they spill instead, where they run.
+template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) void k_stage_heat(
+    MarchArgs a, SpecialArgs sp, StageScalars ss, unsigned sg, unsigned mg, int mode)
+{
+    stage_block<KIND, STAGE, W, TY, VEC, MIXED>(a, sp, ss, blockIdx.x, sg, mg, mode);
+}
+
 // RK4 stage constants (rk4.cpp:13-14), as the host's
 constexpr double kStageTime[4] = {0.0, 0.5, 0.5, 1.0};
 constexpr double kStageWeight[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
@@ -800,10 +824,8 @@ constexpr double kStageWeight[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 // captured launch stays valid at any time.  The stage scalars are formed as
 // the single-problem host path forms them (t = t0 + n dt).
 template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
-__global__ __launch_bounds__(64 * TY) void k_batch_stage(const BatchEntry* __restrict__ ent,
-                                                         const BatchClock* __restrict__ clk,
-                                                         const unsigned* __restrict__ start,
-                                                         int nprob)
+__device__ __forceinline__ void batch_block(const BatchEntry* __restrict__ ent, const BatchClock* __restrict__ clk,
+                                            const unsigned* __restrict__ start, int nprob)
 {
     int lo = 0, hi = nprob;
     while (hi - lo > 1) {
@@ -836,6 +858,23 @@ __global__ __launch_bounds__(64 * TY) void k_batch_stage(const BatchEntry* __res
     stage_block<KIND, STAGE, W, TY, VEC, MIXED, true>(m, sp, ss, blockIdx.x - start[lo], sg, mg, mode);
 }
 
+template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
+__global__ __launch_bounds__(64 * TY) void k_batch_stage(const BatchEntry* __restrict__ ent,
+                                                         const BatchClock* __restrict__ clk,
+                                                         const unsigned* __restrict__ start, int nprob)
+{
+    batch_block<KIND, STAGE, W, TY, VEC, MIXED>(ent, clk, start, nprob);
+}
+
+// The batch stage of the heat systems, bounded as k_stage_heat.
+template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_stage_heat(
+    const BatchEntry* __restrict__ ent, const BatchClock* __restrict__ clk, const unsigned* __restrict__ start,
+    int nprob)
+{
+    batch_block<KIND, STAGE, W, TY, VEC, MIXED>(ent, clk, start, nprob);
+}
+
 template <int KIND, int STAGE, int W, bool MIXED>
 int stage_ty(hipStream_t st, int ty, int mode, const MarchArgs& a, const SpecialArgs& sp,
              const StageScalars& ss, long nitems)
@@ -848,7 +887,10 @@ int stage_ty(hipStream_t st, int ty, int mode, const MarchArgs& a, const Special
         if (grid == 0) return 0;
 #define SHX_STAGE_CASE(TY, VEC)                                                                      \
     case TY * 10 + VEC:                                                                              \
-        k_stage<KIND, STAGE, W, TY, VEC, MIXED><<<grid, 64 * TY, 0, st>>>(a, sp, ss, s_groups, m_groups, md); \
+        if constexpr (kIsHeat<KIND>)                                                                 \
+            k_stage_heat<KIND, STAGE, W, TY, VEC, MIXED><<<grid, 64 * TY, 0, st>>>(a, sp, ss, s_groups, m_groups, md); \
+        else                                                                                         \
+            k_stage<KIND, STAGE, W, TY, VEC, MIXED><<<grid, 64 * TY, 0, st>>>(a, sp, ss, s_groups, m_groups, md); \
         return 0;
         switch (ty * 10 + a.vec) {
             SHX_STAGE_CASE(4, 1)
@@ -909,8 +951,12 @@ int batch_ty(hipStream_t st, int ty, const BatchLaunch& bl)
 {
 #define SHX_BATCH_CASE(TY, VEC)                                                                      \
     case TY * 10 + VEC:                                                                              \
-        k_batch_stage<KIND, STAGE, W, TY, VEC, MIXED><<<bl.grid, 64 * TY, 0, st>>>(bl.ent, bl.clk, bl.start, \
-                                                                                 bl.nprob);          \
+        if constexpr (kIsHeat<KIND>)                                                                 \
+            k_batch_stage_heat<KIND, STAGE, W, TY, VEC, MIXED><<<bl.grid, 64 * TY, 0, st>>>(bl.ent, bl.clk, \
+                                                                                      bl.start, bl.nprob); \
+        else                                                                                         \
+            k_batch_stage<KIND, STAGE, W, TY, VEC, MIXED><<<bl.grid, 64 * TY, 0, st>>>(bl.ent, bl.clk, bl.start, \
+                                                                                     bl.nprob);      \
         return 0;
     switch (ty * 10 + bl.vec) {
         SHX_BATCH_CASE(4, 1)
