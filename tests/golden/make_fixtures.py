@@ -105,6 +105,38 @@ def mesh_cases():
     )
 
 
+def sphere_geometry(ref: str):
+    """object_geometry.t.cpp "sphere intersections": per direction the cut
+    points' positions, psi and solid coordinates (in the reference's R
+    order) and the solid points of that direction's lines (its S order),
+    from the `exact`, `sc`, `psi` and `sx`/`sy`/`sz` initializers of each
+    SECTION.  The mesh and sphere are those of the test's configuration."""
+    src = open(os.path.join(ref, "src", "mesh", "object_geometry.t.cpp")).read()
+    body = src[src.index('TEST_CASE("sphere intersections")'):src.index('TEST_CASE("rect_intersections")')]
+    out = dict(n=[21, 22, 23], lo=[-1.0, -1.0, 0.0], hi=[1.0, 2.0, 2.2], center=[0.01, -0.01, 0.5],
+               radius=0.25, dirs=[])
+    names = ["X", "y", "z"]
+    for d, name in enumerate(names):
+        start = body.index(f'SECTION("{name}")')
+        end = body.index(f'SECTION("{names[d + 1]}")') if d < 2 else len(body)
+        sec = body[start:end]
+
+        def init(var):
+            m = re.search(r"std::vector<\w+>\s+" + var + r"\s*=\s*\{(.*?)\};", sec, re.S)
+            return _numbers(m.group(1))
+
+        pos = init("exact")
+        sc = init("sc")
+        psi = init("psi")
+        solid = init("s" + "xyz"[d])
+        out["dirs"].append(dict(
+            pos=[pos[i:i + 3] for i in range(0, len(pos), 3)],
+            solid_coord=[[int(v) for v in sc[i:i + 3]] for i in range(0, len(sc), 3)],
+            psi=psi,
+            solid=[[int(v) for v in solid[i:i + 3]] for i in range(0, len(solid), 3)]))
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     stencils = stencil_cases(ref) + polye2_cases(ref)
@@ -112,7 +144,11 @@ def main():
         json.dump(stencils, f, indent=1)
     with open(os.path.join(HERE, "mesh_lines.json"), "w") as f:
         json.dump(mesh_cases(), f, indent=1)
-    print(f"{len(stencils)} stencil cases written")
+    geo = sphere_geometry(ref)
+    with open(os.path.join(HERE, "object_geometry_sphere.json"), "w") as f:
+        json.dump(geo, f)
+    print(f"{len(stencils)} stencil cases written; sphere geometry: "
+          + ", ".join(f"{len(d['pos'])} cut points / {len(d['solid'])} solid points" for d in geo["dirs"]))
 
 
 if __name__ == "__main__":

@@ -1,6 +1,7 @@
 """The matrix entry points of the device boundary against the reference's
 own golden vectors (src/matrices/*.t.cpp), through the C ABI:
 shx_block_apply for ccs::matrix::{circulant, dense, inner_block, block}
+(inner_block.t.cpp's identity and random-boundary cases included)
 (one derivative operator whose segments hold the matrices) and
 shx_csr_create / shx_csr_apply for ccs::matrix::csr.  The vectors were
 printed by the reference; the comparisons allow 1e-12 relative.
@@ -178,6 +179,53 @@ def test_block_identity(ctx):
     assert np.array_equal(got, xx)
     got = _block_apply(ctx, 50, segs, left + right, [0.0, 1.0, 0.0], x, xx, 1)
     assert np.array_equal(got, 2 * xx)
+
+
+def test_inner_block_identity(ctx):
+    """inner_block.t.cpp:31-97 "Identity": [I4 | circulant {1} | I2] square
+    (16 rows and columns; the one-coefficient circulant as the stencil
+    {0, 1, 0}) gives b = x, and plus_eq 2x; the non-square block
+    (row offset 1: a shifted 4 x 5 identity, 9 circulant rows, I2 on the
+    last two columns) gives no row at point 0 and x elsewhere."""
+    x = np.random.default_rng(5).uniform(-1, 1, 16)
+    seg = _seg(0, 4, 10, 2, lcols=4, lcol0=0, lcoef=0, rcols=2, rcol0=14, rcoef=16)
+    pool = np.eye(4).ravel().tolist() + np.eye(2).ravel().tolist()
+    got = _block_apply(ctx, 16, [seg], pool, [0.0, 1.0, 0.0], x, np.zeros(16), 0)
+    assert np.array_equal(got, x)
+    got = _block_apply(ctx, 16, [seg], pool, [0.0, 1.0, 0.0], x, got, 1)
+    assert np.array_equal(got, 2 * x)
+    shifted = np.zeros((4, 5))
+    for r in range(4):
+        shifted[r, r + 1] = 1.0
+    seg = _seg(1, 4, 9, 2, lcols=5, lcol0=0, lcoef=0, rcols=2, rcol0=14, rcoef=20)
+    pool = shifted.ravel().tolist() + np.eye(2).ravel().tolist()
+    got = _block_apply(ctx, 16, [seg], pool, [0.0, 1.0, 0.0], x, np.zeros(16), 0)
+    assert got[0] == 0.0
+    assert np.array_equal(got[1:], x[1:])
+
+
+def test_inner_block_random_boundary(ctx):
+    """inner_block.t.cpp:99-168: [dense 4 x 5 | circulant 10 rows | dense
+    2 x 3] with row and column offset 2 on 18 points (the first two without
+    a row)."""
+    left = [2.247323503221594, -5.0275337061235135, -0.9845370624957113, 9.621361506824222, -1.4847274599487292,
+            -5.920356924707782, -2.5821362891416157, 3.4175272453709056, 8.530925204174977, 6.082313908057927,
+            -6.47449633704835, 3.2352002101687702, 4.758308560669512, 1.598859246390333, 0.9341660471574365,
+            -2.405757184769108, -1.0503192473307266, 2.485464032420346, 9.816354187065464, 5.090921023204807]
+    interior = [0.7763383518842382, 2.749405412018632, -0.7966643678385346]
+    right = [-0.12697902221008128, 1.4021179149102307, 0.30625420155821903, 1.288208276267654,
+             1.3345939131355147, -1.4713774812532359]
+    x = [0.0, 0.0, 1.1489955608128035, -9.641815125514444, 0.5975150739415511, 5.321795555035614,
+         -5.467954909319733, 1.4687325444642383, 8.370366165425736, -2.810816575553474, 1.815504174076466,
+         1.1769187743854737, -8.870585410511126, -0.14181686526567816, 7.93970270970086, 0.14183234352470464,
+         -8.054425087325441, -3.809075460321715]
+    want = [0., 0., 109.78978122898833, 32.2780625869145, -32.38838457188155, 33.25158538253831,
+            -12.072197714155793, -6.87521436567669, 26.39304084900066, -2.6761855166330424, 1.8718030426410628,
+            11.712151684565809, -23.362167910509047, -13.601765954771285, 21.606370954127627, 12.97052379948305,
+            -12.477808805315766, -4.962089239867884]
+    seg = _seg(2, 4, 10, 2, lcols=5, lcol0=2, lcoef=0, rcols=3, rcol0=2 + 16 - 3, rcoef=20)
+    got = _block_apply(ctx, 18, [seg], left + right, interior, np.array(x), np.zeros(18), 0)
+    _close(got, want)
 
 
 def test_csr_random(ctx):
