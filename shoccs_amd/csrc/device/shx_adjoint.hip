@@ -68,6 +68,9 @@ struct shx_adjoint_ws {
     double* irr_w = nullptr;
     int8_t* irr_dir = nullptr;
     bool lists_ready = false;
+    // per column, bit c W + m: row q + (W/2 - m) st_c is a circulant row of
+    // direction c with a nonzero weight (built once; the mesh fixes it)
+    unsigned short* tmask = nullptr;
 };
 
 namespace {
@@ -156,6 +159,40 @@ __device__ __forceinline__ bool face_of(const DevSys& s, int a, int q, int n)
     return (s.bc[2 * a] == SHX_BC_DIRICHLET && q == 0) || (s.bc[2 * a + 1] == SHX_BC_DIRICHLET && q == n - 1);
 }
 
+// Which rows of column q contribute to the circulant-row part of the
+// transpose: bit c W + m is set when row p = q + (W/2 - m) st_c lies on the
+// mesh, is a circulant row of direction c, and does not lie on a Dirichlet
+// face (along its own axis, or along another axis, where the point is on one
+// already): its weight is not zero.  Fixed by the mesh, so built once.
+__global__ void k_tap_mask(DevSys s, AdjDirs ad, long N, int W, unsigned short* __restrict__ mask)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N) return;
+    const int hw = W / 2;
+    int ijk[3];
+    point_coords(q, s.n1, s.n2, ijk[0], ijk[1], ijk[2]);
+    const int nn[3] = {s.n0, s.n1, s.n2};
+    bool fa[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) fa[a] = face_of(s, a, ijk[a], nn[a]);
+    unsigned m = 0u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const AdjDir& a = ad.d[c];
+        if (!a.present) continue;
+        const bool other = (c != 0 && fa[0]) || (c != 1 && fa[1]) || (c != 2 && fa[2]);
+        for (int t = 0; t < W; ++t) {
+            const int o = hw - t;
+            const int qc = ijk[c] + o;
+            if (qc < 0 || qc >= nn[c]) continue;
+            if (!bit(a.intr, q + (long)o * a.st)) continue;
+            if (other || face_of(s, c, qc, nn[c])) continue;
+            m |= 1u << (c * W + t);
+        }
+    }
+    mask[q] = (unsigned short)m;
+}
+
 // The circulant-row part of (d rhs / d u)^T kbar at every column q: for each
 // direction c (in order) and tap m (in stencil order), the row p = q + (hw -
 // m) st_c contributes coef_c[m] y_c[p] when it is a circulant row, with y_c =
@@ -164,11 +201,15 @@ __device__ __forceinline__ bool face_of(const DevSys& s, int a, int q, int n)
 // coefficient gradient: row p's term of coefficient m reads w at column q, so
 // y_c[p] w[q] is added to value c W + m; per workgroup the values are summed
 // in a fixed order and added to the workgroup's slot of `part` (one writer
-// per slot and launch: deterministic).
-template <int W, bool GRAD>
-__global__ __launch_bounds__(kT) void k_rhsT(DevSys s, AdjDirs ad, long N, const double* __restrict__ kbar,
-                                             double* __restrict__ wbar, const double* __restrict__ w,
-                                             double* __restrict__ part)
+// per slot and launch: deterministic).  The tap mask says which rows take
+// part; the others add 0.0 (which leaves a sum started from 0.0 unchanged),
+// reading column q itself so that every load stays in bounds.  CONSTW: the
+// weight is constant per direction (heat: k, linear wave: -a_c); else the
+// radial wave's weight at the row.
+template <int W, bool GRAD, bool CONSTW>
+__global__ __launch_bounds__(kT) void k_rhsT(DevSys s, AdjDirs ad, long N, const unsigned short* __restrict__ tmask,
+                                             const double* __restrict__ kbar, double* __restrict__ wbar,
+                                             const double* __restrict__ w, double* __restrict__ part)
 {
     constexpr int hw = W / 2, NV = 3 * W;
     static_assert(NV * 16 <= kT, "one thread per partial sum");
@@ -179,35 +220,29 @@ __global__ __launch_bounds__(kT) void k_rhsT(DevSys s, AdjDirs ad, long N, const
 #pragma unroll
         for (int v = 0; v < NV; ++v) g[v] = 0.0;
     if (in) {
-        int ijk[3];
-        point_coords(q, s.n1, s.n2, ijk[0], ijk[1], ijk[2]);
-        const int nn[3] = {s.n0, s.n1, s.n2};
-        bool fa[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) fa[a] = face_of(s, a, ijk[a], nn[a]);
+        const unsigned mk = tmask[q];
         const double wq = GRAD ? w[q] : 0.0;
+        int ijk[3] = {0, 0, 0};
+        if constexpr (!CONSTW) point_coords(q, s.n1, s.n2, ijk[0], ijk[1], ijk[2]);
         double acc = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const AdjDir& a = ad.d[c];
             if (!a.present) continue;
-            // a row lies on a Dirichlet face when the point does along the
-            // other axes, or when its own coordinate along c does
-            const bool other = (c != 0 && fa[0]) || (c != 1 && fa[1]) || (c != 2 && fa[2]);
+            const double wc = s.kind == SHX_SYS_HEAT ? s.k : -s.vel[c];
             double sc = 0.0;
 #pragma unroll
             for (int m = 0; m < W; ++m) {
                 const int o = hw - m;
-                const int qc = ijk[c] + o;
-                if (qc < 0 || qc >= nn[c]) continue;
-                const long p = q + (long)o * a.st;
-                if (!bit(a.intr, p)) continue;
-                double wgt;
-                if (other || face_of(s, c, qc, nn[c])) wgt = 0.0;
-                else if (s.kind == SHX_SYS_HEAT) wgt = s.k;
-                else if (s.kind == SHX_SYS_WAVE_LINEAR) wgt = -s.vel[c];
-                else wgt = adj_weight(s, c, ijk[0] + (c == 0 ? o : 0), ijk[1] + (c == 1 ? o : 0), ijk[2] + (c == 2 ? o : 0));
-                const double y = wgt * kbar[p];
+                const bool on = (mk >> (c * W + m)) & 1u;
+                const long p = on ? q + (long)o * a.st : q;
+                double wgt = wc;
+                if constexpr (!CONSTW)
+                    wgt = on ? adj_weight(s, c, ijk[0] + (c == 0 ? o : 0), ijk[1] + (c == 1 ? o : 0),
+                                          ijk[2] + (c == 2 ? o : 0))
+                             : 0.0;
+                const double kb = kbar[p];
+                const double y = on ? wgt * kb : 0.0;
                 sc += a.coef[m] * y;
                 if constexpr (GRAD) g[c * W + m] += y * wq;
             }
@@ -676,13 +711,22 @@ int rhs_T(shx_system* s, const shx_field& kbar, shx_field& wbar, const shx_field
     grad = grad && w;
     if (grad) ws->coef_w = W;
     const unsigned nb = blocks_for(s->N);
-#define SHX_RHST(WW)                                                                                      \
-    if (grad) k_rhsT<WW, true><<<nb, kT, 0, st>>>(s->dev, ad, s->N, kbar.D, wbar.D, w->D, ws->part);      \
-    else k_rhsT<WW, false><<<nb, kT, 0, st>>>(s->dev, ad, s->N, kbar.D, wbar.D, nullptr, nullptr);
-    switch (W) {
-    case 3: SHX_RHST(3) break;
-    case 5: SHX_RHST(5) break;
-    default: return fail(s->ctx, SHX_EUNSUPPORTED, "adjoint: interior width must be 3 or 5");
+    if (W != 3 && W != 5) return fail(s->ctx, SHX_EUNSUPPORTED, "adjoint: interior width must be 3 or 5");
+    if (!ws->tmask) {
+        hipError_t e = hipMalloc((void**)&ws->tmask, sizeof(unsigned short) * (size_t)std::max<long>(s->N, 1));
+        if (e != hipSuccess) return hip_fail(s->ctx, e, "adjoint tap mask");
+        k_tap_mask<<<nb, kT, 0, st>>>(s->dev, ad, s->N, W, ws->tmask);
+    }
+    const bool constw = s->kind == SHX_SYS_HEAT || s->kind == SHX_SYS_WAVE_LINEAR;
+#define SHX_RHST(WW, CW)                                                                                  \
+    if (grad) k_rhsT<WW, true, CW><<<nb, kT, 0, st>>>(s->dev, ad, s->N, ws->tmask, kbar.D, wbar.D, w->D, ws->part); \
+    else k_rhsT<WW, false, CW><<<nb, kT, 0, st>>>(s->dev, ad, s->N, ws->tmask, kbar.D, wbar.D, nullptr, nullptr);
+    if (W == 3) {
+        if (constw) { SHX_RHST(3, true) }
+        else { SHX_RHST(3, false) }
+    } else {
+        if (constw) { SHX_RHST(5, true) }
+        else { SHX_RHST(5, false) }
     }
 #undef SHX_RHST
     rc = ensure_lists(s);
@@ -792,6 +836,7 @@ void destroy_adjoint_ws(shx_adjoint_ws* w)
     hipFree(w->irr_row);
     hipFree(w->irr_w);
     hipFree(w->irr_dir);
+    hipFree(w->tmask);
     delete w;
 }
 
