@@ -741,6 +741,31 @@ def main():
                  "linf_error_vs_exact": m_stats[0],
                  "tuning": list(m_tuning[:3])}
         um.free()
+        if not args.headline_only:
+            # the same mixed-form steps replayed from one hipGraph of 10 steps
+            # (a one-problem batch: the batch stage kernel, the clock on the
+            # device), timed over the replays
+            ug = prob.field()
+            prob.initialize(ug, 0.0)
+            prob.update_boundary(ug, 0.0)
+            nsteps_g = 10
+            graph = S.Rk4Graph(prob, ug, 0.0, dt, nsteps_g)
+            graph.launch()  # warm-up replay
+            reps = max(1, args.steps // nsteps_g)
+            ctx.sync()
+            t0w = time.perf_counter()
+            for _ in range(reps):
+                graph.launch(sync=False)
+            ctx.sync()
+            ms_g = (time.perf_counter() - t0w) * 1e3 / (reps * nsteps_g)
+            mixed["graph"] = {"mpts": N / (ms_g * 1e-3) / 1e6, "ms_per_step": ms_g,
+                              "hbm_frac": MIXED_BYTES_PER_PT_STEP * N / (ms_g * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              "steps": reps * nsteps_g,
+                              "note": "the 256^3 mixed form as a captured graph of 10 fused steps (one launch "
+                                      "per stage of the batch stage kernel), replayed; wall time over the "
+                                      "replays"}
+            graph.close()
+            ug.free()
         prob.set_precision(S.F64)
 
     mpts = N / (ms_step * 1e-3) / 1e6

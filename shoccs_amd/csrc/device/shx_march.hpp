@@ -858,17 +858,11 @@ __device__ __forceinline__ void batch_block(const BatchEntry* __restrict__ ent, 
     stage_block<KIND, STAGE, W, TY, VEC, MIXED, true>(m, sp, ss, blockIdx.x - start[lo], sg, mg, mode);
 }
 
+// Held to four waves per SIMD, as k_stage_heat: the batch kernel carries
+// the special work of every system family it serves (the FP64 wave form
+// needs 133-135 registers unbounded, 111-127 and no scratch bounded).
 template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
-__global__ __launch_bounds__(64 * TY) void k_batch_stage(const BatchEntry* __restrict__ ent,
-                                                         const BatchClock* __restrict__ clk,
-                                                         const unsigned* __restrict__ start, int nprob)
-{
-    batch_block<KIND, STAGE, W, TY, VEC, MIXED>(ent, clk, start, nprob);
-}
-
-// The batch stage of the heat systems, bounded as k_stage_heat.
-template <int KIND, int STAGE, int W, int TY, int VEC, bool MIXED>
-__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_stage_heat(
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_stage(
     const BatchEntry* __restrict__ ent, const BatchClock* __restrict__ clk, const unsigned* __restrict__ start,
     int nprob)
 {
@@ -951,12 +945,8 @@ int batch_ty(hipStream_t st, int ty, const BatchLaunch& bl)
 {
 #define SHX_BATCH_CASE(TY, VEC)                                                                      \
     case TY * 10 + VEC:                                                                              \
-        if constexpr (kIsHeat<KIND>)                                                                 \
-            k_batch_stage_heat<KIND, STAGE, W, TY, VEC, MIXED><<<bl.grid, 64 * TY, 0, st>>>(bl.ent, bl.clk, \
-                                                                                      bl.start, bl.nprob); \
-        else                                                                                         \
-            k_batch_stage<KIND, STAGE, W, TY, VEC, MIXED><<<bl.grid, 64 * TY, 0, st>>>(bl.ent, bl.clk, bl.start, \
-                                                                                     bl.nprob);      \
+        k_batch_stage<KIND, STAGE, W, TY, VEC, MIXED><<<bl.grid, 64 * TY, 0, st>>>(bl.ent, bl.clk, bl.start, \
+                                                                                 bl.nprob);          \
         return 0;
     switch (ty * 10 + bl.vec) {
         SHX_BATCH_CASE(4, 1)
