@@ -1900,7 +1900,15 @@ int shx_system_autotune(shx_system* s, const shx_field* u0, double t, double dt,
     if (march_ok(s))
         for (int v = 1; v <= 4; ++v)
             for (int ty : {4, 8})
-                for (int xc : {16, 32, 64, 1 << 16}) cands.push_back({v, ty, std::min(xc, s->n[0])});
+                // chunk lengths that leave the tiles' last round of
+                // workgroups partly free for the special work, as well as
+                // the ones that fill whole rounds
+                for (int xc : {8, 12, 16, 20, 24, 32, 64, 1 << 16}) {
+                    const cand c{v, ty, std::min(xc, s->n[0])};
+                    bool dup = false;
+                    for (const cand& o : cands) dup = dup || (o.v == c.v && o.ty == c.ty && o.xc == c.xc);
+                    if (!dup) cands.push_back(c);
+                }
     if (cands.empty()) {
         for (int c = 0; c < 3; ++c) hipFree(tmp.R[c]);
         hipFree(tmp.D);
