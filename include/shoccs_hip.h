@@ -336,8 +336,9 @@ int shx_system_set_precision(shx_system* s, int precision);
 int shx_system_get_precision(const shx_system* s, int* precision);
 int shx_system_get_tuning(const shx_system* s, int out[3]);
 /* Points of D updated per stage by the timed fused-stage launch (out[0]) and,
- * among them, points evaluated from special-point records (out[1], 0 for
- * variant 0); builds the point classes of variant 1 if needed. */
+ * among them, points the special-point work writes (out[1]: points with a
+ * record and Dirichlet grid-face points; 0 for variant 0); builds the point
+ * classes of variant 1 if needed. */
 int shx_system_stage_points(shx_system* s, int64_t out[2]);
 /* Autotuner: times every candidate on a copy of u0 (u0 is not modified),
  * keeps the fastest and returns its ms per step. */

@@ -699,7 +699,8 @@ class Problem:
 
     def stage_points(self) -> tuple:
         """(points of D updated by the timed fused-stage launch, points among
-        them evaluated from special-point records) for the current variant."""
+        them written by the special-point work: those with a record and the
+        Dirichlet grid-face points) for the current variant."""
         self._need_device()
         out = (C.c_int64 * 2)()
         check(lib().shx_system_stage_points(self.sys, out), self.ctx.p, "stage_points")

@@ -100,9 +100,9 @@ struct RPtrs {
 //                terms whose columns the tile holds (x: the register queue,
 //                y/z: the LDS plane);
 //   kSpecial  -- anything else (rows with B terms, closures out of reach,
-//                Dirichlet faces): evaluated from its record by the
-//                special-point work.  Face records carry no rows: the
-//                right-hand side there is zero and the value exact(t_next).
+//                Dirichlet faces): written by the special-point work, from
+//                a record, or for a face point (whose right-hand side is
+//                zero) as exact(t_next) from its face item.
 // The classification kernel marks raw special points kRawSpecial | flags;
 // the host turns them into a pattern or kSpecial.
 constexpr unsigned char kInterior = 0;
