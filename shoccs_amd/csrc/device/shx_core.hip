@@ -57,6 +57,36 @@ __global__ __launch_bounds__(kBlock) void k_derivative(Tiling t, int n0, DevDeri
     int g, j, k;
     long unused;
     if (!(DIR == 2 ? tile_point_zpatch(t, g, j, k, unused) : tile_point(t, g, j, k, unused))) return;
+    if constexpr (DIR == 1 && XP > 1) {
+        // rows along y: the tiling counts groups of XP rows (t.n1 groups), a
+        // lane takes the XP consecutive rows of group j in its (x, z) column,
+        // and all their taps come from one window of rows (clamped to the
+        // line; the clamped values only reach rows outside the fast
+        // intervals, which the slow-row list evaluates)
+        const int ny = d.nq, nz = t.n2;
+        const int j0 = j * XP;
+        const long colb = (long)g * ny * nz + k;  // (g, 0, k)
+        const unsigned long long f = d.fast[g * nz + k];
+        double win[XP + W - 1];
+#pragma unroll
+        for (int m = 0; m < XP + W - 1; ++m) win[m] = u[colb + (long)min(max(j0 + m - hw, 0), ny - 1) * nz];
+        double old[XP];
+#pragma unroll
+        for (int p = 0; p < XP; ++p) old[p] = PLUS ? du[colb + (long)min(j0 + p, ny - 1) * nz] : 0.0;
+        const int lo0 = (int)(f & 0xffffu), hi0 = (int)((f >> 16) & 0xffffu);
+        const int lo1 = (int)((f >> 32) & 0xffffu), hi1 = (int)(f >> 48);
+#pragma unroll
+        for (int p = 0; p < XP; ++p) {
+            const int q = j0 + p;
+            if (q >= ny) break;
+            if (!((q >= lo0 && q < hi0) || (q >= lo1 && q < hi1))) continue;
+            double dot = 0.0;
+#pragma unroll
+            for (int m = 0; m < W; ++m) dot += d.coef[m] * win[p + m];
+            du[colb + (long)q * nz] = PLUS ? old[p] + dot : dot;
+        }
+        return;
+    }
     const int n1 = t.n1, n2 = t.n2;
     const long plane = (long)n1 * n2, col = (long)j * n2 + k;
     const int i0 = g * XP;
@@ -309,14 +339,17 @@ int launch_derivative(shx_derivative* d, const double* u, const double* uR, doub
 {
     shx_ctx* ctx = d->ctx;
     if (!d->dev.present) return SHX_OK;
-    // four planes per lane and the slow rows from their list (measured at
-    // 256^3: x 4.1, y 3.4, z 2.6 TB/s, against 3.0, 3.0, 1.9 with one plane
-    // and the segment scan in every lane), unless the grid would then hold
-    // fewer than eight workgroups per CU
+    // four planes (rows along y: four rows) per lane and the slow rows from
+    // their list (measured at 256^3: x 4.3, y 4.3 TB/s, against 3.0, 3.0
+    // with one plane and the segment scan in every lane), unless the grid
+    // would then hold fewer than eight workgroups per CU
     const int g4[3] = {(d->n[0] + kDerivPlanes - 1) / kDerivPlanes, d->n[1], d->n[2]};
     const Tiling t4 = make_tiling(g4);
     const bool multi = t4.nblocks >= 8u * 256u;
-    const Tiling t = multi ? t4 : make_tiling(d->n);
+    // rows along y take groups of kDerivPlanes consecutive rows per lane
+    // instead of planes (one window of rows per lane)
+    const int g4y[3] = {d->n[0], (d->n[1] + kDerivPlanes - 1) / kDerivPlanes, d->n[2]};
+    const Tiling t = multi ? (d->dir == 1 ? make_tiling(g4y) : t4) : make_tiling(d->n);
     // rows along z: the vector form when the z extent allows aligned vectors
     // and the grid is large enough for a second launch to pay (at 256^3 it
     // takes the z apply from 2.5 to 4.0 TB/s of 16 B per point)
