@@ -37,13 +37,59 @@ __global__ void k_copy_tail(double* __restrict__ dst, const double* __restrict__
     if (threadIdx.x == 0 && blockIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
 }
 
+// The rows outside the fast intervals, evaluated by workgroups appended to
+// the grid of the interior apply (after its `first` workgroups, so that they
+// fill the slots its last tiles leave idle): lane i takes listed row i
+// (closures with B terms, interior rows of a line's third and later
+// segments) for i < n, and closure end i - n (all its rows) for
+// n <= i < n + nw.
+struct ListArgs {
+    long n;
+    const SlowRow* rows;
+    long nw;
+    const WallSide* walls;
+    unsigned nlb;  // workgroups of the list
+};
+
+template <int W, bool PLUS, bool WITH_B>
+__device__ __forceinline__ void list_item(long i, const ListArgs& la, const DevDeriv& d,
+                                          const double* __restrict__ u, const double* __restrict__ uR,
+                                          double* __restrict__ du)
+{
+    if (i < la.n) {
+        const SlowRow e = la.rows[i];
+        const RowVal r = row_in_block<WITH_B>(d.blocks[e.b], d.pool, d.coef, W, d.stride, u, uR, e.idx, e.q);
+        du[e.idx] = PLUS ? row_plus(du[e.idx], r) : row_eq(r);
+        return;
+    }
+    if (i - la.n >= la.nw) return;
+    const WallSide e = la.walls[i - la.n];
+    const long st = d.stride;
+    double v[kWallCols];
+#pragma unroll
+    for (int m = 0; m < kWallCols; ++m) v[m] = m < e.cols ? u[e.base + (long)(e.col0 + m) * st] : 0.0;
+#pragma unroll
+    for (int r = 0; r < kWallRows; ++r) {
+        if (r < e.rows) {
+            const double* a = d.pool + e.coef + r * e.cols;
+            double dot = 0.0;
+#pragma unroll
+            for (int m = 0; m < kWallCols; ++m)
+                if (m < e.cols) dot += a[m] * v[m];
+            const long idx = e.base + (long)(e.q0 + r) * st;
+            du[idx] = PLUS ? du[idx] + dot : dot;
+        }
+    }
+}
+
 // Interior rows of the fast intervals.  A lane evaluates the points of its
 // (y, z) position in XP consecutive x planes.  Every point's taps (and, for
 // plus_eq, its old value) are loaded before any row is evaluated, so that a
 // lane keeps several points' loads in flight; rows along x share one window
 // of planes.  Each row is summed as row_at sums it.  With XP > 1 (large
-// grids) every other row comes from the slow-row list (k_deriv_list, a launch
-// of its own), so no wave waits on a segment scan; with XP = 1 (grids too
+// grids) every other row comes from the slow-row list and the closure ends
+// (list_item, in workgroups after the tiles), so no wave waits on a segment
+// scan; with XP = 1 (grids too
 // small to fill the chip, where a second launch costs more than the
 // divergence) the lane scans its line's segments itself.  Points with no row
 // are left as they are.  t counts groups of XP planes along x.
@@ -51,12 +97,20 @@ template <int DIR, int W, bool PLUS, bool WITH_B, int XP>
 __global__ __launch_bounds__(kBlock) void k_derivative(Tiling t, int n0, DevDeriv d,
                                                        const double* __restrict__ u,
                                                        const double* __restrict__ uR,
-                                                       double* __restrict__ du)
+                                                       double* __restrict__ du, ListArgs la)
 {
     constexpr int hw = W / 2;
+    unsigned bid = blockIdx.x;
+    if constexpr (XP > 1) {
+        if (bid < la.nlb) {
+            list_item<W, PLUS, WITH_B>((long)bid * kBlock + threadIdx.x, la, d, u, uR, du);
+            return;
+        }
+        bid -= la.nlb;
+    }
     int g, j, k;
     long unused;
-    if (!(DIR == 2 ? tile_point_zpatch(t, g, j, k, unused) : tile_point(t, g, j, k, unused))) return;
+    if (!(DIR == 2 ? tile_point_zpatch(t, bid, g, j, k, unused) : tile_point(t, bid, g, j, k, unused))) return;
     if constexpr (DIR == 1 && XP > 1) {
         // rows along y: the tiling counts groups of XP rows (t.n1 groups), a
         // lane takes the XP consecutive rows of group j in its (x, z) column,
@@ -162,15 +216,22 @@ struct VecT {
 // (the lane's own and its neighbours' positions, clamped into the line; the
 // clamped values only reach rows outside the fast intervals), so a point
 // costs 3 / VEC loads instead of W.  Each row is summed as row_at sums it;
-// every row outside the line's fast intervals is left to the slow-row list.
-template <int W, bool PLUS, int VEC, int NL>
+// every row outside the line's fast intervals is left to list_item, in the
+// workgroups after the first la.first.
+template <int W, bool PLUS, bool WITH_B, int VEC, int NL>
 __global__ __launch_bounds__(256) void k_derivative_z(long nlines, int n2, int nzc, DevDeriv d,
-                                                      const double* __restrict__ u, double* __restrict__ du)
+                                                      const double* __restrict__ u,
+                                                      const double* __restrict__ uR, double* __restrict__ du,
+                                                      ListArgs la)
 {
     constexpr int hw = W / 2;
     static_assert(hw <= VEC, "the taps lie in the three vectors");
     using V = typename VecT<VEC>::type;
-    const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+    if (blockIdx.x < la.nlb) {
+        list_item<W, PLUS, WITH_B>((long)blockIdx.x * 256 + threadIdx.x, la, d, u, uR, du);
+        return;
+    }
+    const unsigned b = xcd_remap(blockIdx.x - la.nlb, gridDim.x - la.nlb);
     const long task = (long)b * 4 + threadIdx.x / 64;  // one wave: one z chunk of NL lines
     const int zc = (int)(task % nzc);
     const long l0 = (task / nzc) * NL;
@@ -225,20 +286,6 @@ __global__ __launch_bounds__(256) void k_derivative_z(long nlines, int n2, int n
                 if (in[e]) p[e] = o[e];
         }
     }
-}
-
-// The listed rows (closures, B terms, interior rows of a line's third and
-// later segments), one lane each.
-template <int W, bool PLUS, bool WITH_B>
-__global__ __launch_bounds__(256) void k_deriv_list(long n, const SlowRow* __restrict__ rows, DevDeriv d,
-                                                    const double* __restrict__ u,
-                                                    const double* __restrict__ uR, double* __restrict__ du)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const SlowRow e = rows[i];
-    const RowVal r = row_in_block<WITH_B>(d.blocks[e.b], d.pool, d.coef, W, d.stride, u, uR, e.idx, e.q);
-    du[e.idx] = PLUS ? row_plus(du[e.idx], r) : row_eq(r);
 }
 
 // N matrix rows, compressed to the rows that hold entries.
@@ -334,6 +381,14 @@ static int launch_copy(shx_ctx* ctx, double* dst, const double* src, long n)
     return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "copy");
 }
 
+// The list's workgroups, appended to an interior apply of `first` workgroups.
+ListArgs list_args(const shx_derivative* d, unsigned first, unsigned& grid)
+{
+    const ListArgs la{d->nslow_list, d->slow, d->nwall, d->walls, (unsigned)((d->nslow_list + d->nwall + 255) / 256)};
+    grid = first + la.nlb;
+    return la;
+}
+
 int launch_derivative(shx_derivative* d, const double* u, const double* uR, double* du, bool plus,
                       bool with_b)
 {
@@ -359,46 +414,38 @@ int launch_derivative(shx_derivative* d, const double* u, const double* uR, doub
         const long nlines = (long)d->n[0] * d->n[1];
         const int nzc = (d->n[2] + 64 * vec - 1) / (64 * vec);
         const long waves = (nlines + NL - 1) / NL * nzc;
-        const unsigned grid = (unsigned)((waves + 3) / 4);
-        const int zkey = d->dev.width * 100 + vec * 10 + (plus ? 1 : 0);
+        unsigned grid = 0;
+        const ListArgs la = list_args(d, (unsigned)((waves + 3) / 4), grid);
+        const int zkey = d->dev.width * 1000 + vec * 100 + (plus ? 10 : 0) + (with_b ? 1 : 0);
         switch (zkey) {
-#define SHX_DZ_CASE(W, V, P) \
-    case W * 100 + V * 10 + (P ? 1 : 0): \
-        k_derivative_z<W, P, V, NL><<<grid, 256, 0, ctx->stream>>>(nlines, d->n[2], nzc, d->dev, u, du); \
+#define SHX_DZ_CASE(W, V, P, B) \
+    case W * 1000 + V * 100 + (P ? 10 : 0) + (B ? 1 : 0): \
+        k_derivative_z<W, P, B, V, NL><<<grid, 256, 0, ctx->stream>>>(nlines, d->n[2], nzc, d->dev, u, uR, du, la); \
         break;
-            SHX_DZ_CASE(3, 2, false) SHX_DZ_CASE(3, 2, true) SHX_DZ_CASE(3, 4, false) SHX_DZ_CASE(3, 4, true)
-            SHX_DZ_CASE(5, 2, false) SHX_DZ_CASE(5, 2, true) SHX_DZ_CASE(5, 4, false) SHX_DZ_CASE(5, 4, true)
+#define SHX_DZ_W(W) \
+    SHX_DZ_CASE(W, 2, false, false) SHX_DZ_CASE(W, 2, false, true) SHX_DZ_CASE(W, 2, true, false) \
+    SHX_DZ_CASE(W, 2, true, true) SHX_DZ_CASE(W, 4, false, false) SHX_DZ_CASE(W, 4, false, true) \
+    SHX_DZ_CASE(W, 4, true, false) SHX_DZ_CASE(W, 4, true, true)
+            SHX_DZ_W(3) SHX_DZ_W(5)
+#undef SHX_DZ_W
 #undef SHX_DZ_CASE
         default:
             return fail(ctx, SHX_EUNSUPPORTED, "derivative: interior width must be 3 or 5");
         }
-        if (d->nslow > 0) {
-            const unsigned g = (unsigned)((d->nslow + 255) / 256);
-            if (d->dev.width == 3) {
-                if (plus && with_b) k_deriv_list<3, true, true><<<g, 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du);
-                else if (plus) k_deriv_list<3, true, false><<<g, 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du);
-                else if (with_b) k_deriv_list<3, false, true><<<g, 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du);
-                else k_deriv_list<3, false, false><<<g, 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du);
-            } else {
-                if (plus && with_b) k_deriv_list<5, true, true><<<g, 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du);
-                else if (plus) k_deriv_list<5, true, false><<<g, 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du);
-                else if (with_b) k_deriv_list<5, false, true><<<g, 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du);
-                else k_deriv_list<5, false, false><<<g, 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du);
-            }
-        }
         hipError_t e = hipGetLastError();
         return e == hipSuccess ? SHX_OK : hip_fail(ctx, e, "derivative kernel");
     }
+    static_assert(kBlock == 256, "the list's workgroups have the tiles' size");
+    unsigned grid = 0;
+    const ListArgs la = multi ? list_args(d, t.nblocks, grid) : ListArgs{0, nullptr, 0, nullptr, 0};
     const int key = d->dir * 100 + d->dev.width * 10 + (plus ? 2 : 0) + (with_b ? 1 : 0);
     switch (key) {
 #define SHX_DERIV_CASE(D, W, P, B) \
     case D * 100 + W * 10 + (P ? 2 : 0) + (B ? 1 : 0): \
         if (multi) \
-            k_derivative<D, W, P, B, kDerivPlanes><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->n[0], d->dev, u, uR, du); \
+            k_derivative<D, W, P, B, kDerivPlanes><<<grid, kBlock, 0, ctx->stream>>>(t, d->n[0], d->dev, u, uR, du, la); \
         else \
-            k_derivative<D, W, P, B, 1><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->n[0], d->dev, u, uR, du); \
-        if (multi && d->nslow > 0) \
-            k_deriv_list<W, P, B><<<(unsigned)((d->nslow + 255) / 256), 256, 0, ctx->stream>>>(d->nslow, d->slow, d->dev, u, uR, du); \
+            k_derivative<D, W, P, B, 1><<<t.nblocks, kBlock, 0, ctx->stream>>>(t, d->n[0], d->dev, u, uR, du, la); \
         break;
 #define SHX_DERIV_DIR(D, W) \
     SHX_DERIV_CASE(D, W, false, false) SHX_DERIV_CASE(D, W, false, true) \
@@ -715,8 +762,13 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
     }
     for (long p = 0; p < npairs; ++p) pair_ptr[p + 1] += pair_ptr[p];
 
-    // every row outside its line's fast intervals, sorted by point
-    std::vector<SlowRow> slow;
+    // every row outside its line's fast intervals, sorted by point: first
+    // those that the list evaluates, then the closure rows of the ends that
+    // one lane each evaluates (no B term at the end, closures that do not
+    // overlap, at most kWallRows rows of kWallCols columns).  The Euler flux
+    // derivative takes the whole list.
+    std::vector<SlowRow> slow, slow_wall;
+    std::vector<WallSide> walls;
     for (long b = 0; b < desc->nblocks; ++b) {
         const shx_line_block& blk = desc->blocks[b];
         const unsigned long long fd = fast[blk.pair];
@@ -728,16 +780,36 @@ int shx_derivative_create(shx_ctx* ctx, const shx_derivative_desc* desc, shx_der
         else if (dir == 1) base = (long)s * desc->n[1] * desc->n[2] + f;
         else base = ((long)s * desc->n[1] + f) * desc->n[2];
         const int total = blk.nl + blk.ni + blk.nr;
+        const bool lw = blk.nl > 0 && blk.ni >= 0 && blk.lobj < 0 && blk.nl <= kWallRows &&
+                        blk.lcols > 0 && blk.lcols <= kWallCols;
+        const bool rw = blk.nr > 0 && blk.ni >= 0 && blk.robj < 0 && blk.nr <= kWallRows &&
+                        blk.rcols > 0 && blk.rcols <= kWallCols;
+        if (lw) walls.push_back(WallSide{base, blk.lcoef, blk.lo, blk.lcol0, (short)blk.nl, (short)blk.lcols});
+        if (rw)
+            walls.push_back(WallSide{base, blk.rcoef, blk.lo + blk.nl + blk.ni, blk.rcol0, (short)blk.nr,
+                                     (short)blk.rcols});
         for (int q = blk.lo; q < blk.lo + total; ++q) {
             if ((q >= lo0 && q < hi0) || (q >= lo1 && q < hi1)) continue;
-            slow.push_back(SlowRow{base + (long)q * d->dev.stride, (int)b, q});
+            const int lr = q - blk.lo;
+            const bool end = (lw && lr < blk.nl) || (rw && lr >= blk.nl + blk.ni);
+            (end ? slow_wall : slow).push_back(SlowRow{base + (long)q * d->dev.stride, (int)b, q});
         }
     }
-    std::sort(slow.begin(), slow.end(), [](const SlowRow& a, const SlowRow& b) { return a.idx < b.idx; });
+    const auto by_idx = [](const SlowRow& a, const SlowRow& b) { return a.idx < b.idx; };
+    std::sort(slow.begin(), slow.end(), by_idx);
+    std::sort(slow_wall.begin(), slow_wall.end(), by_idx);
+    d->nslow_list = (long)slow.size();
+    slow.insert(slow.end(), slow_wall.begin(), slow_wall.end());
     d->nslow = (long)slow.size();
+    // neighbouring lanes take the same end of neighbouring lines
+    std::sort(walls.begin(), walls.end(), [](const WallSide& a, const WallSide& b) {
+        return a.q0 != b.q0 ? a.q0 < b.q0 : a.base < b.base;
+    });
+    d->nwall = (long)walls.size();
 
     int rc = upload(ctx, &d->pair_ptr, pair_ptr.data(), pair_ptr.size());
     if (rc == SHX_OK && d->nslow) rc = upload(ctx, &d->slow, slow.data(), slow.size());
+    if (rc == SHX_OK && d->nwall) rc = upload(ctx, &d->walls, walls.data(), walls.size());
     if (rc == SHX_OK) rc = upload(ctx, &d->fast, fast.data(), fast.size());
     if (rc == SHX_OK && desc->nblocks)
         rc = upload(ctx, &d->blocks, desc->blocks, (size_t)desc->nblocks);
@@ -824,6 +896,7 @@ int shx_derivative_destroy(shx_derivative* d)
     hipFree(d->pair_ptr);
     hipFree(d->fast);
     hipFree(d->slow);
+    hipFree(d->walls);
     hipFree(d->blocks);
     hipFree(d->pool);
     hipFree(d->coef);

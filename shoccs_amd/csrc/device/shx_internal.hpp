@@ -167,6 +167,18 @@ struct SlowRow {
     int q;
 };
 
+// The closure rows of one end of a segment with no B term at that end (the
+// grid walls, above all), evaluated by one lane: the end's columns are loaded
+// once and every row of the end sums over them, from 0.0 in column order.
+constexpr int kWallRows = 8, kWallCols = 8;
+struct WallSide {
+    long base;  // point of position 0 of the line
+    int coef;   // pool offset of the rows (row-major, `cols` values per row)
+    int q0;     // position of the first row
+    int col0;   // position of the first column
+    short rows, cols;
+};
+
 // Row of direction DIR (stencil width W) at point idx = (i, j, k).  The W taps
 // are loaded before the interval test, at positions clamped to the line, so
 // that their loads are issued together; they are used only when q lies in one
@@ -249,10 +261,11 @@ inline Tiling make_tiling(const int n[3])
     return t;
 }
 
-// Point handled by this thread; returns false outside the mesh.
-__device__ __forceinline__ bool tile_point(const Tiling& t, int& i, int& j, int& k, long& idx)
+// Point handled by this thread of tile workgroup b; returns false outside the
+// mesh.
+__device__ __forceinline__ bool tile_point(const Tiling& t, unsigned b, int& i, int& j, int& k, long& idx)
 {
-    const unsigned L = xcd_remap(blockIdx.x, t.nblocks);
+    const unsigned L = xcd_remap(b, t.nblocks);
     i = (int)(L % (unsigned)t.n0);
     const unsigned rest = L / (unsigned)t.n0;
     const int bz = (int)(rest % (unsigned)t.nbz);
@@ -268,11 +281,12 @@ __device__ __forceinline__ bool tile_point(const Tiling& t, int& i, int& j, int&
 // 64 / kTileY points along z, for rows along z: the closure rows at the ends
 // of a z line then fall in the end waves of the end tiles only, instead of
 // diverging every wave of those tiles.
-__device__ __forceinline__ bool tile_point_zpatch(const Tiling& t, int& i, int& j, int& k, long& idx)
+__device__ __forceinline__ bool tile_point_zpatch(const Tiling& t, unsigned b, int& i, int& j, int& k,
+                                                  long& idx)
 {
     constexpr int ZW = 64 / kTileY;
     static_assert(kTileZ == 64 && 64 % kTileY == 0, "a wave holds whole rows of its z patch");
-    const unsigned L = xcd_remap(blockIdx.x, t.nblocks);
+    const unsigned L = xcd_remap(b, t.nblocks);
     i = (int)(L % (unsigned)t.n0);
     const unsigned rest = L / (unsigned)t.n0;
     const int bz = (int)(rest % (unsigned)t.nbz);
@@ -282,6 +296,17 @@ __device__ __forceinline__ bool tile_point_zpatch(const Tiling& t, int& i, int& 
     if (k >= t.n2 || j >= t.n1) return false;
     idx = ((long)i * t.n1 + j) * t.n2 + k;
     return true;
+}
+
+// The same for the workgroups of a launch that holds tiles only.
+__device__ __forceinline__ bool tile_point(const Tiling& t, int& i, int& j, int& k, long& idx)
+{
+    return tile_point(t, blockIdx.x, i, j, k, idx);
+}
+
+__device__ __forceinline__ bool tile_point_zpatch(const Tiling& t, int& i, int& j, int& k, long& idx)
+{
+    return tile_point_zpatch(t, blockIdx.x, i, j, k, idx);
 }
 
 }  // namespace shx

@@ -29,7 +29,10 @@ struct shx_derivative {
     int* pair_ptr = nullptr;
     unsigned long long* fast = nullptr;
     long nslow = 0;                  // rows outside the fast intervals
-    shx::SlowRow* slow = nullptr;    // sorted by point
+    long nslow_list = 0;             // those not in a WallSide: the first of `slow`
+    shx::SlowRow* slow = nullptr;    // sorted by point (the first nslow_list, then the rest)
+    long nwall = 0;                  // closure ends evaluated one lane each
+    shx::WallSide* walls = nullptr;  // sorted by first row position, then line
     shx_line_block* blocks = nullptr;
     double* pool = nullptr;
     double* coef = nullptr;
