@@ -230,6 +230,21 @@ def derivative_microbench(ctx, sides=(64, 256), reps=50):
     return out
 
 
+def derivative_only(args):
+    """The derivative microbenchmark alone (profiling runs): the reference's
+    bench_derivative case (E4 second derivative, floating faces) at 64^3 and
+    256^3, each direction's apply timed over 50 launches."""
+    import shoccs_amd as S
+
+    ctx = S.Context(0)
+    r = derivative_microbench(ctx)
+    ctx.close()
+    print(json.dumps({"metric": "derivative_apply_bandwidth", "unit": "GB/s at 16 B/pt",
+                      "value": r["256^3"]["z"]["gbs_16B_per_pt"], "higher_is_better": True,
+                      "dtype": "f64", "data": "synthetic", "config": {"workload": "derivative"},
+                      "derivative_microbench": r}))
+
+
 def heat_batch64_result(ctx, steps):
     """Config of 64 independent 3-D heat problems (per-axis sizes drawn from
     seed 0 in 32..128, a Dirichlet sphere each with centre in [0.3, 0.7]^3 and
@@ -637,10 +652,11 @@ def main():
                     help="only the headline forms (profiling runs): no accuracy companions, "
                          "derivative microbenchmark or batch")
     ap.add_argument("--workload", default="wave256",
-                    choices=["wave256", "heat-batch64", "adjoint128", "euler192"],
+                    choices=["wave256", "heat-batch64", "adjoint128", "euler192", "derivative"],
                     help="wave256: the headline 256^3 scalar wave; heat-batch64: the batched heat "
                          "configuration; adjoint128: the adjoint gradient of a 128^3 solve; euler192: "
-                         "the 192^3 compressible Euler configuration (each its own JSON line)")
+                         "the 192^3 compressible Euler configuration; derivative: the derivative "
+                         "microbenchmark alone, its value the 256^3 z apply (each its own JSON line)")
     ap.add_argument("--n", type=int, default=N_SIDE)
     ap.add_argument("--adj-cfl", type=float, default=0.05, help="CFL of the adjoint workload")
     args = ap.parse_args()
@@ -657,6 +673,9 @@ def main():
         return
     if args.workload == "euler192":
         euler192(args)
+        return
+    if args.workload == "derivative":
+        derivative_only(args)
         return
 
     import numpy as np
