@@ -37,17 +37,20 @@ __global__ void k_copy_tail(double* __restrict__ dst, const double* __restrict__
     if (threadIdx.x == 0 && blockIdx.x == 0 && (n & 1)) dst[n - 1] = src[n - 1];
 }
 
-// The rows outside the fast intervals, evaluated by workgroups appended to
-// the grid of the interior apply (after its `first` workgroups, so that they
-// fill the slots its last tiles leave idle): lane i takes listed row i
+// The rows outside the fast intervals, evaluated by the first nlb
+// workgroups of the interior apply's launch: lane i takes listed row i
 // (closures with B terms, interior rows of a line's third and later
-// segments) for i < n, and closure end i - n (all its rows) for
-// n <= i < n + nw.
+// segments) for i < n; the L = 2^lsh lanes n + L w .. n + L w + L - 1 take
+// closure end w, lane r of them its rows r, r + L, ...  Along x and y one
+// lane takes an end (neighbouring lanes take neighbouring lines, whose
+// columns are adjacent in memory); along z four (the four load the same
+// columns, so that a load instruction touches 16 lines, not 64).
 struct ListArgs {
     long n;
     const SlowRow* rows;
     long nw;
     const WallSide* walls;
+    int lsh;
     unsigned nlb;  // workgroups of the list
 };
 
@@ -62,14 +65,15 @@ __device__ __forceinline__ void list_item(long i, const ListArgs& la, const DevD
         du[e.idx] = PLUS ? row_plus(du[e.idx], r) : row_eq(r);
         return;
     }
-    if (i - la.n >= la.nw) return;
-    const WallSide e = la.walls[i - la.n];
+    const long w = (i - la.n) >> la.lsh;
+    const int r0 = (int)((i - la.n) & ((1 << la.lsh) - 1));
+    if (w >= la.nw) return;
+    const WallSide e = la.walls[w];
     const long st = d.stride;
     double v[kWallCols];
 #pragma unroll
     for (int m = 0; m < kWallCols; ++m) v[m] = m < e.cols ? u[e.base + (long)(e.col0 + m) * st] : 0.0;
-#pragma unroll
-    for (int r = 0; r < kWallRows; ++r) {
+    for (int r = r0; r < kWallRows; r += 1 << la.lsh) {
         if (r < e.rows) {
             const double* a = d.pool + e.coef + r * e.cols;
             double dot = 0.0;
@@ -384,7 +388,9 @@ static int launch_copy(shx_ctx* ctx, double* dst, const double* src, long n)
 // The list's workgroups, appended to an interior apply of `first` workgroups.
 ListArgs list_args(const shx_derivative* d, unsigned first, unsigned& grid)
 {
-    const ListArgs la{d->nslow_list, d->slow, d->nwall, d->walls, (unsigned)((d->nslow_list + d->nwall + 255) / 256)};
+    const int lsh = d->dir == 2 ? 2 : 0;
+    const long lanes = d->nslow_list + (d->nwall << lsh);
+    const ListArgs la{d->nslow_list, d->slow, d->nwall, d->walls, lsh, (unsigned)((lanes + 255) / 256)};
     grid = first + la.nlb;
     return la;
 }
@@ -437,7 +443,7 @@ int launch_derivative(shx_derivative* d, const double* u, const double* uR, doub
     }
     static_assert(kBlock == 256, "the list's workgroups have the tiles' size");
     unsigned grid = 0;
-    const ListArgs la = multi ? list_args(d, t.nblocks, grid) : ListArgs{0, nullptr, 0, nullptr, 0};
+    const ListArgs la = multi ? list_args(d, t.nblocks, grid) : ListArgs{0, nullptr, 0, nullptr, 0, 0};
     const int key = d->dir * 100 + d->dev.width * 10 + (plus ? 2 : 0) + (with_b ? 1 : 0);
     switch (key) {
 #define SHX_DERIV_CASE(D, W, P, B) \
