@@ -31,13 +31,18 @@ ALL = {**WAVE_CFGS, **HEAT_CFGS}
 # of the tile or of the four planes
 MULTI_PLANE = {"heat3d_multi_plane": H.heat3d_cfg(n=(130, 66, 258), center=(0.5, 0.45, 0.55), radius=0.3)}
 # z extent a multiple of 4 (four points per lane in the vector z kernel), with
-# floating and with Dirichlet grid faces: the lines the sphere misses are one
-# segment whose wall-closure rows the z kernel evaluates in its edge lanes,
-# the others keep theirs in the slow-row list
+# floating and with Dirichlet grid faces: every grid-wall closure end is
+# evaluated by its own lanes (four per end along z), the closures at the
+# sphere (B terms) come from the slow-row list
 for _faces, _bc in (("floating", [O.FLOATING] * 6), ("dirichlet", [O.DIRICHLET, O.FLOATING] * 3)):
     _cfg = H.heat3d_cfg(n=(130, 66, 256), center=(0.5, 0.45, 0.55), radius=0.3)
     _cfg["grid_bc"] = _bc
     MULTI_PLANE[f"heat3d_multi_plane_z4_{_faces}"] = _cfg
+# odd z extent: the z rows take the point-wise kernel with four planes per
+# lane, and its grid-wall ends the four-lane form
+_cfg = H.heat3d_cfg(n=(130, 66, 257), center=(0.5, 0.45, 0.55), radius=0.3)
+_cfg["grid_bc"] = [O.FLOATING] * 6
+MULTI_PLANE["heat3d_multi_plane_zodd_floating"] = _cfg
 
 
 def _rand_field(osys, seed):
