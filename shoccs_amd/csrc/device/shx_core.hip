@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_derivative(Tiling t, int n0, DevDeri
         // lane takes the XP consecutive rows of group j in its (x, z) column,
         // and all their taps come from one window of rows (clamped to the
         // line; the clamped values only reach rows outside the fast
-        // intervals, which the slow-row list evaluates)
+        // intervals, which list_item evaluates)
         const int ny = d.nq, nz = t.n2;
         const int j0 = j * XP;
         const long colb = (long)g * ny * nz + k;  // (g, 0, k)
